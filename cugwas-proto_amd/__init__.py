@@ -1,0 +1,23 @@
+"""MI355X-native Cholesky FGLS (drop-in for the reference's fgls_chol_gpu path).
+
+Import name: ``cugwas_proto_amd`` (the directory name carries a hyphen, so
+``load_package()`` in the repository's entry points registers it).
+"""
+from .fgls import (  # noqa: F401
+    DeviceContext,
+    FglsError,
+    colmajor,
+    device_count,
+    exported_symbols,
+    fgls_chol_gpu,
+    last_error,
+    lib,
+    LIB_PATH,
+    FGLS_OK,
+    FGLS_ERR_ARG,
+    FGLS_ERR_HIP,
+    FGLS_ERR_NOMEM,
+    FGLS_ERR_NOTSPD,
+    FGLS_ERR_IO,
+    FGLS_ERR_STATE,
+)

@@ -1,0 +1,96 @@
+"""Build recipe of the package's native code (no CMake; hipcc + gcc).
+
+Outputs (git-ignored, shipped with the tree):
+  lib/libfgls_amd.so   HIP kernels + C-ABI (include/fgls_hip.h) + the C host
+                       solver fgls_chol_gpu (include/fgls_chol_gpu.h)
+  lib/driver.x         the reference's CLI contract (driver.c) over that solver
+"""
+from __future__ import annotations
+
+import glob
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIBDIR = os.path.join(PKG, "lib")
+INCLUDE = os.path.join(ROOT, "include")
+ARCH = "gfx950"
+
+# All HIP code is compiled without packed FP32 instructions.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result"]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def openblas_lib() -> str:
+    """LP64 OpenBLAS bundled with scipy (exports scipy_-prefixed LAPACK)."""
+    import scipy
+    libs = glob.glob(os.path.join(os.path.dirname(scipy.__file__) + ".libs", "libscipy_openblas*.so"))
+    libs = [p for p in libs if "64_" not in os.path.basename(p)]
+    if not libs:
+        raise RuntimeError("scipy's bundled OpenBLAS not found")
+    return sorted(libs)[0]
+
+
+def run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"build step failed: {' '.join(cmd[:3])} ...")
+    return res
+
+
+def build(verbose: bool = False) -> dict:
+    os.makedirs(LIBDIR, exist_ok=True)
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    blas = openblas_lib()
+    blasdir = os.path.dirname(blas)
+    blasname = os.path.basename(blas)
+
+    kobj = os.path.join(objdir, "fgls_kernels.o")
+    run([hipcc(), *HIP_FLAGS, "-I", INCLUDE, "-c", os.path.join(CSRC, "fgls_kernels.hip"), "-o", kobj], verbose)
+
+    host_objs = []
+    for src in sorted(glob.glob(os.path.join(CSRC, "*.c"))):
+        if os.path.basename(src) == "driver.c":
+            continue
+        obj = os.path.join(objdir, os.path.basename(src)[:-2] + ".o")
+        run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
+             "-fopenmp", "-I", INCLUDE, "-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+             "-c", src, "-o", obj], verbose)
+        host_objs.append(obj)
+
+    so = os.path.join(LIBDIR, "libfgls_amd.so")
+    tmp = so + ".tmp"
+    run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, kobj, *host_objs,
+         "-L", blasdir, f"-l:{blasname}", f"-Wl,-rpath,{blasdir}", "-fopenmp", "-lpthread", "-lrt"],
+        verbose)
+    os.replace(tmp, so)
+
+    out = {"lib": so}
+    drv = os.path.join(CSRC, "driver.c")
+    if os.path.exists(drv):
+        exe = os.path.join(LIBDIR, "driver.x")
+        run(["gcc", "-O2", "-std=gnu11", "-Wall", "-I", INCLUDE, drv, "-o", exe,
+             "-L", LIBDIR, "-lfgls_amd", "-Wl,-rpath,$ORIGIN",
+             "-Wl,-rpath-link," + blasdir], verbose)
+        out["driver"] = exe
+    return out
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,440 @@
+/*
+ * fgls_host.c -- C host side of the Cholesky FGLS solver for one MI355X.
+ *
+ * Implements fgls_chol_gpu() (include/fgls_chol_gpu.h) with the reference's
+ * arguments, files and output layout (src/fgls_chol_gpu.c:97-795), as a
+ * single-GPU streaming pipeline:
+ *
+ *   reader thread:  pread(XR.in block c)  -> pinned host slot c%2
+ *   copy stream:    hipMemcpy2DAsync slot -> device X_R buffer c%2
+ *   compute stream: fgls_hip_solve_block (TRMM + per-SNP stage) -> device B/V c%2
+ *                   hipMemcpyAsync B/V -> pinned host B/V c%2
+ *   main thread:    pwrite B/V of block c-1 at the reference's offsets
+ *
+ * Ordering is by hipEvents; each buffer pair is double buffered, so disk
+ * reads, PCIe uploads, device compute and output writes of neighbouring
+ * blocks overlap.  Per trait: M is formed and factorised on the host
+ * (LAPACK dpotrf, as fgls_chol_gpu.c:406-430), L is uploaded once and the
+ * device precompute runs (fgls_hip_set_trait).  X_R is streamed once per
+ * trait (the CPU twin's trait loop, fgls_chol.c:223 and :336-337).
+ */
+#define _GNU_SOURCE
+#include <errno.h>
+#include <fcntl.h>
+#include <math.h>
+#include <pthread.h>
+#include <sched.h>
+#include <semaphore.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <time.h>
+#include <unistd.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "fgls_chol_gpu.h"
+#include "fgls_hip.h"
+#include "fgls_internal.h"
+
+/* LP64 LAPACK of the OpenBLAS build the library is linked with. */
+extern void scipy_dpotrf_(const char *uplo, const int *n, double *a, const int *lda, int *info);
+extern void scipy_openblas_set_num_threads(int n);
+
+typedef struct {
+  int64_t block;     /* SNPs per device launch */
+  int device;
+  int serial;
+  int stats;
+} options_t;
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static int host_cores(void) {
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) return CPU_COUNT(&set);
+  long n = sysconf(_SC_NPROCESSORS_ONLN);
+  return n > 0 ? (int)n : 1;
+}
+
+static void parse_options(const char *s, options_t *o) {
+  if (!s) return;
+  char buf[512];
+  snprintf(buf, sizeof(buf), "%s", s);
+  char *save = NULL;
+  for (char *tok = strtok_r(buf, ",", &save); tok; tok = strtok_r(NULL, ",", &save)) {
+    char *eq = strchr(tok, '=');
+    if (!eq) continue;
+    *eq = 0;
+    long long v = atoll(eq + 1);
+    if (!strcmp(tok, "block")) o->block = v;
+    else if (!strcmp(tok, "device")) o->device = (int)v;
+    else if (!strcmp(tok, "serial")) o->serial = (int)v;
+    else if (!strcmp(tok, "stats")) o->stats = (int)v;
+  }
+}
+
+/* Read exactly `count` doubles at element offset `start` (io.c:7-19, but a
+ * short read is an error here instead of a printed warning). */
+static int read_exact(const char *path, double *buf, size_t count, size_t start) {
+  int fd = open(path, O_RDONLY);
+  if (fd < 0) return fgls_set_error(FGLS_ERR_IO, "cannot open %s: %s", path, strerror(errno));
+  size_t want = count * sizeof(double), got = 0;
+  off_t off = (off_t)(start * sizeof(double));
+  while (got < want) {
+    ssize_t r = pread(fd, (char *)buf + got, want - got, off + (off_t)got);
+    if (r < 0 && errno == EINTR) continue;
+    if (r <= 0) break;
+    got += (size_t)r;
+  }
+  close(fd);
+  if (got != want)
+    return fgls_set_error(FGLS_ERR_IO, "%s: read %zu elements - %zu expected", path,
+                          got / sizeof(double), count);
+  return FGLS_OK;
+}
+
+static int write_at(int fd, const char *name, const void *buf, size_t bytes, off_t off) {
+  size_t done = 0;
+  while (done < bytes) {
+    ssize_t w = pwrite(fd, (const char *)buf + done, bytes - done, off + (off_t)done);
+    if (w < 0 && errno == EINTR) continue;
+    if (w <= 0) return fgls_set_error(FGLS_ERR_IO, "write to %s failed: %s", name, strerror(errno));
+    done += (size_t)w;
+  }
+  return FGLS_OK;
+}
+
+#define HCHK(call)                                                                       \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      rc = fgls_set_error(FGLS_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));  \
+      goto done;                                                                         \
+    }                                                                                    \
+  } while (0)
+
+#define RCHK(expr)          \
+  do {                      \
+    rc = (expr);            \
+    if (rc != FGLS_OK) goto done; \
+  } while (0)
+
+/* ---------------------------------------------------------------- reader */
+typedef struct {
+  int fd;
+  const char *path;
+  int n, wXR, t;
+  int64_t m, block, nchunks;
+  double *slot[2];          /* pinned */
+  sem_t free_[2], full[2];
+  int status[2];            /* FGLS_OK or FGLS_ERR_IO per filled slot */
+  char err[2][256];
+  volatile int stop;
+  double busy_s;            /* time spent in pread */
+} reader_t;
+
+static void *reader_main(void *arg) {
+  reader_t *R = (reader_t *)arg;
+  const size_t col_bytes = (size_t)R->n * sizeof(double);
+  for (int64_t it = 0; it < (int64_t)R->t * R->nchunks; ++it) {
+    const int s = (int)(it & 1);
+    sem_wait(&R->free_[s]);
+    if (R->stop) break;
+    const int64_t c = it % R->nchunks;
+    const int64_t i0 = c * R->block;
+    const int64_t cnt = (R->m - i0) < R->block ? (R->m - i0) : R->block;
+    const size_t bytes = (size_t)cnt * R->wXR * col_bytes;
+    const off_t off = (off_t)i0 * R->wXR * (off_t)col_bytes;
+    double t0 = now_s();
+    size_t got = 0;
+    while (got < bytes) {
+      ssize_t r = pread(R->fd, (char *)R->slot[s] + got, bytes - got, off + (off_t)got);
+      if (r < 0 && errno == EINTR) continue;
+      if (r <= 0) break;
+      got += (size_t)r;
+    }
+    R->busy_s += now_s() - t0;
+    if (got != bytes) {
+      R->status[s] = FGLS_ERR_IO;
+      snprintf(R->err[s], sizeof(R->err[s]),
+               "%s: data not read completely (SNPs %lld..%lld: %zu of %zu bytes)", R->path,
+               (long long)i0, (long long)(i0 + cnt - 1), got, bytes);
+    } else {
+      R->status[s] = FGLS_OK;
+    }
+    sem_post(&R->full[s]);
+  }
+  return NULL;
+}
+
+/* ---------------------------------------------------------------- solver */
+int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int y_b,
+                     int num_threads, const char *Phi_path, const char *h2_path,
+                     const char *sigma2_path, const char *XL_path, const char *XR_path,
+                     const char *Y_path, const char *B_path, const char *V_path,
+                     const char *options) {
+  int rc = FGLS_OK;
+  options_t opt = {0, 0, 0, 0};
+  parse_options(options ? options : getenv("FGLS_OPTIONS"), &opt);
+
+  if (n < 1 || m < 0 || t < 1 || wXL < 0 || wXR < 1 || p != wXL + wXR || x_b < 1)
+    return fgls_set_error(FGLS_ERR_ARG,
+                          "invalid dimensions n=%d p=%d m=%d t=%d wXL=%d wXR=%d x_b=%d", n, p, m,
+                          t, wXL, wXR, x_b);
+  if (!Phi_path || !h2_path || !sigma2_path || !XR_path || !Y_path || !B_path || !V_path ||
+      (wXL > 0 && !XL_path))
+    return fgls_set_error(FGLS_ERR_ARG, "NULL path");
+  if (y_b != 1) fprintf(stderr, "[Warning] y_b not used (set to 1)\n");
+
+  /* SNPs per device launch: a multiple of x_b near 16384 (bounded by m) */
+  int64_t block = opt.block > 0 ? opt.block : 0;
+  if (block <= 0) {
+    block = x_b;
+    while (block < 16384 && block + x_b <= (int64_t)m) block += x_b;
+  }
+  if (m > 0 && block > m) block = m;
+  if (block < 1) block = 1;
+  const int64_t nchunks = m > 0 ? ((int64_t)m + block - 1) / block : 0;
+  const int ldx = (n + 3) & ~3;  /* device X_R leading dimension (32-byte rows) */
+  const int64_t pp = (int64_t)p * p;
+  const int threads = num_threads > 0 ? num_threads : host_cores();
+
+  double *Phi = NULL, *M = NULL, *h = NULL, *sig = NULL, *XL = NULL, *y = NULL;
+  double *hB[2] = {NULL, NULL}, *hV[2] = {NULL, NULL};
+  unsigned long long *hErr = NULL;  /* pinned copies of the failure record, one per slot */
+  double *dL = NULL, *dXL = NULL, *dy = NULL, *dX[2] = {NULL, NULL};
+  double *dB[2] = {NULL, NULL}, *dV[2] = {NULL, NULL};
+  hipStream_t s_copy = NULL, s_comp = NULL;
+  hipEvent_t ev_h2d[2] = {NULL, NULL}, ev_out[2] = {NULL, NULL}, ev_free[2] = {NULL, NULL};
+  fgls_hip_ctx *ctx = NULL;
+  reader_t R;
+  memset(&R, 0, sizeof(R));
+  R.fd = -1;
+  int fdB = -1, fdV = -1;
+  pthread_t rthread;
+  int reader_started = 0;
+  double t_potrf = 0.0, t_trait = 0.0, t_loop = 0.0;
+
+  const size_t nn = (size_t)n * n;
+  Phi = (double *)malloc(nn * sizeof(double));
+  M = (double *)malloc(nn * sizeof(double));
+  h = (double *)calloc((size_t)t, sizeof(double));
+  sig = (double *)calloc((size_t)t, sizeof(double));
+  XL = (double *)calloc((size_t)n * (wXL > 0 ? wXL : 1), sizeof(double));
+  y = (double *)calloc((size_t)n, sizeof(double));
+  if (!Phi || !M || !h || !sig || !XL || !y) {
+    rc = fgls_set_error(FGLS_ERR_NOMEM, "host allocation of %zu bytes failed", 2 * nn * sizeof(double));
+    goto done;
+  }
+  RCHK(read_exact(Phi_path, Phi, nn, 0));
+  RCHK(read_exact(h2_path, h, (size_t)t, 0));
+  RCHK(read_exact(sigma2_path, sig, (size_t)t, 0));
+  if (wXL > 0) RCHK(read_exact(XL_path, XL, (size_t)wXL * n, 0));
+
+  R.fd = open(XR_path, O_RDONLY);
+  if (R.fd < 0) {
+    rc = fgls_set_error(FGLS_ERR_IO, "cannot open %s: %s", XR_path, strerror(errno));
+    goto done;
+  }
+  fdB = open(B_path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+  fdV = open(V_path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+  if (fdB < 0 || fdV < 0) {
+    rc = fgls_set_error(FGLS_ERR_IO, "cannot create %s / %s: %s", B_path, V_path, strerror(errno));
+    goto done;
+  }
+
+  /* device state */
+  HCHK(hipSetDevice(opt.device));
+  RCHK(fgls_hip_ctx_create(&ctx, n, wXL, wXR, (int)(block > 0 ? block : 1), opt.device));
+  HCHK(hipStreamCreateWithFlags(&s_copy, hipStreamNonBlocking));
+  HCHK(hipStreamCreateWithFlags(&s_comp, hipStreamNonBlocking));
+  for (int s = 0; s < 2; ++s) {
+    HCHK(hipEventCreateWithFlags(&ev_h2d[s], hipEventDisableTiming));
+    HCHK(hipEventCreateWithFlags(&ev_out[s], hipEventDisableTiming));
+    HCHK(hipEventCreateWithFlags(&ev_free[s], hipEventDisableTiming));
+  }
+  HCHK(hipMalloc((void **)&dL, nn * sizeof(double)));
+  HCHK(hipMalloc((void **)&dXL, (size_t)n * (wXL > 0 ? wXL : 1) * sizeof(double)));
+  HCHK(hipMalloc((void **)&dy, (size_t)n * sizeof(double)));
+  if (wXL > 0) HCHK(hipMemcpy(dXL, XL, (size_t)n * wXL * sizeof(double), hipMemcpyHostToDevice));
+  HCHK(hipHostMalloc((void **)&hErr, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+  hErr[0] = hErr[1] = ~0ull;
+  if (nchunks > 0) {
+    const size_t xbytes = (size_t)block * wXR * ldx * sizeof(double);
+    for (int s = 0; s < 2; ++s) {
+      HCHK(hipMalloc((void **)&dX[s], xbytes));
+      HCHK(hipMemset(dX[s], 0, xbytes));
+      HCHK(hipMalloc((void **)&dB[s], (size_t)block * p * sizeof(double)));
+      HCHK(hipMalloc((void **)&dV[s], (size_t)block * pp * sizeof(double)));
+      HCHK(hipHostMalloc((void **)&R.slot[s], (size_t)block * wXR * n * sizeof(double), hipHostMallocDefault));
+      HCHK(hipHostMalloc((void **)&hB[s], (size_t)block * p * sizeof(double), hipHostMallocDefault));
+      HCHK(hipHostMalloc((void **)&hV[s], (size_t)block * pp * sizeof(double), hipHostMallocDefault));
+    }
+    R.path = XR_path;
+    R.n = n;
+    R.wXR = wXR;
+    R.t = t;
+    R.m = m;
+    R.block = block;
+    R.nchunks = nchunks;
+    for (int s = 0; s < 2; ++s) {
+      sem_init(&R.free_[s], 0, 1);
+      sem_init(&R.full[s], 0, 0);
+    }
+    if (pthread_create(&rthread, NULL, reader_main, &R) != 0) {
+      rc = fgls_set_error(FGLS_ERR_IO, "cannot start the reader thread");
+      goto done;
+    }
+    reader_started = 1;
+  }
+  scipy_openblas_set_num_threads(threads);
+
+  int64_t it = 0;  /* global chunk counter across traits (reader order) */
+  for (int j = 0; j < t; ++j) {
+    /* M := sigma*h*Phi; M_ii += sigma*(1-h)   (fgls_chol_gpu.c:406-414) */
+    const double alpha = h[j] * sig[j], beta = (1.0 - h[j]) * sig[j];
+    double t0 = now_s();
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t c = 0; c < n; ++c) {
+      const double *src = Phi + (size_t)c * n;
+      double *dst = M + (size_t)c * n;
+      for (int64_t r = 0; r < n; ++r) dst[r] = alpha * src[r];
+      dst[c] += beta;
+    }
+    int info = 0;
+    scipy_dpotrf_("L", &n, M, &n, &info);                 /* fgls_chol_gpu.c:416-430 */
+    if (info != 0) {
+      rc = fgls_set_error(FGLS_ERR_NOTSPD, "dpotrf(M) failed (info: %d)", info);
+      goto done;
+    }
+    t_potrf += now_s() - t0;
+    t0 = now_s();
+    RCHK(read_exact(Y_path, y, (size_t)n, (size_t)j * n));
+    HCHK(hipMemcpyAsync(dL, M, nn * sizeof(double), hipMemcpyHostToDevice, s_comp));
+    HCHK(hipMemcpyAsync(dy, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s_comp));
+    RCHK(fgls_hip_set_trait(ctx, dL, n, dXL, n, dy, s_comp));
+    t_trait += now_s() - t0;
+
+    double tl0 = now_s();
+    for (int64_t c = 0; c <= nchunks; ++c, ++it) {
+      const int s = (int)(it & 1);
+      if (c < nchunks) {
+        const int64_t i0 = c * block;
+        const int cnt = (int)((m - i0) < block ? (m - i0) : block);
+        sem_wait(&R.full[s]);
+        if (R.status[s] != FGLS_OK) {
+          rc = fgls_set_error(FGLS_ERR_IO, "%s", R.err[s]);
+          goto done;
+        }
+        /* device X_R slot s is free once the solve of chunk c-2 has run */
+        HCHK(hipStreamWaitEvent(s_copy, ev_free[s], 0));
+        HCHK(hipMemcpy2DAsync(dX[s], (size_t)ldx * sizeof(double), R.slot[s],
+                              (size_t)n * sizeof(double), (size_t)n * sizeof(double),
+                              (size_t)cnt * wXR, hipMemcpyHostToDevice, s_copy));
+        HCHK(hipEventRecord(ev_h2d[s], s_copy));
+        HCHK(hipStreamWaitEvent(s_comp, ev_h2d[s], 0));
+        RCHK(fgls_hip_solve_block(ctx, dX[s], ldx, cnt, dB[s], dV[s], (int64_t)j * m + i0, s_comp));
+        HCHK(hipEventRecord(ev_free[s], s_comp));
+        HCHK(hipMemcpyAsync(hB[s], dB[s], (size_t)cnt * p * sizeof(double), hipMemcpyDeviceToHost, s_comp));
+        HCHK(hipMemcpyAsync(hV[s], dV[s], (size_t)cnt * pp * sizeof(double), hipMemcpyDeviceToHost, s_comp));
+        HCHK(hipMemcpyAsync(&hErr[s], fgls_ctx_err_ptr(ctx), sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, s_comp));
+        HCHK(hipEventRecord(ev_out[s], s_comp));
+        if (opt.serial) HCHK(hipDeviceSynchronize());
+        /* the pinned read slot can be refilled once its upload has landed */
+        HCHK(hipEventSynchronize(ev_h2d[s]));
+        sem_post(&R.free_[s]);
+      }
+      if (c > 0) {  /* write back chunk c-1 */
+        const int ps = (int)((it - 1) & 1);
+        const int64_t i0 = (c - 1) * block;
+        const int cnt = (int)((m - i0) < block ? (m - i0) : block);
+        HCHK(hipEventSynchronize(ev_out[ps]));
+        if (hErr[ps] != ~0ull) {  /* first failing SNP so far (fgls_chol_gpu.c:664-670) */
+          const long long bad = (long long)(hErr[ps] >> 8) - (long long)j * m;
+          rc = fgls_set_error(FGLS_ERR_NOTSPD, "dpotrf(V) failed (info: %d) - i: %lld",
+                              (int)(hErr[ps] & 0xff), bad);
+          goto done;
+        }
+        RCHK(write_at(fdB, B_path, hB[ps], (size_t)cnt * p * sizeof(double),
+                      (off_t)(((int64_t)j * m + i0) * p) * (off_t)sizeof(double)));
+        RCHK(write_at(fdV, V_path, hV[ps], (size_t)cnt * pp * sizeof(double),
+                      (off_t)(((int64_t)j * m + i0) * pp) * (off_t)sizeof(double)));
+      }
+    }
+    --it;  /* the c == nchunks pass consumed no slot */
+    t_loop += now_s() - tl0;
+  }
+  HCHK(hipStreamSynchronize(s_comp));
+  if (opt.stats) {
+    const double snps = (double)m * t;
+    fprintf(stderr,
+            "[fgls] n=%d p=%d m=%d t=%d block=%lld: potrf %.3fs, trait precompute %.3fs, "
+            "block loop %.3fs (%.0f SNPs/s, %.2f GB/s X_R), reader busy %.3fs, threads %d\n",
+            n, p, m, t, (long long)block, t_potrf, t_trait, t_loop, t_loop > 0 ? snps / t_loop : 0.0,
+            t_loop > 0 ? snps * wXR * n * 8.0 / t_loop / 1e9 : 0.0, R.busy_s, threads);
+  }
+
+done:
+  if (reader_started) {
+    R.stop = 1;
+    for (int s = 0; s < 2; ++s) sem_post(&R.free_[s]);
+    pthread_join(rthread, NULL);
+  }
+  if (nchunks > 0 && R.slot[0] != NULL) {
+    for (int s = 0; s < 2; ++s) {
+      sem_destroy(&R.free_[s]);
+      sem_destroy(&R.full[s]);
+    }
+  }
+  if (s_comp) hipStreamSynchronize(s_comp);
+  if (s_copy) hipStreamSynchronize(s_copy);
+  for (int s = 0; s < 2; ++s) {
+    if (ev_h2d[s]) hipEventDestroy(ev_h2d[s]);
+    if (ev_out[s]) hipEventDestroy(ev_out[s]);
+    if (ev_free[s]) hipEventDestroy(ev_free[s]);
+    if (dX[s]) hipFree(dX[s]);
+    if (dB[s]) hipFree(dB[s]);
+    if (dV[s]) hipFree(dV[s]);
+    if (R.slot[s]) hipHostFree(R.slot[s]);
+    if (hB[s]) hipHostFree(hB[s]);
+    if (hV[s]) hipHostFree(hV[s]);
+  }
+  if (hErr) hipHostFree(hErr);
+  if (s_copy) hipStreamDestroy(s_copy);
+  if (s_comp) hipStreamDestroy(s_comp);
+  if (dL) hipFree(dL);
+  if (dXL) hipFree(dXL);
+  if (dy) hipFree(dy);
+  if (ctx) fgls_hip_ctx_destroy(ctx);
+  if (R.fd >= 0) close(R.fd);
+  if (fdB >= 0) close(fdB);
+  if (fdV >= 0) close(fdV);
+  free(Phi);
+  free(M);
+  free(h);
+  free(sig);
+  free(XL);
+  free(y);
+  return rc;
+}
+
+int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR, int x_b, int y_b, int num_threads,
+                  char *Phi_path, char *h2_path, char *sigma2_path, char *XL_path, char *XR_path,
+                  char *Y_path, char *B_path, char *V_path) {
+  int rc = fgls_chol_gpu_ex(n, p, m, t, wXL, wXR, x_b, y_b, num_threads, Phi_path, h2_path,
+                            sigma2_path, XL_path, XR_path, Y_path, B_path, V_path, NULL);
+  if (rc != FGLS_OK) {
+    fprintf(stderr, "[Error]: %s\n", fgls_hip_last_error());   /* common.c:11-17 */
+    exit(EXIT_FAILURE);
+  }
+  return 0;
+}

@@ -1,0 +1,997 @@
+// fgls_kernels.hip -- gfx950 (MI355X, CDNA4) kernels of the Cholesky FGLS
+// hot path and the C-ABI declared in include/fgls_hip.h.
+//
+// Per SNP i the solver needs, with L L' = M (reference fgls_chol.c:251-368):
+//   z_i  = L^-1 x_i                     (the block TRSM, fgls_chol_gpu.c:550)
+//   S_i  = [[V_tl, .], [X~_L' z_i, z_i' z_i]],  r_i = [B_t ; z_i' y~]
+//   b_i  = S_i^-1 r_i,  V_i = res_sigma * S_i^-1 (sqrt on the diagonal)
+//                                        (per-SNP stage, fgls_chol.c:386-439)
+//
+// Device data layout (all FP64, padded to ldn = ceil(n/128)*128 rows, zero
+// padding):
+//   L     ldn x ldn  column-major copy of the factor (lower triangle used)
+//   Linv  ldn x ldn  ROW-major L^-1, so that one MFMA A-fragment row of 16
+//                    consecutive k is a contiguous 128-byte line
+//   Dinv  nblk x 64 x 64  inverses of the 64x64 diagonal blocks of L
+//   W     ldn x (wXL+1)   [L^-1 X_L | L^-1 y]
+//   part  RT x (p+1) x max_block   per-128-row partial reductions
+//
+// MFMA: v_mfma_f64_16x16x4_f64.  Operand A (16x4): lane l holds A[l%16][l/16];
+// B (4x16): lane l holds B[l/16][l%16]; D (16x16): lane l, element v holds
+// D[l/16 + 4v][l%16] (checked on the device by the parity tests: any other
+// assignment permutes rows inside each 16-row tile).  Every K=16 step is
+// issued as four MFMAs in which
+// lane group g = l/16 supplies k = 4g+s (s = 0..3): the contraction over k
+// does not depend on the order, and it lets each lane fetch its four k
+// values with one 32-byte load from both operands.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdarg.h>
+#include <string.h>
+#include <math.h>
+
+#include "fgls_hip.h"
+#include "fgls_internal.h"
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int NB = 64;    // diagonal block of the TRSM
+constexpr int TM = 128;   // TRMM tile rows (2 waves of 64)
+constexpr int TN = 128;   // TRMM tile columns (2 waves of 64)
+constexpr int PMAX = 16;  // largest p supported by the per-SNP kernels
+
+__device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ v4d ld4(const double* p) {
+  return *reinterpret_cast<const v4d*>(p);
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// Inverses of the 64x64 diagonal blocks of L, in LDS, column-parallel
+// forward substitution.  Block I covers rows/cols [64I, 64I+64); rows beyond
+// n are completed with the identity so that the padded solve is harmless.
+// One wave per block; Dinv block I is stored column-major at Dinv + I*4096.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_diag_inv(int n, const double* __restrict__ L,
+                                                 int64_t ldl, double* __restrict__ Dinv) {
+  __shared__ double Lb[NB][NB + 1];
+  __shared__ double Db[NB][NB + 1];
+  const int I = blockIdx.x, t = threadIdx.x;
+  const int base = I * NB;
+  const int r = base + t;
+  for (int c = 0; c < NB; ++c) {
+    double v;
+    if (c > t) v = 0.0;
+    else if (r < n) v = L[(size_t)(base + c) * ldl + r];
+    else v = (c == t) ? 1.0 : 0.0;
+    Lb[t][c] = v;
+  }
+  __syncthreads();
+  // thread t owns column t of the inverse: Lb * d = e_t
+  for (int row = 0; row < NB; ++row) {
+    double d = 0.0;
+    if (row >= t) {
+      double s = (row == t) ? 1.0 : 0.0;
+      for (int q = t; q < row; ++q) s -= Lb[row][q] * Db[q][t];
+      d = s / Lb[row][row];
+    }
+    Db[row][t] = d;
+  }
+  __syncthreads();
+  double* out = Dinv + (size_t)I * NB * NB;
+  for (int c = 0; c < NB; ++c) out[(size_t)c * NB + t] = Db[t][c];
+}
+
+// ---------------------------------------------------------------------------
+// Blocked left-looking TRSM, L Z = B, one workgroup per 64-column panel.
+// For every 64-row block I:   R = B_I - sum_{J<I} L_IJ Z_J   (FP64 MFMA)
+//                             Z_I = Dinv_I R                 (FP64 MFMA, R in LDS)
+// IDENT: B is the identity, panel P's rows above block P are zero (L^-1 is
+// lower triangular), so the sweep starts at block P.
+// ROWMAJOR: Z(r, c) is stored at Z[r*ldz + c] (used for Linv), else at
+// Z[c*ldz + r]; B is always column-major and may alias Z (in place) when
+// !ROWMAJOR.
+// ---------------------------------------------------------------------------
+template <bool IDENT, bool ROWMAJOR>
+__global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const double* __restrict__ L,
+                                                    int64_t ldl, const double* __restrict__ Dinv,
+                                                    const double* B, int64_t ldb, double* Z,
+                                                    int64_t ldz) {
+  __shared__ double Rs[NB][NB + 2];
+  const int P = blockIdx.x;
+  const int c0 = P * NB;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int nblk = (n + NB - 1) / NB;
+  const int Istart = IDENT ? P : 0;
+  int colc[4];
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) colc[tj] = min(c0 + 16 * tj + li, nrhs - 1);
+
+  for (int I = Istart; I < nblk; ++I) {
+    v4d acc[4];
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) acc[tj] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int rowA = I * NB + 16 * w + li;
+    const bool rowok = rowA < n;
+    for (int k0 = Istart * NB; k0 < I * NB; k0 += 16) {
+      double a[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        a[s] = rowok ? L[(size_t)(k0 + 4 * lg + s) * ldl + rowA] : 0.0;
+      double b[4][4];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        if (ROWMAJOR) {
+#pragma unroll
+          for (int s = 0; s < 4; ++s) b[tj][s] = Z[(size_t)(k0 + 4 * lg + s) * ldz + colc[tj]];
+        } else {
+#pragma unroll
+          for (int s = 0; s < 4; ++s) b[tj][s] = Z[(size_t)colc[tj] * ldz + k0 + 4 * lg + s];
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) acc[tj] = mfma_f64(a[s], b[tj][s], acc[tj]);
+    }
+    // R = B_I - acc into LDS (natural 64x64 layout)
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * w + lg + 4 * v, c = 16 * tj + li;
+        const int grow = I * NB + r, gcol = c0 + c;
+        double bv;
+        if (IDENT) bv = (grow == gcol) ? 1.0 : 0.0;
+        else bv = (grow < n && gcol < nrhs) ? B[(size_t)gcol * ldb + grow] : 0.0;
+        Rs[r][c] = bv - acc[tj][v];
+      }
+    }
+    __syncthreads();
+    // Z_I = Dinv_I * R
+    v4d zc[4];
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) zc[tj] = v4d{0.0, 0.0, 0.0, 0.0};
+    const double* D = Dinv + (size_t)I * NB * NB;
+#pragma unroll
+    for (int k0 = 0; k0 < NB; k0 += 16) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int k = k0 + 4 * lg + s;
+        const double a = D[(size_t)k * NB + 16 * w + li];
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) zc[tj] = mfma_f64(a, Rs[k][16 * tj + li], zc[tj]);
+      }
+    }
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int grow = I * NB + 16 * w + lg + 4 * v, gcol = c0 + 16 * tj + li;
+        if (grow < n && gcol < nrhs) {
+          if (ROWMAJOR) Z[(size_t)grow * ldz + gcol] = zc[tj][v];
+          else Z[(size_t)gcol * ldz + grow] = zc[tj][v];
+        }
+      }
+    }
+    // Z_I is read back by other waves of this workgroup in the next blocks.
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular product Z = Linv * X over 128x128 tiles (2x2 waves of 64x64),
+// FP64 MFMA, operands fetched straight into registers (one K=16 step ahead).
+// Linv is row-major (ld ldli >= RT*128) and lower triangular: row tile rt
+// only needs k < (rt+1)*128.  X is column-major with ld ldx (% 4 == 0);
+// rows k >= n are masked.  Epilogue:
+//   EPI_STORE : Z written column-major (ld ldz)
+//   EPI_REDUCE: per column, the NQ sums over the tile's rows of
+//               z*z and z*W[:,k] (k < NQ-1) go to part[(rt*NQ+q)*ldp + col]
+// Grid: (column tiles, row tiles), heaviest row tiles first.
+// ---------------------------------------------------------------------------
+enum { EPI_STORE = 0, EPI_REDUCE = 1 };
+
+template <int EPI, int NQ>
+__global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
+                                              const double* __restrict__ Linv, int64_t ldli,
+                                              const double* __restrict__ X, int64_t ldx,
+                                              double* __restrict__ Zout, int64_t ldz,
+                                              const double* __restrict__ W, int64_t ldw,
+                                              double* __restrict__ part, int64_t ldp) {
+  const int ct = blockIdx.x;
+  const int rt = RT - 1 - (int)blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  const int li = lane & 15, lg = lane >> 4;
+  const int r0 = rt * TM + wr * 64;
+  const int c0 = ct * TN + wc * 64;
+  const int n16 = (n + 15) & ~15;
+  const int kend = (r0 < n16) ? min(r0 + 64, n16) : 0;   // multiple of 16
+  const int kfull = min(kend, n & ~15);                   // steps with no masking
+
+  const double* Ap[4];
+  const double* Bp[4];
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti) Ap[ti] = Linv + (size_t)(r0 + 16 * ti + li) * ldli + 4 * lg;
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+    const int col = min(c0 + 16 * tj + li, count - 1);
+    Bp[tj] = X + (size_t)col * ldx + 4 * lg;
+  }
+
+  v4d acc[4][4];
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  v4d a[4], b[4];
+  if (kfull > 0) {
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) a[ti] = ld4(Ap[ti]);
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) b[tj] = ld4(Bp[tj]);
+  }
+  for (int k0 = 0; k0 < kfull; k0 += 16) {
+    v4d an[4], bn[4];
+    const bool more = (k0 + 16) < kfull;
+    if (more) {
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) an[ti] = ld4(Ap[ti] + k0 + 16);
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) bn[tj] = ld4(Bp[tj] + k0 + 16);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+          acc[ti][tj] = (EPI == EPI_STORE) ? mfma_f64(b[tj][s], a[ti][s], acc[ti][tj])
+                                           : mfma_f64(a[ti][s], b[tj][s], acc[ti][tj]);
+    if (more) {
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) a[ti] = an[ti];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) b[tj] = bn[tj];
+    }
+  }
+  if (kend > kfull) {  // last partial K step: rows k >= n of X are masked
+    const int k0 = kfull;
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) a[ti] = ld4(Ap[ti] + k0);
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int k = k0 + 4 * lg + s;
+        b[tj][s] = (k < n) ? Bp[tj][k0 + s] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+          acc[ti][tj] = (EPI == EPI_STORE) ? mfma_f64(b[tj][s], a[ti][s], acc[ti][tj])
+                                           : mfma_f64(a[ti][s], b[tj][s], acc[ti][tj]);
+  }
+
+  if (EPI == EPI_STORE) {
+    // operands swapped: element v of acc[ti][tj] is z(row r0+16ti+li, column
+    // c0+16tj+lg+4v), so 16 lanes store 16 consecutive rows of one column.
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int col = c0 + 16 * tj + lg + 4 * v;
+        if (col < count) {
+#pragma unroll
+          for (int ti = 0; ti < 4; ++ti) Zout[(size_t)col * ldz + r0 + 16 * ti + li] = acc[ti][tj][v];
+        }
+      }
+    }
+  } else {
+    // element v of acc[ti][tj] is z(row r0+16ti+lg+4v, column c0+16tj+li)
+    __shared__ double red[2][64][NQ];
+    double q[4][NQ];
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int qi = 0; qi < NQ; ++qi) q[tj][qi] = 0.0;
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + 16 * ti + lg + 4 * v;
+        double wv[NQ - 1];
+#pragma unroll
+        for (int k = 0; k < NQ - 1; ++k) wv[k] = W[(size_t)k * ldw + row];
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) {
+          const double z = acc[ti][tj][v];
+          q[tj][0] = fma(z, z, q[tj][0]);
+#pragma unroll
+          for (int k = 0; k < NQ - 1; ++k) q[tj][1 + k] = fma(z, wv[k], q[tj][1 + k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int qi = 0; qi < NQ; ++qi) {
+        double x = q[tj][qi];
+        x += __shfl_xor(x, 16, 64);
+        x += __shfl_xor(x, 32, 64);
+        q[tj][qi] = x;
+      }
+    if (wr == 1 && lg == 0) {
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) red[wc][16 * tj + li][qi] = q[tj][qi];
+    }
+    __syncthreads();
+    if (wr == 0 && lg == 0) {
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        const int col = c0 + 16 * tj + li;
+        if (col < count) {
+#pragma unroll
+          for (int qi = 0; qi < NQ; ++qi)
+            part[((size_t)rt * NQ + qi) * ldp + col] = q[tj][qi] + red[wc][16 * tj + li][qi];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// p x p tail of the per-SNP stage (fgls_chol.c:401-438): Cholesky of the
+// lower triangle of S, two triangular solves for b, inverse through the
+// factor (trtri + L'L product, as dpotri does), scaling by res_sigma and
+// sqrt of the diagonal.  S is column-major-lower in registers; all indices
+// are compile-time.  Returns the LAPACK-style info (0 = success).
+// ---------------------------------------------------------------------------
+template <int P>
+__device__ __forceinline__ int finalize_snp(double (&S)[P][P], double (&b)[P], double res_sigma,
+                                            double* __restrict__ Bo, double* __restrict__ Vo) {
+  int info = 0;
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    double d = S[j][j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= S[j][k] * S[j][k];
+    if (!(d > 0.0) && info == 0) info = j + 1;
+    d = sqrt(d);
+    S[j][j] = d;
+#pragma unroll
+    for (int i = j + 1; i < P; ++i) {
+      double s = S[i][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) s -= S[i][k] * S[j][k];
+      S[i][j] = s / d;
+    }
+  }
+  if (info) {
+    const double nan = __builtin_nan("");
+#pragma unroll
+    for (int i = 0; i < P; ++i) Bo[i] = nan;
+#pragma unroll
+    for (int i = 0; i < P * P; ++i) Vo[i] = nan;
+    return info;
+  }
+  // C y = b, then C' x = y
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    double s = b[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= S[i][k] * b[k];
+    b[i] = s / S[i][i];
+  }
+#pragma unroll
+  for (int i = P - 1; i >= 0; --i) {
+    double s = b[i];
+#pragma unroll
+    for (int k = i + 1; k < P; ++k) s -= S[k][i] * b[k];
+    b[i] = s / S[i][i];
+  }
+#pragma unroll
+  for (int i = 0; i < P; ++i) Bo[i] = b[i];
+  // C^-1 in place (lower, non-unit), columns right to left
+#pragma unroll
+  for (int j = P - 1; j >= 0; --j) {
+    S[j][j] = 1.0 / S[j][j];
+    const double ajj = -S[j][j];
+    // x = S[j+1:, j]; x <- T x with T = S[j+1:, j+1:] (already inverted), bottom-up
+#pragma unroll
+    for (int i = P - 1; i > j; --i) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = j + 1; k <= i; ++k) s += S[i][k] * S[k][j];
+      S[i][j] = s;
+    }
+#pragma unroll
+    for (int i = j + 1; i < P; ++i) S[i][j] *= ajj;
+  }
+  // lower(C^-T C^-1) in place, rows top to bottom (dlauu2, lower)
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    const double aii = S[i][i];
+    double d = 0.0;
+#pragma unroll
+    for (int k = i; k < P; ++k) d += S[k][i] * S[k][i];
+    S[i][i] = d;
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      double s = aii * S[i][j];
+#pragma unroll
+      for (int k = i + 1; k < P; ++k) s += S[k][i] * S[k][j];
+      S[i][j] = s;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+      double v;
+      if (r > c) v = res_sigma * S[r][c];
+      else if (r == c) v = sqrt(res_sigma * S[r][r]);
+      else v = 0.0;
+      Vo[c * P + r] = v;
+    }
+  }
+  return 0;
+}
+
+__device__ __forceinline__ void record_failure(unsigned long long* err, int64_t snp, int info) {
+  const unsigned long long key = ((unsigned long long)snp << 8) | (unsigned long long)(info & 0xff);
+  atomicMin(err, key);
+}
+
+// Per-trait scalars live in a small device array:
+//   [0, wXL)               B_t
+//   [wXL, wXL + wXL^2)     V_tl (column-major, upper triangle 0)
+//   [wXL + wXL^2]          res_sigma
+//   [wXL + wXL^2 + 1, +wXL) b_L
+__host__ __device__ inline int trait_res_off(int wXL) { return wXL + wXL * wXL; }
+
+// Finalize from the TRMM partials (wXR = 1, p = P): thread per SNP.
+template <int P>
+__global__ __launch_bounds__(128) void k_finalize(int count, int RT, const double* __restrict__ part,
+                                                  int64_t ldp, const double* __restrict__ trait,
+                                                  double* __restrict__ Bout, double* __restrict__ Vout,
+                                                  unsigned long long* __restrict__ err, int64_t snp_base) {
+  constexpr int NQ = P + 1;
+  constexpr int WXL = P - 1;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  double sums[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) sums[q] = 0.0;
+  for (int rt = 0; rt < RT; ++rt) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) sums[q] += part[((size_t)rt * NQ + q) * ldp + i];
+  }
+  double S[P][P], b[P];
+#pragma unroll
+  for (int r = 0; r < P; ++r)
+#pragma unroll
+    for (int c = 0; c < P; ++c) S[r][c] = 0.0;
+#pragma unroll
+  for (int r = 0; r < WXL; ++r) {
+#pragma unroll
+    for (int c = 0; c <= r; ++c) S[r][c] = trait[WXL + c * WXL + r];
+    b[r] = trait[r];
+  }
+#pragma unroll
+  for (int c = 0; c < WXL; ++c) S[WXL][c] = sums[1 + c];
+  S[WXL][WXL] = sums[0];
+  b[WXL] = sums[NQ - 1];
+  const double res = trait[trait_res_off(WXL)];
+  const int info = finalize_snp<P>(S, b, res, Bout + (size_t)i * P, Vout + (size_t)i * P * P);
+  if (info) record_failure(err, snp_base + i, info);
+}
+
+// Per-SNP stage from transformed columns Z (any wXR): one wave per SNP,
+// wavefront reductions over the n rows, then the p x p tail on lane 0.
+template <int P>
+__global__ __launch_bounds__(256) void k_snp_stage(int n, int count, int wXL, int wXR,
+                                                   const double* __restrict__ Z, int64_t ldz,
+                                                   const double* __restrict__ W, int64_t ldw,
+                                                   const double* __restrict__ trait,
+                                                   double* __restrict__ Bout, double* __restrict__ Vout,
+                                                   unsigned long long* __restrict__ err, int64_t snp_base) {
+  __shared__ double Ssh[4][P * P + P];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int snp = blockIdx.x * 4 + w;
+  if (snp >= count) return;  // no workgroup barrier below
+  double* Sw = Ssh[w];
+  for (int a = 0; a < wXR; ++a) {
+    const double* za = Z + (size_t)(snp * wXR + a) * ldz;
+    double t[P], u[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) t[i] = u[i] = 0.0;
+    for (int r = lane; r < n; r += 64) {
+      const double x = za[r];
+#pragma unroll
+      for (int bb = 0; bb < P; ++bb)
+        if (bb <= a) t[bb] = fma(x, Z[(size_t)(snp * wXR + bb) * ldz + r], t[bb]);
+#pragma unroll
+      for (int k = 0; k < P; ++k)
+        if (k <= wXL) u[k] = fma(x, W[(size_t)k * ldw + r], u[k]);
+    }
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+      t[i] = wave_sum(t[i]);
+      u[i] = wave_sum(u[i]);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int bb = 0; bb < P; ++bb)
+        if (bb <= a) Sw[(wXL + a) + (wXL + bb) * P] = t[bb];
+#pragma unroll
+      for (int k = 0; k < P; ++k)
+        if (k < wXL) Sw[(wXL + a) + k * P] = u[k];
+      double uy = 0.0;
+#pragma unroll
+      for (int k = 0; k < P; ++k)
+        if (k == wXL) uy = u[k];
+      Sw[P * P + wXL + a] = uy;
+    }
+  }
+  if (lane != 0) return;
+  for (int r = 0; r < wXL; ++r) {
+    for (int c = 0; c <= r; ++c) Sw[r + c * P] = trait[wXL + c * wXL + r];
+    Sw[P * P + r] = trait[r];
+  }
+  double S[P][P], b[P];
+#pragma unroll
+  for (int r = 0; r < P; ++r) {
+#pragma unroll
+    for (int c = 0; c < P; ++c) S[r][c] = (c <= r) ? Sw[r + c * P] : 0.0;
+    b[r] = Sw[P * P + r];
+  }
+  const double res = trait[trait_res_off(wXL)];
+  const int info = finalize_snp<P>(S, b, res, Bout + (size_t)snp * P, Vout + (size_t)snp * P * P);
+  if (info) record_failure(err, snp_base + snp, info);
+}
+
+// Per-trait scalars from W = [X~_L | y~]: B_t, V_tl, b_L, res_sigma.
+// One workgroup; fixed summation order.
+__device__ double block_sum(double x, double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  x = wave_sum(x);
+  __syncthreads();
+  if (lane == 0) red[w] = x;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+  return s;
+}
+
+__global__ __launch_bounds__(256) void k_trait_stats(int n, int wXL, const double* __restrict__ W,
+                                                     int64_t ldw, double* __restrict__ trait,
+                                                     int* __restrict__ status) {
+  __shared__ double red[4];
+  __shared__ double S[PMAX * PMAX];
+  __shared__ double bl[PMAX];
+  const int tid = threadIdx.x;
+  // V_tl (lower) and B_t
+  for (int a = 0; a < wXL; ++a) {
+    for (int c = 0; c <= a; ++c) {
+      double s = 0.0;
+      for (int r = tid; r < n; r += blockDim.x) s = fma(W[(size_t)a * ldw + r], W[(size_t)c * ldw + r], s);
+      s = block_sum(s, red);
+      if (tid == 0) S[a + c * PMAX] = s;
+    }
+    double s = 0.0;
+    for (int r = tid; r < n; r += blockDim.x) s = fma(W[(size_t)a * ldw + r], W[(size_t)wXL * ldw + r], s);
+    s = block_sum(s, red);
+    if (tid == 0) bl[a] = s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int a = 0; a < wXL; ++a) {
+      trait[a] = bl[a];
+      for (int c = 0; c < wXL; ++c) trait[wXL + c * wXL + a] = (c <= a) ? S[a + c * PMAX] : 0.0;
+    }
+    // b_L = V_tl^-1 B_t by Cholesky (fgls_chol.c:306-316)
+    int info = 0;
+    for (int j = 0; j < wXL && !info; ++j) {
+      double d = S[j + j * PMAX];
+      for (int k = 0; k < j; ++k) d -= S[j + k * PMAX] * S[j + k * PMAX];
+      if (!(d > 0.0)) { info = j + 1; break; }
+      d = sqrt(d);
+      S[j + j * PMAX] = d;
+      for (int i = j + 1; i < wXL; ++i) {
+        double s = S[i + j * PMAX];
+        for (int k = 0; k < j; ++k) s -= S[i + k * PMAX] * S[j + k * PMAX];
+        S[i + j * PMAX] = s / d;
+      }
+    }
+    status[0] = info;
+    if (!info) {
+      for (int i = 0; i < wXL; ++i) {
+        double s = bl[i];
+        for (int k = 0; k < i; ++k) s -= S[i + k * PMAX] * bl[k];
+        bl[i] = s / S[i + i * PMAX];
+      }
+      for (int i = wXL - 1; i >= 0; --i) {
+        double s = bl[i];
+        for (int k = i + 1; k < wXL; ++k) s -= S[k + i * PMAX] * bl[k];
+        bl[i] = s / S[i + i * PMAX];
+      }
+    }
+    for (int i = 0; i < wXL; ++i) trait[trait_res_off(wXL) + 1 + i] = bl[i];
+  }
+  __syncthreads();
+  // res_sigma = |y~ - X~_L b_L|^2 / (n - wXL)   (L^-1 is linear: L^-1(y - X_L b_L))
+  double s = 0.0;
+  for (int r = tid; r < n; r += blockDim.x) {
+    double e = W[(size_t)wXL * ldw + r];
+    for (int k = 0; k < wXL; ++k) e -= W[(size_t)k * ldw + r] * bl[k];
+    s = fma(e, e, s);
+  }
+  s = block_sum(s, red);
+  if (tid == 0) trait[trait_res_off(wXL)] = s / (double)(n - wXL);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(FGLS_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));          \
+  } while (0)
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+struct fgls_hip_ctx {
+  int device;
+  int n, wXL, wXR, p;
+  int ldn;        // padded rows (multiple of 128)
+  int nblk;       // 64-row diagonal blocks
+  int RT;         // 128-row TRMM tiles
+  int max_block;
+  bool have_trait;
+  double* L;
+  double* Linv;
+  double* Dinv;
+  double* W;
+  double* trait;
+  double* part;
+  double* z;      // lazily allocated (generic per-SNP path)
+  unsigned long long* err;
+  int* status;
+};
+
+namespace {
+
+int check_ctx(const fgls_hip_ctx* c) {
+  if (!c) return fail(FGLS_ERR_ARG, "NULL context");
+  return FGLS_OK;
+}
+
+int set_device(const fgls_hip_ctx* c) {
+  HIP_TRY(hipSetDevice(c->device));
+  return FGLS_OK;
+}
+
+template <int P>
+void launch_finalize(const fgls_hip_ctx* c, int count, double* dB, double* dV, int64_t base,
+                     hipStream_t st) {
+  dim3 grid((count + 127) / 128);
+  hipLaunchKernelGGL(k_finalize<P>, grid, dim3(128), 0, st, count, c->RT, c->part,
+                     (int64_t)c->max_block, c->trait, dB, dV, c->err, base);
+}
+
+template <int P>
+void launch_trmm_reduce(const fgls_hip_ctx* c, const double* X, int64_t ldx, int count,
+                        hipStream_t st) {
+  dim3 grid((count + TN - 1) / TN, c->RT);
+  hipLaunchKernelGGL((k_trmm<EPI_REDUCE, P + 1>), grid, dim3(256), 0, st, c->n, count, c->RT,
+                     c->Linv, (int64_t)c->ldn, X, ldx, (double*)nullptr, (int64_t)0, c->W,
+                     (int64_t)c->ldn, c->part, (int64_t)c->max_block);
+}
+
+template <int P>
+void launch_snp_stage(const fgls_hip_ctx* c, const double* Z, int64_t ldz, int count, double* dB,
+                      double* dV, int64_t base, hipStream_t st) {
+  dim3 grid((count + 3) / 4);
+  hipLaunchKernelGGL(k_snp_stage<P>, grid, dim3(256), 0, st, c->n, count, c->wXL, c->wXR, Z, ldz,
+                     c->W, (int64_t)c->ldn, c->trait, dB, dV, c->err, base);
+}
+
+#define P_SWITCH(p, FN, ...)                    \
+  switch (p) {                                  \
+    case 1: FN<1>(__VA_ARGS__); break;          \
+    case 2: FN<2>(__VA_ARGS__); break;          \
+    case 3: FN<3>(__VA_ARGS__); break;          \
+    case 4: FN<4>(__VA_ARGS__); break;          \
+    case 5: FN<5>(__VA_ARGS__); break;          \
+    case 6: FN<6>(__VA_ARGS__); break;          \
+    case 7: FN<7>(__VA_ARGS__); break;          \
+    case 8: FN<8>(__VA_ARGS__); break;          \
+    case 9: FN<9>(__VA_ARGS__); break;          \
+    case 10: FN<10>(__VA_ARGS__); break;        \
+    case 11: FN<11>(__VA_ARGS__); break;        \
+    case 12: FN<12>(__VA_ARGS__); break;        \
+    case 13: FN<13>(__VA_ARGS__); break;        \
+    case 14: FN<14>(__VA_ARGS__); break;        \
+    case 15: FN<15>(__VA_ARGS__); break;        \
+    case 16: FN<16>(__VA_ARGS__); break;        \
+    default: break;                             \
+  }
+
+bool aligned32(const void* p) { return ((uintptr_t)p & 31u) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+const char* fgls_hip_last_error(void) { return g_err; }
+
+int fgls_set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int fgls_hip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int fgls_hip_ctx_create(fgls_hip_ctx** out, int n, int wXL, int wXR, int max_block, int device) {
+  if (!out) return fail(FGLS_ERR_ARG, "NULL output pointer");
+  *out = nullptr;
+  if (n < 1 || wXL < 0 || wXR < 1 || wXL + wXR > PMAX || max_block < 1)
+    return fail(FGLS_ERR_ARG, "invalid sizes n=%d wXL=%d wXR=%d max_block=%d (need n>=1, wXR>=1, p<=%d)",
+                n, wXL, wXR, max_block, PMAX);
+  if (wXL >= n) return fail(FGLS_ERR_ARG, "wXL=%d must be smaller than n=%d", wXL, n);
+  int ndev = fgls_hip_device_count();
+  if (device < 0 || device >= ndev)
+    return fail(FGLS_ERR_ARG, "device %d not available (%d visible)", device, ndev);
+  HIP_TRY(hipSetDevice(device));
+  fgls_hip_ctx* c = new fgls_hip_ctx();
+  memset(c, 0, sizeof(*c));
+  c->device = device;
+  c->n = n;
+  c->wXL = wXL;
+  c->wXR = wXR;
+  c->p = wXL + wXR;
+  c->ldn = (int)round_up(n, TM);
+  c->nblk = (n + NB - 1) / NB;
+  c->RT = c->ldn / TM;
+  c->max_block = (int)round_up(max_block, 4);
+  const size_t ldn = (size_t)c->ldn;
+  const size_t nq = (size_t)c->p + 1;
+  struct { void** ptr; size_t bytes; } allocs[] = {
+      {(void**)&c->L, ldn * ldn * sizeof(double)},
+      {(void**)&c->Linv, ldn * ldn * sizeof(double)},
+      {(void**)&c->Dinv, (size_t)c->nblk * NB * NB * sizeof(double)},
+      {(void**)&c->W, ldn * (size_t)(wXL + 1) * sizeof(double)},
+      {(void**)&c->trait, (size_t)(2 * PMAX + PMAX * PMAX + 8) * sizeof(double)},
+      {(void**)&c->part, (size_t)c->RT * nq * (size_t)c->max_block * sizeof(double)},
+      {(void**)&c->err, sizeof(unsigned long long)},
+      {(void**)&c->status, 4 * sizeof(int)},
+  };
+  for (auto& a : allocs) {
+    if (hipMalloc(a.ptr, a.bytes) != hipSuccess) {
+      fgls_hip_ctx_destroy(c);
+      return fail(FGLS_ERR_NOMEM, "hipMalloc of %zu bytes failed", a.bytes);
+    }
+    if (hipMemset(*a.ptr, 0, a.bytes) != hipSuccess) {
+      fgls_hip_ctx_destroy(c);
+      return fail(FGLS_ERR_HIP, "hipMemset failed");
+    }
+  }
+  HIP_TRY(hipMemset(c->err, 0xff, sizeof(unsigned long long)));
+  HIP_TRY(hipDeviceSynchronize());
+  *out = c;
+  return FGLS_OK;
+}
+
+int fgls_hip_ctx_destroy(fgls_hip_ctx* c) {
+  if (!c) return FGLS_OK;
+  hipSetDevice(c->device);
+  hipDeviceSynchronize();
+  void* ptrs[] = {c->L, c->Linv, c->Dinv, c->W, c->trait, c->part, c->z, c->err, c->status};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  delete c;
+  return FGLS_OK;
+}
+
+int fgls_hip_ctx_ldn(const fgls_hip_ctx* c) { return c ? c->ldn : 0; }
+
+unsigned long long* fgls_ctx_err_ptr(fgls_hip_ctx* c) { return c ? c->err : nullptr; }
+
+int fgls_hip_set_trait(fgls_hip_ctx* c, const double* dL, int64_t ldl, const double* dXL,
+                       int64_t ldxl, const double* dy, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!dL || !dy || (c->wXL > 0 && !dXL)) return fail(FGLS_ERR_ARG, "NULL input pointer");
+  if (ldl < c->n || (c->wXL > 0 && ldxl < c->n)) return fail(FGLS_ERR_ARG, "leading dimension below n");
+  if ((rc = set_device(c))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int n = c->n, wXL = c->wXL;
+  const size_t ldn = (size_t)c->ldn;
+  HIP_TRY(hipMemcpy2DAsync(c->L, ldn * sizeof(double), dL, (size_t)ldl * sizeof(double),
+                           (size_t)n * sizeof(double), (size_t)n, hipMemcpyDeviceToDevice, st));
+  if (wXL > 0)
+    HIP_TRY(hipMemcpy2DAsync(c->W, ldn * sizeof(double), dXL, (size_t)ldxl * sizeof(double),
+                             (size_t)n * sizeof(double), (size_t)wXL, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(c->W + (size_t)wXL * ldn, dy, (size_t)n * sizeof(double),
+                         hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_diag_inv, dim3(c->nblk), dim3(64), 0, st, n, c->L, (int64_t)ldn, c->Dinv);
+  hipLaunchKernelGGL((k_trsm_panel<true, true>), dim3(c->nblk), dim3(256), 0, st, n, n, c->L,
+                     (int64_t)ldn, c->Dinv, (const double*)nullptr, (int64_t)0, c->Linv,
+                     (int64_t)ldn);
+  hipLaunchKernelGGL((k_trsm_panel<false, false>), dim3((wXL + 1 + NB - 1) / NB), dim3(256), 0, st,
+                     n, wXL + 1, c->L, (int64_t)ldn, c->Dinv, c->W, (int64_t)ldn, c->W,
+                     (int64_t)ldn);
+  hipLaunchKernelGGL(k_trait_stats, dim3(1), dim3(256), 0, st, n, wXL, c->W, (int64_t)ldn, c->trait,
+                     c->status);
+  HIP_TRY(hipGetLastError());
+  int status = 0;
+  HIP_TRY(hipMemcpyAsync(&status, c->status, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (status != 0) {
+    c->have_trait = false;
+    return fail(FGLS_ERR_NOTSPD, "dpotrf(scoreV) failed (info: %d)", status);
+  }
+  c->have_trait = true;
+  return FGLS_OK;
+}
+
+int fgls_hip_get_trait(fgls_hip_ctx* c, double* hBt, double* hVtl, double* hres) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->have_trait) return fail(FGLS_ERR_STATE, "no trait set");
+  if ((rc = set_device(c))) return rc;
+  const int wXL = c->wXL;
+  double buf[2 * PMAX + PMAX * PMAX + 8];
+  HIP_TRY(hipMemcpy(buf, c->trait, sizeof(buf), hipMemcpyDeviceToHost));
+  if (hBt) memcpy(hBt, buf, (size_t)wXL * sizeof(double));
+  if (hVtl) memcpy(hVtl, buf + wXL, (size_t)wXL * wXL * sizeof(double));
+  if (hres) *hres = buf[trait_res_off(wXL)];
+  return FGLS_OK;
+}
+
+int fgls_hip_trmm(fgls_hip_ctx* c, const double* dX, int64_t ldx, int ncols, double* dZ,
+                  int64_t ldz, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->have_trait) return fail(FGLS_ERR_STATE, "no trait set");
+  if (ncols < 0) return fail(FGLS_ERR_ARG, "ncols < 0");
+  if (ncols == 0) return FGLS_OK;
+  if (!dX || !dZ) return fail(FGLS_ERR_ARG, "NULL pointer");
+  if (ldx < c->n || (ldx & 3) || !aligned32(dX))
+    return fail(FGLS_ERR_ARG, "X needs ld >= n, ld %% 4 == 0 and 32-byte alignment");
+  if (ldz < c->ldn || (ldz & 3) || !aligned32(dZ))
+    return fail(FGLS_ERR_ARG, "Z needs ld >= %d, ld %% 4 == 0 and 32-byte alignment", c->ldn);
+  if ((rc = set_device(c))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  dim3 grid((ncols + TN - 1) / TN, c->RT);
+  hipLaunchKernelGGL((k_trmm<EPI_STORE, 2>), grid, dim3(256), 0, st, c->n, ncols, c->RT, c->Linv,
+                     (int64_t)c->ldn, dX, ldx, dZ, ldz, c->W, (int64_t)c->ldn, (double*)nullptr,
+                     (int64_t)0);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_trsm(fgls_hip_ctx* c, double* dX, int64_t ldx, int ncols, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->have_trait) return fail(FGLS_ERR_STATE, "no trait set");
+  if (ncols < 0) return fail(FGLS_ERR_ARG, "ncols < 0");
+  if (ncols == 0) return FGLS_OK;
+  if (!dX || ldx < c->n) return fail(FGLS_ERR_ARG, "X needs ld >= n");
+  if ((rc = set_device(c))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL((k_trsm_panel<false, false>), dim3((ncols + NB - 1) / NB), dim3(256), 0, st,
+                     c->n, ncols, c->L, (int64_t)c->ldn, c->Dinv, dX, ldx, dX, ldx);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_snp_stage(fgls_hip_ctx* c, const double* dZ, int64_t ldz, int count, double* dB,
+                       double* dV, int64_t snp_base, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->have_trait) return fail(FGLS_ERR_STATE, "no trait set");
+  if (count < 0) return fail(FGLS_ERR_ARG, "count < 0");
+  if (count == 0) return FGLS_OK;
+  if (!dZ || !dB || !dV || ldz < c->n) return fail(FGLS_ERR_ARG, "bad Z/B/V arguments");
+  if ((rc = set_device(c))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  P_SWITCH(c->p, launch_snp_stage, c, dZ, ldz, count, dB, dV, snp_base, st);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_solve_block(fgls_hip_ctx* c, const double* dXR, int64_t ldxr, int count, double* dB,
+                         double* dV, int64_t snp_base, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->have_trait) return fail(FGLS_ERR_STATE, "no trait set");
+  if (count < 0 || count > c->max_block)
+    return fail(FGLS_ERR_ARG, "count=%d outside [0, %d]", count, c->max_block);
+  if (count == 0) return FGLS_OK;
+  if (!dXR || !dB || !dV) return fail(FGLS_ERR_ARG, "NULL pointer");
+  if (ldxr < c->n || (ldxr & 3) || !aligned32(dXR))
+    return fail(FGLS_ERR_ARG, "X_R needs ld >= n, ld %% 4 == 0 and 32-byte alignment");
+  if ((rc = set_device(c))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (c->wXR == 1) {
+    P_SWITCH(c->p, launch_trmm_reduce, c, dXR, ldxr, count, st);
+    P_SWITCH(c->p, launch_finalize, c, count, dB, dV, snp_base, st);
+  } else {
+    const int ncols = count * c->wXR;
+    if (!c->z) {
+      const size_t bytes = (size_t)c->ldn * (size_t)c->max_block * c->wXR * sizeof(double);
+      if (hipMalloc((void**)&c->z, bytes) != hipSuccess)
+        return fail(FGLS_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+    }
+    rc = fgls_hip_trmm(c, dXR, ldxr, ncols, c->z, c->ldn, stream);
+    if (rc) return rc;
+    P_SWITCH(c->p, launch_snp_stage, c, c->z, (int64_t)c->ldn, count, dB, dV, snp_base, st);
+  }
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_check(fgls_hip_ctx* c, void* stream, int64_t* bad_snp, int* info) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = set_device(c))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long key = 0;
+  HIP_TRY(hipMemcpyAsync(&key, c->err, sizeof(key), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (key == ~0ull) return FGLS_OK;
+  const int64_t snp = (int64_t)(key >> 8);
+  const int inf = (int)(key & 0xff);
+  if (bad_snp) *bad_snp = snp;
+  if (info) *info = inf;
+  HIP_TRY(hipMemsetAsync(c->err, 0xff, sizeof(unsigned long long), st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return fail(FGLS_ERR_NOTSPD, "dpotrf(V) failed (info: %d) - i: %lld", inf, (long long)snp);
+}
+
+}  // extern "C"

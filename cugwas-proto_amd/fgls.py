@@ -1,0 +1,258 @@
+"""Python host mirror of the reference's FGLS interface over the C-ABI.
+
+The native library (lib/libfgls_amd.so, built by build.py) is required: there
+is no CPU fallback, and every entry point raises ``FglsError`` when the HIP
+library or a GPU is missing.
+
+Two levels are exposed:
+
+* ``fgls_chol_gpu(n, p, m, t, wXL, wXR, x_b, y_b, num_threads, Phi_path, ...)``
+  -- the reference solver's 17-argument entry point (fgls_chol_gpu.h:4-10),
+  files in, files out, running the C host pipeline of the library.
+* ``DeviceContext`` -- the device layer (include/fgls_hip.h) on torch CUDA
+  tensors: per-trait precompute, fused block solve, TRSM, TRMM and the
+  per-SNP stage.
+
+Matrix convention on tensors: a column-major n x k matrix with leading
+dimension ld is a contiguous tensor of shape (k, ld) (row c = column c).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libfgls_amd.so")
+
+FGLS_OK = 0
+FGLS_ERR_ARG = -1
+FGLS_ERR_HIP = -2
+FGLS_ERR_NOMEM = -3
+FGLS_ERR_NOTSPD = -4
+FGLS_ERR_IO = -5
+FGLS_ERR_STATE = -6
+
+_lib = None
+
+
+class FglsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+        self.msg = msg
+
+
+def lib() -> ctypes.CDLL:
+    """Load the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FglsError(FGLS_ERR_STATE, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
+        sig = {
+            "fgls_hip_last_error": (ctypes.c_char_p, []),
+            "fgls_hip_device_count": (i32, []),
+            "fgls_hip_ctx_create": (i32, [ctypes.POINTER(vp), i32, i32, i32, i32, i32]),
+            "fgls_hip_ctx_destroy": (i32, [vp]),
+            "fgls_hip_ctx_ldn": (i32, [vp]),
+            "fgls_hip_set_trait": (i32, [vp, dp, i64, dp, i64, dp, vp]),
+            "fgls_hip_get_trait": (i32, [vp, dp, dp, dp]),
+            "fgls_hip_solve_block": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
+            "fgls_hip_trsm": (i32, [vp, dp, i64, i32, vp]),
+            "fgls_hip_trmm": (i32, [vp, dp, i64, i32, dp, i64, vp]),
+            "fgls_hip_snp_stage": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
+            "fgls_hip_check": (i32, [vp, vp, ctypes.POINTER(i64), ctypes.POINTER(i32)]),
+            "fgls_chol_gpu": (i32, [i32] * 9 + [ctypes.c_char_p] * 8),
+            "fgls_chol_gpu_ex": (i32, [i32] * 9 + [ctypes.c_char_p] * 8 + [ctypes.c_void_p]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return [s for s in ("fgls_hip_last_error", "fgls_hip_device_count", "fgls_hip_ctx_create",
+                        "fgls_hip_ctx_destroy", "fgls_hip_ctx_ldn", "fgls_hip_set_trait",
+                        "fgls_hip_get_trait", "fgls_hip_solve_block", "fgls_hip_trsm",
+                        "fgls_hip_trmm", "fgls_hip_snp_stage", "fgls_hip_check")
+            if hasattr(lib(), s)]
+
+
+def last_error() -> str:
+    return (lib().fgls_hip_last_error() or b"").decode()
+
+
+def _check(rc: int):
+    if rc != FGLS_OK:
+        raise FglsError(rc, last_error())
+
+
+def device_count() -> int:
+    return int(lib().fgls_hip_device_count())
+
+
+def _ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+def _stream(stream) -> int:
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return int(stream.cuda_stream)
+
+
+def _need_cuda_f64(t, name: str):
+    import torch
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != torch.float64:
+        raise FglsError(FGLS_ERR_ARG, f"{name} must be a float64 CUDA tensor")
+    if not t.is_contiguous():
+        raise FglsError(FGLS_ERR_ARG, f"{name} must be contiguous")
+
+
+def colmajor(a, ld: int | None = None, device="cuda"):
+    """numpy n x k (any order) -> contiguous float64 tensor of shape (k, ld)
+    holding the matrix column-major with leading dimension ld (zero padded)."""
+    import numpy as np
+    import torch
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim == 1:
+        a = a[:, None]
+    n, k = a.shape
+    ld = n if ld is None else ld
+    out = torch.zeros((k, ld), dtype=torch.float64)
+    out[:, :n] = torch.from_numpy(np.ascontiguousarray(a.T))
+    return out.to(device)
+
+
+class DeviceContext:
+    """One device context (include/fgls_hip.h, fgls_hip_ctx_*)."""
+
+    def __init__(self, n: int, wXL: int, wXR: int = 1, max_block: int = 4096, device: int = 0):
+        self.n, self.wXL, self.wXR, self.p = n, wXL, wXR, wXL + wXR
+        self.max_block = max_block
+        self.device = device
+        h = ctypes.c_void_p()
+        _check(lib().fgls_hip_ctx_create(ctypes.byref(h), n, wXL, wXR, max_block, device))
+        self._h = h
+        self.ldn = int(lib().fgls_hip_ctx_ldn(h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fgls_hip_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- per trait ---------------------------------------------------------
+    def set_trait(self, L, XL, y, stream=None):
+        """L: (n, ldl) column-major lower factor; XL: (wXL, ldxl) or None; y: (>=n,)."""
+        _need_cuda_f64(L, "L")
+        _need_cuda_f64(y, "y")
+        ldxl = self.n
+        if self.wXL > 0:
+            _need_cuda_f64(XL, "XL")
+            ldxl = XL.shape[1]
+        _check(lib().fgls_hip_set_trait(self._h, _ptr(L), L.shape[1], _ptr(XL) if self.wXL else 0,
+                                        ldxl, _ptr(y), _stream(stream)))
+
+    def trait(self):
+        import numpy as np
+        Bt = np.zeros(max(self.wXL, 1))
+        Vtl = np.zeros(max(self.wXL * self.wXL, 1))
+        res = np.zeros(1)
+        _check(lib().fgls_hip_get_trait(self._h, Bt.ctypes.data, Vtl.ctypes.data, res.ctypes.data))
+        return Bt[:self.wXL], Vtl[:self.wXL * self.wXL].reshape(self.wXL, self.wXL, order="F"), float(res[0])
+
+    # -- per block ---------------------------------------------------------
+    def solve_block(self, XR, count: int | None = None, B=None, V=None, snp_base: int = 0, stream=None):
+        """XR: (count*wXR, ldx) tensor.  Returns (B (count, p), V (count, p*p))."""
+        import torch
+        _need_cuda_f64(XR, "XR")
+        if count is None:
+            count = XR.shape[0] // self.wXR
+        if B is None:
+            B = torch.empty((count, self.p), dtype=torch.float64, device=XR.device)
+        if V is None:
+            V = torch.empty((count, self.p * self.p), dtype=torch.float64, device=XR.device)
+        _check(lib().fgls_hip_solve_block(self._h, _ptr(XR), XR.shape[1], count, _ptr(B), _ptr(V),
+                                          snp_base, _stream(stream)))
+        return B, V
+
+    def trsm(self, X, ncols: int | None = None, stream=None):
+        """In place X <- L^-1 X (X: (ncols, ldx))."""
+        _need_cuda_f64(X, "X")
+        ncols = X.shape[0] if ncols is None else ncols
+        _check(lib().fgls_hip_trsm(self._h, _ptr(X), X.shape[1], ncols, _stream(stream)))
+        return X
+
+    def trmm(self, X, ncols: int | None = None, Z=None, stream=None):
+        """Z = L^-1 X through the explicit inverse; Z has shape (ncols, ldn)."""
+        import torch
+        _need_cuda_f64(X, "X")
+        ncols = X.shape[0] if ncols is None else ncols
+        if Z is None:
+            Z = torch.empty((ncols, self.ldn), dtype=torch.float64, device=X.device)
+        _check(lib().fgls_hip_trmm(self._h, _ptr(X), X.shape[1], ncols, _ptr(Z), Z.shape[1], _stream(stream)))
+        return Z
+
+    def snp_stage(self, Z, count: int | None = None, B=None, V=None, snp_base: int = 0, stream=None):
+        import torch
+        _need_cuda_f64(Z, "Z")
+        if count is None:
+            count = Z.shape[0] // self.wXR
+        if B is None:
+            B = torch.empty((count, self.p), dtype=torch.float64, device=Z.device)
+        if V is None:
+            V = torch.empty((count, self.p * self.p), dtype=torch.float64, device=Z.device)
+        _check(lib().fgls_hip_snp_stage(self._h, _ptr(Z), Z.shape[1], count, _ptr(B), _ptr(V),
+                                        snp_base, _stream(stream)))
+        return B, V
+
+    def check(self, stream=None):
+        """Raise FglsError(FGLS_ERR_NOTSPD) carrying .bad_snp / .info if a per-SNP
+        factorisation failed since the last check."""
+        snp = ctypes.c_int64(-1)
+        info = ctypes.c_int(0)
+        rc = lib().fgls_hip_check(self._h, _stream(stream), ctypes.byref(snp), ctypes.byref(info))
+        if rc != FGLS_OK:
+            err = FglsError(rc, last_error())
+            err.bad_snp = int(snp.value)
+            err.info = int(info.value)
+            raise err
+
+
+def fgls_chol_gpu(n, p, m, t, wXL, wXR, x_b, y_b, num_threads,
+                  Phi_path, h2_path, sigma2_path, XL_path, XR_path, Y_path, B_path, V_path,
+                  options: dict | None = None) -> int:
+    """The reference entry point (fgls_chol_gpu.h:4-10), files in and out.
+
+    Unlike the reference (which prints "[Error]: ..." and exits the process,
+    common.c:11-17) this Python mirror raises FglsError; the C symbol used by
+    the CLI driver keeps the exit behaviour."""
+    L = lib()
+    args = [int(v) for v in (n, p, m, t, wXL, wXR, x_b, y_b, num_threads)]
+    paths = [os.fsencode(s) for s in (Phi_path, h2_path, sigma2_path, XL_path, XR_path, Y_path,
+                                      B_path, V_path)]
+    opts = None
+    if options:
+        opts = ",".join(f"{k}={v}" for k, v in options.items()).encode()
+    rc = L.fgls_chol_gpu_ex(*args, *paths, opts)
+    _check(rc)
+    return rc

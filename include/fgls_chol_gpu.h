@@ -1,0 +1,66 @@
+/*
+ * fgls_chol_gpu.h -- the solver entry point of the reference, rebuilt for one
+ * MI355X.
+ *
+ * int fgls_chol_gpu(...) has the reference's signature and file contract
+ * (src/fgls_chol_gpu.h:4-10, identical to src/fgls_chol.h:4-10):
+ *   inputs  Phi_path (n*n doubles, column-major), h2_path and sigma2_path
+ *           (>= t doubles), XL_path (wXL*n, column k at k*n), XR_path
+ *           (m*wXR*n, SNP i at i*wXR*n), Y_path (t*n, trait j at j*n);
+ *   outputs B_path (t*m*p doubles, (j*m+i)*p + k) and V_path (t*m*p*p,
+ *           (j*m+i)*p*p + c*p + r, lower triangle, sqrt on the diagonal,
+ *           strict upper exactly 0), truncated first ("wb").
+ * All files are raw little-endian doubles (io.c:7-33).
+ *
+ * Differences from the reference, all deliberate:
+ *   - t >= 1 traits are supported (the reference GPU path exits for t != 1,
+ *     fgls_chol_gpu.c:169-174; its CPU twin loops over traits, re-reading
+ *     X_R per trait, fgls_chol.c:223 and :336-337, which is what is done here);
+ *   - missing/short input files, non-SPD matrices and device failures are
+ *     reported as "[Error]: <message>" on stderr followed by
+ *     exit(EXIT_FAILURE) (the reference's error_msg, common.c:11-17) instead
+ *     of crashing; no LXR.out file is created (fgls_chol_gpu.c:338);
+ *   - no per-block progress lines on stdout (fgls_chol_gpu.c:501-504).
+ * y_b is ignored with the reference's warning (fgls_chol_gpu.c:185-186);
+ * num_threads sets the host BLAS threads for the per-trait Cholesky of M.
+ */
+#ifndef FGLS_CHOL_GPU_H
+#define FGLS_CHOL_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Replaces fgls_chol_gpu (src/fgls_chol_gpu.h:4-10).  Returns 0; on failure
+ * prints "[Error]: ..." to stderr and calls exit(EXIT_FAILURE). */
+int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
+                  int x_b, int y_b, int num_threads,
+                  char *Phi_path, char *h2_path, char *sigma2_path,
+                  char *XL_path, char *XR_path, char *Y_path,
+                  char *B_path, char *V_path);
+
+/*
+ * Same solver returning a status instead of exiting: FGLS_OK (0) or a
+ * negative FGLS_ERR_* code from fgls_hip.h, with the message available from
+ * fgls_hip_last_error().  `options` (may be NULL) is a comma-separated list
+ * of key=value settings; when NULL the FGLS_OPTIONS environment variable is
+ * read instead:
+ *   block=<SNPs>   SNPs per device launch (default: a multiple of x_b near
+ *                  16384, at most m)
+ *   device=<id>    HIP device (default 0)
+ *   serial=1       synchronise after every stage (debug aid, like the
+ *                  reference's FGLS_GPU_SERIAL, fgls_chol_gpu.c:44-46)
+ *   stats=1        print stage timings and rates to stderr
+ * No counterpart in the reference (its GPU entry only exits on error).
+ */
+int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR,
+                     int x_b, int y_b, int num_threads,
+                     const char *Phi_path, const char *h2_path, const char *sigma2_path,
+                     const char *XL_path, const char *XR_path, const char *Y_path,
+                     const char *B_path, const char *V_path, const char *options);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FGLS_CHOL_GPU_H */

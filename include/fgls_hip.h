@@ -1,0 +1,139 @@
+/*
+ * fgls_hip.h -- C-ABI of the MI355X (gfx950) device layer of the Cholesky
+ * FGLS solver.
+ *
+ * Everything here takes plain pointers and sizes.  Matrices are column-major
+ * FP64 unless stated.  Pointers named d* are device (HBM) pointers; `stream`
+ * is a hipStream_t passed as void* (NULL = the legacy default stream).
+ *
+ * Each entry point replaces a host BLAS/LAPACK call or a cuBLAS call of the
+ * reference's GPU solver (fgls_chol_gpu.c), as cited per function.
+ *
+ * Error convention: every function returns FGLS_OK (0) or a negative code;
+ * fgls_hip_last_error() gives a message for the calling thread.  The host
+ * solver (fgls_chol_gpu) turns codes into the reference's
+ * "[Error]: <msg>" + exit(EXIT_FAILURE) behaviour (common.c:11-17).
+ */
+#ifndef FGLS_HIP_H
+#define FGLS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FGLS_OK            0
+#define FGLS_ERR_ARG      -1   /* invalid argument (sizes, strides, NULL)      */
+#define FGLS_ERR_HIP      -2   /* a HIP runtime call failed                    */
+#define FGLS_ERR_NOMEM    -3   /* device or pinned allocation failed           */
+#define FGLS_ERR_NOTSPD   -4   /* a Cholesky factorisation met a non-positive pivot */
+#define FGLS_ERR_IO       -5   /* file open / read / write failure (host layer) */
+#define FGLS_ERR_STATE    -6   /* call order violated (e.g. no trait set)      */
+
+/* Arithmetic of the per-SNP hot path. */
+#define FGLS_MODE_FP64     0   /* FP64 inputs, FP64 MFMA                       */
+
+typedef struct fgls_hip_ctx fgls_hip_ctx;
+
+/* Message describing the last failure on the calling thread. */
+const char *fgls_hip_last_error(void);
+
+/* Number of visible HIP devices (0 when there is none); never fails. */
+int fgls_hip_device_count(void);
+
+/*
+ * Create a device context for problems of n individuals, wXL covariate
+ * columns and wXR SNP columns per SNP (p = wXL + wXR), processing at most
+ * max_block SNPs per solve call.  Allocates all device state up front
+ * (replaces the cudaMalloc of L and the X_R buffers, fgls_chol_gpu.c:230-265,
+ * and the stream creation, :212-228).
+ */
+int fgls_hip_ctx_create(fgls_hip_ctx **out, int n, int wXL, int wXR,
+                        int max_block, int device);
+int fgls_hip_ctx_destroy(fgls_hip_ctx *ctx);
+
+/* Leading dimension (rows, a multiple of 128) that the context pads n to. */
+int fgls_hip_ctx_ldn(const fgls_hip_ctx *ctx);
+
+/*
+ * Per-trait precompute from a factor already in device memory.
+ * dL: n x n lower Cholesky factor of M (upper triangle ignored), ld ldl.
+ * dXL: n x wXL covariates (ld ldxl), dy: the trait vector (n).
+ * Computes on the device, in FP64:
+ *   - the inverses of the 64x64 diagonal blocks of L (LDS-staged solves),
+ *   - L^-1 (blocked TRSM against the identity; MFMA off-diagonal updates),
+ *   - X~_L = L^-1 X_L and y~ = L^-1 y,
+ *   - B_t = X~_L' y~, V_tl = lower(X~_L' X~_L) and the null-model residual
+ *     variance res_sigma = |L^-1 (y - X_L b_L)|^2 / (n - wXL).
+ * Replaces: cublasSetVector(L) fgls_chol_gpu.c:434-445; dtrsm(XL) :447-449;
+ * dtrsv(y), dgemv, dsyrk :461-468; the res_sigma block :469-493.
+ * Synchronises `stream`; returns FGLS_ERR_NOTSPD when X~_L' X~_L is not SPD.
+ */
+int fgls_hip_set_trait(fgls_hip_ctx *ctx, const double *dL, int64_t ldl,
+                       const double *dXL, int64_t ldxl, const double *dy,
+                       void *stream);
+
+/* Copy the per-trait scalars back: B_t (wXL), V_tl (wXL*wXL, col-major,
+ * upper triangle 0) and res_sigma.  Any output pointer may be NULL. */
+int fgls_hip_get_trait(fgls_hip_ctx *ctx, double *hBt, double *hVtl,
+                       double *hres_sigma);
+
+/*
+ * Fused per-block solve (FP64): for `count` SNPs whose columns are in
+ * dXR (column-major, ld ldxr >= n, ldxr % 4 == 0, SNP i's wXR columns at
+ * column i*wXR), compute b_i (p values) and V_i (p*p, column-major, lower
+ * triangle = res_sigma * S_i^-1 with sqrt on the diagonal, upper = 0) into
+ * dB (count*p) and dV (count*p*p), the layout of B.out/V.out
+ * (fgls_chol.c:458-474).
+ * Replaces cublasDtrsm fgls_chol_gpu.c:531-557, the D2H of L^-1 X_R
+ * :564-576 and the CPU OpenMP per-SNP stage :619-687.
+ * Asynchronous on `stream`.  A non-SPD per-SNP system is recorded in the
+ * context (first failing SNP, counted from snp_base) and reported by
+ * fgls_hip_check().
+ */
+int fgls_hip_solve_block(fgls_hip_ctx *ctx, const double *dXR, int64_t ldxr,
+                         int count, double *dB, double *dV, int64_t snp_base,
+                         void *stream);
+
+/*
+ * In-place X <- L^-1 X for ncols columns (ld ldx), with the factor of the
+ * current trait: the direct replacement of
+ * cublasDtrsm(LEFT, LOWER, N, NON_UNIT, n, ncols, 1, L, n, X, n)
+ * at fgls_chol_gpu.c:550 (blocked left-looking TRSM, diagonal blocks via
+ * their LDS-computed inverses, FP64 MFMA updates).  Asynchronous.
+ */
+int fgls_hip_trsm(fgls_hip_ctx *ctx, double *dX, int64_t ldx, int ncols,
+                  void *stream);
+
+/*
+ * Z <- L^-1 X through the explicit inverse (triangular MFMA product), for
+ * ncols columns; dZ may not alias dX.  ldx % 4 == 0, ldz % 4 == 0,
+ * ldz >= fgls_hip_ctx_ldn(ctx).  Asynchronous.
+ */
+int fgls_hip_trmm(fgls_hip_ctx *ctx, const double *dX, int64_t ldx,
+                  int ncols, double *dZ, int64_t ldz, void *stream);
+
+/*
+ * Per-SNP stage from already transformed columns Z = L^-1 X_R (count SNPs,
+ * wXR columns each, ld ldz): the batched replacement of the OpenMP loop at
+ * fgls_chol_gpu.c:619-687 (= fgls_chol.c:386-439) for any wXR.  Asynchronous.
+ */
+int fgls_hip_snp_stage(fgls_hip_ctx *ctx, const double *dZ, int64_t ldz,
+                       int count, double *dB, double *dV, int64_t snp_base,
+                       void *stream);
+
+/*
+ * Synchronise the context's work on `stream` and report a recorded per-SNP
+ * factorisation failure: returns FGLS_ERR_NOTSPD with *bad_snp (0-based,
+ * global index) and *info (LAPACK-style pivot index, 1-based), after which
+ * the record is cleared; FGLS_OK otherwise.
+ */
+int fgls_hip_check(fgls_hip_ctx *ctx, void *stream, int64_t *bad_snp,
+                   int *info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FGLS_HIP_H */

@@ -1,0 +1,189 @@
+"""FP64 CPU oracle of the Cholesky FGLS path -- TEST INFRASTRUCTURE ONLY.
+
+This module is the checker for the MI355X implementation.  Only tests,
+``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg may import
+it; the product path never calls it and has no CPU fallback.
+
+It restates, with numpy/scipy LAPACK calls in the same order, the reference
+solver ``fgls_chol`` (src/fgls_chol.c) and the identical arithmetic of the GPU
+variant (src/fgls_chol_gpu.c):
+
+* ``build_M``           fgls_chol.c:251-257  (M = sigma*(h*Phi) + sigma*(1-h) I)
+* ``trait_precompute``  fgls_chol.c:259-325  (dpotrf, dtrsm, dtrsv, dgemv, dsyrk,
+                                              null-model res_sigma)
+* ``solve_snps``        fgls_chol.c:365-368  (dtrsm of the X_R block) and
+                        fgls_chol.c:386-439  (per-SNP stage)
+* ``fgls_chol``         fgls_chol.c:223-491  (trait loop, output layout
+                                              fgls_chol.c:458-474)
+* ``read_inputs`` / ``write_outputs``  io.c:7-33 raw little-endian doubles.
+
+How each function is pinned (beyond this restatement itself) is listed in
+tests/test_oracle.py: the reference fixture test_cases/test_XL/B_exp.out
+(B only, through a synthetic X_R), closed forms (brute-force inverse of M at
+small n, GLS normal equations), invariances (sigma scale, SNP scaling and
+permutation, intercept shift), and LAPACK identities.  V has no reference
+fixture: its layout and values are pinned by the closed form
+V = res_sigma * (X' M^-1 X)^-1 (dense brute force) only.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.linalg as sla
+
+
+@dataclass
+class Trait:
+    """Per-trait quantities of fgls_chol.c:259-325."""
+
+    L: np.ndarray          # lower Cholesky factor of M
+    XLt: np.ndarray        # L^-1 X_L  (n x wXL)
+    yt: np.ndarray         # L^-1 y    (n)
+    B_t: np.ndarray        # X~_L' y~  (wXL)
+    V_tl: np.ndarray       # lower(X~_L' X~_L), upper 0 (wXL x wXL)
+    res_sigma: float
+
+
+def build_M(Phi: np.ndarray, h: float, sigma: float) -> np.ndarray:
+    """M := sigma*h*Phi; M_ii += sigma*(1-h)  (fgls_chol.c:251-257).
+
+    H.in holds h^2 directly; the code uses it linearly (the comment at
+    fgls_chol.c:251 is misleading, the arithmetic is what counts)."""
+    M = (h * sigma) * np.array(Phi, dtype=np.float64, copy=True)
+    M[np.diag_indices_from(M)] += (1.0 - h) * sigma
+    return M
+
+
+def cholesky_lower(M: np.ndarray) -> np.ndarray:
+    """dpotrf_("L") (fgls_chol.c:262); raises on a non-SPD matrix."""
+    return sla.cholesky(M, lower=True, check_finite=False)
+
+
+def trait_precompute(L: np.ndarray, XL: np.ndarray, y: np.ndarray) -> Trait:
+    """fgls_chol.c:278-325 for one trait.
+
+    XL is n x wXL (column k = covariate k), y has n entries."""
+    n = L.shape[0]
+    wXL = XL.shape[1]
+    XLt = sla.solve_triangular(L, XL, lower=True, check_finite=False)   # dtrsm  :279
+    yt = sla.solve_triangular(L, y, lower=True, check_finite=False)     # dtrsv  :291
+    B_t = XLt.T @ yt                                                    # dgemv  :294
+    V_tl = np.tril(XLt.T @ XLt)                                         # dsyrk  :297 (upper stays 0)
+    # sigma2.score, fgls_chol.c:301-325
+    if wXL > 0:
+        C = np.linalg.cholesky(V_tl + np.tril(V_tl, -1).T)              # dpotrf :307
+        bL = sla.solve_triangular(C, B_t, lower=True)                   # dtrsv  :314
+        bL = sla.solve_triangular(C.T, bL, lower=False)                 # dtrsv  :315
+        r = y - XL @ bL                                                 # dgemv  :317-320
+    else:
+        r = np.array(y, dtype=np.float64, copy=True)
+    rt = sla.solve_triangular(L, r, lower=True, check_finite=False)     # dtrsv  :322-323
+    res = float(rt @ rt) / (n - wXL)                                    # ddot   :324
+    return Trait(L=L, XLt=XLt, yt=yt, B_t=B_t, V_tl=V_tl, res_sigma=res)
+
+
+def per_snp_from_z(Z: np.ndarray, tr: Trait, wXR: int):
+    """Per-SNP stage (fgls_chol.c:386-439) for transformed columns Z = L^-1 X_R.
+
+    Z is n x (count*wXR), SNP i's columns at [i*wXR, (i+1)*wXR).
+    Returns B (count x p) and V (count x p x p) where V[i] is stored so that
+    V[i].ravel(order="F") is the on-disk order (column-major, lower)."""
+    n = Z.shape[0]
+    wXL = tr.XLt.shape[1]
+    p = wXL + wXR
+    count = Z.shape[1] // wXR
+    Zs = Z.reshape(n, count, wXR).transpose(1, 0, 2)           # count x n x wXR
+    S = np.zeros((count, p, p))
+    S[:, :wXL, :wXL] = tr.V_tl + np.tril(tr.V_tl, -1).T        # V_TL (symmetrised copy)
+    VBL = np.einsum("cna,nk->cak", Zs, tr.XLt)                 # dgemm "T","N" :409-412
+    S[:, wXL:, :wXL] = VBL
+    S[:, :wXL, wXL:] = VBL.transpose(0, 2, 1)
+    S[:, wXL:, wXL:] = np.einsum("cna,cnb->cab", Zs, Zs)       # dsyrk :414-417
+    rhs = np.empty((count, p))
+    rhs[:, :wXL] = tr.B_t                                      # memcpy B_t :398
+    rhs[:, wXL:] = np.einsum("cna,n->ca", Zs, tr.yt)           # dgemv "T" :400-403
+    C = np.linalg.cholesky(S)                                  # dpotrf :420
+    b = np.linalg.solve(C, rhs[..., None])                     # dtrsv N :427
+    b = np.linalg.solve(np.swapaxes(C, 1, 2), b)[..., 0]       # dtrsv T :428
+    Ci = np.linalg.inv(C)
+    Sinv = np.swapaxes(Ci, 1, 2) @ Ci                          # dpotri :431
+    V = tr.res_sigma * np.tril(Sinv)                           # dscal over p^2, upper stays 0
+    idx = np.arange(p)
+    V[:, idx, idx] = np.sqrt(V[:, idx, idx])                   # sqrt of the diagonal :437-438
+    return b, V
+
+
+def solve_snps(XR_block: np.ndarray, tr: Trait, wXR: int):
+    """dtrsm of one X_R block (fgls_chol.c:365-368) followed by the per-SNP
+    stage.  XR_block is n x (count*wXR) column-major data as a numpy array."""
+    Z = sla.solve_triangular(tr.L, XR_block, lower=True, check_finite=False)
+    return per_snp_from_z(Z, tr, wXR)
+
+
+def fgls_chol(Phi, H, Sig, XL, XR, Y, wXR: int = 1, x_b: int | None = None):
+    """Whole solver over t traits, outputs in the reference layout.
+
+    XL: n x wXL, XR: n x (m*wXR), Y: n x t.  Returns flat B (t*m*p) and
+    V (t*m*p*p) exactly as B.out / V.out are laid out (fgls_chol.c:458-474):
+    B[(j*m + i)*p + k], V[(j*m + i)*p*p + c*p + r]."""
+    n = Phi.shape[0]
+    wXL = XL.shape[1]
+    p = wXL + wXR
+    m = XR.shape[1] // wXR
+    t = Y.shape[1]
+    x_b = m if x_b is None else x_b
+    Bout = np.empty(t * m * p)
+    Vout = np.empty(t * m * p * p)
+    for j in range(t):
+        L = cholesky_lower(build_M(Phi, float(H[j]), float(Sig[j])))
+        tr = trait_precompute(L, XL, Y[:, j])
+        for ib in range(0, m, x_b):
+            cnt = min(x_b, m - ib)
+            b, V = solve_snps(XR[:, ib * wXR:(ib + cnt) * wXR], tr, wXR)
+            o = (j * m + ib) * p
+            Bout[o:o + cnt * p] = b.ravel()
+            ov = (j * m + ib) * p * p
+            Vout[ov:ov + cnt * p * p] = V.transpose(0, 2, 1).ravel()
+    return Bout, Vout
+
+
+def read_doubles(path: str, count: int, offset: int = 0) -> np.ndarray:
+    """sync_read (io.c:7-19): raw little-endian doubles, no header."""
+    a = np.fromfile(path, dtype="<f8", count=count, offset=offset * 8)
+    if a.size != count:
+        raise IOError(f"{path}: read {a.size} elements - {count} expected")
+    return a
+
+
+def read_inputs(d: str, n: int, p: int, m: int, t: int, wXL: int):
+    """The driver's fixed file names (driver.c:56-66) and layouts."""
+    wXR = p - wXL
+    Phi = read_doubles(os.path.join(d, "Phi.in"), n * n).reshape(n, n, order="F")
+    H = read_doubles(os.path.join(d, "H.in"), t)
+    Sig = read_doubles(os.path.join(d, "Sig.in"), t)
+    XL = read_doubles(os.path.join(d, "XL.in"), wXL * n).reshape(n, wXL, order="F")
+    XR = read_doubles(os.path.join(d, "XR.in"), m * wXR * n).reshape(n, m * wXR, order="F")
+    Y = read_doubles(os.path.join(d, "Y.in"), t * n).reshape(n, t, order="F")
+    return Phi, H, Sig, XL, XR, Y
+
+
+def max_rel_err(a: np.ndarray, b: np.ndarray, p: int | None = None, lower_only: bool = False) -> float:
+    """driver.c:159-177: max_i |a_i - b_i| / |b_i|.
+
+    With lower_only the comparison of a flat V is restricted to the lower
+    triangle of each p x p matrix (the strict upper is 0 in both)."""
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    if lower_only:
+        assert p is not None
+        r, c = np.tril_indices(p)
+        mask = np.zeros((p, p), dtype=bool)
+        mask[c, r] = True            # flat index c*p + r with r >= c
+        sel = np.tile(mask.ravel(), a.size // (p * p))
+        a, b = a[sel], b[sel]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rel = np.abs(a - b) / np.abs(b)
+    rel = rel[~np.isnan(rel)]      # 0/0 is skipped, as NaN > max is false in driver.c:171
+    return float(rel.max()) if rel.size else 0.0

@@ -1,0 +1,115 @@
+"""The drop-in boundary on the GPU: the C host solver fgls_chol_gpu and the
+reference CLI contract (driver.x), files in and files out."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import fgls_oracle as orc
+from tests.fgls_data import make_problem, write_problem_dir
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "cugwas-proto_amd", "lib", "driver.x")
+
+
+def run_driver(d, stdin, variant="g", extra=(), env=None):
+    e = dict(os.environ)
+    if env:
+        e.update(env)
+    return subprocess.run([DRIVER, variant, d, "1", *extra], input=stdin, capture_output=True,
+                          text=True, env=e, timeout=300)
+
+
+def read_out(d, name, count):
+    return np.fromfile(os.path.join(d, name), dtype="<f8", count=count)
+
+
+def test_driver_reference_fixture(gpu, tmp_path):
+    """test_cases/test_XL end to end through the CLI (t=20 traits, x_b=2000)."""
+    from tests.golden.test_xl_fixture import write_dir
+    d = str(tmp_path / "test_XL")
+    prm = write_dir(d)
+    stdin = open(os.path.join(d, "input.in")).read()
+    res = run_driver(d, stdin, extra=("--check",))
+    assert res.returncode == 0, res.stderr
+    lines = res.stdout.strip().splitlines()
+    assert lines[-1].strip().isdigit()          # mean ms per repetition is the last line
+    n, p, m, t = prm["n"], prm["p"], prm["m"], prm["t"]
+    B = read_out(d, "B.out", t * m * p)
+    Bexp = read_out(d, "B_exp.out", t * m * p)
+    assert orc.max_rel_err(B, Bexp) < 1e-10
+    # V against the oracle on the same synthetic X_R
+    Phi, H, Sig, XL, XR, Y = orc.read_inputs(d, n, p, m, t, prm["wXL"])
+    Bo, Vo = orc.fgls_chol(Phi, H, Sig, XL, XR, Y, wXR=1, x_b=prm["x_b"])
+    V = read_out(d, "V.out", t * m * p * p)
+    assert orc.max_rel_err(B, Bo) < 1e-10
+    assert orc.max_rel_err(V, Vo, p=p, lower_only=True) < 1e-10
+    Vm = V.reshape(-1, p, p)
+    r, c = np.triu_indices(p, 1)
+    assert np.all(Vm[:, c, r] == 0.0)
+
+
+@pytest.mark.parametrize("n,wXL,m,t,x_b,wXR", [(1000, 3, 10000, 1, 256, 1),
+                                                 (333, 2, 517, 3, 100, 1),
+                                                 (200, 3, 64, 2, 7, 2)])
+def test_fgls_chol_gpu_matches_oracle(pkg, gpu, tmp_path, n, wXL, m, t, x_b, wXR):
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=n + m + t, t=t, wXR=wXR)
+    d = str(tmp_path)
+    H = np.linspace(0.3, 0.7, t)
+    Sig = np.linspace(0.8, 1.5, t)
+    write_problem_dir(prob, d, H=H, Sig=Sig)
+    p = wXL + wXR
+    f = lambda s: os.path.join(d, s)
+    pkg.fgls_chol_gpu(n, p, m, t, wXL, wXR, x_b, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                      f("XL.in"), f("XR.in"), f("Y.in"), f("B.out"), f("V.out"))
+    Bo, Vo = orc.fgls_chol(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y, wXR=wXR)
+    B = read_out(d, "B.out", t * m * p)
+    V = read_out(d, "V.out", t * m * p * p)
+    assert B.size == Bo.size and V.size == Vo.size
+    assert orc.max_rel_err(B, Bo) < 1e-10
+    assert orc.max_rel_err(V, Vo, p=p, lower_only=True) < 1e-10
+    # block size and the serialised debug mode do not change a single bit
+    for opts in ({"block": x_b}, {"serial": 1, "block": 3 * x_b}):
+        pkg.fgls_chol_gpu(n, p, m, t, wXL, wXR, x_b, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f("XR.in"), f("Y.in"), f("B2.out"), f("V2.out"), options=opts)
+        assert np.array_equal(read_out(d, "B2.out", t * m * p), B)
+        assert np.array_equal(read_out(d, "V2.out", t * m * p * p), V)
+
+
+def test_driver_errors(gpu, tmp_path):
+    """Defined errors instead of the reference's crashes (common.c:11-17)."""
+    prob = make_problem(n=128, wXL=3, m=40, seed=3)
+    d = str(tmp_path)
+    write_problem_dir(prob, d)
+    stdin = "128 4 40 1 3 16 1 1"
+    ok = run_driver(d, stdin)
+    assert ok.returncode == 0, ok.stderr
+    # truncated X_R: the last block cannot be read completely
+    xr = os.path.join(d, "XR.in")
+    data = open(xr, "rb").read()
+    open(xr, "wb").write(data[: len(data) - 8 * 128 * 3])
+    res = run_driver(d, stdin)
+    assert res.returncode == 1 and "[Error]:" in res.stderr and "not read completely" in res.stderr
+    open(xr, "wb").write(data)
+    # missing file
+    os.rename(os.path.join(d, "Phi.in"), os.path.join(d, "Phi.bak"))
+    res = run_driver(d, stdin)
+    assert res.returncode == 1 and "[Error]:" in res.stderr and "Phi.in" in res.stderr
+    os.rename(os.path.join(d, "Phi.bak"), os.path.join(d, "Phi.in"))
+    # singular per-SNP system: an all-zero SNP column
+    X = prob.XR.copy()
+    X[:, 17] = 0.0
+    np.asarray(X.ravel(order="F"), dtype="<f8").tofile(xr)
+    res = run_driver(d, stdin)
+    assert res.returncode == 1 and "dpotrf(V) failed (info: 4) - i: 17" in res.stderr
+    # M not positive definite: h2 = 1 and a duplicated individual makes Phi singular
+    Phi = prob.Phi.copy()
+    Phi[:, 1] = Phi[:, 0]
+    Phi[1, :] = Phi[0, :]
+    np.asarray(Phi.ravel(order="F"), dtype="<f8").tofile(os.path.join(d, "Phi.in"))
+    np.asarray([1.0], dtype="<f8").tofile(os.path.join(d, "H.in"))
+    res = run_driver(d, stdin)
+    assert res.returncode == 1 and "dpotrf(M) failed" in res.stderr
