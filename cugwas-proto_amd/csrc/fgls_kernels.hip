@@ -684,6 +684,7 @@ struct fgls_hip_ctx {
   int nblk;       // 64-row diagonal blocks
   int RT;         // 128-row TRMM tiles
   int max_block;
+  int reduced;    // SNP count of the last block_reduce (finalize must match)
   bool have_trait;
   double* L;
   double* Linv;
@@ -945,35 +946,62 @@ int fgls_hip_snp_stage(fgls_hip_ctx* c, const double* dZ, int64_t ldz, int count
   return FGLS_OK;
 }
 
-int fgls_hip_solve_block(fgls_hip_ctx* c, const double* dXR, int64_t ldxr, int count, double* dB,
-                         double* dV, int64_t snp_base, void* stream) {
+int fgls_hip_block_reduce(fgls_hip_ctx* c, const double* dXR, int64_t ldxr, int count,
+                          void* stream) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!c->have_trait) return fail(FGLS_ERR_STATE, "no trait set");
   if (count < 0 || count > c->max_block)
     return fail(FGLS_ERR_ARG, "count=%d outside [0, %d]", count, c->max_block);
   if (count == 0) return FGLS_OK;
-  if (!dXR || !dB || !dV) return fail(FGLS_ERR_ARG, "NULL pointer");
+  if (!dXR) return fail(FGLS_ERR_ARG, "NULL pointer");
   if (ldxr < c->n || (ldxr & 3) || !aligned32(dXR))
     return fail(FGLS_ERR_ARG, "X_R needs ld >= n, ld %% 4 == 0 and 32-byte alignment");
   if ((rc = set_device(c))) return rc;
   hipStream_t st = (hipStream_t)stream;
   if (c->wXR == 1) {
     P_SWITCH(c->p, launch_trmm_reduce, c, dXR, ldxr, count, st);
-    P_SWITCH(c->p, launch_finalize, c, count, dB, dV, snp_base, st);
   } else {
-    const int ncols = count * c->wXR;
     if (!c->z) {
       const size_t bytes = (size_t)c->ldn * (size_t)c->max_block * c->wXR * sizeof(double);
       if (hipMalloc((void**)&c->z, bytes) != hipSuccess)
         return fail(FGLS_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
     }
-    rc = fgls_hip_trmm(c, dXR, ldxr, ncols, c->z, c->ldn, stream);
+    rc = fgls_hip_trmm(c, dXR, ldxr, count * c->wXR, c->z, c->ldn, stream);
     if (rc) return rc;
+  }
+  HIP_TRY(hipGetLastError());
+  c->reduced = count;
+  return FGLS_OK;
+}
+
+int fgls_hip_block_finalize(fgls_hip_ctx* c, int count, double* dB, double* dV, int64_t snp_base,
+                            void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (count < 0 || count != c->reduced)
+    return fail(FGLS_ERR_STATE, "finalize of %d SNPs after a reduce of %d", count, c->reduced);
+  if (count == 0) return FGLS_OK;
+  if (!dB || !dV) return fail(FGLS_ERR_ARG, "NULL pointer");
+  if ((rc = set_device(c))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (c->wXR == 1) {
+    P_SWITCH(c->p, launch_finalize, c, count, dB, dV, snp_base, st);
+  } else {
     P_SWITCH(c->p, launch_snp_stage, c, c->z, (int64_t)c->ldn, count, dB, dV, snp_base, st);
   }
   HIP_TRY(hipGetLastError());
   return FGLS_OK;
+}
+
+int fgls_hip_solve_block(fgls_hip_ctx* c, const double* dXR, int64_t ldxr, int count, double* dB,
+                         double* dV, int64_t snp_base, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (count > 0 && (!dB || !dV)) return fail(FGLS_ERR_ARG, "NULL pointer");
+  rc = fgls_hip_block_reduce(c, dXR, ldxr, count, stream);
+  if (rc) return rc;
+  return fgls_hip_block_finalize(c, count, dB, dV, snp_base, stream);
 }
 
 int fgls_hip_check(fgls_hip_ctx* c, void* stream, int64_t* bad_snp, int* info) {

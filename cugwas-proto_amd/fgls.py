@@ -59,6 +59,8 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_set_trait": (i32, [vp, dp, i64, dp, i64, dp, vp]),
             "fgls_hip_get_trait": (i32, [vp, dp, dp, dp]),
             "fgls_hip_solve_block": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
+            "fgls_hip_block_reduce": (i32, [vp, dp, i64, i32, vp]),
+            "fgls_hip_block_finalize": (i32, [vp, i32, dp, dp, i64, vp]),
             "fgls_hip_trsm": (i32, [vp, dp, i64, i32, vp]),
             "fgls_hip_trmm": (i32, [vp, dp, i64, i32, dp, i64, vp]),
             "fgls_hip_snp_stage": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
@@ -79,7 +81,8 @@ def lib() -> ctypes.CDLL:
 def exported_symbols() -> list[str]:
     return [s for s in ("fgls_hip_last_error", "fgls_hip_device_count", "fgls_hip_ctx_create",
                         "fgls_hip_ctx_destroy", "fgls_hip_ctx_ldn", "fgls_hip_set_trait",
-                        "fgls_hip_get_trait", "fgls_hip_solve_block", "fgls_hip_trsm",
+                        "fgls_hip_get_trait", "fgls_hip_solve_block", "fgls_hip_block_reduce",
+                        "fgls_hip_block_finalize", "fgls_hip_trsm",
                         "fgls_hip_trmm", "fgls_hip_snp_stage", "fgls_hip_check")
             if hasattr(lib(), s)]
 
@@ -193,6 +196,16 @@ class DeviceContext:
             V = torch.empty((count, self.p * self.p), dtype=torch.float64, device=XR.device)
         _check(lib().fgls_hip_solve_block(self._h, _ptr(XR), XR.shape[1], count, _ptr(B), _ptr(V),
                                           snp_base, _stream(stream)))
+        return B, V
+
+    def block_reduce(self, XR, count: int | None = None, stream=None):
+        _need_cuda_f64(XR, "XR")
+        count = XR.shape[0] // self.wXR if count is None else count
+        _check(lib().fgls_hip_block_reduce(self._h, _ptr(XR), XR.shape[1], count, _stream(stream)))
+        return count
+
+    def block_finalize(self, count: int, B, V, snp_base: int = 0, stream=None):
+        _check(lib().fgls_hip_block_finalize(self._h, count, _ptr(B), _ptr(V), snp_base, _stream(stream)))
         return B, V
 
     def trsm(self, X, ncols: int | None = None, stream=None):

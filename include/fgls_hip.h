@@ -97,6 +97,19 @@ int fgls_hip_solve_block(fgls_hip_ctx *ctx, const double *dXR, int64_t ldxr,
                          void *stream);
 
 /*
+ * The two stages of fgls_hip_solve_block, for callers that overlap or time
+ * them separately.  block_reduce runs the triangular MFMA product of the
+ * block with the per-row-tile reductions (X~_L' z, z'z, z'y~) kept in the
+ * context (for wXR > 1: z itself); block_finalize turns them into b_i and
+ * V_i.  A finalize must follow the reduce of the same `count` on the same
+ * stream before the next reduce.
+ */
+int fgls_hip_block_reduce(fgls_hip_ctx *ctx, const double *dXR, int64_t ldxr,
+                          int count, void *stream);
+int fgls_hip_block_finalize(fgls_hip_ctx *ctx, int count, double *dB,
+                            double *dV, int64_t snp_base, void *stream);
+
+/*
  * In-place X <- L^-1 X for ncols columns (ld ldx), with the factor of the
  * current trait: the direct replacement of
  * cublasDtrsm(LEFT, LOWER, N, NON_UNIT, n, ncols, 1, L, n, X, n)

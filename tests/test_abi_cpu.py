@@ -1,0 +1,86 @@
+"""CPU-side checks of the C-ABI library (no compute calls without a GPU)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def declared_functions():
+    names = []
+    for fn in sorted(os.listdir(INCLUDE)):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^[A-Za-z_][\w \*]*?\b(fgls_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_headers_declare_the_boundary():
+    names = declared_functions()
+    for required in ("fgls_chol_gpu", "fgls_chol_gpu_ex", "fgls_hip_ctx_create", "fgls_hip_set_trait",
+                     "fgls_hip_solve_block", "fgls_hip_trsm", "fgls_hip_snp_stage", "fgls_hip_check"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    lib = pkg.lib()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_no_device_is_reported_cleanly(pkg):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    assert pkg.device_count() == 0
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.DeviceContext(100, 3, 1, 64)
+    assert ei.value.code == pkg.FGLS_ERR_ARG
+    assert "device 0 not available" in ei.value.msg
+
+
+def test_argument_errors(pkg):
+    lib = pkg.lib()
+    h = ctypes.c_void_p()
+    assert lib.fgls_hip_ctx_create(ctypes.byref(h), 0, 3, 1, 64, 0) == pkg.FGLS_ERR_ARG
+    assert lib.fgls_hip_ctx_create(ctypes.byref(h), 100, 3, 0, 64, 0) == pkg.FGLS_ERR_ARG
+    assert lib.fgls_hip_ctx_create(ctypes.byref(h), 100, 15, 2, 64, 0) == pkg.FGLS_ERR_ARG  # p > 16
+    assert lib.fgls_hip_solve_block(None, None, 0, 1, None, None, 0, None) == pkg.FGLS_ERR_ARG
+    assert b"NULL context" in lib.fgls_hip_last_error()
+
+
+def test_host_solver_input_errors(pkg, tmp_path):
+    """File errors surface before any device work (defined errors, not crashes)."""
+    d = str(tmp_path)
+    f = lambda s: os.path.join(d, s)
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.fgls_chol_gpu(10, 4, 5, 1, 3, 1, 5, 1, 1, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f("XR.in"), f("Y.in"), f("B.out"), f("V.out"))
+    assert ei.value.code == pkg.FGLS_ERR_IO and "Phi.in" in ei.value.msg
+    np.zeros(50).tofile(f("Phi.in"))                      # short: 50 of 100 doubles
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.fgls_chol_gpu(10, 4, 5, 1, 3, 1, 5, 1, 1, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f("XR.in"), f("Y.in"), f("B.out"), f("V.out"))
+    assert ei.value.code == pkg.FGLS_ERR_IO and "read 50 elements - 100 expected" in ei.value.msg
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.fgls_chol_gpu(10, 5, 5, 1, 3, 1, 5, 1, 1, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f("XR.in"), f("Y.in"), f("B.out"), f("V.out"))
+    assert ei.value.code == pkg.FGLS_ERR_ARG                # p != wXL + wXR
+
+
+def test_driver_usage_and_exit_codes(pkg, tmp_path):
+    drv = os.path.join(ROOT, "cugwas-proto_amd", "lib", "driver.x")
+    assert os.path.exists(drv)
+    r = subprocess.run([drv], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    r = subprocess.run([drv, "e", str(tmp_path), "1"], input="10 4 5 1 3 5 1 1", capture_output=True, text=True)
+    assert r.returncode == 1 and "fgls_eigen" in r.stderr
+    r = subprocess.run([drv, "g", str(tmp_path), "1"], input="10 4 5 1 3 5 1 1", capture_output=True, text=True)
+    assert r.returncode == 1 and r.stderr.startswith("[Error]: ") and "Phi.in" in r.stderr
