@@ -240,21 +240,33 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
 #pragma unroll
     for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
 
+  // Main K loop: operands of step k0 + 16 are loaded while the 64 MFMAs of
+  // step k0 run.  The prefetch is unconditional inside the loop and the last
+  // full step is peeled, so the accumulators stay in AGPRs across iterations.
   v4d a[4], b[4];
   if (kfull > 0) {
 #pragma unroll
     for (int ti = 0; ti < 4; ++ti) a[ti] = ld4(Ap[ti]);
 #pragma unroll
     for (int tj = 0; tj < 4; ++tj) b[tj] = ld4(Bp[tj]);
-  }
-  for (int k0 = 0; k0 < kfull; k0 += 16) {
-    v4d an[4], bn[4];
-    const bool more = (k0 + 16) < kfull;
-    if (more) {
+    for (int k0 = 16; k0 < kfull; k0 += 16) {
+      v4d an[4], bn[4];
 #pragma unroll
-      for (int ti = 0; ti < 4; ++ti) an[ti] = ld4(Ap[ti] + k0 + 16);
+      for (int ti = 0; ti < 4; ++ti) an[ti] = ld4(Ap[ti] + k0);
 #pragma unroll
-      for (int tj = 0; tj < 4; ++tj) bn[tj] = ld4(Bp[tj] + k0 + 16);
+      for (int tj = 0; tj < 4; ++tj) bn[tj] = ld4(Bp[tj] + k0);
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < 4; ++tj)
+            acc[ti][tj] = (EPI == EPI_STORE) ? mfma_f64(b[tj][s], a[ti][s], acc[ti][tj])
+                                             : mfma_f64(a[ti][s], b[tj][s], acc[ti][tj]);
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) a[ti] = an[ti];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) b[tj] = bn[tj];
     }
 #pragma unroll
     for (int s = 0; s < 4; ++s)
@@ -264,12 +276,6 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
         for (int tj = 0; tj < 4; ++tj)
           acc[ti][tj] = (EPI == EPI_STORE) ? mfma_f64(b[tj][s], a[ti][s], acc[ti][tj])
                                            : mfma_f64(a[ti][s], b[tj][s], acc[ti][tj]);
-    if (more) {
-#pragma unroll
-      for (int ti = 0; ti < 4; ++ti) a[ti] = an[ti];
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj) b[tj] = bn[tj];
-    }
   }
   if (kend > kfull) {  // last partial K step: rows k >= n of X are masked
     const int k0 = kfull;
