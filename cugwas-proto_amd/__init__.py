@@ -20,4 +20,6 @@ from .fgls import (  # noqa: F401
     FGLS_ERR_NOTSPD,
     FGLS_ERR_IO,
     FGLS_ERR_STATE,
+    FGLS_MODE_FP64,
+    FGLS_MODE_MIXED_I8,
 )

@@ -369,6 +369,228 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
 }
 
 // ---------------------------------------------------------------------------
+// Mixed precision: L^-1 as S signed 7-bit slices, genotypes as int8.
+//
+// Row i of L^-1 is scaled by sc_i = 2^(6-e_i) (e_i = frexp exponent of the
+// row's largest magnitude, so |L^-1_ik| sc_i < 64) and written as
+//   L^-1_ik sc_i = sum_{s<S} a_s,ik 2^(-7s) + O(2^(-7S+6)),  a_s,ik in [-64, 64]
+// (a_s = rint(r_s), r_{s+1} = 128 (r_s - a_s); every step is exact in FP64).
+// z_i = L^-1 x for integer genotypes x is then sum_s 2^(-7s) (A_s x)_i / sc_i,
+// where every A_s x is an exact int32 product on the int8 MFMA
+// (|a x| <= 64*127 per term, so the sum stays exact up to n ~ 2.6e5).
+//
+// Slice layout: for row r and 16-column chunk kc, the S slices' 16 bytes are
+// contiguous: A8[((r*KC + kc)*S + s)*16 + b], KC = ldn/16, so each lane of the
+// MFMA A operand fetches all S slices of its 16 k with S 16-byte loads.
+// ---------------------------------------------------------------------------
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4i mfma_i8(v4i a, v4i b, v4i c) {
+  return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+  return x;
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_slice_linv(int ldn, const double* __restrict__ Linv,
+                                                    int8_t* __restrict__ A8,
+                                                    double* __restrict__ inv_scale) {
+  __shared__ double red[4];
+  const int row = blockIdx.x, tid = threadIdx.x;
+  const double* Lr = Linv + (size_t)row * ldn;
+  double mx = 0.0;
+  for (int k = tid; k < ldn; k += 256) mx = fmax(mx, fabs(Lr[k]));
+  mx = wave_max(mx);
+  if ((tid & 63) == 0) red[tid >> 6] = mx;
+  __syncthreads();
+  mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  int e = 0;
+  if (mx > 0.0) frexp(mx, &e);         // mx in [2^(e-1), 2^e)
+  const double sc = ldexp(1.0, 6 - e); // mx * sc in [32, 64)
+  if (tid == 0) inv_scale[row] = ldexp(1.0, e - 6);
+  const int KC = ldn / 16;
+  for (int kc = tid; kc < KC; kc += 256) {
+    int8_t out[S][16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      double r = Lr[kc * 16 + b] * sc;
+#pragma unroll
+      for (int s2 = 0; s2 < S; ++s2) {
+        const double a = rint(r);
+        out[s2][b] = (int8_t)(int)a;
+        r = (r - a) * 128.0;
+      }
+    }
+    int8_t* dst = A8 + ((size_t)row * KC + kc) * S * 16;
+#pragma unroll
+    for (int s2 = 0; s2 < S; ++s2) *reinterpret_cast<v4i*>(dst + s2 * 16) = *reinterpret_cast<const v4i*>(out[s2]);
+  }
+}
+
+// Int8 triangular product with the same fused reductions as k_trmm<EPI_REDUCE>.
+// Workgroup tile 64 rows x 128 columns, 2x2 waves of 32 x 64, one int32
+// accumulator set per slice.  MFMA v_mfma_i32_16x16x64_i8: A lane l holds 16
+// consecutive k of row l%16 (k chunk l/16 of the 64), B lane l the same 16 k
+// of column l%16; D element v of lane l is D[4*(l/16)+v][l%16].
+// X8 is column-major int8 (ld ldx8 >= ceil(n/64)*64, ldx8 % 16 == 0); its
+// rows >= n may hold anything, they meet all-zero slice columns.
+template <int NQ, int S>
+__global__ __launch_bounds__(256) void k_trmm_i8(int n, int count, int RT8, int KC,
+                                                 const int8_t* __restrict__ A8,
+                                                 const double* __restrict__ inv_scale,
+                                                 const int8_t* __restrict__ X8, int64_t ldx8,
+                                                 const double* __restrict__ W, int64_t ldw,
+                                                 double* __restrict__ part, int64_t ldp) {
+  const int ct = blockIdx.x;
+  const int rt = RT8 - 1 - (int)blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  const int li = lane & 15, lg = lane >> 4;
+  const int r0 = rt * 64 + wr * 32;
+  const int c0 = ct * 128 + wc * 64;
+  const int n64 = (n + 63) & ~63;
+  const int kend = (r0 < n64) ? min((r0 + 32 + 63) & ~63, n64) : 0;   // multiple of 64
+
+  const int8_t* Ap[2];
+  const int8_t* Bp[4];
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti) Ap[ti] = A8 + ((size_t)(r0 + 16 * ti + li) * KC + lg) * S * 16;
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+    const int col = min(c0 + 16 * tj + li, count - 1);
+    Bp[tj] = X8 + (size_t)col * ldx8 + 16 * lg;
+  }
+  constexpr int ASTEP = 4 * S * 16;   // bytes of A per lane per 64-k step
+
+  v4i acc[S][2][4];
+#pragma unroll
+  for (int s2 = 0; s2 < S; ++s2)
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = v4i{0, 0, 0, 0};
+
+  if (kend > 0) {
+    v4i a[2][S], b[4];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int s2 = 0; s2 < S; ++s2) a[ti][s2] = *reinterpret_cast<const v4i*>(Ap[ti] + s2 * 16);
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) b[tj] = *reinterpret_cast<const v4i*>(Bp[tj]);
+    int step = 1;
+    for (int k0 = 64; k0 < kend; k0 += 64, ++step) {
+      v4i an[2][S], bn[4];
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int s2 = 0; s2 < S; ++s2)
+          an[ti][s2] = *reinterpret_cast<const v4i*>(Ap[ti] + (size_t)step * ASTEP + s2 * 16);
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) bn[tj] = *reinterpret_cast<const v4i*>(Bp[tj] + k0);
+#pragma unroll
+      for (int s2 = 0; s2 < S; ++s2)
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(a[ti][s2], b[tj], acc[s2][ti][tj]);
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int s2 = 0; s2 < S; ++s2) a[ti][s2] = an[ti][s2];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) b[tj] = bn[tj];
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < S; ++s2)
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(a[ti][s2], b[tj], acc[s2][ti][tj]);
+  }
+
+  // z(row r0+16ti+4lg+v, col c0+16tj+li) = inv_scale[row] * sum_s 2^(-7s) acc[s]
+  __shared__ double red[2][64][NQ];
+  double q[4][NQ];
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+    for (int qi = 0; qi < NQ; ++qi) q[tj][qi] = 0.0;
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = r0 + 16 * ti + 4 * lg + v;
+      const double isc = inv_scale[row];
+      double wv[NQ - 1];
+#pragma unroll
+      for (int k = 0; k < NQ - 1; ++k) wv[k] = W[(size_t)k * ldw + row];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        double t = (double)acc[S - 1][ti][tj][v];
+#pragma unroll
+        for (int s2 = S - 2; s2 >= 0; --s2) t = fma(t, 1.0 / 128.0, (double)acc[s2][ti][tj][v]);
+        const double z = t * isc;
+        q[tj][0] = fma(z, z, q[tj][0]);
+#pragma unroll
+        for (int k = 0; k < NQ - 1; ++k) q[tj][1 + k] = fma(z, wv[k], q[tj][1 + k]);
+      }
+    }
+  }
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+    for (int qi = 0; qi < NQ; ++qi) {
+      double x = q[tj][qi];
+      x += __shfl_xor(x, 16, 64);
+      x += __shfl_xor(x, 32, 64);
+      q[tj][qi] = x;
+    }
+  if (wr == 1 && lg == 0) {
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int qi = 0; qi < NQ; ++qi) red[wc][16 * tj + li][qi] = q[tj][qi];
+  }
+  __syncthreads();
+  if (wr == 0 && lg == 0) {
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) {
+      const int col = c0 + 16 * tj + li;
+      if (col < count) {
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi)
+          part[((size_t)rt * NQ + qi) * ldp + col] = q[tj][qi] + red[wc][16 * tj + li][qi];
+      }
+    }
+  }
+}
+
+// FP64 genotypes -> int8 (column-major, ld ldx8), exact only: a value that is
+// not an integer in [-127, 127] is counted in *bad.
+__global__ __launch_bounds__(256) void k_to_i8(int n, const double* __restrict__ X,
+                                               int64_t ldx, int8_t* __restrict__ X8, int64_t ldx8,
+                                               unsigned int* __restrict__ bad) {
+  const int col = blockIdx.x;   // one workgroup per column
+  unsigned int nb = 0;
+  for (int r = threadIdx.x; r < ldx8; r += 256) {
+    int8_t o = 0;
+    if (r < n) {
+      const double v = X[(size_t)col * ldx + r];
+      const double rv = rint(v);
+      if (!(rv == v && fabs(v) <= 127.0)) ++nb;
+      else o = (int8_t)(int)rv;
+    }
+    X8[(size_t)col * ldx8 + r] = o;
+  }
+  if (nb) atomicAdd(bad, nb);
+}
+
+// ---------------------------------------------------------------------------
 // p x p tail of the per-SNP stage (fgls_chol.c:401-438): Cholesky of the
 // lower triangle of S, two triangular solves for b, inverse through the
 // factor (trtri + L'L product, as dpotri does), scaling by res_sigma and
@@ -691,7 +913,14 @@ struct fgls_hip_ctx {
   int RT;         // 128-row TRMM tiles
   int max_block;
   int reduced;    // SNP count of the last block_reduce (finalize must match)
+  int last_rt;    // row tiles of the partials written by the last block_reduce
   bool have_trait;
+  int mode;       // FGLS_MODE_FP64 or FGLS_MODE_MIXED_I8
+  int RT8;        // 64-row tiles of the int8 product
+  int part_rt;    // row-tile capacity of `part`
+  int8_t* A8;     // slices of L^-1 (mixed mode)
+  double* invsc;  // per-row 2^(e_i - 6)
+  unsigned int* bad;  // count of genotypes that are not int8-exact
   double* L;
   double* Linv;
   double* Dinv;
@@ -719,7 +948,7 @@ template <int P>
 void launch_finalize(const fgls_hip_ctx* c, int count, double* dB, double* dV, int64_t base,
                      hipStream_t st) {
   dim3 grid((count + 127) / 128);
-  hipLaunchKernelGGL(k_finalize<P>, grid, dim3(128), 0, st, count, c->RT, c->part,
+  hipLaunchKernelGGL(k_finalize<P>, grid, dim3(128), 0, st, count, c->last_rt, c->part,
                      (int64_t)c->max_block, c->trait, dB, dV, c->err, base);
 }
 
@@ -730,6 +959,24 @@ void launch_trmm_reduce(const fgls_hip_ctx* c, const double* X, int64_t ldx, int
   hipLaunchKernelGGL((k_trmm<EPI_REDUCE, P + 1>), grid, dim3(256), 0, st, c->n, count, c->RT,
                      c->Linv, (int64_t)c->ldn, X, ldx, (double*)nullptr, (int64_t)0, c->W,
                      (int64_t)c->ldn, c->part, (int64_t)c->max_block);
+}
+
+constexpr int NSLICE = 6;   // 7-bit slices of L^-1 in the mixed mode
+
+template <int P>
+void launch_trmm_i8(const fgls_hip_ctx* c, const int8_t* X8, int64_t ldx8, int count,
+                    hipStream_t st) {
+  dim3 grid((count + 127) / 128, c->RT8);
+  hipLaunchKernelGGL((k_trmm_i8<P + 1, NSLICE>), grid, dim3(256), 0, st, c->n, count, c->RT8,
+                     c->ldn / 16, c->A8, c->invsc, X8, ldx8, c->W, (int64_t)c->ldn, c->part,
+                     (int64_t)c->max_block);
+}
+
+int build_slices(fgls_hip_ctx* c, hipStream_t st) {
+  hipLaunchKernelGGL((k_slice_linv<NSLICE>), dim3(c->ldn), dim3(256), 0, st, c->ldn, c->Linv, c->A8,
+                     c->invsc);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
 }
 
 template <int P>
@@ -805,6 +1052,9 @@ int fgls_hip_ctx_create(fgls_hip_ctx** out, int n, int wXL, int wXR, int max_blo
   c->nblk = (n + NB - 1) / NB;
   c->RT = c->ldn / TM;
   c->max_block = (int)round_up(max_block, 4);
+  c->RT8 = c->ldn / 64;
+  c->part_rt = c->RT;
+  c->mode = FGLS_MODE_FP64;
   const size_t ldn = (size_t)c->ldn;
   const size_t nq = (size_t)c->p + 1;
   struct { void** ptr; size_t bytes; } allocs[] = {
@@ -816,6 +1066,7 @@ int fgls_hip_ctx_create(fgls_hip_ctx** out, int n, int wXL, int wXR, int max_blo
       {(void**)&c->part, (size_t)c->RT * nq * (size_t)c->max_block * sizeof(double)},
       {(void**)&c->err, sizeof(unsigned long long)},
       {(void**)&c->status, 4 * sizeof(int)},
+      {(void**)&c->bad, 4 * sizeof(unsigned int)},
   };
   for (auto& a : allocs) {
     if (hipMalloc(a.ptr, a.bytes) != hipSuccess) {
@@ -837,7 +1088,8 @@ int fgls_hip_ctx_destroy(fgls_hip_ctx* c) {
   if (!c) return FGLS_OK;
   hipSetDevice(c->device);
   hipDeviceSynchronize();
-  void* ptrs[] = {c->L, c->Linv, c->Dinv, c->W, c->trait, c->part, c->z, c->err, c->status};
+  void* ptrs[] = {c->L, c->Linv, c->Dinv, c->W, c->trait, c->part, c->z, c->err, c->status,
+                  c->A8, c->invsc, c->bad};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete c;
@@ -875,6 +1127,7 @@ int fgls_hip_set_trait(fgls_hip_ctx* c, const double* dL, int64_t ldl, const dou
   hipLaunchKernelGGL(k_trait_stats, dim3(1), dim3(256), 0, st, n, wXL, c->W, (int64_t)ldn, c->trait,
                      c->status);
   HIP_TRY(hipGetLastError());
+  if (c->mode == FGLS_MODE_MIXED_I8 && (rc = build_slices(c, st))) return rc;
   int status = 0;
   HIP_TRY(hipMemcpyAsync(&status, c->status, sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -978,7 +1231,84 @@ int fgls_hip_block_reduce(fgls_hip_ctx* c, const double* dXR, int64_t ldxr, int 
   }
   HIP_TRY(hipGetLastError());
   c->reduced = count;
+  c->last_rt = c->RT;
   return FGLS_OK;
+}
+
+int fgls_hip_ctx_set_mode(fgls_hip_ctx* c, int mode) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (mode != FGLS_MODE_FP64 && mode != FGLS_MODE_MIXED_I8) return fail(FGLS_ERR_ARG, "unknown mode %d", mode);
+  if (mode == FGLS_MODE_MIXED_I8 && c->wXR != 1)
+    return fail(FGLS_ERR_ARG, "the mixed mode needs wXR = 1 (got %d)", c->wXR);
+  if ((rc = set_device(c))) return rc;
+  if (mode == FGLS_MODE_MIXED_I8) {
+    if (!c->A8) {
+      const size_t bytes = (size_t)c->ldn * c->ldn * NSLICE;
+      if (hipMalloc((void**)&c->A8, bytes) != hipSuccess)
+        return fail(FGLS_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+      if (hipMalloc((void**)&c->invsc, (size_t)c->ldn * sizeof(double)) != hipSuccess)
+        return fail(FGLS_ERR_NOMEM, "hipMalloc of the row scales failed");
+    }
+    if (c->part_rt < c->RT8) {
+      double* np = nullptr;
+      const size_t bytes = (size_t)c->RT8 * (c->p + 1) * (size_t)c->max_block * sizeof(double);
+      HIP_TRY(hipDeviceSynchronize());
+      if (hipMalloc((void**)&np, bytes) != hipSuccess)
+        return fail(FGLS_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+      HIP_TRY(hipFree(c->part));
+      c->part = np;
+      c->part_rt = c->RT8;
+    }
+    if (c->have_trait) {
+      if ((rc = build_slices(c, nullptr))) return rc;
+      HIP_TRY(hipDeviceSynchronize());
+    }
+  }
+  c->mode = mode;
+  return FGLS_OK;
+}
+
+int fgls_hip_to_i8(const double* dX, int64_t ldx, int n, int ncols, int8_t* dX8, int64_t ldx8,
+                   unsigned int* dbad, void* stream) {
+  if (n < 1 || ncols < 0 || ldx < n || ldx8 < n || !dX || !dX8 || !dbad)
+    return fail(FGLS_ERR_ARG, "bad arguments to fgls_hip_to_i8");
+  if (ncols == 0) return FGLS_OK;
+  hipLaunchKernelGGL(k_to_i8, dim3(ncols), dim3(256), 0, (hipStream_t)stream, n, dX, ldx, dX8, ldx8,
+                     dbad);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_block_reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count,
+                             void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->have_trait) return fail(FGLS_ERR_STATE, "no trait set");
+  if (c->mode != FGLS_MODE_MIXED_I8) return fail(FGLS_ERR_STATE, "context is not in the mixed mode");
+  if (count < 0 || count > c->max_block)
+    return fail(FGLS_ERR_ARG, "count=%d outside [0, %d]", count, c->max_block);
+  if (count == 0) return FGLS_OK;
+  if (!dX8) return fail(FGLS_ERR_ARG, "NULL pointer");
+  if (ldx8 < (int64_t)round_up(c->n, 64) || (ldx8 & 15) || ((uintptr_t)dX8 & 15))
+    return fail(FGLS_ERR_ARG, "int8 X_R needs ld >= ceil(n/64)*64, ld %% 16 == 0 and 16-byte alignment");
+  if ((rc = set_device(c))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  P_SWITCH(c->p, launch_trmm_i8, c, dX8, ldx8, count, st);
+  HIP_TRY(hipGetLastError());
+  c->reduced = count;
+  c->last_rt = c->RT8;
+  return FGLS_OK;
+}
+
+int fgls_hip_solve_block_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count, double* dB,
+                            double* dV, int64_t snp_base, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (count > 0 && (!dB || !dV)) return fail(FGLS_ERR_ARG, "NULL pointer");
+  rc = fgls_hip_block_reduce_i8(c, dX8, ldx8, count, stream);
+  if (rc) return rc;
+  return fgls_hip_block_finalize(c, count, dB, dV, snp_base, stream);
 }
 
 int fgls_hip_block_finalize(fgls_hip_ctx* c, int count, double* dB, double* dV, int64_t snp_base,

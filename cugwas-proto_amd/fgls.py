@@ -32,6 +32,9 @@ FGLS_ERR_NOTSPD = -4
 FGLS_ERR_IO = -5
 FGLS_ERR_STATE = -6
 
+FGLS_MODE_FP64 = 0
+FGLS_MODE_MIXED_I8 = 1
+
 _lib = None
 
 
@@ -61,6 +64,10 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_solve_block": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
             "fgls_hip_block_reduce": (i32, [vp, dp, i64, i32, vp]),
             "fgls_hip_block_finalize": (i32, [vp, i32, dp, dp, i64, vp]),
+            "fgls_hip_ctx_set_mode": (i32, [vp, i32]),
+            "fgls_hip_block_reduce_i8": (i32, [vp, dp, i64, i32, vp]),
+            "fgls_hip_solve_block_i8": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
+            "fgls_hip_to_i8": (i32, [dp, i64, i32, i32, dp, i64, dp, vp]),
             "fgls_hip_trsm": (i32, [vp, dp, i64, i32, vp]),
             "fgls_hip_trmm": (i32, [vp, dp, i64, i32, dp, i64, vp]),
             "fgls_hip_snp_stage": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
@@ -82,7 +89,8 @@ def exported_symbols() -> list[str]:
     return [s for s in ("fgls_hip_last_error", "fgls_hip_device_count", "fgls_hip_ctx_create",
                         "fgls_hip_ctx_destroy", "fgls_hip_ctx_ldn", "fgls_hip_set_trait",
                         "fgls_hip_get_trait", "fgls_hip_solve_block", "fgls_hip_block_reduce",
-                        "fgls_hip_block_finalize", "fgls_hip_trsm",
+                        "fgls_hip_block_finalize", "fgls_hip_ctx_set_mode", "fgls_hip_block_reduce_i8",
+                        "fgls_hip_solve_block_i8", "fgls_hip_to_i8", "fgls_hip_trsm",
                         "fgls_hip_trmm", "fgls_hip_snp_stage", "fgls_hip_check")
             if hasattr(lib(), s)]
 
@@ -206,6 +214,49 @@ class DeviceContext:
 
     def block_finalize(self, count: int, B, V, snp_base: int = 0, stream=None):
         _check(lib().fgls_hip_block_finalize(self._h, count, _ptr(B), _ptr(V), snp_base, _stream(stream)))
+        return B, V
+
+    # -- mixed mode (int8 genotypes, sliced L^-1) ---------------------------
+    def set_mode(self, mode: int):
+        _check(lib().fgls_hip_ctx_set_mode(self._h, mode))
+        self.mode = mode
+
+    @property
+    def ld8(self) -> int:
+        """Smallest valid leading dimension of an int8 genotype block."""
+        return (self.n + 63) // 64 * 64
+
+    def to_i8(self, X, ncols: int | None = None, X8=None, stream=None):
+        """FP64 block (ncols, ldx) -> int8 block (ncols, ld8); raises if a value
+        is not an integer in [-127, 127]."""
+        import torch
+        _need_cuda_f64(X, "X")
+        ncols = X.shape[0] if ncols is None else ncols
+        if X8 is None:
+            X8 = torch.empty((ncols, self.ld8), dtype=torch.int8, device=X.device)
+        bad = torch.zeros(1, dtype=torch.int32, device=X.device)
+        _check(lib().fgls_hip_to_i8(_ptr(X), X.shape[1], self.n, ncols, _ptr(X8), X8.shape[1],
+                                    _ptr(bad), _stream(stream)))
+        nb = int(bad.item())
+        if nb:
+            raise FglsError(FGLS_ERR_ARG, f"{nb} genotype values are not integers in [-127, 127]")
+        return X8
+
+    def block_reduce_i8(self, X8, count: int | None = None, stream=None):
+        count = X8.shape[0] if count is None else count
+        _check(lib().fgls_hip_block_reduce_i8(self._h, _ptr(X8), X8.shape[1], count, _stream(stream)))
+        return count
+
+    def solve_block_i8(self, X8, count: int | None = None, B=None, V=None, snp_base: int = 0, stream=None):
+        import torch
+        if count is None:
+            count = X8.shape[0]
+        if B is None:
+            B = torch.empty((count, self.p), dtype=torch.float64, device=X8.device)
+        if V is None:
+            V = torch.empty((count, self.p * self.p), dtype=torch.float64, device=X8.device)
+        _check(lib().fgls_hip_solve_block_i8(self._h, _ptr(X8), X8.shape[1], count, _ptr(B), _ptr(V),
+                                             snp_base, _stream(stream)))
         return B, V
 
     def trsm(self, X, ncols: int | None = None, stream=None):

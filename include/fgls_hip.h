@@ -31,8 +31,11 @@ extern "C" {
 #define FGLS_ERR_IO       -5   /* file open / read / write failure (host layer) */
 #define FGLS_ERR_STATE    -6   /* call order violated (e.g. no trait set)      */
 
-/* Arithmetic of the per-SNP hot path. */
-#define FGLS_MODE_FP64     0   /* FP64 inputs, FP64 MFMA                       */
+/* Arithmetic of the per-SNP hot path (fgls_hip_ctx_set_mode). */
+#define FGLS_MODE_FP64     0   /* FP64 genotypes, FP64 MFMA                    */
+#define FGLS_MODE_MIXED_I8 1   /* int8 genotypes; L^-1 as 6 signed 7-bit slices;
+                                  exact int32 accumulation on the int8 MFMA;
+                                  FP64 per-row recombination and per-SNP solve */
 
 typedef struct fgls_hip_ctx fgls_hip_ctx;
 
@@ -108,6 +111,27 @@ int fgls_hip_block_reduce(fgls_hip_ctx *ctx, const double *dXR, int64_t ldxr,
                           int count, void *stream);
 int fgls_hip_block_finalize(fgls_hip_ctx *ctx, int count, double *dB,
                             double *dV, int64_t snp_base, void *stream);
+
+/*
+ * Mixed mode (no counterpart in the reference, whose arithmetic is FP64 only).
+ * set_mode(FGLS_MODE_MIXED_I8) allocates ldn*ldn*6 bytes of slices of L^-1
+ * (built at set_trait, or at once if a trait is set) and needs wXR = 1.
+ * The genotypes of a block are int8, column-major, ld ldx8 >=
+ * ceil(n/64)*64 with ldx8 % 16 == 0 (rows n..ld-1 are ignored).  b and V
+ * have the layout of fgls_hip_solve_block; their error against the FP64
+ * path is set by the slice count (<= 1e-9 relative at n = 1e4, see the tests).
+ */
+int fgls_hip_ctx_set_mode(fgls_hip_ctx *ctx, int mode);
+int fgls_hip_block_reduce_i8(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ldx8,
+                             int count, void *stream);
+int fgls_hip_solve_block_i8(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ldx8,
+                            int count, double *dB, double *dV, int64_t snp_base,
+                            void *stream);
+/* FP64 -> int8 genotypes (ld ldx8, rows n..ldx8-1 zeroed); values that are
+ * not integers in [-127, 127] are counted into *dbad (device counter,
+ * caller-cleared) and written as 0.  Asynchronous. */
+int fgls_hip_to_i8(const double *dX, int64_t ldx, int n, int ncols,
+                   int8_t *dX8, int64_t ldx8, unsigned int *dbad, void *stream);
 
 /*
  * In-place X <- L^-1 X for ncols columns (ld ldx), with the factor of the

@@ -203,6 +203,8 @@ def parity_errors(B, V, Bref, Vref, p: int) -> dict:
       standard error (the diagonal of V_ref).
     * ``V_scaled``: diagonal (standard errors) element-wise; off-diagonal
       covariances as |V - V_ref| / max(|V_ref|, se_r * se_c).
+    * ``var_rel``: element-wise relative error of the variances se^2, i.e.
+      of var(b_ik), never near zero.
     The scaled forms are the parity bars; both are reported."""
     B = np.asarray(B, dtype=np.float64).reshape(-1, p)
     Bref = np.asarray(Bref, dtype=np.float64).reshape(-1, p)
@@ -213,7 +215,10 @@ def parity_errors(B, V, Bref, Vref, p: int) -> dict:
     scale = np.maximum(np.abs(Vref), se[:, :, None] * se[:, None, :])
     ev = np.abs(V - Vref) / scale
     r, c = np.tril_indices(p)
+    se_new = np.einsum("icc->ic", V)
+    var_rel = np.abs(se_new ** 2 - se ** 2) / (se ** 2)
     return {"B_rel": max_rel_err(B, Bref), "V_rel": max_rel_err(V, Vref, p=p, lower_only=True),
+            "var_rel": float(np.max(var_rel)) if var_rel.size else 0.0,
             "B_scaled": float(np.max(eb)) if eb.size else 0.0,
             "V_scaled": float(np.max(ev[:, c, r])) if ev.size else 0.0,
             "V_upper_zero": bool(np.all(V[:, r[r != c], c[r != c]] == 0.0)) if p > 1 else True}

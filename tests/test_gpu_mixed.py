@@ -1,0 +1,87 @@
+"""GPU tests of the mixed-precision path (int8 genotypes, 6 seven-bit slices
+of L^-1, exact int32 MFMA accumulation): its b and V against the FP64 oracle
+within the stated bound, and against the device's own FP64 path."""
+import numpy as np
+import pytest
+
+from oracle import fgls_oracle as orc
+from tests.fgls_data import make_problem
+from tests.test_gpu_parity import setup, v_flat
+
+pytestmark = pytest.mark.gpu
+
+# Stated bound of the mixed path: b relative to max(|b|, se), standard errors
+# element-wise and covariances relative to max(|V_rc|, se_r se_c), <= 1e-9;
+# the variances var(b_ik) element-wise <= 1e-8.  The plain element-wise error
+# of b and of the covariances is reported, not gated: it is unbounded for
+# entries near zero (at any precision).
+TOL_SCALED = 1e-9
+TOL_VAR = 1e-8
+
+
+@pytest.mark.parametrize("n,m,wXL,h2", [(1000, 2000, 3, 0.4), (333, 257, 3, 0.6), (2500, 700, 5, 0.3),
+                                        (64, 10, 1, 0.5), (4096, 300, 3, 0.5)])
+def test_mixed_matches_oracle(pkg, gpu, n, m, wXL, h2):
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=7 * n + m, h2=h2)
+    tr, ctx = setup(pkg, prob, max_block=max(m, 64))
+    ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)
+    b_ref, V_ref = orc.solve_snps(prob.XR, tr, 1)
+    X8 = ctx.to_i8(pkg.colmajor(prob.XR))
+    B, V = ctx.solve_block_i8(X8, snp_base=3)
+    ctx.check()
+    e = orc.parity_errors(B.cpu().numpy(), V.cpu().numpy(), b_ref, v_flat(V_ref), wXL + 1)
+    assert e["V_upper_zero"]
+    assert e["B_scaled"] < TOL_SCALED and e["V_scaled"] < TOL_SCALED, e
+    assert e["var_rel"] < TOL_VAR, e
+    # the FP64 path of the same context agrees too (mode switch keeps the trait)
+    ctx.set_mode(pkg.FGLS_MODE_FP64)
+    B64, V64 = ctx.solve_block(pkg.colmajor(prob.XR, ld=(n + 3) // 4 * 4))
+    e64 = orc.parity_errors(B.cpu().numpy(), V.cpu().numpy(), B64.cpu().numpy(), V64.cpu().numpy(), wXL + 1)
+    assert e64["B_scaled"] < TOL_SCALED and e64["V_scaled"] < TOL_SCALED, e64
+    ctx.close()
+
+
+def test_mixed_mode_set_before_trait_and_ragged_blocks(pkg, gpu):
+    import torch
+    prob = make_problem(n=777, wXL=3, m=300, seed=5)
+    L = orc.cholesky_lower(orc.build_M(prob.Phi, prob.h2, 1.0))
+    tr = orc.trait_precompute(L, prob.XL, prob.y)
+    ctx = pkg.DeviceContext(prob.n, 3, 1, max_block=128)
+    ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)
+    ctx.set_trait(torch.from_numpy(np.ascontiguousarray(L.T)).cuda(), pkg.colmajor(prob.XL),
+                  torch.from_numpy(prob.y).cuda())
+    X8 = ctx.to_i8(pkg.colmajor(prob.XR))
+    Bs, Vs = [], []
+    for i0 in range(0, prob.m, 128):
+        B, V = ctx.solve_block_i8(X8[i0:i0 + 128], snp_base=i0)
+        Bs.append(B.cpu().numpy())
+        Vs.append(V.cpu().numpy())
+    ctx.check()
+    b_ref, V_ref = orc.solve_snps(prob.XR, tr, 1)
+    e = orc.parity_errors(np.concatenate(Bs), np.concatenate(Vs), b_ref, v_flat(V_ref), 4)
+    assert e["B_scaled"] < TOL_SCALED and e["V_scaled"] < TOL_SCALED, e
+    ctx.close()
+
+
+def test_to_i8_rejects_non_integer_genotypes(pkg, gpu):
+    prob = make_problem(n=200, wXL=3, m=10, seed=1)
+    _, ctx = setup(pkg, prob, max_block=16)
+    X = prob.XR.copy()
+    X[5, 3] = 0.5
+    X[7, 4] = 300.0
+    with pytest.raises(pkg.FglsError) as ei:
+        ctx.to_i8(pkg.colmajor(X))
+    assert "2 genotype values" in ei.value.msg
+    ctx.close()
+
+
+def test_mixed_singular_snp_reports_index(pkg, gpu):
+    prob = make_problem(n=300, wXL=3, m=100, seed=9)
+    prob.XR[:, 41] = 0.0
+    tr, ctx = setup(pkg, prob, max_block=128)
+    ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)
+    ctx.solve_block_i8(ctx.to_i8(pkg.colmajor(prob.XR)), snp_base=10)
+    with pytest.raises(pkg.FglsError) as ei:
+        ctx.check()
+    assert ei.value.bad_snp == 51 and ei.value.info == 4
+    ctx.close()
