@@ -8,11 +8,17 @@ resident in HBM.  Synthetic data from the configuration's generators:
 Phi = ZZ'/(2n) rescaled to unit diagonal, X_L = [1, N(0,1), N(0,1)],
 X_R ~ Binomial(2, f_j), f_j ~ U(0.05, 0.5), y = X_L beta + eps, eps ~ N(0, M).
 
-A step is one pass of the hot path over all m SNPs: per block of SNPs,
-fgls_hip_block_reduce (triangular MFMA product L^-1 X_R with the per-SNP
-reductions) then fgls_hip_block_finalize (p x p solves, b_i and V_i).  The
-per-trait precompute (L^-1, X~_L, y~, res_sigma) is a one-off reported as
-`trait_setup_s`, outside the steps.
+Two paths run on the same genotypes (drawn as int8 hard calls; the FP64 path
+reads their exact FP64 copy, 80 GB, the mixed path the int8 copy, 10 GB):
+  fp64  -- FP64 MFMA triangular product through L^-1;
+  mixed -- int8 MFMA over six 7-bit slices of L^-1, exact int32 accumulation.
+A step is one pass of a path over all m SNPs: per block of SNPs the reduce
+launch (triangular product L^-1 X_R with the per-SNP reductions) then the
+finalize launch (p x p solves, b_i and V_i).  The per-trait precompute
+(L^-1, X~_L, y~, res_sigma; for mixed also the slicing) is a one-off reported
+outside the steps.  `value` is the mixed path's SNPs/s; both paths, their
+speed ratio and the mixed path's error against the FP64 path over all m SNPs
+are in `paths`.
 
 Prints one JSON line (rank 0).  The CPU baseline is the reference's own
 fgls_chol (compiled from its sources by oracle/Makefile) on a bounded sample
@@ -32,7 +38,16 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SNPs/s at n=10k,p=4,m=1M: FP64 vs mixed-prec (rel err ≤1e-8); MFMA % peak"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_INT8_TOPS = 5000.0
 PEAK_HBM_GBS = 8000.0
+NSLICE = 6
+
+
+def orc_parity(res, ref, p):
+    """Errors of one device path against another over all SNPs."""
+    from oracle import fgls_oracle as orc
+    return orc.parity_errors(res["B"].cpu().numpy(), res["V"].cpu().numpy(),
+                             ref["B"].cpu().numpy(), ref["V"].cpu().numpy(), p)
 
 
 def log(msg):
@@ -66,6 +81,10 @@ def make_inputs(torch, n, m, wXL, h2, seed, ldx, dev):
     XL[:, 1:] = torch.randn((n, wXL - 1), dtype=f64, device=dev, generator=g)
     beta = torch.randn(wXL, dtype=f64, device=dev, generator=g)
     y = XL @ beta + L @ torch.randn(n, dtype=f64, device=dev, generator=g)
+    # genotypes are drawn once as int8 (hard calls 0/1/2, the mixed path's
+    # input); the FP64 path's input is the exact FP64 copy of the same values
+    ld8 = (n + 63) // 64 * 64
+    X8 = torch.zeros((m, ld8), dtype=torch.int8, device=dev)
     X = torch.empty((m, ldx), dtype=f64, device=dev)
     if ldx > n:
         X[:, n:].zero_()
@@ -76,9 +95,10 @@ def make_inputs(torch, n, m, wXL, h2, seed, ldx, dev):
         fc = f[i0:i0 + cnt, None]
         a = torch.rand((cnt, n), dtype=torch.float32, device=dev, generator=g) < fc
         b = torch.rand((cnt, n), dtype=torch.float32, device=dev, generator=g) < fc
-        X[i0:i0 + cnt, :n] = a.to(f64) + b.to(f64)
+        X8[i0:i0 + cnt, :n] = a.to(torch.int8) + b.to(torch.int8)
+        X[i0:i0 + cnt, :n] = X8[i0:i0 + cnt, :n].to(f64)
         del a, b
-    return Phi, L, XL, y, X
+    return Phi, L, XL, y, X, X8
 
 
 def cpu_baseline(kind, Phi, L, XL, y, X, n, wXL, h2, sample, threads):
@@ -136,6 +156,7 @@ def main():
     ap.add_argument("--check-snps", type=int, default=512)
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "none"], default="auto")
     ap.add_argument("--cpu-snps", type=int, default=16384)
+    ap.add_argument("--paths", default="fp64,mixed", help="comma-separated: fp64, mixed")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -156,9 +177,10 @@ def main():
     ldx = (n + 3) // 4 * 4
 
     t0 = time.perf_counter()
-    Phi, L, XL, y, X = make_inputs(torch, n, m, wXL, args.h2, args.seed, ldx, dev)
+    Phi, L, XL, y, X, X8 = make_inputs(torch, n, m, wXL, args.h2, args.seed, ldx, dev)
     torch.cuda.synchronize()
-    log(f"inputs ready: n={n} m={m} X_R {X.numel() * 8 / 1e9:.1f} GB in HBM ({time.perf_counter() - t0:.1f}s)")
+    log(f"inputs ready: n={n} m={m} X_R {X.numel() * 8 / 1e9:.1f} GB FP64 + {X8.numel() / 1e9:.1f} GB int8 "
+        f"in HBM ({time.perf_counter() - t0:.1f}s)")
 
     block = min(args.block, m)
     ctx = pkg.DeviceContext(n, wXL, 1, max_block=block)
@@ -166,48 +188,64 @@ def main():
     ctx.set_trait(L.T.contiguous(), XL.T.contiguous(), y.contiguous())
     torch.cuda.synchronize()
     trait_setup = time.perf_counter() - t0
-    Bout = torch.empty((m, p), dtype=torch.float64, device=dev)
-    Vout = torch.empty((m, p * p), dtype=torch.float64, device=dev)
     starts = list(range(0, m, block))
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in starts]
 
-    def step(record):
-        for bi, i0 in enumerate(starts):
-            cnt = min(block, m - i0)
-            if record:
-                evs[bi][0].record()
-            ctx.block_reduce(X[i0:i0 + cnt], cnt)
-            if record:
-                evs[bi][1].record()
-            ctx.block_finalize(cnt, Bout[i0:i0 + cnt], Vout[i0:i0 + cnt], snp_base=i0)
-            if record:
-                evs[bi][2].record()
+    def run_path(mode):
+        """Time `args.steps` passes of one path; returns outputs and timings."""
+        if mode == "mixed":
+            t0 = time.perf_counter()
+            ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)      # slices L^-1 of the current trait
+            torch.cuda.synchronize()
+            setup_s = time.perf_counter() - t0
+            reduce = lambda i0, cnt: ctx.block_reduce_i8(X8[i0:i0 + cnt], cnt)
+        else:
+            ctx.set_mode(pkg.FGLS_MODE_FP64)
+            setup_s = 0.0
+            reduce = lambda i0, cnt: ctx.block_reduce(X[i0:i0 + cnt], cnt)
+        Bo = torch.empty((m, p), dtype=torch.float64, device=dev)
+        Vo = torch.empty((m, p * p), dtype=torch.float64, device=dev)
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in starts]
 
-    for _ in range(args.warmup):
-        step(False)
-    ctx.check()
-    torch.cuda.synchronize()
-    reduce_ms, fin_ms, reduce_flops, nfull = 0.0, 0.0, 0.0, 0
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(True)
+        def step(record):
+            for bi, i0 in enumerate(starts):
+                cnt = min(block, m - i0)
+                if record:
+                    evs[bi][0].record()
+                reduce(i0, cnt)
+                if record:
+                    evs[bi][1].record()
+                ctx.block_finalize(cnt, Bo[i0:i0 + cnt], Vo[i0:i0 + cnt], snp_base=i0)
+                if record:
+                    evs[bi][2].record()
+
+        for _ in range(args.warmup):
+            step(False)
+        ctx.check()
         torch.cuda.synchronize()
-        for bi, i0 in enumerate(starts):
-            cnt = min(block, m - i0)
-            if cnt == block:
-                reduce_ms += evs[bi][0].elapsed_time(evs[bi][1])
-                fin_ms += evs[bi][1].elapsed_time(evs[bi][2])
-                reduce_flops += float(n) * n * cnt
-                nfull += 1
-        log(f"step {k + 1}/{args.steps} done")
-    elapsed = time.perf_counter() - t0
-    ctx.check()
-    # the event reads inside the loop add host time: use the event totals too
-    value = m * args.steps / elapsed
-    avg_reduce_ms = reduce_ms / max(nfull, 1)
-    achieved_tflops = reduce_flops / (reduce_ms * 1e-3) / 1e12 if reduce_ms > 0 else 0.0
+        red_ms, fin_ms, nfull = 0.0, 0.0, 0
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step(True)
+            torch.cuda.synchronize()
+            for bi, i0 in enumerate(starts):
+                if min(block, m - i0) == block:
+                    red_ms += evs[bi][0].elapsed_time(evs[bi][1])
+                    fin_ms += evs[bi][1].elapsed_time(evs[bi][2])
+                    nfull += 1
+            log(f"{mode} step {k + 1}/{args.steps} done")
+        elapsed = time.perf_counter() - t0
+        ctx.check()
+        return {"B": Bo, "V": Vo, "elapsed": elapsed, "snps_per_s": m * args.steps / elapsed,
+                "reduce_ms": red_ms / max(nfull, 1), "finalize_ms": fin_ms / max(nfull, 1),
+                "setup_s": setup_s}
 
-    # parity spot check against the FP64 oracle on a sample of SNPs
+    paths = {}
+    for mode in args.paths.split(","):
+        paths[mode] = run_path(mode)
+        log(f"{mode}: {paths[mode]['snps_per_s']:.0f} SNPs/s, reduce kernel {paths[mode]['reduce_ms']:.2f} ms "
+            f"per {block} SNPs")
+
+    # parity spot check of each path against the FP64 oracle on a sample of SNPs
     check = {}
     if args.check_snps > 0:
         from oracle import fgls_oracle as orc
@@ -217,13 +255,12 @@ def main():
         tr = orc.trait_precompute(Lc, XL.cpu().numpy(), y.cpu().numpy())
         Xs = X[torch.as_tensor(idx, device=dev), :n].cpu().numpy().T.copy()
         b_ref, V_ref = orc.solve_snps(Xs, tr, 1)
-        Bg = Bout[torch.as_tensor(idx, device=dev)].cpu().numpy()
-        Vg = Vout[torch.as_tensor(idx, device=dev)].cpu().numpy()
-        check = {"snps": int(idx.size),
-                 "max_rel_err_B": orc.max_rel_err(Bg, b_ref),
-                 "max_rel_err_V": orc.max_rel_err(Vg, V_ref.transpose(0, 2, 1).reshape(idx.size, -1),
-                                                  p=p, lower_only=True)}
-        log(f"parity vs oracle on {idx.size} SNPs: {check}")
+        Vr = V_ref.transpose(0, 2, 1).reshape(idx.size, -1)
+        sel = torch.as_tensor(idx, device=dev)
+        for mode, res in paths.items():
+            e = orc.parity_errors(res["B"][sel].cpu().numpy(), res["V"][sel].cpu().numpy(), b_ref, Vr, p)
+            check[mode] = dict(e, snps=int(idx.size))
+            log(f"{mode} parity vs oracle on {idx.size} SNPs: {e}")
 
     cpu = None
     if args.cpu_baseline != "none":
@@ -232,29 +269,57 @@ def main():
         cpu = cpu_baseline(args.cpu_baseline, Phi, L, XL, y, X, n, wXL, args.h2, args.cpu_snps, threads)
         log(f"CPU baseline: {cpu['value']:.1f} SNPs/s")
 
+    def roofline(mode, res):
+        flop = float(n) * n * block                 # algorithmic: n^2 per SNP (TRSM count)
+        if mode == "fp64":
+            ach = flop / (res["reduce_ms"] * 1e-3) / 1e12
+            return {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": None,
+                    "kernel": "k_trmm<EPI_REDUCE> (fgls_hip_block_reduce)",
+                    "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block, "flop_per_snp": float(n) * n}
+        ops = NSLICE * flop                         # int8 ops the 6-slice product executes
+        ach = ops / (res["reduce_ms"] * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": ach, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
+                "frac": ach / PEAK_INT8_TOPS, "traffic": None,
+                "kernel": "k_trmm_i8 (fgls_hip_block_reduce_i8)", "achieved_is": "int8 TOP/s",
+                "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block,
+                "int8_ops_per_snp": NSLICE * float(n) * n,
+                "fp64_equivalent_tflops": flop / (res["reduce_ms"] * 1e-3) / 1e12}
+
+    head = "mixed" if "mixed" in paths else "fp64"
+    res = paths[head]
+    summary = {}
+    for mode, r in paths.items():
+        summary[mode] = {"snps_per_s": r["snps_per_s"], "ms_per_step": r["elapsed"] / args.steps * 1e3,
+                         "reduce_kernel_ms": r["reduce_ms"], "finalize_kernel_ms": r["finalize_ms"],
+                         "mode_setup_s": r["setup_s"], "roofline": roofline(mode, r)}
+    if "mixed" in paths and "fp64" in paths:
+        e = orc_parity(paths["mixed"], paths["fp64"], p)
+        summary["mixed_vs_fp64_all_snps"] = e
+        summary["speedup_mixed_over_fp64"] = paths["mixed"]["snps_per_s"] / paths["fp64"]["snps_per_s"]
+        log(f"mixed vs fp64 over all {m} SNPs: {e}; speedup {summary['speedup_mixed_over_fp64']:.2f}x")
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": res["snps_per_s"],
         "unit": "SNPs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": res["elapsed"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "i8" if head == "mixed" else "f64",
         "data": "synthetic (configuration generators, seeded); X_R resident in HBM",
-        "config": {"workload": f"Cholesky FGLS, FP64 path: n={n}, p={p}, m={m}, t=1, h2={args.h2}",
-                   "n": n, "p": p, "m": m, "block": block, "path": "fp64"},
-        "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "k_trmm<EPI_REDUCE> (fgls_hip_block_reduce)",
-                     "avg_launch_ms": avg_reduce_ms, "snps_per_launch": block,
-                     "flop_per_snp": float(n) * n},
+        "config": {"workload": f"Cholesky FGLS, n={n}, p={p}, m={m}, t=1, h2={args.h2}; value = {head} path",
+                   "n": n, "p": p, "m": m, "block": block, "path": head,
+                   "paths": {"fp64": "FP64 genotypes, FP64 MFMA triangular product through L^-1",
+                             "mixed": "int8 genotypes x 6 signed 7-bit slices of the FP64 L^-1 on the int8 "
+                                      "MFMA, exact int32 accumulation, FP64 recombination and p x p solves"}},
+        "roofline": roofline(head, res),
         "cpu_baseline": cpu,
         "trait_setup_s": trait_setup,
-        "finalize_avg_ms": fin_ms / max(nfull, 1),
+        "paths": summary,
         "parity_vs_oracle": check,
     }
     print(json.dumps(out), flush=True)
