@@ -431,40 +431,78 @@ __global__ __launch_bounds__(256) void k_slice_linv(int ldn, const double* __res
   }
 }
 
+// Tile order shared by the hot kernels: workgroup b of a 1-D grid runs on XCD
+// b % 8 (round-robin dispatch), so each XCD takes whole groups of GR x GC
+// tiles (one group = the 32 workgroups its 32 CUs hold at once, sharing row
+// panels of L^-1 and column panels of X_R in that XCD's L2).  Groups are
+// dealt to the XCDs in turn, heaviest row tiles (longest K) first.  Returns
+// false for padding workgroups.
+template <int GR, int GC>
+__device__ __forceinline__ bool tile_of_block(int b, int RT, int CT, int& rt, int& ct) {
+  constexpr int GS = GR * GC;
+  const int xcd = b & 7, k = b >> 3;
+  const int group = (k / GS) * 8 + xcd, j = k % GS;
+  const int NGC = (CT + GC - 1) / GC, NGR = (RT + GR - 1) / GR;
+  if (group >= NGR * NGC) return false;
+  const int gr = NGR - 1 - group / NGC, gc = group % NGC;
+  rt = gr * GR + j / GC;
+  ct = gc * GC + j % GC;
+  return rt < RT && ct < CT;
+}
+__host__ __device__ constexpr int tile_grid(int RT, int CT, int GR, int GC) {
+  return (((RT + GR - 1) / GR) * ((CT + GC - 1) / GC) + 7) / 8 * 8 * GR * GC;
+}
+
 // Int8 triangular product with the same fused reductions as k_trmm<EPI_REDUCE>.
-// Workgroup tile 64 rows x 128 columns, 2x2 waves of 32 x 64, one int32
-// accumulator set per slice.  MFMA v_mfma_i32_16x16x64_i8: A lane l holds 16
-// consecutive k of row l%16 (k chunk l/16 of the 64), B lane l the same 16 k
-// of column l%16; D element v of lane l is D[4*(l/16)+v][l%16].
-// X8 is column-major int8 (ld ldx8 >= ceil(n/64)*64, ldx8 % 16 == 0); its
-// rows >= n may hold anything, they meet all-zero slice columns.
+// Workgroup tile: 32 rows of L^-1 x 256 SNP columns; wave w owns columns
+// [64w, 64w+64) and all 32 rows, one int32 accumulator set per slice.
+// Per 64-k step the 32 x S x 64 slice bytes (12 KB at S = 6) are loaded once
+// per workgroup (3 x 16 B per thread), staged through registers into a
+// double-buffered LDS block laid out in MFMA lane order, and read back by all
+// four waves; the genotype fragments go straight to registers, two steps
+// ahead.  MFMA v_mfma_i32_16x16x64_i8: A lane l holds 16 consecutive k of
+// row l%16 (k chunk l/16 of the 64), B lane l the same 16 k of column l%16;
+// D element v of lane l is D[4*(l/16)+v][l%16].  X8 is column-major int8
+// (ld ldx8 >= ceil(n/64)*64, ldx8 % 16 == 0); its rows >= n may hold
+// anything, they meet all-zero slice columns.  All loads are unconditional
+// (addresses clamped to the last step), so the loop has no branches.
+constexpr int I8_TM = 32, I8_TN = 256, I8_GR = 8, I8_GC = 4;
+
 template <int NQ, int S>
-__global__ __launch_bounds__(256) void k_trmm_i8(int n, int count, int RT8, int KC,
+__global__ __launch_bounds__(256) void k_trmm_i8(int n, int count, int RT32, int CT, int KC,
                                                  const int8_t* __restrict__ A8,
                                                  const double* __restrict__ inv_scale,
                                                  const int8_t* __restrict__ X8, int64_t ldx8,
                                                  const double* __restrict__ W, int64_t ldw,
                                                  double* __restrict__ part, int64_t ldp) {
-  const int ct = blockIdx.x;
-  const int rt = RT8 - 1 - (int)blockIdx.y;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int wr = w >> 1, wc = w & 1;
+  constexpr int NBLK = 2 * S;            // (row tile, slice) blocks of 64 lanes x 16 B
+  constexpr int ASTEP = 4 * S * 16;      // bytes of one row's slices per 64-k step
+  __shared__ v4i As[2][NBLK][64];
+  int rt, ct;
+  if (!tile_of_block<I8_GR, I8_GC>(blockIdx.x, RT32, CT, rt, ct)) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
-  const int r0 = rt * 64 + wr * 32;
-  const int c0 = ct * 128 + wc * 64;
+  const int r0 = rt * I8_TM;
+  const int c0 = ct * I8_TN + w * 64;
   const int n64 = (n + 63) & ~63;
-  const int kend = (r0 < n64) ? min((r0 + 32 + 63) & ~63, n64) : 0;   // multiple of 64
+  const int nsteps = (r0 < n64) ? min((r0 + I8_TM + 63) & ~63, n64) / 64 : 0;
 
-  const int8_t* Ap[2];
-  const int8_t* Bp[4];
+  // this thread's share of the slice block: chunks c = tid + 256 j
+  const int8_t* asrc[NBLK / 4];
+  int ablk[NBLK / 4];
 #pragma unroll
-  for (int ti = 0; ti < 2; ++ti) Ap[ti] = A8 + ((size_t)(r0 + 16 * ti + li) * KC + lg) * S * 16;
+  for (int j = 0; j < NBLK / 4; ++j) {
+    const int c = tid + 256 * j, blk = c >> 6, ll = c & 63;
+    const int ti = blk / S, s2 = blk % S;
+    asrc[j] = A8 + (((size_t)(r0 + 16 * ti + (ll & 15)) * KC + (ll >> 4)) * S + s2) * 16;
+    ablk[j] = blk;
+  }
+  const int8_t* Bp[4];
 #pragma unroll
   for (int tj = 0; tj < 4; ++tj) {
     const int col = min(c0 + 16 * tj + li, count - 1);
     Bp[tj] = X8 + (size_t)col * ldx8 + 16 * lg;
   }
-  constexpr int ASTEP = 4 * S * 16;   // bytes of A per lane per 64-k step
 
   v4i acc[S][2][4];
 #pragma unroll
@@ -474,47 +512,53 @@ __global__ __launch_bounds__(256) void k_trmm_i8(int n, int count, int RT8, int 
 #pragma unroll
       for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = v4i{0, 0, 0, 0};
 
-  if (kend > 0) {
-    v4i a[2][S], b[4];
+  if (nsteps > 0) {
+    const int last = nsteps - 1;
+    v4i ga0[NBLK / 4], ga1[NBLK / 4];     // slices of steps k+1, k+2
+    v4i gb0[4], gb1[4], gb2[4];            // genotypes of steps k, k+1, k+2
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-      for (int s2 = 0; s2 < S; ++s2) a[ti][s2] = *reinterpret_cast<const v4i*>(Ap[ti] + s2 * 16);
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj) b[tj] = *reinterpret_cast<const v4i*>(Bp[tj]);
-    int step = 1;
-    for (int k0 = 64; k0 < kend; k0 += 64, ++step) {
-      v4i an[2][S], bn[4];
-#pragma unroll
-      for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-        for (int s2 = 0; s2 < S; ++s2)
-          an[ti][s2] = *reinterpret_cast<const v4i*>(Ap[ti] + (size_t)step * ASTEP + s2 * 16);
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj) bn[tj] = *reinterpret_cast<const v4i*>(Bp[tj] + k0);
-#pragma unroll
-      for (int s2 = 0; s2 < S; ++s2)
-#pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-          for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(a[ti][s2], b[tj], acc[s2][ti][tj]);
-#pragma unroll
-      for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-        for (int s2 = 0; s2 < S; ++s2) a[ti][s2] = an[ti][s2];
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj) b[tj] = bn[tj];
+    for (int j = 0; j < NBLK / 4; ++j) {
+      const v4i t0 = *reinterpret_cast<const v4i*>(asrc[j]);
+      As[0][ablk[j]][lane] = t0;
+      ga0[j] = *reinterpret_cast<const v4i*>(asrc[j] + (size_t)min(1, last) * ASTEP);
     }
 #pragma unroll
-    for (int s2 = 0; s2 < S; ++s2)
+    for (int tj = 0; tj < 4; ++tj) {
+      gb0[tj] = *reinterpret_cast<const v4i*>(Bp[tj]);
+      gb1[tj] = *reinterpret_cast<const v4i*>(Bp[tj] + 64 * min(1, last));
+    }
+    __syncthreads();
+    for (int k = 0; k < nsteps; ++k) {
+      const int k2 = min(k + 2, last);
 #pragma unroll
-      for (int ti = 0; ti < 2; ++ti)
+      for (int j = 0; j < NBLK / 4; ++j) ga1[j] = *reinterpret_cast<const v4i*>(asrc[j] + (size_t)k2 * ASTEP);
 #pragma unroll
-        for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(a[ti][s2], b[tj], acc[s2][ti][tj]);
+      for (int tj = 0; tj < 4; ++tj) gb2[tj] = *reinterpret_cast<const v4i*>(Bp[tj] + 64 * k2);
+      // stage k+1 of the slices (stage (k+1)&1 was last read in step k-1)
+#pragma unroll
+      for (int j = 0; j < NBLK / 4; ++j) As[(k + 1) & 1][ablk[j]][lane] = ga0[j];
+      const int st = k & 1;
+#pragma unroll
+      for (int s2 = 0; s2 < S; ++s2) {
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti) {
+          const v4i a = As[st][ti * S + s2][lane];
+#pragma unroll
+          for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(a, gb0[tj], acc[s2][ti][tj]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < NBLK / 4; ++j) ga0[j] = ga1[j];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        gb0[tj] = gb1[tj];
+        gb1[tj] = gb2[tj];
+      }
+    }
   }
 
   // z(row r0+16ti+4lg+v, col c0+16tj+li) = inv_scale[row] * sum_s 2^(-7s) acc[s]
-  __shared__ double red[2][64][NQ];
   double q[4][NQ];
 #pragma unroll
   for (int tj = 0; tj < 4; ++tj)
@@ -542,7 +586,7 @@ __global__ __launch_bounds__(256) void k_trmm_i8(int n, int count, int RT8, int 
     }
   }
 #pragma unroll
-  for (int tj = 0; tj < 4; ++tj)
+  for (int tj = 0; tj < 4; ++tj) {
 #pragma unroll
     for (int qi = 0; qi < NQ; ++qi) {
       double x = q[tj][qi];
@@ -550,22 +594,10 @@ __global__ __launch_bounds__(256) void k_trmm_i8(int n, int count, int RT8, int 
       x += __shfl_xor(x, 32, 64);
       q[tj][qi] = x;
     }
-  if (wr == 1 && lg == 0) {
+    const int col = c0 + 16 * tj + li;
+    if (lg == 0 && col < count) {
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj)
-#pragma unroll
-      for (int qi = 0; qi < NQ; ++qi) red[wc][16 * tj + li][qi] = q[tj][qi];
-  }
-  __syncthreads();
-  if (wr == 0 && lg == 0) {
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj) {
-      const int col = c0 + 16 * tj + li;
-      if (col < count) {
-#pragma unroll
-        for (int qi = 0; qi < NQ; ++qi)
-          part[((size_t)rt * NQ + qi) * ldp + col] = q[tj][qi] + red[wc][16 * tj + li][qi];
-      }
+      for (int qi = 0; qi < NQ; ++qi) part[((size_t)rt * NQ + qi) * ldp + col] = q[tj][qi];
     }
   }
 }
@@ -916,7 +948,7 @@ struct fgls_hip_ctx {
   int last_rt;    // row tiles of the partials written by the last block_reduce
   bool have_trait;
   int mode;       // FGLS_MODE_FP64 or FGLS_MODE_MIXED_I8
-  int RT8;        // 64-row tiles of the int8 product
+  int RT8;        // 32-row tiles of the int8 product
   int part_rt;    // row-tile capacity of `part`
   int8_t* A8;     // slices of L^-1 (mixed mode)
   double* invsc;  // per-row 2^(e_i - 6)
@@ -966,8 +998,9 @@ constexpr int NSLICE = 6;   // 7-bit slices of L^-1 in the mixed mode
 template <int P>
 void launch_trmm_i8(const fgls_hip_ctx* c, const int8_t* X8, int64_t ldx8, int count,
                     hipStream_t st) {
-  dim3 grid((count + 127) / 128, c->RT8);
-  hipLaunchKernelGGL((k_trmm_i8<P + 1, NSLICE>), grid, dim3(256), 0, st, c->n, count, c->RT8,
+  const int CT = (count + I8_TN - 1) / I8_TN;
+  const int G = tile_grid(c->RT8, CT, I8_GR, I8_GC);
+  hipLaunchKernelGGL((k_trmm_i8<P + 1, NSLICE>), dim3(G), dim3(256), 0, st, c->n, count, c->RT8, CT,
                      c->ldn / 16, c->A8, c->invsc, X8, ldx8, c->W, (int64_t)c->ldn, c->part,
                      (int64_t)c->max_block);
 }
@@ -1052,7 +1085,7 @@ int fgls_hip_ctx_create(fgls_hip_ctx** out, int n, int wXL, int wXR, int max_blo
   c->nblk = (n + NB - 1) / NB;
   c->RT = c->ldn / TM;
   c->max_block = (int)round_up(max_block, 4);
-  c->RT8 = c->ldn / 64;
+  c->RT8 = c->ldn / I8_TM;
   c->part_rt = c->RT;
   c->mode = FGLS_MODE_FP64;
   const size_t ldn = (size_t)c->ldn;
