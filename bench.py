@@ -157,6 +157,9 @@ def main():
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "none"], default="auto")
     ap.add_argument("--cpu-snps", type=int, default=16384)
     ap.add_argument("--paths", default="fp64,mixed", help="comma-separated: fp64, mixed")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (a known-size kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -176,6 +179,15 @@ def main():
     p = wXL + 1
     ldx = (n + 3) // 4 * 4
 
+    if args.copy_calibration_mb > 0:
+        nel = args.copy_calibration_mb * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {nel * 8} bytes read and written")
+        del src, dst
     t0 = time.perf_counter()
     Phi, L, XL, y, X, X8 = make_inputs(torch, n, m, wXL, args.h2, args.seed, ldx, dev)
     torch.cuda.synchronize()

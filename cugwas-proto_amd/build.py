@@ -63,6 +63,10 @@ def build(verbose: bool = False) -> dict:
 
     kobj = os.path.join(objdir, "fgls_kernels.o")
     run([hipcc(), *HIP_FLAGS, "-I", INCLUDE, "-c", os.path.join(CSRC, "fgls_kernels.hip"), "-o", kobj], verbose)
+    # the int8 kernel keeps its MFMA accumulators in VGPRs (see fgls_i8.hip)
+    i8obj = os.path.join(objdir, "fgls_i8.o")
+    run([hipcc(), *HIP_FLAGS, "-mllvm", "-amdgpu-mfma-vgpr-form=1", "-I", INCLUDE, "-c",
+         os.path.join(CSRC, "fgls_i8.hip"), "-o", i8obj], verbose)
 
     host_objs = []
     for src in sorted(glob.glob(os.path.join(CSRC, "*.c"))):
@@ -76,7 +80,7 @@ def build(verbose: bool = False) -> dict:
 
     so = os.path.join(LIBDIR, "libfgls_amd.so")
     tmp = so + ".tmp"
-    run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, kobj, *host_objs,
+    run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, kobj, i8obj, *host_objs,
          "-L", blasdir, f"-l:{blasname}", f"-Wl,-rpath,{blasdir}", "-fopenmp", "-lpthread", "-lrt"],
         verbose)
     os.replace(tmp, so)

@@ -1,0 +1,56 @@
+// fgls_device.h -- device helpers shared by the translation units of the
+// FGLS device layer (internal; not part of the C-ABI).
+#ifndef FGLS_DEVICE_H
+#define FGLS_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double fgls_v4d __attribute__((ext_vector_type(4)));
+typedef int fgls_v4i __attribute__((ext_vector_type(4)));
+
+// Mixed mode: L^-1 as NSLICE signed 7-bit slices; the int8 product runs on
+// workgroup tiles of I8_TM rows x I8_TN SNP columns, grouped I8_GR x I8_GC
+// per XCD (see fgls_tile_of_block).
+constexpr int FGLS_NSLICE = 6;
+constexpr int I8_TM = 32, I8_TN = 256, I8_GR = 8, I8_GC = 4;
+
+// Tile order of the hot kernels: workgroup b of a 1-D grid runs on XCD b % 8
+// (round-robin dispatch), so each XCD takes whole groups of GR x GC tiles (one
+// group = the workgroups its 32 CUs hold at once, sharing row panels of L^-1
+// and column panels of X_R in that XCD's L2).  Groups are dealt to the XCDs
+// in turn, heaviest row tiles (longest K) first.  Returns false for padding
+// workgroups.
+template <int GR, int GC>
+__device__ __forceinline__ bool fgls_tile_of_block(int b, int RT, int CT, int& rt, int& ct) {
+  constexpr int GS = GR * GC;
+  const int xcd = b & 7, k = b >> 3;
+  const int group = (k / GS) * 8 + xcd, j = k % GS;
+  const int NGC = (CT + GC - 1) / GC, NGR = (RT + GR - 1) / GR;
+  if (group >= NGR * NGC) return false;
+  const int gr = NGR - 1 - group / NGC, gc = group % NGC;
+  rt = gr * GR + j / GC;
+  ct = gc * GC + j % GC;
+  return rt < RT && ct < CT;
+}
+
+__host__ __device__ constexpr int fgls_tile_grid(int RT, int CT, int GR, int GC) {
+  return (((RT + GR - 1) / GR) * ((CT + GC - 1) / GC) + 7) / 8 * 8 * GR * GC;
+}
+
+// Slice layout of the int8 product: 1 KB blocks, one per (16-row tile rt16,
+// 64-k step ks, slice s), each in the A-operand lane order of
+// v_mfma_i32_16x16x64_i8: lane 16*g + i holds k = 64 ks + 16 g .. +15 of row
+// 16 rt16 + i.  KS = ldn / 64 steps per row tile; one step of a row tile
+// is S contiguous blocks.
+__host__ __device__ constexpr size_t fgls_a8_offset(int row, int s, int kc, int S, int KS) {
+  return (((((size_t)(row >> 4) * KS + (kc >> 2)) * S + s) * 64) + 16 * (kc & 3) + (row & 15)) * 16;
+}
+
+// Launch of the int8 triangular product with fused reductions (fgls_i8.hip),
+// for p = 1..16.  part receives (p+1) sums per SNP per 32-row tile.
+void fgls_launch_trmm_i8(int p, int n, int count, int RT32, int KS, const int8_t* A8,
+                         const double* inv_scale, const int8_t* X8, int64_t ldx8, const double* W,
+                         int64_t ldw, double* part, int64_t ldp, hipStream_t st);
+
+#endif

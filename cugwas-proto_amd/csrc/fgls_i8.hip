@@ -1,0 +1,220 @@
+// fgls_i8.hip -- the mixed-precision hot kernel: int8 genotypes times six
+// signed 7-bit slices of L^-1 on v_mfma_i32_16x16x64_i8, exact int32
+// accumulation, FP64 recombination and the per-SNP reductions.
+//
+// This translation unit is compiled with -mllvm -amdgpu-mfma-vgpr-form=1:
+// the int32 accumulators then stay in place in VGPRs (with AGPR accumulators
+// the allocator copies accumulators through scratch AGPRs around the MFMAs
+// once more than a few dozen of them are live).
+//
+// Slice representation (built by k_slice_linv in fgls_kernels.hip): row i of
+// L^-1 times sc_i = 2^(6-e_i) is sum_s a_s 2^(-7s), a_s in [-64, 64], stored
+// in 1 KB blocks (fgls_a8_offset).  For integer genotypes x, (A_s x)_i is an
+// exact int32 (|a x| <= 64*127 per term, exact up to n ~ 2.6e5), and
+// z_i = sum_s 2^(-7s) (A_s x)_i / sc_i.
+
+#include "fgls_device.h"
+
+namespace {
+
+#define FGLS_LDS(p) ((__attribute__((address_space(3))) void*)(p))
+
+__device__ __forceinline__ fgls_v4i mfma_i8(fgls_v4i a, fgls_v4i b, fgls_v4i c) {
+  return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+}
+
+// s_waitcnt immediate that waits until at most n vector-memory operations of
+// this wave are outstanding (expcnt and lgkmcnt are not waited for).
+constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); }
+
+// Workgroup: 4 waves, tile I8_TM = 32 rows of L^-1 x I8_TN = 256 SNP columns;
+// wave w owns columns [64w, 64w + 64) and all 32 rows: 6 slices x 2 x 4 MFMA
+// tiles of 16 x 16 int32 accumulators (192 VGPRs, two waves per SIMD).
+//
+// Per 64-k step the tile's 2 x 6 slice blocks (12 KB) go from HBM straight
+// into LDS (global_load_lds_dwordx4: no staging registers; each wave moves
+// three 1 KB blocks) into one half of a double-buffered LDS area; every wave
+// then reads all twelve blocks as MFMA A operands.  The genotype fragments
+// (B operands, 4 x 1 KB per wave) go straight to registers.
+//
+// Step k (reading half k & 1): wait for this wave's loads of step k, barrier
+// (every wave's slices of step k are in LDS and every wave has finished
+// reading the other half in step k - 1), issue the genotypes and slices of
+// step k + 1, then 48 MFMAs.  The loop runs two steps per trip so that the
+// LDS halves and the genotype registers alternate without copies (a copy of
+// a register that a load is filling would wait for that load).
+// Prefetch addresses are clamped to the last step, so every load is in
+// bounds.  The barrier carries no memory fence: a workgroup-scope release
+// fence would also wait for the LDS loads just issued; the vmcnt wait before
+// it orders them instead.
+//
+// MFMA v_mfma_i32_16x16x64_i8: A lane l holds 16 consecutive k of row l%16
+// (k chunk l/16 of the 64), B lane l the same 16 k of column l%16; D element
+// v of lane l is D[4*(l/16)+v][l%16].  X8: column-major int8, ld ldx8 >=
+// ceil(n/64)*64, ldx8 % 16 == 0; its rows >= n meet all-zero slice columns.
+template <int S>
+struct SliceStage {
+  fgls_v4i blk[2 * S][64];
+};
+
+template <int S>
+__device__ __forceinline__ void slice_dma(const int8_t* const* src, int64_t off,
+                                          SliceStage<S>& dst, int w) {
+#pragma unroll
+  for (int j = 0; j < S / 2; ++j)
+    __builtin_amdgcn_global_load_lds(src[j] + off, FGLS_LDS(&dst.blk[w + 4 * j][0]), 16, 0, 0);
+}
+
+// One 64-k step: rd = LDS half of step k, wr = half that receives step k + 1,
+// b = genotype fragments of step k, bn receives those of step k + 1.
+template <int S>
+__device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i (&b)[4],
+                                        fgls_v4i (&bn)[4], const SliceStage<S>& rd,
+                                        SliceStage<S>& wr, const int8_t* const* asrc,
+                                        const int8_t* Xw, const int (&boff)[4], int k, int last,
+                                        int w, int lane) {
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_sched_barrier(0);
+  const int k1 = min(k + 1, last);
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj)
+    bn[tj] = *reinterpret_cast<const fgls_v4i*>(Xw + 64 * k1 + boff[tj]);
+  slice_dma<S>(asrc, (int64_t)k1 * S * 1024, wr, w);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int s2 = 0; s2 < S; ++s2) {
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti) {
+      const fgls_v4i a = rd.blk[ti * S + s2][lane];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(a, b[tj], acc[s2][ti][tj]);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int NQ, int S>
+__global__ __launch_bounds__(256, 2) void k_trmm_i8(
+    int n, int count, int RT32, int CT, int KS, const int8_t* __restrict__ A8,
+    const double* __restrict__ inv_scale, const int8_t* __restrict__ X8, int64_t ldx8,
+    const double* __restrict__ W, int64_t ldw, double* __restrict__ part, int64_t ldp) {
+  static_assert(S % 2 == 0, "the 2 x S slice blocks are spread over 4 waves");
+  __shared__ SliceStage<S> st0, st1;
+  int rt, ct;
+  if (!fgls_tile_of_block<I8_GR, I8_GC>(blockIdx.x, RT32, CT, rt, ct)) return;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 15, lg = lane >> 4;
+  const int r0 = rt * I8_TM;
+  const int c0 = ct * I8_TN + w * 64;
+  const int n64 = (n + 63) & ~63;
+  const int nsteps = (r0 < n64) ? min((r0 + I8_TM + 63) & ~63, n64) / 64 : 0;
+
+  // wave w moves blocks w, w + 4, w + 8 of each step: block = ti * S + s
+  const int8_t* asrc[S / 2];
+#pragma unroll
+  for (int j = 0; j < S / 2; ++j) {
+    const int blk = w + 4 * j, ti = blk / S, s2 = blk % S;
+    asrc[j] = A8 + fgls_a8_offset(r0 + 16 * ti, s2, 0, S, KS) + 16 * lane;
+  }
+  // genotype columns of this wave: Xw + boff[tj] (+ 64 k) for fragment tj;
+  // columns past count re-read the last one (64 ldx8 < 2^31 is checked by
+  // the host)
+  const int cb = min(c0, count - 1);
+  const int8_t* Xw = X8 + (size_t)cb * ldx8;
+  int boff[4];
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) boff[tj] = (min(c0 + 16 * tj + li, count - 1) - cb) * (int)ldx8 + 16 * lg;
+
+  fgls_v4i acc[S][2][4];
+#pragma unroll
+  for (int s2 = 0; s2 < S; ++s2)
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = fgls_v4i{0, 0, 0, 0};
+
+  if (nsteps > 0) {
+    const int last = nsteps - 1;
+    fgls_v4i b0[4], b1[4];
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) b0[tj] = *reinterpret_cast<const fgls_v4i*>(Xw + boff[tj]);
+    slice_dma<S>(asrc, 0, st0, w);
+    int k = 0;
+    for (; k + 2 <= nsteps; k += 2) {
+      step_i8<S>(acc, b0, b1, st0, st1, asrc, Xw, boff, k, last, w, lane);
+      step_i8<S>(acc, b1, b0, st1, st0, asrc, Xw, boff, k + 1, last, w, lane);
+    }
+    if (k < nsteps) step_i8<S>(acc, b0, b1, st0, st1, asrc, Xw, boff, k, last, w, lane);
+    // no LDS load may still be in flight when the workgroup ends
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+  }
+
+  // z(row r0+16ti+4lg+v, col c0+16tj+li) = inv_scale[row] * sum_s 2^(-7s) acc[s];
+  // one 16-column fragment at a time keeps the FP64 sums to 2 NQ registers
+  double isc[2][4];
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) isc[ti][v] = inv_scale[r0 + 16 * ti + 4 * lg + v];
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+    double q[NQ];
+#pragma unroll
+    for (int qi = 0; qi < NQ; ++qi) q[qi] = 0.0;
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + 16 * ti + 4 * lg + v;
+        double t = (double)acc[S - 1][ti][tj][v];
+#pragma unroll
+        for (int s2 = S - 2; s2 >= 0; --s2) t = fma(t, 1.0 / 128.0, (double)acc[s2][ti][tj][v]);
+        const double z = t * isc[ti][v];
+        q[0] = fma(z, z, q[0]);
+#pragma unroll
+        for (int k = 0; k < NQ - 1; ++k) q[1 + k] = fma(z, W[(size_t)k * ldw + row], q[1 + k]);
+      }
+    }
+#pragma unroll
+    for (int qi = 0; qi < NQ; ++qi) {
+      q[qi] += __shfl_xor(q[qi], 16, 64);
+      q[qi] += __shfl_xor(q[qi], 32, 64);
+    }
+    const int col = c0 + 16 * tj + li;
+    if (lg == 0 && col < count) {
+#pragma unroll
+      for (int qi = 0; qi < NQ; ++qi) part[((size_t)rt * NQ + qi) * ldp + col] = q[qi];
+    }
+  }
+}
+
+template <int P>
+void launch(int n, int count, int RT32, int KS, const int8_t* A8, const double* isc,
+            const int8_t* X8, int64_t ldx8, const double* W, int64_t ldw, double* part, int64_t ldp,
+            hipStream_t st) {
+  const int CT = (count + I8_TN - 1) / I8_TN;
+  const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
+  hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE>), dim3(G), dim3(256), 0, st, n, count, RT32, CT,
+                     KS, A8, isc, X8, ldx8, W, ldw, part, ldp);
+}
+
+}  // namespace
+
+void fgls_launch_trmm_i8(int p, int n, int count, int RT32, int KS, const int8_t* A8,
+                         const double* inv_scale, const int8_t* X8, int64_t ldx8, const double* W,
+                         int64_t ldw, double* part, int64_t ldp, hipStream_t st) {
+#define FGLS_I8_CASE(P)                                                                      \
+  case P:                                                                                    \
+    launch<P>(n, count, RT32, KS, A8, inv_scale, X8, ldx8, W, ldw, part, ldp, st);           \
+    break;
+  switch (p) {
+    FGLS_I8_CASE(1) FGLS_I8_CASE(2) FGLS_I8_CASE(3) FGLS_I8_CASE(4) FGLS_I8_CASE(5)
+    FGLS_I8_CASE(6) FGLS_I8_CASE(7) FGLS_I8_CASE(8) FGLS_I8_CASE(9) FGLS_I8_CASE(10)
+    FGLS_I8_CASE(11) FGLS_I8_CASE(12) FGLS_I8_CASE(13) FGLS_I8_CASE(14) FGLS_I8_CASE(15)
+    FGLS_I8_CASE(16)
+    default: break;
+  }
+#undef FGLS_I8_CASE
+}

@@ -32,6 +32,7 @@
 #include <string.h>
 #include <math.h>
 
+#include "fgls_device.h"
 #include "fgls_hip.h"
 #include "fgls_internal.h"
 
@@ -379,15 +380,10 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
 // where every A_s x is an exact int32 product on the int8 MFMA
 // (|a x| <= 64*127 per term, so the sum stays exact up to n ~ 2.6e5).
 //
-// Slice layout: for row r and 16-column chunk kc, the S slices' 16 bytes are
-// contiguous: A8[((r*KC + kc)*S + s)*16 + b], KC = ldn/16, so each lane of the
-// MFMA A operand fetches all S slices of its 16 k with S 16-byte loads.
+// Slice layout: fgls_a8_offset (1 KB blocks in MFMA A-operand lane order, so
+// one 64-k step of a 16-row tile and one slice is a single contiguous load).
 // ---------------------------------------------------------------------------
-typedef int v4i __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ v4i mfma_i8(v4i a, v4i b, v4i c) {
-  return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
-}
+typedef fgls_v4i v4i;
 
 __device__ __forceinline__ double wave_max(double x) {
 #pragma unroll
@@ -425,180 +421,10 @@ __global__ __launch_bounds__(256) void k_slice_linv(int ldn, const double* __res
         r = (r - a) * 128.0;
       }
     }
-    int8_t* dst = A8 + ((size_t)row * KC + kc) * S * 16;
 #pragma unroll
-    for (int s2 = 0; s2 < S; ++s2) *reinterpret_cast<v4i*>(dst + s2 * 16) = *reinterpret_cast<const v4i*>(out[s2]);
-  }
-}
-
-// Tile order shared by the hot kernels: workgroup b of a 1-D grid runs on XCD
-// b % 8 (round-robin dispatch), so each XCD takes whole groups of GR x GC
-// tiles (one group = the 32 workgroups its 32 CUs hold at once, sharing row
-// panels of L^-1 and column panels of X_R in that XCD's L2).  Groups are
-// dealt to the XCDs in turn, heaviest row tiles (longest K) first.  Returns
-// false for padding workgroups.
-template <int GR, int GC>
-__device__ __forceinline__ bool tile_of_block(int b, int RT, int CT, int& rt, int& ct) {
-  constexpr int GS = GR * GC;
-  const int xcd = b & 7, k = b >> 3;
-  const int group = (k / GS) * 8 + xcd, j = k % GS;
-  const int NGC = (CT + GC - 1) / GC, NGR = (RT + GR - 1) / GR;
-  if (group >= NGR * NGC) return false;
-  const int gr = NGR - 1 - group / NGC, gc = group % NGC;
-  rt = gr * GR + j / GC;
-  ct = gc * GC + j % GC;
-  return rt < RT && ct < CT;
-}
-__host__ __device__ constexpr int tile_grid(int RT, int CT, int GR, int GC) {
-  return (((RT + GR - 1) / GR) * ((CT + GC - 1) / GC) + 7) / 8 * 8 * GR * GC;
-}
-
-// Int8 triangular product with the same fused reductions as k_trmm<EPI_REDUCE>.
-// Workgroup tile: 32 rows of L^-1 x 256 SNP columns; wave w owns columns
-// [64w, 64w+64) and all 32 rows, one int32 accumulator set per slice.
-// Per 64-k step the 32 x S x 64 slice bytes (12 KB at S = 6) are loaded once
-// per workgroup (3 x 16 B per thread), staged through registers into a
-// double-buffered LDS block laid out in MFMA lane order, and read back by all
-// four waves; the genotype fragments go straight to registers, two steps
-// ahead.  MFMA v_mfma_i32_16x16x64_i8: A lane l holds 16 consecutive k of
-// row l%16 (k chunk l/16 of the 64), B lane l the same 16 k of column l%16;
-// D element v of lane l is D[4*(l/16)+v][l%16].  X8 is column-major int8
-// (ld ldx8 >= ceil(n/64)*64, ldx8 % 16 == 0); its rows >= n may hold
-// anything, they meet all-zero slice columns.  All loads are unconditional
-// (addresses clamped to the last step), so the loop has no branches.
-constexpr int I8_TM = 32, I8_TN = 256, I8_GR = 8, I8_GC = 4;
-
-template <int NQ, int S>
-__global__ __launch_bounds__(256) void k_trmm_i8(int n, int count, int RT32, int CT, int KC,
-                                                 const int8_t* __restrict__ A8,
-                                                 const double* __restrict__ inv_scale,
-                                                 const int8_t* __restrict__ X8, int64_t ldx8,
-                                                 const double* __restrict__ W, int64_t ldw,
-                                                 double* __restrict__ part, int64_t ldp) {
-  constexpr int NBLK = 2 * S;            // (row tile, slice) blocks of 64 lanes x 16 B
-  constexpr int ASTEP = 4 * S * 16;      // bytes of one row's slices per 64-k step
-  __shared__ v4i As[2][NBLK][64];
-  int rt, ct;
-  if (!tile_of_block<I8_GR, I8_GC>(blockIdx.x, RT32, CT, rt, ct)) return;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int li = lane & 15, lg = lane >> 4;
-  const int r0 = rt * I8_TM;
-  const int c0 = ct * I8_TN + w * 64;
-  const int n64 = (n + 63) & ~63;
-  const int nsteps = (r0 < n64) ? min((r0 + I8_TM + 63) & ~63, n64) / 64 : 0;
-
-  // this thread's share of the slice block: chunks c = tid + 256 j
-  const int8_t* asrc[NBLK / 4];
-  int ablk[NBLK / 4];
-#pragma unroll
-  for (int j = 0; j < NBLK / 4; ++j) {
-    const int c = tid + 256 * j, blk = c >> 6, ll = c & 63;
-    const int ti = blk / S, s2 = blk % S;
-    asrc[j] = A8 + (((size_t)(r0 + 16 * ti + (ll & 15)) * KC + (ll >> 4)) * S + s2) * 16;
-    ablk[j] = blk;
-  }
-  const int8_t* Bp[4];
-#pragma unroll
-  for (int tj = 0; tj < 4; ++tj) {
-    const int col = min(c0 + 16 * tj + li, count - 1);
-    Bp[tj] = X8 + (size_t)col * ldx8 + 16 * lg;
-  }
-
-  v4i acc[S][2][4];
-#pragma unroll
-  for (int s2 = 0; s2 < S; ++s2)
-#pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = v4i{0, 0, 0, 0};
-
-  if (nsteps > 0) {
-    const int last = nsteps - 1;
-    v4i ga0[NBLK / 4], ga1[NBLK / 4];     // slices of steps k+1, k+2
-    v4i gb0[4], gb1[4], gb2[4];            // genotypes of steps k, k+1, k+2
-#pragma unroll
-    for (int j = 0; j < NBLK / 4; ++j) {
-      const v4i t0 = *reinterpret_cast<const v4i*>(asrc[j]);
-      As[0][ablk[j]][lane] = t0;
-      ga0[j] = *reinterpret_cast<const v4i*>(asrc[j] + (size_t)min(1, last) * ASTEP);
-    }
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj) {
-      gb0[tj] = *reinterpret_cast<const v4i*>(Bp[tj]);
-      gb1[tj] = *reinterpret_cast<const v4i*>(Bp[tj] + 64 * min(1, last));
-    }
-    __syncthreads();
-    for (int k = 0; k < nsteps; ++k) {
-      const int k2 = min(k + 2, last);
-#pragma unroll
-      for (int j = 0; j < NBLK / 4; ++j) ga1[j] = *reinterpret_cast<const v4i*>(asrc[j] + (size_t)k2 * ASTEP);
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj) gb2[tj] = *reinterpret_cast<const v4i*>(Bp[tj] + 64 * k2);
-      // stage k+1 of the slices (stage (k+1)&1 was last read in step k-1)
-#pragma unroll
-      for (int j = 0; j < NBLK / 4; ++j) As[(k + 1) & 1][ablk[j]][lane] = ga0[j];
-      const int st = k & 1;
-#pragma unroll
-      for (int s2 = 0; s2 < S; ++s2) {
-#pragma unroll
-        for (int ti = 0; ti < 2; ++ti) {
-          const v4i a = As[st][ti * S + s2][lane];
-#pragma unroll
-          for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(a, gb0[tj], acc[s2][ti][tj]);
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < NBLK / 4; ++j) ga0[j] = ga1[j];
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj) {
-        gb0[tj] = gb1[tj];
-        gb1[tj] = gb2[tj];
-      }
-    }
-  }
-
-  // z(row r0+16ti+4lg+v, col c0+16tj+li) = inv_scale[row] * sum_s 2^(-7s) acc[s]
-  double q[4][NQ];
-#pragma unroll
-  for (int tj = 0; tj < 4; ++tj)
-#pragma unroll
-    for (int qi = 0; qi < NQ; ++qi) q[tj][qi] = 0.0;
-#pragma unroll
-  for (int ti = 0; ti < 2; ++ti) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = r0 + 16 * ti + 4 * lg + v;
-      const double isc = inv_scale[row];
-      double wv[NQ - 1];
-#pragma unroll
-      for (int k = 0; k < NQ - 1; ++k) wv[k] = W[(size_t)k * ldw + row];
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj) {
-        double t = (double)acc[S - 1][ti][tj][v];
-#pragma unroll
-        for (int s2 = S - 2; s2 >= 0; --s2) t = fma(t, 1.0 / 128.0, (double)acc[s2][ti][tj][v]);
-        const double z = t * isc;
-        q[tj][0] = fma(z, z, q[tj][0]);
-#pragma unroll
-        for (int k = 0; k < NQ - 1; ++k) q[tj][1 + k] = fma(z, wv[k], q[tj][1 + k]);
-      }
-    }
-  }
-#pragma unroll
-  for (int tj = 0; tj < 4; ++tj) {
-#pragma unroll
-    for (int qi = 0; qi < NQ; ++qi) {
-      double x = q[tj][qi];
-      x += __shfl_xor(x, 16, 64);
-      x += __shfl_xor(x, 32, 64);
-      q[tj][qi] = x;
-    }
-    const int col = c0 + 16 * tj + li;
-    if (lg == 0 && col < count) {
-#pragma unroll
-      for (int qi = 0; qi < NQ; ++qi) part[((size_t)rt * NQ + qi) * ldp + col] = q[tj][qi];
-    }
+    for (int s2 = 0; s2 < S; ++s2)
+      *reinterpret_cast<v4i*>(A8 + fgls_a8_offset(row, s2, kc, S, ldn / 64)) =
+          *reinterpret_cast<const v4i*>(out[s2]);
   }
 }
 
@@ -993,17 +819,7 @@ void launch_trmm_reduce(const fgls_hip_ctx* c, const double* X, int64_t ldx, int
                      (int64_t)c->ldn, c->part, (int64_t)c->max_block);
 }
 
-constexpr int NSLICE = 6;   // 7-bit slices of L^-1 in the mixed mode
-
-template <int P>
-void launch_trmm_i8(const fgls_hip_ctx* c, const int8_t* X8, int64_t ldx8, int count,
-                    hipStream_t st) {
-  const int CT = (count + I8_TN - 1) / I8_TN;
-  const int G = tile_grid(c->RT8, CT, I8_GR, I8_GC);
-  hipLaunchKernelGGL((k_trmm_i8<P + 1, NSLICE>), dim3(G), dim3(256), 0, st, c->n, count, c->RT8, CT,
-                     c->ldn / 16, c->A8, c->invsc, X8, ldx8, c->W, (int64_t)c->ldn, c->part,
-                     (int64_t)c->max_block);
-}
+constexpr int NSLICE = FGLS_NSLICE;   // 7-bit slices of L^-1 in the mixed mode
 
 int build_slices(fgls_hip_ctx* c, hipStream_t st) {
   hipLaunchKernelGGL((k_slice_linv<NSLICE>), dim3(c->ldn), dim3(256), 0, st, c->ldn, c->Linv, c->A8,
@@ -1325,9 +1141,12 @@ int fgls_hip_block_reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, i
   if (!dX8) return fail(FGLS_ERR_ARG, "NULL pointer");
   if (ldx8 < (int64_t)round_up(c->n, 64) || (ldx8 & 15) || ((uintptr_t)dX8 & 15))
     return fail(FGLS_ERR_ARG, "int8 X_R needs ld >= ceil(n/64)*64, ld %% 16 == 0 and 16-byte alignment");
+  if (ldx8 > ((int64_t)1 << 31) / 64)  // 32-bit offsets within a wave's 64 columns
+    return fail(FGLS_ERR_ARG, "int8 X_R ld %lld too large", (long long)ldx8);
   if ((rc = set_device(c))) return rc;
   hipStream_t st = (hipStream_t)stream;
-  P_SWITCH(c->p, launch_trmm_i8, c, dX8, ldx8, count, st);
+  fgls_launch_trmm_i8(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, dX8, ldx8, c->W,
+                      (int64_t)c->ldn, c->part, (int64_t)c->max_block, st);
   HIP_TRY(hipGetLastError());
   c->reduced = count;
   c->last_rt = c->RT8;
