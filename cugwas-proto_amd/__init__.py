@@ -9,6 +9,7 @@ from .fgls import (  # noqa: F401
     colmajor,
     device_count,
     exported_symbols,
+    factor_M,
     fgls_chol_gpu,
     last_error,
     lib,
@@ -23,3 +24,4 @@ from .fgls import (  # noqa: F401
     FGLS_MODE_FP64,
     FGLS_MODE_MIXED_I8,
 )
+from .adjoint import SnpLoss, snp_loss  # noqa: F401

@@ -1,0 +1,93 @@
+// fgls_ctx.h -- the device context shared by the translation units of the
+// HIP layer (internal; callers see only the opaque fgls_hip_ctx).
+#ifndef FGLS_CTX_H
+#define FGLS_CTX_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fgls_hip.h"
+
+struct fgls_adjoint_state;
+// Releases the adjoint buffers (fgls_adjoint.hip); NULL is allowed.
+void fgls_adjoint_free(fgls_adjoint_state* a);
+// Marks the adjoint state stale (a new trait was set); NULL is allowed.
+void fgls_adjoint_invalidate(fgls_adjoint_state* a);
+
+struct fgls_hip_ctx {
+  int device;
+  int n, wXL, wXR, p;
+  int ldn;        // padded rows (multiple of 128)
+  int nblk;       // 64-row diagonal blocks
+  int RT;         // 128-row TRMM tiles
+  int max_block;
+  int reduced;    // SNP count of the last block_reduce (finalize must match)
+  int last_rt;    // row tiles of the partials written by the last block_reduce
+  bool have_trait;
+  int mode;       // FGLS_MODE_FP64 or FGLS_MODE_MIXED_I8
+  int RT8;        // 32-row tiles of the int8 product
+  int part_rt;    // row-tile capacity of `part`
+  int8_t* A8;     // slices of L^-1 (mixed mode)
+  double* invsc;  // per-row 2^(e_i - 6)
+  unsigned int* bad;  // count of genotypes that are not int8-exact
+  double* L;
+  double* Linv;
+  double* Dinv;
+  double* W;
+  double* trait;
+  double* part;
+  double* z;      // lazily allocated (generic per-SNP path, adjoint)
+  fgls_adjoint_state* adj;  // lazily allocated (adjoint)
+  unsigned long long* err;
+  int* status;
+};
+
+// Calls FN<p>(...) for p = 1..16.
+#define FGLS_P_SWITCH(p, FN, ...)                    \
+  switch (p) {                                  \
+    case 1: FN<1>(__VA_ARGS__); break;          \
+    case 2: FN<2>(__VA_ARGS__); break;          \
+    case 3: FN<3>(__VA_ARGS__); break;          \
+    case 4: FN<4>(__VA_ARGS__); break;          \
+    case 5: FN<5>(__VA_ARGS__); break;          \
+    case 6: FN<6>(__VA_ARGS__); break;          \
+    case 7: FN<7>(__VA_ARGS__); break;          \
+    case 8: FN<8>(__VA_ARGS__); break;          \
+    case 9: FN<9>(__VA_ARGS__); break;          \
+    case 10: FN<10>(__VA_ARGS__); break;        \
+    case 11: FN<11>(__VA_ARGS__); break;        \
+    case 12: FN<12>(__VA_ARGS__); break;        \
+    case 13: FN<13>(__VA_ARGS__); break;        \
+    case 14: FN<14>(__VA_ARGS__); break;        \
+    case 15: FN<15>(__VA_ARGS__); break;        \
+    case 16: FN<16>(__VA_ARGS__); break;        \
+    default: break;                             \
+  }
+
+namespace fgls {
+
+constexpr int TM = 128;   // FP64 TRMM tile rows
+constexpr int TN = 128;   // FP64 TRMM tile columns
+constexpr int PMAX = 16;  // largest p of the per-SNP kernels
+
+// Sets the calling thread's error message; returns code.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int set_device(const fgls_hip_ctx* c);
+// Offset of res_sigma in the per-trait scalars (ctx->trait).
+__host__ __device__ inline int trait_res_off(int wXL) { return wXL + wXL * wXL; }
+
+// Z = A X, column-major out (ld ldz): A = ctx->Linv (lower) or, if dense, a
+// full row-major ldn x ldn matrix.
+void launch_product_store(const fgls_hip_ctx* c, const double* A, bool dense, const double* X,
+                          int64_t ldx, int ncols, double* Z, int64_t ldz, hipStream_t st);
+// Y = A X for a dense row-major A; per 128-row tile and column, the sums of
+// y * X[row, col] and y * ctx->W[:, k] go to ctx->part (p + 1 per column).
+void launch_product_dot(const fgls_hip_ctx* c, const double* A, const double* X, int64_t ldx,
+                        int count, hipStream_t st);
+// b and V of every SNP of a block from its transformed columns Z.
+void launch_snp_stage_any(const fgls_hip_ctx* c, const double* Z, int64_t ldz, int count,
+                          double* dB, double* dV, int64_t base, hipStream_t st);
+
+}  // namespace fgls
+
+#endif

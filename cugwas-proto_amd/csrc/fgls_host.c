@@ -176,6 +176,27 @@ static void *reader_main(void *arg) {
 }
 
 /* ---------------------------------------------------------------- solver */
+int fgls_factor_M(int n, const double *Phi, int64_t ldphi, double h2, double sigma2, double *L,
+                  int64_t ldl, int threads) {
+  if (n < 1 || !Phi || !L || ldphi < n || ldl < n)
+    return fgls_set_error(FGLS_ERR_ARG, "fgls_factor_M: bad arguments");
+  if (threads < 1) threads = host_cores();
+  /* M := sigma*h*Phi; M_ii += sigma*(1-h)   (fgls_chol_gpu.c:406-414) */
+  const double alpha = h2 * sigma2, beta = (1.0 - h2) * sigma2;
+#pragma omp parallel for num_threads(threads) schedule(static)
+  for (int64_t c = 0; c < n; ++c) {
+    const double *src = Phi + (size_t)c * ldphi;
+    double *dst = L + (size_t)c * ldl;
+    for (int64_t r = 0; r < n; ++r) dst[r] = alpha * src[r];
+    dst[c] += beta;
+  }
+  scipy_openblas_set_num_threads(threads);
+  int info = 0, ld = (int)ldl;
+  scipy_dpotrf_("L", &n, L, &ld, &info);                 /* fgls_chol_gpu.c:416-430 */
+  if (info != 0) return fgls_set_error(FGLS_ERR_NOTSPD, "dpotrf(M) failed (info: %d)", info);
+  return FGLS_OK;
+}
+
 int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int y_b,
                      int num_threads, const char *Phi_path, const char *h2_path,
                      const char *sigma2_path, const char *XL_path, const char *XR_path,
@@ -299,22 +320,8 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
 
   int64_t it = 0;  /* global chunk counter across traits (reader order) */
   for (int j = 0; j < t; ++j) {
-    /* M := sigma*h*Phi; M_ii += sigma*(1-h)   (fgls_chol_gpu.c:406-414) */
-    const double alpha = h[j] * sig[j], beta = (1.0 - h[j]) * sig[j];
     double t0 = now_s();
-#pragma omp parallel for num_threads(threads) schedule(static)
-    for (int64_t c = 0; c < n; ++c) {
-      const double *src = Phi + (size_t)c * n;
-      double *dst = M + (size_t)c * n;
-      for (int64_t r = 0; r < n; ++r) dst[r] = alpha * src[r];
-      dst[c] += beta;
-    }
-    int info = 0;
-    scipy_dpotrf_("L", &n, M, &n, &info);                 /* fgls_chol_gpu.c:416-430 */
-    if (info != 0) {
-      rc = fgls_set_error(FGLS_ERR_NOTSPD, "dpotrf(M) failed (info: %d)", info);
-      goto done;
-    }
+    if ((rc = fgls_factor_M(n, Phi, n, h[j], sig[j], M, n, threads)) != FGLS_OK) goto done;
     t_potrf += now_s() - t0;
     t0 = now_s();
     RCHK(read_exact(Y_path, y, (size_t)n, (size_t)j * n));

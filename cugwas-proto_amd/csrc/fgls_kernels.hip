@@ -35,15 +35,17 @@
 #include "fgls_device.h"
 #include "fgls_hip.h"
 #include "fgls_internal.h"
+#include "fgls_ctx.h"
 
 namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int NB = 64;    // diagonal block of the TRSM
-constexpr int TM = 128;   // TRMM tile rows (2 waves of 64)
-constexpr int TN = 128;   // TRMM tile columns (2 waves of 64)
-constexpr int PMAX = 16;  // largest p supported by the per-SNP kernels
+using fgls::TM;    // TRMM tile rows (2 waves of 64)
+using fgls::TN;    // TRMM tile columns (2 waves of 64)
+using fgls::PMAX;  // largest p supported by the per-SNP kernels
+using fgls::trait_res_off;
 
 __device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -198,16 +200,18 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const doubl
 // Triangular product Z = Linv * X over 128x128 tiles (2x2 waves of 64x64),
 // FP64 MFMA, operands fetched straight into registers (one K=16 step ahead).
 // Linv is row-major (ld ldli >= RT*128) and lower triangular: row tile rt
-// only needs k < (rt+1)*128.  X is column-major with ld ldx (% 4 == 0);
-// rows k >= n are masked.  Epilogue:
+// only needs k < (rt+1)*128 (DENSE: a full row-major matrix, all k < n).
+// X is column-major with ld ldx (% 4 == 0); rows k >= n are masked.
+// Epilogue:
 //   EPI_STORE : Z written column-major (ld ldz)
 //   EPI_REDUCE: per column, the NQ sums over the tile's rows of
 //               z*z and z*W[:,k] (k < NQ-1) go to part[(rt*NQ+q)*ldp + col]
+//   EPI_DOT   : as EPI_REDUCE with z*X[row, col] (rows < n) in place of z*z
 // Grid: (column tiles, row tiles), heaviest row tiles first.
 // ---------------------------------------------------------------------------
-enum { EPI_STORE = 0, EPI_REDUCE = 1 };
+enum { EPI_STORE = 0, EPI_REDUCE = 1, EPI_DOT = 2 };
 
-template <int EPI, int NQ>
+template <int EPI, int NQ, bool DENSE = false>
 __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
                                               const double* __restrict__ Linv, int64_t ldli,
                                               const double* __restrict__ X, int64_t ldx,
@@ -222,7 +226,7 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
   const int r0 = rt * TM + wr * 64;
   const int c0 = ct * TN + wc * 64;
   const int n16 = (n + 15) & ~15;
-  const int kend = (r0 < n16) ? min(r0 + 64, n16) : 0;   // multiple of 16
+  const int kend = (r0 < n16) ? (DENSE ? n16 : min(r0 + 64, n16)) : 0;   // multiple of 16
   const int kfull = min(kend, n & ~15);                   // steps with no masking
 
   const double* Ap[4];
@@ -333,7 +337,12 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
 #pragma unroll
         for (int tj = 0; tj < 4; ++tj) {
           const double z = acc[ti][tj][v];
-          q[tj][0] = fma(z, z, q[tj][0]);
+          if (EPI == EPI_DOT) {
+            const double x = (row < n) ? Bp[tj][(size_t)row - 4 * lg] : 0.0;
+            q[tj][0] = fma(z, x, q[tj][0]);
+          } else {
+            q[tj][0] = fma(z, z, q[tj][0]);
+          }
 #pragma unroll
           for (int k = 0; k < NQ - 1; ++k) q[tj][1 + k] = fma(z, wv[k], q[tj][1 + k]);
         }
@@ -556,7 +565,6 @@ __device__ __forceinline__ void record_failure(unsigned long long* err, int64_t 
 //   [wXL, wXL + wXL^2)     V_tl (column-major, upper triangle 0)
 //   [wXL + wXL^2]          res_sigma
 //   [wXL + wXL^2 + 1, +wXL) b_L
-__host__ __device__ inline int trait_res_off(int wXL) { return wXL + wXL * wXL; }
 
 // Finalize from the TRMM partials (wXR = 1, p = P): thread per SNP.
 template <int P>
@@ -763,32 +771,6 @@ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 }  // namespace
 
-struct fgls_hip_ctx {
-  int device;
-  int n, wXL, wXR, p;
-  int ldn;        // padded rows (multiple of 128)
-  int nblk;       // 64-row diagonal blocks
-  int RT;         // 128-row TRMM tiles
-  int max_block;
-  int reduced;    // SNP count of the last block_reduce (finalize must match)
-  int last_rt;    // row tiles of the partials written by the last block_reduce
-  bool have_trait;
-  int mode;       // FGLS_MODE_FP64 or FGLS_MODE_MIXED_I8
-  int RT8;        // 32-row tiles of the int8 product
-  int part_rt;    // row-tile capacity of `part`
-  int8_t* A8;     // slices of L^-1 (mixed mode)
-  double* invsc;  // per-row 2^(e_i - 6)
-  unsigned int* bad;  // count of genotypes that are not int8-exact
-  double* L;
-  double* Linv;
-  double* Dinv;
-  double* W;
-  double* trait;
-  double* part;
-  double* z;      // lazily allocated (generic per-SNP path)
-  unsigned long long* err;
-  int* status;
-};
 
 namespace {
 
@@ -836,30 +818,55 @@ void launch_snp_stage(const fgls_hip_ctx* c, const double* Z, int64_t ldz, int c
                      c->W, (int64_t)c->ldn, c->trait, dB, dV, c->err, base);
 }
 
-#define P_SWITCH(p, FN, ...)                    \
-  switch (p) {                                  \
-    case 1: FN<1>(__VA_ARGS__); break;          \
-    case 2: FN<2>(__VA_ARGS__); break;          \
-    case 3: FN<3>(__VA_ARGS__); break;          \
-    case 4: FN<4>(__VA_ARGS__); break;          \
-    case 5: FN<5>(__VA_ARGS__); break;          \
-    case 6: FN<6>(__VA_ARGS__); break;          \
-    case 7: FN<7>(__VA_ARGS__); break;          \
-    case 8: FN<8>(__VA_ARGS__); break;          \
-    case 9: FN<9>(__VA_ARGS__); break;          \
-    case 10: FN<10>(__VA_ARGS__); break;        \
-    case 11: FN<11>(__VA_ARGS__); break;        \
-    case 12: FN<12>(__VA_ARGS__); break;        \
-    case 13: FN<13>(__VA_ARGS__); break;        \
-    case 14: FN<14>(__VA_ARGS__); break;        \
-    case 15: FN<15>(__VA_ARGS__); break;        \
-    case 16: FN<16>(__VA_ARGS__); break;        \
-    default: break;                             \
-  }
-
 bool aligned32(const void* p) { return ((uintptr_t)p & 31u) == 0; }
 
+template <int P>
+void launch_trmm_dot(const fgls_hip_ctx* c, const double* A, const double* X, int64_t ldx,
+                     int count, hipStream_t st) {
+  dim3 grid((count + TN - 1) / TN, c->RT);
+  hipLaunchKernelGGL((k_trmm<EPI_DOT, P + 1, true>), grid, dim3(256), 0, st, c->n, count, c->RT, A,
+                     (int64_t)c->ldn, X, ldx, (double*)nullptr, (int64_t)0, c->W, (int64_t)c->ldn,
+                     c->part, (int64_t)c->max_block);
+}
+
 }  // namespace
+
+namespace fgls {
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int set_device(const fgls_hip_ctx* c) { return ::set_device(c); }
+
+void launch_product_store(const fgls_hip_ctx* c, const double* A, bool dense, const double* X,
+                          int64_t ldx, int ncols, double* Z, int64_t ldz, hipStream_t st) {
+  dim3 grid((ncols + TN - 1) / TN, c->RT);
+  if (dense)
+    hipLaunchKernelGGL((k_trmm<EPI_STORE, 2, true>), grid, dim3(256), 0, st, c->n, ncols, c->RT, A,
+                       (int64_t)c->ldn, X, ldx, Z, ldz, c->W, (int64_t)c->ldn, (double*)nullptr,
+                       (int64_t)0);
+  else
+    hipLaunchKernelGGL((k_trmm<EPI_STORE, 2>), grid, dim3(256), 0, st, c->n, ncols, c->RT, A,
+                       (int64_t)c->ldn, X, ldx, Z, ldz, c->W, (int64_t)c->ldn, (double*)nullptr,
+                       (int64_t)0);
+}
+
+void launch_product_dot(const fgls_hip_ctx* c, const double* A, const double* X, int64_t ldx,
+                        int count, hipStream_t st) {
+  FGLS_P_SWITCH(c->p, launch_trmm_dot, c, A, X, ldx, count, st);
+}
+
+void launch_snp_stage_any(const fgls_hip_ctx* c, const double* Z, int64_t ldz, int count,
+                          double* dB, double* dV, int64_t base, hipStream_t st) {
+  FGLS_P_SWITCH(c->p, launch_snp_stage, c, Z, ldz, count, dB, dV, base, st);
+}
+
+}  // namespace fgls
 
 extern "C" {
 
@@ -941,6 +948,7 @@ int fgls_hip_ctx_destroy(fgls_hip_ctx* c) {
                   c->A8, c->invsc, c->bad};
   for (void* p : ptrs)
     if (p) hipFree(p);
+  fgls_adjoint_free(c->adj);
   delete c;
   return FGLS_OK;
 }
@@ -959,6 +967,7 @@ int fgls_hip_set_trait(fgls_hip_ctx* c, const double* dL, int64_t ldl, const dou
   hipStream_t st = (hipStream_t)stream;
   const int n = c->n, wXL = c->wXL;
   const size_t ldn = (size_t)c->ldn;
+  fgls_adjoint_invalidate(c->adj);
   HIP_TRY(hipMemcpy2DAsync(c->L, ldn * sizeof(double), dL, (size_t)ldl * sizeof(double),
                            (size_t)n * sizeof(double), (size_t)n, hipMemcpyDeviceToDevice, st));
   if (wXL > 0)
@@ -1049,7 +1058,7 @@ int fgls_hip_snp_stage(fgls_hip_ctx* c, const double* dZ, int64_t ldz, int count
   if (!dZ || !dB || !dV || ldz < c->n) return fail(FGLS_ERR_ARG, "bad Z/B/V arguments");
   if ((rc = set_device(c))) return rc;
   hipStream_t st = (hipStream_t)stream;
-  P_SWITCH(c->p, launch_snp_stage, c, dZ, ldz, count, dB, dV, snp_base, st);
+  FGLS_P_SWITCH(c->p, launch_snp_stage, c, dZ, ldz, count, dB, dV, snp_base, st);
   HIP_TRY(hipGetLastError());
   return FGLS_OK;
 }
@@ -1068,7 +1077,7 @@ int fgls_hip_block_reduce(fgls_hip_ctx* c, const double* dXR, int64_t ldxr, int 
   if ((rc = set_device(c))) return rc;
   hipStream_t st = (hipStream_t)stream;
   if (c->wXR == 1) {
-    P_SWITCH(c->p, launch_trmm_reduce, c, dXR, ldxr, count, st);
+    FGLS_P_SWITCH(c->p, launch_trmm_reduce, c, dXR, ldxr, count, st);
   } else {
     if (!c->z) {
       const size_t bytes = (size_t)c->ldn * (size_t)c->max_block * c->wXR * sizeof(double);
@@ -1174,9 +1183,9 @@ int fgls_hip_block_finalize(fgls_hip_ctx* c, int count, double* dB, double* dV, 
   if ((rc = set_device(c))) return rc;
   hipStream_t st = (hipStream_t)stream;
   if (c->wXR == 1) {
-    P_SWITCH(c->p, launch_finalize, c, count, dB, dV, snp_base, st);
+    FGLS_P_SWITCH(c->p, launch_finalize, c, count, dB, dV, snp_base, st);
   } else {
-    P_SWITCH(c->p, launch_snp_stage, c, c->z, (int64_t)c->ldn, count, dB, dV, snp_base, st);
+    FGLS_P_SWITCH(c->p, launch_snp_stage, c, c->z, (int64_t)c->ldn, count, dB, dV, snp_base, st);
   }
   HIP_TRY(hipGetLastError());
   return FGLS_OK;

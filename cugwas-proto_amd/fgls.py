@@ -72,6 +72,11 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_trmm": (i32, [vp, dp, i64, i32, dp, i64, vp]),
             "fgls_hip_snp_stage": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
             "fgls_hip_check": (i32, [vp, vp, ctypes.POINTER(i64), ctypes.POINTER(i32)]),
+            "fgls_hip_adjoint_begin": (i32, [vp, dp, i64, ctypes.c_double, vp]),
+            "fgls_hip_adjoint_block": (i32, [vp, dp, i64, i32, dp, i64, vp]),
+            "fgls_hip_adjoint_end": (i32, [vp, dp, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double), vp]),
+            "fgls_factor_M": (i32, [i32, dp, i64, ctypes.c_double, ctypes.c_double, dp, i64, i32]),
             "fgls_chol_gpu": (i32, [i32] * 9 + [ctypes.c_char_p] * 8),
             "fgls_chol_gpu_ex": (i32, [i32] * 9 + [ctypes.c_char_p] * 8 + [ctypes.c_void_p]),
         }
@@ -91,7 +96,9 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_get_trait", "fgls_hip_solve_block", "fgls_hip_block_reduce",
                         "fgls_hip_block_finalize", "fgls_hip_ctx_set_mode", "fgls_hip_block_reduce_i8",
                         "fgls_hip_solve_block_i8", "fgls_hip_to_i8", "fgls_hip_trsm",
-                        "fgls_hip_trmm", "fgls_hip_snp_stage", "fgls_hip_check")
+                        "fgls_hip_trmm", "fgls_hip_snp_stage", "fgls_hip_check",
+                        "fgls_hip_adjoint_begin", "fgls_hip_adjoint_block", "fgls_hip_adjoint_end",
+                        "fgls_factor_M")
             if hasattr(lib(), s)]
 
 
@@ -289,6 +296,35 @@ class DeviceContext:
                                         snp_base, _stream(stream)))
         return B, V
 
+    # -- adjoint of l = sum_i w_i b_i[p-1]^2 (FP64, wXR = 1) ----------------
+    def adjoint_begin(self, Phi, sigma2: float, stream=None):
+        """Phi: (n, ldphi) column-major (symmetric) on the device."""
+        _need_cuda_f64(Phi, "Phi")
+        _check(lib().fgls_hip_adjoint_begin(self._h, _ptr(Phi), Phi.shape[1], float(sigma2),
+                                            _stream(stream)))
+
+    def adjoint_block(self, XR, w, count: int | None = None, snp_base: int = 0, stream=None):
+        """XR: (count, ldx) genotype block; w: (count,) loss weights."""
+        _need_cuda_f64(XR, "XR")
+        _need_cuda_f64(w, "w")
+        count = XR.shape[0] if count is None else count
+        if w.numel() < count:
+            raise FglsError(FGLS_ERR_ARG, "w has fewer than count weights")
+        _check(lib().fgls_hip_adjoint_block(self._h, _ptr(XR), XR.shape[1], count, _ptr(w), snp_base,
+                                            _stream(stream)))
+
+    def adjoint_end(self, grad_y=None, stream=None):
+        """Returns (loss, dl/dy as a (n,) tensor, dl/dh2)."""
+        import torch
+        if grad_y is None:
+            grad_y = torch.empty(self.n, dtype=torch.float64, device="cuda")
+        _need_cuda_f64(grad_y, "grad_y")
+        loss = ctypes.c_double(0.0)
+        gh = ctypes.c_double(0.0)
+        _check(lib().fgls_hip_adjoint_end(self._h, _ptr(grad_y), ctypes.byref(loss), ctypes.byref(gh),
+                                          _stream(stream)))
+        return loss.value, grad_y, gh.value
+
     def check(self, stream=None):
         """Raise FglsError(FGLS_ERR_NOTSPD) carrying .bad_snp / .info if a per-SNP
         factorisation failed since the last check."""
@@ -300,6 +336,18 @@ class DeviceContext:
             err.bad_snp = int(snp.value)
             err.info = int(info.value)
             raise err
+
+
+def factor_M(Phi, h2: float, sigma2: float, threads: int = 0):
+    """Host Cholesky factor of M = sigma2 (h2 Phi + (1 - h2) I) (fgls_chol_gpu.c:406-430):
+    numpy n x n in, numpy lower factor (column-major, Fortran order) out."""
+    import numpy as np
+    Phi = np.asfortranarray(Phi, dtype=np.float64)
+    n = Phi.shape[0]
+    L = np.empty((n, n), dtype=np.float64, order="F")
+    _check(lib().fgls_factor_M(n, Phi.ctypes.data, n, float(h2), float(sigma2), L.ctypes.data, n,
+                               int(threads)))
+    return np.tril(L)
 
 
 def fgls_chol_gpu(n, p, m, t, wXL, wXR, x_b, y_b, num_threads,

@@ -27,6 +27,8 @@
 #ifndef FGLS_CHOL_GPU_H
 #define FGLS_CHOL_GPU_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -58,6 +60,16 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR,
                      const char *Phi_path, const char *h2_path, const char *sigma2_path,
                      const char *XL_path, const char *XR_path, const char *Y_path,
                      const char *B_path, const char *V_path, const char *options);
+
+/*
+ * Host factor of one trait's covariance: L (column-major, ld ldl) receives
+ * M = sigma2 (h2 Phi + (1 - h2) I) and is overwritten by its lower Cholesky
+ * factor (fgls_chol_gpu.c:406-430: M built in place, then dpotrf "L"; the
+ * strict upper triangle keeps sigma2 h2 Phi).  threads < 1 uses every core.
+ * FGLS_OK, or FGLS_ERR_NOTSPD with "dpotrf(M) failed (info: i)".
+ */
+int fgls_factor_M(int n, const double *Phi, int64_t ldphi, double h2, double sigma2,
+                  double *L, int64_t ldl, int threads);
 
 #ifdef __cplusplus
 }

@@ -169,6 +169,32 @@ int fgls_hip_snp_stage(fgls_hip_ctx *ctx, const double *dZ, int64_t ldz,
 int fgls_hip_check(fgls_hip_ctx *ctx, void *stream, int64_t *bad_snp,
                    int *info);
 
+/*
+ * Adjoint of the loss l = sum_i w_i beta_i^2 over the SNPs of the current
+ * trait, beta_i = b_i[p-1] (the SNP coefficient of the per-SNP solve,
+ * fgls_chol.c:386-439), with respect to y and h^2, where
+ * M = sigma2 (h^2 Phi + (1 - h^2) I) is the matrix whose factor L was given
+ * to fgls_hip_set_trait (fgls_chol_gpu.c:406-414).  The reference has no
+ * counterpart: this is an addition on top of its forward path.
+ * FP64 mode and wXR = 1 only.
+ *
+ * begin: Phi (device, column-major, ld ldphi >= n) and sigma2 of the trait;
+ *        builds G = L^-1 (Phi - I) L^-T once and clears the accumulators.
+ * block: one block of SNPs (device X_R as for fgls_hip_solve_block, count
+ *        <= max_block) with their weights dw (device, count doubles); the
+ *        block's L^-1 X_R is recomputed here (nothing is kept from a forward
+ *        pass).  Per-SNP dpotrf failures are recorded as by solve_block.
+ * end:   dgrad_y (device, n doubles) = dl/dy; *hloss = l and
+ *        *hgrad_h2 = dl/dh^2 (host); synchronizes `stream`.
+ */
+int fgls_hip_adjoint_begin(fgls_hip_ctx *ctx, const double *dPhi, int64_t ldphi,
+                           double sigma2, void *stream);
+int fgls_hip_adjoint_block(fgls_hip_ctx *ctx, const double *dXR, int64_t ldxr,
+                           int count, const double *dw, int64_t snp_base,
+                           void *stream);
+int fgls_hip_adjoint_end(fgls_hip_ctx *ctx, double *dgrad_y, double *hloss,
+                         double *hgrad_h2, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,11 @@ variant (src/fgls_chol_gpu.c):
 * ``fgls_chol``         fgls_chol.c:223-491  (trait loop, output layout
                                               fgls_chol.c:458-474)
 * ``read_inputs`` / ``write_outputs``  io.c:7-33 raw little-endian doubles.
+* ``snp_loss_adjoint``  no reference counterpart: gradients of
+                        l = sum_i w_i b_i[p-1]^2 w.r.t. y and h (an addition
+                        on top of the per-SNP stage); pinned only by
+                        torch.autograd through a dense FP64 torch restatement
+                        and by central finite differences (tests/test_oracle.py).
 
 How each function is pinned (beyond this restatement itself) is listed in
 tests/test_oracle.py: the reference fixture test_cases/test_XL/B_exp.out
@@ -147,6 +152,45 @@ def fgls_chol(Phi, H, Sig, XL, XR, Y, wXR: int = 1, x_b: int | None = None):
             ov = (j * m + ib) * p * p
             Vout[ov:ov + cnt * p * p] = V.transpose(0, 2, 1).ravel()
     return Bout, Vout
+
+
+def snp_loss_adjoint(Phi, h: float, sigma: float, XL, XR, y, w):
+    """l = sum_i w_i b_i[p-1]^2 and (dl/dy, dl/dh), wXR = 1.
+
+    XL is n x wXL, XR n x m (column i = SNP i).  With X_i = [X_L, x_i],
+    S_i = X_i' M^-1 X_i, u_i = S_i^-1 e_p, g_i = 2 w_i b_i[p-1] u_i,
+    r_i = y - X_i b_i and G = L^-1 (Phi - I) L^-T:
+        dl/dy = M^-1 sum_i X_i g_i = L^-T sum_i p~_i,   p~_i = L^-1 X_i g_i
+        dl/dh = -sigma sum_i p~_i' G q~_i,              q~_i = L^-1 r_i
+    (from d b_i = -S_i^-1 X_i' M^-1 dM M^-1 r_i and dM = sigma (Phi - I) dh)."""
+    Phi = np.asarray(Phi, dtype=np.float64)
+    XL = np.asarray(XL, dtype=np.float64).reshape(Phi.shape[0], -1)
+    XR = np.asarray(XR, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    w = np.asarray(w, dtype=np.float64)
+    n, wXL = XL.shape
+    p = wXL + 1
+    L = cholesky_lower(build_M(Phi, h, sigma))
+    tr = trait_precompute(L, XL, y)
+    Z = sla.solve_triangular(L, XR, lower=True, check_finite=False)     # n x m
+    b, V = per_snp_from_z(Z, tr, 1)
+    m = Z.shape[1]
+    S = np.zeros((m, p, p))
+    S[:, :wXL, :wXL] = tr.XLt.T @ tr.XLt
+    S[:, wXL, :wXL] = (Z.T @ tr.XLt)
+    S[:, :wXL, wXL] = S[:, wXL, :wXL]
+    S[:, wXL, wXL] = np.einsum("nm,nm->m", Z, Z)
+    u = np.linalg.solve(S, np.broadcast_to(np.eye(p)[:, -1], (m, p))[..., None])[..., 0]
+    beta = b[:, -1]
+    g = 2.0 * (w * beta)[:, None] * u                                   # m x p
+    Pt = tr.XLt @ g[:, :wXL].T + Z * g[:, -1]                           # n x m
+    Qt = tr.yt[:, None] - tr.XLt @ b[:, :wXL].T - Z * beta              # n x m
+    grad_y = sla.solve_triangular(L.T, Pt.sum(axis=1), lower=False, check_finite=False)
+    T = sla.solve_triangular(L, Phi - np.eye(n), lower=True, check_finite=False)
+    G = sla.solve_triangular(L, T.T, lower=True, check_finite=False)
+    grad_h = -sigma * float(np.einsum("nm,nm->", Pt, G @ Qt))
+    loss = float(w @ (beta * beta))
+    return loss, grad_y, grad_h
 
 
 def read_doubles(path: str, count: int, offset: int = 0) -> np.ndarray:

@@ -169,3 +169,55 @@ def test_parity_metric_catches_planted_errors():
     assert not orc.parity_errors(b, bad, b, Vf, 4)["V_upper_zero"]
     bb = b.copy(); bb[7, 3] += 1e-6 * Vf[7, 15]                # SNP coefficient off by 1e-6 se
     assert orc.parity_errors(bb, Vf, b, Vf, 4)["B_scaled"] > 1e-7
+
+
+# --- adjoint of l = sum_i w_i b_i[p-1]^2 (no reference counterpart) ---------
+
+def _adj_problem(n=48, wXL=3, m=40, seed=7, h2=0.45):
+    pr = make_problem(n=n, wXL=wXL, m=m, seed=seed, h2=h2)
+    w = np.random.default_rng(seed + 1).uniform(0.0, 1.0, m)
+    return pr, w
+
+
+@pytest.mark.parametrize("sigma", [1.0, 2.5])
+def test_adjoint_oracle_matches_torch_autograd(sigma):
+    from tests.adjoint_ref import dense_grads
+    pr, w = _adj_problem()
+    loss, gy, gh = orc.snp_loss_adjoint(pr.Phi, pr.h2, sigma, pr.XL, pr.XR, pr.y, w)
+    loss_r, gy_r, gh_r = dense_grads(pr.y, pr.h2, pr.Phi, pr.XL, pr.XR, w, sigma)
+    assert abs(loss - loss_r) <= 1e-10 * abs(loss_r)
+    assert np.max(np.abs(gy - gy_r)) <= 1e-9 * np.max(np.abs(gy_r))
+    assert abs(gh - gh_r) <= 1e-9 * abs(gh_r)
+
+
+def test_adjoint_oracle_matches_central_differences():
+    """Step 1e-6 central differences of the oracle's forward loss."""
+    pr, w = _adj_problem(n=40, m=30, seed=11)
+    sigma = 1.0
+
+    def loss_of(y, h):
+        L = orc.cholesky_lower(orc.build_M(pr.Phi, h, sigma))
+        tr = orc.trait_precompute(L, pr.XL, y)
+        b, _ = orc.solve_snps(pr.XR, tr, 1)
+        return float(w @ (b[:, -1] ** 2))
+
+    loss, gy, gh = orc.snp_loss_adjoint(pr.Phi, pr.h2, sigma, pr.XL, pr.XR, pr.y, w)
+    eps = 1e-6
+    fd_h = (loss_of(pr.y, pr.h2 + eps) - loss_of(pr.y, pr.h2 - eps)) / (2 * eps)
+    assert abs(fd_h - gh) <= 1e-6 * abs(gh)
+    d = np.random.default_rng(3).standard_normal(pr.n)
+    fd_y = (loss_of(pr.y + eps * d, pr.h2) - loss_of(pr.y - eps * d, pr.h2)) / (2 * eps)
+    assert abs(fd_y - gy @ d) <= 1e-6 * abs(gy @ d)
+
+
+def test_adjoint_oracle_catches_planted_mistakes():
+    """The tolerances above separate the slips a restatement could make: a
+    dropped factor 2 of the square or a flipped sign of dM/dh."""
+    from tests.adjoint_ref import dense_grads
+    pr, w = _adj_problem(seed=5)
+    _, gy_r, gh_r = dense_grads(pr.y, pr.h2, pr.Phi, pr.XL, pr.XR, w)
+    _, gy, gh = orc.snp_loss_adjoint(pr.Phi, pr.h2, 1.0, pr.XL, pr.XR, pr.y, w)
+    tol_y = 1e-9 * np.max(np.abs(gy_r))
+    assert np.max(np.abs(gy - gy_r)) <= tol_y and abs(gh - gh_r) <= 1e-9 * abs(gh_r)
+    assert np.max(np.abs(0.5 * gy - gy_r)) > 1e6 * tol_y
+    assert abs(-gh - gh_r) > 1e6 * 1e-9 * abs(gh_r)
