@@ -42,6 +42,17 @@ PEAK_INT8_TOPS = 5000.0
 PEAK_HBM_GBS = 8000.0
 NSLICE = 6
 
+# Bytes per launch of the two reduce kernels at n = 10,000 and 65,536 SNPs per
+# launch, from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of
+# this benchmark (KiB per dispatch, averaged over the launches of a 131,072-SNP
+# pass).  Scale factors from a plain 1 GiB device copy measured in the same
+# runs: it reads as FETCH_SIZE 524,313 KiB (x 2.0) and writes as WRITE_SIZE
+# 1,048,576 KiB (x 1.0).  FETCH_SIZE counts L2 misses sent to the fabric
+# (some are served by the MALL), so this is an upper bound on HBM reads.
+TRAFFIC_N, TRAFFIC_BLOCK = 10000, 65536
+TRAFFIC_KIB = {"fp64": (106695885.3, 202240.0), "mixed": (38137528.2, 1370998.3)}
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+
 
 def orc_parity(res, ref, p):
     """Errors of one device path against another over all SNPs."""
@@ -142,6 +153,56 @@ def cpu_baseline(kind, Phi, L, XL, y, X, n, wXL, h2, sample, threads):
     return {"value": rate, "unit": "SNPs/s", "cores": threads, "kind": kind, "sample": desc}
 
 
+def run_adjoint(torch, pkg, dev, m, steps):
+    """Time the adjoint of l = sum_i w_i b_i[p-1]^2 (dl/dy, dl/dh2) over m SNPs
+    at n = 5000, p = 4, h2 = 0.6, w_i ~ U(0, 1), blocks of 2048 SNPs, next to
+    the forward solve of the same blocks."""
+    n, wXL, block = 5000, 3, 2048
+    ldx = (n + 3) // 4 * 4
+    Phi, L, XL, y, X, X8 = make_inputs(torch, n, m, wXL, 0.6, 3, ldx, dev)
+    del X8
+    g = torch.Generator(device=dev)
+    g.manual_seed(4)
+    w = torch.rand(m, dtype=torch.float64, device=dev, generator=g)
+    ctx = pkg.DeviceContext(n, wXL, 1, max_block=block)
+    ctx.set_trait(L.T.contiguous(), XL.T.contiguous(), y.contiguous())
+    B = torch.empty((block, wXL + 1), dtype=torch.float64, device=dev)
+    V = torch.empty((block, (wXL + 1) ** 2), dtype=torch.float64, device=dev)
+
+    def adjoint_pass():
+        ctx.adjoint_begin(Phi, 1.0)
+        for i0 in range(0, m, block):
+            cnt = min(block, m - i0)
+            ctx.adjoint_block(X[i0:i0 + cnt], w[i0:i0 + cnt], cnt, snp_base=i0)
+        return ctx.adjoint_end()
+
+    def forward_pass():
+        for i0 in range(0, m, block):
+            cnt = min(block, m - i0)
+            ctx.solve_block(X[i0:i0 + cnt], cnt, B, V, snp_base=i0)
+
+    adjoint_pass()
+    forward_pass()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss, gy, gh = adjoint_pass()
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        forward_pass()
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / steps
+    ctx.check()
+    ctx.close()
+    return {"workload": f"adjoint of sum_i w_i b_i[p-1]^2: n={n}, p={wXL + 1}, m={m}, h2=0.6, "
+                        f"blocks of {block}, L^-1 X_R recomputed per block",
+            "snps_per_s": m / t_adj, "seconds": t_adj, "forward_snps_per_s": m / t_fwd,
+            "forward_seconds": t_fwd, "loss": loss, "grad_h2": gh,
+            "grad_y_norm": float(torch.linalg.vector_norm(gy))}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +218,8 @@ def main():
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "none"], default="auto")
     ap.add_argument("--cpu-snps", type=int, default=16384)
     ap.add_argument("--paths", default="fp64,mixed", help="comma-separated: fp64, mixed")
+    ap.add_argument("--adjoint-m", type=int, default=200_000,
+                    help="SNPs of the adjoint measurement at n=5000, p=4, blocks of 2048 (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (a known-size kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -281,18 +344,24 @@ def main():
         cpu = cpu_baseline(args.cpu_baseline, Phi, L, XL, y, X, n, wXL, args.h2, args.cpu_snps, threads)
         log(f"CPU baseline: {cpu['value']:.1f} SNPs/s")
 
+    def traffic(mode):
+        if n != TRAFFIC_N or block != TRAFFIC_BLOCK or mode not in TRAFFIC_KIB:
+            return None
+        f, w = TRAFFIC_KIB[mode]
+        return (f * FETCH_SCALE + w * WRITE_SCALE) * 1024.0
+
     def roofline(mode, res):
         flop = float(n) * n * block                 # algorithmic: n^2 per SNP (TRSM count)
         if mode == "fp64":
             ach = flop / (res["reduce_ms"] * 1e-3) / 1e12
             return {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": None,
+                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": traffic(mode),
                     "kernel": "k_trmm<EPI_REDUCE> (fgls_hip_block_reduce)",
                     "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block, "flop_per_snp": float(n) * n}
         ops = NSLICE * flop                         # int8 ops the 6-slice product executes
         ach = ops / (res["reduce_ms"] * 1e-3) / 1e12
         return {"bound": "mfma", "achieved": ach, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_INT8_TOPS, "traffic": None,
+                "frac": ach / PEAK_INT8_TOPS, "traffic": traffic(mode),
                 "kernel": "k_trmm_i8 (fgls_hip_block_reduce_i8)", "achieved_is": "int8 TOP/s",
                 "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block,
                 "int8_ops_per_snp": NSLICE * float(n) * n,
@@ -310,6 +379,13 @@ def main():
         summary["mixed_vs_fp64_all_snps"] = e
         summary["speedup_mixed_over_fp64"] = paths["mixed"]["snps_per_s"] / paths["fp64"]["snps_per_s"]
         log(f"mixed vs fp64 over all {m} SNPs: {e}; speedup {summary['speedup_mixed_over_fp64']:.2f}x")
+    adjoint = None
+    if args.adjoint_m > 0:
+        del X, X8, paths
+        torch.cuda.empty_cache()
+        adjoint = run_adjoint(torch, pkg, dev, args.adjoint_m, args.steps)
+        log(f"adjoint (n=5000, m={args.adjoint_m}, block 2048): {adjoint['snps_per_s']:.0f} SNPs/s, "
+            f"forward {adjoint['forward_snps_per_s']:.0f} SNPs/s")
     out = {
         "metric": METRIC,
         "value": res["snps_per_s"],
@@ -333,6 +409,7 @@ def main():
         "trait_setup_s": trait_setup,
         "paths": summary,
         "parity_vs_oracle": check,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

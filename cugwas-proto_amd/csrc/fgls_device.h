@@ -13,7 +13,14 @@ typedef int fgls_v4i __attribute__((ext_vector_type(4)));
 // workgroup tiles of I8_TM rows x I8_TN SNP columns, grouped I8_GR x I8_GC
 // per XCD (see fgls_tile_of_block).
 constexpr int FGLS_NSLICE = 6;
-constexpr int I8_TM = 32, I8_TN = 256, I8_GR = 8, I8_GC = 4;
+// The grouping is a build-time tunable (-DFGLS_I8_GR=.. -DFGLS_I8_GC=..).
+#ifndef FGLS_I8_GR
+#define FGLS_I8_GR 16
+#endif
+#ifndef FGLS_I8_GC
+#define FGLS_I8_GC 4
+#endif
+constexpr int I8_TM = 32, I8_TN = 256, I8_GR = FGLS_I8_GR, I8_GC = FGLS_I8_GC;
 
 // Tile order of the hot kernels: workgroup b of a 1-D grid runs on XCD b % 8
 // (round-robin dispatch), so each XCD takes whole groups of GR x GC tiles (one

@@ -49,6 +49,7 @@ typedef struct {
   int device;
   int serial;
   int stats;
+  int mixed;         /* precision=mixed: int8 genotypes x sliced L^-1 */
 } options_t;
 
 static double now_s(void) {
@@ -73,6 +74,10 @@ static void parse_options(const char *s, options_t *o) {
     char *eq = strchr(tok, '=');
     if (!eq) continue;
     *eq = 0;
+    if (!strcmp(tok, "precision")) {
+      o->mixed = !strcmp(eq + 1, "mixed") ? 1 : (!strcmp(eq + 1, "fp64") ? 0 : -1);
+      continue;
+    }
     long long v = atoll(eq + 1);
     if (!strcmp(tok, "block")) o->block = v;
     else if (!strcmp(tok, "device")) o->device = (int)v;
@@ -203,8 +208,11 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
                      const char *Y_path, const char *B_path, const char *V_path,
                      const char *options) {
   int rc = FGLS_OK;
-  options_t opt = {0, 0, 0, 0};
+  options_t opt = {0, 0, 0, 0, 0};
   parse_options(options ? options : getenv("FGLS_OPTIONS"), &opt);
+  if (opt.mixed < 0) return fgls_set_error(FGLS_ERR_ARG, "precision must be fp64 or mixed");
+  if (opt.mixed && wXR != 1)
+    return fgls_set_error(FGLS_ERR_ARG, "precision=mixed needs wXR = 1 (got %d)", wXR);
 
   if (n < 1 || m < 0 || t < 1 || wXL < 0 || wXR < 1 || p != wXL + wXR || x_b < 1)
     return fgls_set_error(FGLS_ERR_ARG,
@@ -231,6 +239,10 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   double *Phi = NULL, *M = NULL, *h = NULL, *sig = NULL, *XL = NULL, *y = NULL;
   double *hB[2] = {NULL, NULL}, *hV[2] = {NULL, NULL};
   unsigned long long *hErr = NULL;  /* pinned copies of the failure record, one per slot */
+  unsigned int *hBad = NULL;        /* pinned: genotypes of the slot that are not int8-exact */
+  int8_t *dX8[2] = {NULL, NULL};
+  unsigned int *dBad = NULL;
+  const int ld8 = (n + 63) & ~63;
   double *dL = NULL, *dXL = NULL, *dy = NULL, *dX[2] = {NULL, NULL};
   double *dB[2] = {NULL, NULL}, *dV[2] = {NULL, NULL};
   hipStream_t s_copy = NULL, s_comp = NULL;
@@ -275,6 +287,7 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   /* device state */
   HCHK(hipSetDevice(opt.device));
   RCHK(fgls_hip_ctx_create(&ctx, n, wXL, wXR, (int)(block > 0 ? block : 1), opt.device));
+  if (opt.mixed) RCHK(fgls_hip_ctx_set_mode(ctx, FGLS_MODE_MIXED_I8));
   HCHK(hipStreamCreateWithFlags(&s_copy, hipStreamNonBlocking));
   HCHK(hipStreamCreateWithFlags(&s_comp, hipStreamNonBlocking));
   for (int s = 0; s < 2; ++s) {
@@ -288,6 +301,12 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   if (wXL > 0) HCHK(hipMemcpy(dXL, XL, (size_t)n * wXL * sizeof(double), hipMemcpyHostToDevice));
   HCHK(hipHostMalloc((void **)&hErr, 2 * sizeof(unsigned long long), hipHostMallocDefault));
   hErr[0] = hErr[1] = ~0ull;
+  HCHK(hipHostMalloc((void **)&hBad, 2 * sizeof(unsigned int), hipHostMallocDefault));
+  hBad[0] = hBad[1] = 0;
+  if (opt.mixed && nchunks > 0) {
+    HCHK(hipMalloc((void **)&dBad, 2 * sizeof(unsigned int)));
+    for (int s = 0; s < 2; ++s) HCHK(hipMalloc((void **)&dX8[s], (size_t)block * ld8));
+  }
   if (nchunks > 0) {
     const size_t xbytes = (size_t)block * wXR * ldx * sizeof(double);
     for (int s = 0; s < 2; ++s) {
@@ -348,8 +367,18 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
                               (size_t)cnt * wXR, hipMemcpyHostToDevice, s_copy));
         HCHK(hipEventRecord(ev_h2d[s], s_copy));
         HCHK(hipStreamWaitEvent(s_comp, ev_h2d[s], 0));
-        RCHK(fgls_hip_solve_block(ctx, dX[s], ldx, cnt, dB[s], dV[s], (int64_t)j * m + i0, s_comp));
-        HCHK(hipEventRecord(ev_free[s], s_comp));
+        if (opt.mixed) {
+          /* exact FP64 -> int8 on the device; a value that is not an integer
+           * in [-127, 127] is counted and fails the run at write-back */
+          HCHK(hipMemsetAsync(dBad + s, 0, sizeof(unsigned int), s_comp));
+          RCHK(fgls_hip_to_i8(dX[s], ldx, n, cnt, dX8[s], ld8, dBad + s, s_comp));
+          HCHK(hipEventRecord(ev_free[s], s_comp));
+          RCHK(fgls_hip_solve_block_i8(ctx, dX8[s], ld8, cnt, dB[s], dV[s], (int64_t)j * m + i0, s_comp));
+          HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
+        } else {
+          RCHK(fgls_hip_solve_block(ctx, dX[s], ldx, cnt, dB[s], dV[s], (int64_t)j * m + i0, s_comp));
+          HCHK(hipEventRecord(ev_free[s], s_comp));
+        }
         HCHK(hipMemcpyAsync(hB[s], dB[s], (size_t)cnt * p * sizeof(double), hipMemcpyDeviceToHost, s_comp));
         HCHK(hipMemcpyAsync(hV[s], dV[s], (size_t)cnt * pp * sizeof(double), hipMemcpyDeviceToHost, s_comp));
         HCHK(hipMemcpyAsync(&hErr[s], fgls_ctx_err_ptr(ctx), sizeof(unsigned long long),
@@ -365,6 +394,12 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
         const int64_t i0 = (c - 1) * block;
         const int cnt = (int)((m - i0) < block ? (m - i0) : block);
         HCHK(hipEventSynchronize(ev_out[ps]));
+        if (hBad[ps] != 0) {
+          rc = fgls_set_error(FGLS_ERR_ARG,
+                              "precision=mixed: %u genotype values of SNPs %lld.. are not integers in "
+                              "[-127, 127] (use precision=fp64)", hBad[ps], (long long)i0);
+          goto done;
+        }
         if (hErr[ps] != ~0ull) {  /* first failing SNP so far (fgls_chol_gpu.c:664-670) */
           const long long bad = (long long)(hErr[ps] >> 8) - (long long)j * m;
           rc = fgls_set_error(FGLS_ERR_NOTSPD, "dpotrf(V) failed (info: %d) - i: %lld",
@@ -416,6 +451,10 @@ done:
     if (hV[s]) hipHostFree(hV[s]);
   }
   if (hErr) hipHostFree(hErr);
+  if (hBad) hipHostFree(hBad);
+  if (dBad) hipFree(dBad);
+  for (int s = 0; s < 2; ++s)
+    if (dX8[s]) hipFree(dX8[s]);
   if (s_copy) hipStreamDestroy(s_copy);
   if (s_comp) hipStreamDestroy(s_comp);
   if (dL) hipFree(dL);

@@ -53,6 +53,10 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *   serial=1       synchronise after every stage (debug aid, like the
  *                  reference's FGLS_GPU_SERIAL, fgls_chol_gpu.c:44-46)
  *   stats=1        print stage timings and rates to stderr
+ *   precision=mixed  int8 genotypes x six 7-bit slices of L^-1 on the int8
+ *                  MFMA (wXR = 1; every X_R value must be an integer in
+ *                  [-127, 127], else FGLS_ERR_ARG); precision=fp64 (default)
+ *                  is the reference's arithmetic
  * No counterpart in the reference (its GPU entry only exits on error).
  */
 int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR,

@@ -110,3 +110,40 @@ def test_driver_errors(gpu, tmp_path):
     np.asarray([1.0], dtype="<f8").tofile(os.path.join(d, "H.in"))
     res = run_driver(d, stdin)
     assert res.returncode == 1 and "dpotrf(M) failed" in res.stderr
+
+
+@pytest.mark.parametrize("n,m,t,x_b", [(1000, 5000, 1, 256), (333, 517, 2, 100)])
+def test_fgls_chol_gpu_mixed_precision(pkg, gpu, tmp_path, n, m, t, x_b):
+    """precision=mixed through the file pipeline: int8 genotypes on the
+    device, same files out; within the mixed path's tolerance of the oracle
+    (b relative to max(|b|, se) 1e-9, var(b) element-wise 1e-8)."""
+    print(f"fgls_chol_gpu precision=mixed n={n} m={m} t={t}", flush=True)
+    wXL = 3
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=n + m + 7, t=t)
+    d = str(tmp_path)
+    H = np.linspace(0.35, 0.65, t)
+    Sig = np.linspace(0.9, 1.3, t)
+    write_problem_dir(prob, d, H=H, Sig=Sig)
+    p = wXL + 1
+    f = lambda s: os.path.join(d, s)
+    pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, x_b, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                      f("XL.in"), f("XR.in"), f("Y.in"), f("B.out"), f("V.out"),
+                      options={"precision": "mixed", "block": 2 * x_b})
+    Bo, Vo = orc.fgls_chol(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y, wXR=1)
+    B = read_out(d, "B.out", t * m * p)
+    V = read_out(d, "V.out", t * m * p * p)
+    e = orc.parity_errors(B, V, Bo, Vo, p)
+    assert e["B_scaled"] < 1e-9 and e["var_rel"] < 1e-8 and e["V_upper_zero"], e
+
+
+def test_fgls_chol_gpu_mixed_rejects_dosages(pkg, gpu, tmp_path):
+    """Imputed (non-integer) dosages cannot take the int8 path: a defined error."""
+    prob = make_problem(n=200, wXL=3, m=300, seed=4, dosage="imputed")
+    d = str(tmp_path)
+    write_problem_dir(prob, d, H=np.array([0.5]), Sig=np.array([1.0]))
+    f = lambda s: os.path.join(d, s)
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.fgls_chol_gpu(200, 4, 300, 1, 3, 1, 100, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f("XR.in"), f("Y.in"), f("B.out"), f("V.out"),
+                          options={"precision": "mixed"})
+    assert ei.value.code == pkg.FGLS_ERR_ARG and "precision=fp64" in ei.value.msg
