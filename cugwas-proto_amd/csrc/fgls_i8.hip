@@ -37,12 +37,14 @@ constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); }
 // then reads all twelve blocks as MFMA A operands.  The genotype fragments
 // (B operands, 4 x 1 KB per wave) go straight to registers.
 //
-// Step k (reading half k & 1): wait for this wave's loads of step k, barrier
-// (every wave's slices of step k are in LDS and every wave has finished
-// reading the other half in step k - 1), issue the genotypes and slices of
-// step k + 1, then 48 MFMAs.  The loop runs two steps per trip so that the
-// LDS halves and the genotype registers alternate without copies (a copy of
-// a register that a load is filling would wait for that load).
+// Step k (reading half k & 1): wait for this wave's loads of step k, issue
+// its genotype loads of step k + 1, barrier (every wave's slices of step k
+// are in LDS and every wave has finished reading the other half in step
+// k - 1), issue the slices of step k + 1, then 12 LDS fragment reads and 48
+// MFMAs, each read one fragment ahead of the four MFMAs that use it.  The
+// loop runs two steps per trip so that the LDS halves and the genotype
+// registers alternate without copies (a copy of a register that a load is
+// filling would wait for that load).
 // Prefetch addresses are clamped to the last step, so every load is in
 // bounds.  The barrier carries no memory fence: a workgroup-scope release
 // fence would also wait for the LDS loads just issued; the vmcnt wait before
@@ -74,22 +76,32 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i
                                         const int8_t* Xw, const int (&boff)[4], int k, int last,
                                         int w, int lane) {
   __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_sched_barrier(0);
   const int k1 = min(k + 1, last);
 #pragma unroll
   for (int tj = 0; tj < 4; ++tj)
     bn[tj] = *reinterpret_cast<const fgls_v4i*>(Xw + 64 * k1 + boff[tj]);
+  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_sched_barrier(0);
   slice_dma<S>(asrc, (int64_t)k1 * S * 1024, wr, w);
   __builtin_amdgcn_sched_barrier(0);
+  // A fragments one ahead: fragment f = 2 s + ti
+  fgls_v4i af[2];
+  af[0] = rd.blk[0][lane];
+  __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
 #pragma unroll
-  for (int s2 = 0; s2 < S; ++s2) {
+  for (int f = 0; f < 2 * S; ++f) {
+    const int s2 = f >> 1, ti = f & 1;
+    if (f + 1 < 2 * S) af[(f + 1) & 1] = rd.blk[((f + 1) & 1) * S + ((f + 1) >> 1)][lane];
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti) {
-      const fgls_v4i a = rd.blk[ti * S + s2][lane];
+    for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(af[f & 1], b[tj], acc[s2][ti][tj]);
+  }
+  // issue order: one LDS read (the next fragment), then the four MFMAs of
+  // the current one
 #pragma unroll
-      for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(a, b[tj], acc[s2][ti][tj]);
-    }
+  for (int f = 0; f < 2 * S; ++f) {
+    if (f + 1 < 2 * S) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
   }
   __builtin_amdgcn_sched_barrier(0);
 }
@@ -152,12 +164,8 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   }
 
   // z(row r0+16ti+4lg+v, col c0+16tj+li) = inv_scale[row] * sum_s 2^(-7s) acc[s];
-  // one 16-column fragment at a time keeps the FP64 sums to 2 NQ registers
-  double isc[2][4];
-#pragma unroll
-  for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) isc[ti][v] = inv_scale[r0 + 16 * ti + 4 * lg + v];
+  // one 16-column fragment at a time, so that the accumulators of finished
+  // fragments are free and the FP64 sums stay within the register budget
 #pragma unroll
   for (int tj = 0; tj < 4; ++tj) {
     double q[NQ];
@@ -171,7 +179,7 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
         double t = (double)acc[S - 1][ti][tj][v];
 #pragma unroll
         for (int s2 = S - 2; s2 >= 0; --s2) t = fma(t, 1.0 / 128.0, (double)acc[s2][ti][tj][v]);
-        const double z = t * isc[ti][v];
+        const double z = t * inv_scale[row];
         q[0] = fma(z, z, q[0]);
 #pragma unroll
         for (int k = 0; k < NQ - 1; ++k) q[1 + k] = fma(z, W[(size_t)k * ldw + row], q[1 + k]);
