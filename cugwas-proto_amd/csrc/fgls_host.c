@@ -22,6 +22,7 @@
 #include <errno.h>
 #include <fcntl.h>
 #include <math.h>
+#include <omp.h>
 #include <pthread.h>
 #include <sched.h>
 #include <semaphore.h>
@@ -50,6 +51,7 @@ typedef struct {
   int serial;
   int stats;
   int mixed;         /* precision=mixed: int8 genotypes x sliced L^-1 */
+  int io_threads;    /* parallel preads of X_R per block */
 } options_t;
 
 static double now_s(void) {
@@ -83,6 +85,7 @@ static void parse_options(const char *s, options_t *o) {
     else if (!strcmp(tok, "device")) o->device = (int)v;
     else if (!strcmp(tok, "serial")) o->serial = (int)v;
     else if (!strcmp(tok, "stats")) o->stats = (int)v;
+    else if (!strcmp(tok, "io_threads")) o->io_threads = (int)v;
   }
 }
 
@@ -143,6 +146,7 @@ typedef struct {
   int status[2];            /* FGLS_OK or FGLS_ERR_IO per filled slot */
   char err[2][256];
   volatile int stop;
+  int io_threads;           /* parallel preads per slot */
   double busy_s;            /* time spent in pread */
 } reader_t;
 
@@ -159,12 +163,23 @@ static void *reader_main(void *arg) {
     const size_t bytes = (size_t)cnt * R->wXR * col_bytes;
     const off_t off = (off_t)i0 * R->wXR * (off_t)col_bytes;
     double t0 = now_s();
+    /* io_threads contiguous pieces in parallel (page-cache reads are
+     * memory-copy bound per thread) */
     size_t got = 0;
-    while (got < bytes) {
-      ssize_t r = pread(R->fd, (char *)R->slot[s] + got, bytes - got, off + (off_t)got);
-      if (r < 0 && errno == EINTR) continue;
-      if (r <= 0) break;
-      got += (size_t)r;
+    const int T = R->io_threads > 0 ? R->io_threads : 1;
+#pragma omp parallel num_threads(T) reduction(+ : got)
+    {
+      const int tid = omp_get_thread_num();
+      const size_t lo = bytes / (size_t)T * (size_t)tid;
+      const size_t hi = tid == T - 1 ? bytes : bytes / (size_t)T * (size_t)(tid + 1);
+      size_t g = 0;
+      while (lo + g < hi) {
+        ssize_t r = pread(R->fd, (char *)R->slot[s] + lo + g, hi - lo - g, off + (off_t)(lo + g));
+        if (r < 0 && errno == EINTR) continue;
+        if (r <= 0) break;
+        g += (size_t)r;
+      }
+      got += g;
     }
     R->busy_s += now_s() - t0;
     if (got != bytes) {
@@ -208,7 +223,7 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
                      const char *Y_path, const char *B_path, const char *V_path,
                      const char *options) {
   int rc = FGLS_OK;
-  options_t opt = {0, 0, 0, 0, 0};
+  options_t opt = {0, 0, 0, 0, 0, 0};
   parse_options(options ? options : getenv("FGLS_OPTIONS"), &opt);
   if (opt.mixed < 0) return fgls_set_error(FGLS_ERR_ARG, "precision must be fp64 or mixed");
   if (opt.mixed && wXR != 1)
@@ -325,6 +340,7 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
     R.m = m;
     R.block = block;
     R.nchunks = nchunks;
+    R.io_threads = opt.io_threads > 0 ? opt.io_threads : 4;
     for (int s = 0; s < 2; ++s) {
       sem_init(&R.free_[s], 0, 1);
       sem_init(&R.full[s], 0, 0);
