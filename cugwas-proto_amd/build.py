@@ -68,9 +68,12 @@ def build(verbose: bool = False) -> dict:
     run([hipcc(), *HIP_FLAGS, "-mllvm", "-amdgpu-mfma-vgpr-form=1", "-I", INCLUDE, "-c",
          os.path.join(CSRC, "fgls_i8.hip"), "-o", i8obj], verbose)
 
-    adjobj = os.path.join(objdir, "fgls_adjoint.o")
-    run([hipcc(), *HIP_FLAGS, "-I", INCLUDE, "-c", os.path.join(CSRC, "fgls_adjoint.hip"), "-o", adjobj],
-        verbose)
+    extra_objs = []
+    for name in ("fgls_adjoint", "fgls_potrf"):
+        obj = os.path.join(objdir, name + ".o")
+        run([hipcc(), *HIP_FLAGS, "-I", INCLUDE, "-c", os.path.join(CSRC, name + ".hip"), "-o", obj],
+            verbose)
+        extra_objs.append(obj)
 
     host_objs = []
     for src in sorted(glob.glob(os.path.join(CSRC, "*.c"))):
@@ -84,7 +87,7 @@ def build(verbose: bool = False) -> dict:
 
     so = os.path.join(LIBDIR, "libfgls_amd.so")
     tmp = so + ".tmp"
-    run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, kobj, i8obj, adjobj, *host_objs,
+    run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, kobj, i8obj, *extra_objs, *host_objs,
          "-L", blasdir, f"-l:{blasname}", f"-Wl,-rpath,{blasdir}", "-fopenmp", "-lpthread", "-lrt"],
         verbose)
     os.replace(tmp, so)

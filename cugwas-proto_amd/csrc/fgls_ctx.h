@@ -34,10 +34,12 @@ struct fgls_hip_ctx {
   double* Linv;
   double* Dinv;
   double* W;
+  double* Wtmp;   // [X_L | y] of the trait, zero padded (input of W = L^-1 Wtmp)
   double* trait;
   double* part;
   double* z;      // lazily allocated (generic per-SNP path, adjoint)
   fgls_adjoint_state* adj;  // lazily allocated (adjoint)
+  double* panel;  // lazily allocated (device Cholesky of M): (ldn + 128) x 64 row-major
   unsigned long long* err;
   int* status;
 };
@@ -87,6 +89,12 @@ void launch_product_dot(const fgls_hip_ctx* c, const double* A, const double* X,
 // b and V of every SNP of a block from its transformed columns Z.
 void launch_snp_stage_any(const fgls_hip_ctx* c, const double* Z, int64_t ldz, int count,
                           double* dB, double* dV, int64_t base, hipStream_t st);
+
+// The per-trait precompute from the factor already in ctx->L (L^-1, W,
+// B_t, V_tl, res_sigma, slices); synchronizes `st`.  With potrf_info set,
+// ctx->status[1] holds the device factorisation's LAPACK-style info.
+int finish_trait(fgls_hip_ctx* c, const double* dXL, int64_t ldxl, const double* dy, hipStream_t st,
+                 const int* potrf_info);
 
 }  // namespace fgls
 

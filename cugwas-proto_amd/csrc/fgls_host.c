@@ -52,6 +52,7 @@ typedef struct {
   int stats;
   int mixed;         /* precision=mixed: int8 genotypes x sliced L^-1 */
   int io_threads;    /* parallel preads of X_R per block */
+  int host_potrf;    /* potrf=host: M and its factor on the host (OpenBLAS) */
 } options_t;
 
 static double now_s(void) {
@@ -76,6 +77,10 @@ static void parse_options(const char *s, options_t *o) {
     char *eq = strchr(tok, '=');
     if (!eq) continue;
     *eq = 0;
+    if (!strcmp(tok, "potrf")) {
+      o->host_potrf = !strcmp(eq + 1, "host") ? 1 : (!strcmp(eq + 1, "gpu") ? 0 : -1);
+      continue;
+    }
     if (!strcmp(tok, "precision")) {
       o->mixed = !strcmp(eq + 1, "mixed") ? 1 : (!strcmp(eq + 1, "fp64") ? 0 : -1);
       continue;
@@ -223,9 +228,10 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
                      const char *Y_path, const char *B_path, const char *V_path,
                      const char *options) {
   int rc = FGLS_OK;
-  options_t opt = {0, 0, 0, 0, 0, 0};
+  options_t opt = {0, 0, 0, 0, 0, 0, 0};
   parse_options(options ? options : getenv("FGLS_OPTIONS"), &opt);
   if (opt.mixed < 0) return fgls_set_error(FGLS_ERR_ARG, "precision must be fp64 or mixed");
+  if (opt.host_potrf < 0) return fgls_set_error(FGLS_ERR_ARG, "potrf must be gpu or host");
   if (opt.mixed && wXR != 1)
     return fgls_set_error(FGLS_ERR_ARG, "precision=mixed needs wXR = 1 (got %d)", wXR);
 
@@ -273,7 +279,7 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
 
   const size_t nn = (size_t)n * n;
   Phi = (double *)malloc(nn * sizeof(double));
-  M = (double *)malloc(nn * sizeof(double));
+  M = opt.host_potrf ? (double *)malloc(nn * sizeof(double)) : Phi;
   h = (double *)calloc((size_t)t, sizeof(double));
   sig = (double *)calloc((size_t)t, sizeof(double));
   XL = (double *)calloc((size_t)n * (wXL > 0 ? wXL : 1), sizeof(double));
@@ -310,7 +316,10 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
     HCHK(hipEventCreateWithFlags(&ev_out[s], hipEventDisableTiming));
     HCHK(hipEventCreateWithFlags(&ev_free[s], hipEventDisableTiming));
   }
+  /* device potrf: Phi is uploaded once and every trait's M is built and
+   * factored on the device; host potrf: each trait's factor is uploaded */
   HCHK(hipMalloc((void **)&dL, nn * sizeof(double)));
+  if (!opt.host_potrf) HCHK(hipMemcpy(dL, Phi, nn * sizeof(double), hipMemcpyHostToDevice));
   HCHK(hipMalloc((void **)&dXL, (size_t)n * (wXL > 0 ? wXL : 1) * sizeof(double)));
   HCHK(hipMalloc((void **)&dy, (size_t)n * sizeof(double)));
   if (wXL > 0) HCHK(hipMemcpy(dXL, XL, (size_t)n * wXL * sizeof(double), hipMemcpyHostToDevice));
@@ -356,13 +365,19 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   int64_t it = 0;  /* global chunk counter across traits (reader order) */
   for (int j = 0; j < t; ++j) {
     double t0 = now_s();
-    if ((rc = fgls_factor_M(n, Phi, n, h[j], sig[j], M, n, threads)) != FGLS_OK) goto done;
-    t_potrf += now_s() - t0;
-    t0 = now_s();
-    RCHK(read_exact(Y_path, y, (size_t)n, (size_t)j * n));
-    HCHK(hipMemcpyAsync(dL, M, nn * sizeof(double), hipMemcpyHostToDevice, s_comp));
-    HCHK(hipMemcpyAsync(dy, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s_comp));
-    RCHK(fgls_hip_set_trait(ctx, dL, n, dXL, n, dy, s_comp));
+    if (opt.host_potrf) {
+      if ((rc = fgls_factor_M(n, Phi, n, h[j], sig[j], M, n, threads)) != FGLS_OK) goto done;
+      t_potrf += now_s() - t0;
+      t0 = now_s();
+      RCHK(read_exact(Y_path, y, (size_t)n, (size_t)j * n));
+      HCHK(hipMemcpyAsync(dL, M, nn * sizeof(double), hipMemcpyHostToDevice, s_comp));
+      HCHK(hipMemcpyAsync(dy, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s_comp));
+      RCHK(fgls_hip_set_trait(ctx, dL, n, dXL, n, dy, s_comp));
+    } else {
+      RCHK(read_exact(Y_path, y, (size_t)n, (size_t)j * n));
+      HCHK(hipMemcpyAsync(dy, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s_comp));
+      RCHK(fgls_hip_set_trait_phi(ctx, dL, n, h[j], sig[j], dXL, n, dy, s_comp));
+    }
     t_trait += now_s() - t0;
 
     double tl0 = now_s();
@@ -435,7 +450,7 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   if (opt.stats) {
     const double snps = (double)m * t;
     fprintf(stderr,
-            "[fgls] n=%d p=%d m=%d t=%d block=%lld: potrf %.3fs, trait precompute %.3fs, "
+            "[fgls] n=%d p=%d m=%d t=%d block=%lld: host potrf %.3fs, trait setup %.3fs, "
             "block loop %.3fs (%.0f SNPs/s, %.2f GB/s X_R), reader busy %.3fs, threads %d\n",
             n, p, m, t, (long long)block, t_potrf, t_trait, t_loop, t_loop > 0 ? snps / t_loop : 0.0,
             t_loop > 0 ? snps * wXR * n * 8.0 / t_loop / 1e9 : 0.0, R.busy_s, threads);
@@ -481,7 +496,7 @@ done:
   if (fdB >= 0) close(fdB);
   if (fdV >= 0) close(fdV);
   free(Phi);
-  free(M);
+  if (M != Phi) free(M);
   free(h);
   free(sig);
   free(XL);

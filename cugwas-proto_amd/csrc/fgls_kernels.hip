@@ -866,6 +866,40 @@ void launch_snp_stage_any(const fgls_hip_ctx* c, const double* Z, int64_t ldz, i
   FGLS_P_SWITCH(c->p, launch_snp_stage, c, Z, ldz, count, dB, dV, base, st);
 }
 
+int finish_trait(fgls_hip_ctx* c, const double* dXL, int64_t ldxl, const double* dy, hipStream_t st,
+                 const int* potrf_info) {
+  int rc;
+  const int n = c->n, wXL = c->wXL;
+  const size_t ldn = (size_t)c->ldn;
+  fgls_adjoint_invalidate(c->adj);
+  c->have_trait = false;
+  if (wXL > 0)
+    HIP_TRY(hipMemcpy2DAsync(c->Wtmp, ldn * sizeof(double), dXL, (size_t)ldxl * sizeof(double),
+                             (size_t)n * sizeof(double), (size_t)wXL, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(c->Wtmp + (size_t)wXL * ldn, dy, (size_t)n * sizeof(double),
+                         hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_diag_inv, dim3(c->nblk), dim3(64), 0, st, n, c->L, (int64_t)ldn, c->Dinv);
+  hipLaunchKernelGGL((k_trsm_panel<true, true>), dim3(c->nblk), dim3(256), 0, st, n, n, c->L,
+                     (int64_t)ldn, c->Dinv, (const double*)nullptr, (int64_t)0, c->Linv,
+                     (int64_t)ldn);
+  // [X~_L | y~] = L^-1 [X_L | y] through the explicit inverse: wXL + 1 columns
+  // of the triangular product spread over all row tiles (a TRSM would run
+  // the n rows one after another in a single workgroup)
+  launch_product_store(c, c->Linv, false, c->Wtmp, (int64_t)ldn, wXL + 1, c->W, (int64_t)ldn, st);
+  hipLaunchKernelGGL(k_trait_stats, dim3(1), dim3(256), 0, st, n, wXL, c->W, (int64_t)ldn, c->trait,
+                     c->status);
+  HIP_TRY(hipGetLastError());
+  if (c->mode == FGLS_MODE_MIXED_I8 && (rc = build_slices(c, st))) return rc;
+  int status[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(status, c->status, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (potrf_info && status[1] != 0)
+    return fail(FGLS_ERR_NOTSPD, "dpotrf(M) failed (info: %d)", status[1]);
+  if (status[0] != 0) return fail(FGLS_ERR_NOTSPD, "dpotrf(scoreV) failed (info: %d)", status[0]);
+  c->have_trait = true;
+  return FGLS_OK;
+}
+
 }  // namespace fgls
 
 extern "C" {
@@ -918,6 +952,7 @@ int fgls_hip_ctx_create(fgls_hip_ctx** out, int n, int wXL, int wXR, int max_blo
       {(void**)&c->Linv, ldn * ldn * sizeof(double)},
       {(void**)&c->Dinv, (size_t)c->nblk * NB * NB * sizeof(double)},
       {(void**)&c->W, ldn * (size_t)(wXL + 1) * sizeof(double)},
+      {(void**)&c->Wtmp, ldn * (size_t)(wXL + 1) * sizeof(double)},
       {(void**)&c->trait, (size_t)(2 * PMAX + PMAX * PMAX + 8) * sizeof(double)},
       {(void**)&c->part, (size_t)c->RT * nq * (size_t)c->max_block * sizeof(double)},
       {(void**)&c->err, sizeof(unsigned long long)},
@@ -944,8 +979,8 @@ int fgls_hip_ctx_destroy(fgls_hip_ctx* c) {
   if (!c) return FGLS_OK;
   hipSetDevice(c->device);
   hipDeviceSynchronize();
-  void* ptrs[] = {c->L, c->Linv, c->Dinv, c->W, c->trait, c->part, c->z, c->err, c->status,
-                  c->A8, c->invsc, c->bad};
+  void* ptrs[] = {c->L, c->Linv, c->Dinv, c->W, c->Wtmp, c->trait, c->part, c->z, c->err, c->status,
+                  c->A8, c->invsc, c->bad, c->panel};
   for (void* p : ptrs)
     if (p) hipFree(p);
   fgls_adjoint_free(c->adj);
@@ -965,36 +1000,10 @@ int fgls_hip_set_trait(fgls_hip_ctx* c, const double* dL, int64_t ldl, const dou
   if (ldl < c->n || (c->wXL > 0 && ldxl < c->n)) return fail(FGLS_ERR_ARG, "leading dimension below n");
   if ((rc = set_device(c))) return rc;
   hipStream_t st = (hipStream_t)stream;
-  const int n = c->n, wXL = c->wXL;
   const size_t ldn = (size_t)c->ldn;
-  fgls_adjoint_invalidate(c->adj);
   HIP_TRY(hipMemcpy2DAsync(c->L, ldn * sizeof(double), dL, (size_t)ldl * sizeof(double),
-                           (size_t)n * sizeof(double), (size_t)n, hipMemcpyDeviceToDevice, st));
-  if (wXL > 0)
-    HIP_TRY(hipMemcpy2DAsync(c->W, ldn * sizeof(double), dXL, (size_t)ldxl * sizeof(double),
-                             (size_t)n * sizeof(double), (size_t)wXL, hipMemcpyDeviceToDevice, st));
-  HIP_TRY(hipMemcpyAsync(c->W + (size_t)wXL * ldn, dy, (size_t)n * sizeof(double),
-                         hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(k_diag_inv, dim3(c->nblk), dim3(64), 0, st, n, c->L, (int64_t)ldn, c->Dinv);
-  hipLaunchKernelGGL((k_trsm_panel<true, true>), dim3(c->nblk), dim3(256), 0, st, n, n, c->L,
-                     (int64_t)ldn, c->Dinv, (const double*)nullptr, (int64_t)0, c->Linv,
-                     (int64_t)ldn);
-  hipLaunchKernelGGL((k_trsm_panel<false, false>), dim3((wXL + 1 + NB - 1) / NB), dim3(256), 0, st,
-                     n, wXL + 1, c->L, (int64_t)ldn, c->Dinv, c->W, (int64_t)ldn, c->W,
-                     (int64_t)ldn);
-  hipLaunchKernelGGL(k_trait_stats, dim3(1), dim3(256), 0, st, n, wXL, c->W, (int64_t)ldn, c->trait,
-                     c->status);
-  HIP_TRY(hipGetLastError());
-  if (c->mode == FGLS_MODE_MIXED_I8 && (rc = build_slices(c, st))) return rc;
-  int status = 0;
-  HIP_TRY(hipMemcpyAsync(&status, c->status, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  if (status != 0) {
-    c->have_trait = false;
-    return fail(FGLS_ERR_NOTSPD, "dpotrf(scoreV) failed (info: %d)", status);
-  }
-  c->have_trait = true;
-  return FGLS_OK;
+                           (size_t)c->n * sizeof(double), (size_t)c->n, hipMemcpyDeviceToDevice, st));
+  return fgls::finish_trait(c, dXL, ldxl, dy, st, nullptr);
 }
 
 int fgls_hip_get_trait(fgls_hip_ctx* c, double* hBt, double* hVtl, double* hres) {

@@ -72,6 +72,8 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_trmm": (i32, [vp, dp, i64, i32, dp, i64, vp]),
             "fgls_hip_snp_stage": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
             "fgls_hip_check": (i32, [vp, vp, ctypes.POINTER(i64), ctypes.POINTER(i32)]),
+            "fgls_hip_set_trait_phi": (i32, [vp, dp, i64, ctypes.c_double, ctypes.c_double, dp, i64, dp, vp]),
+            "fgls_hip_get_factor": (i32, [vp, dp, i64, vp]),
             "fgls_hip_adjoint_begin": (i32, [vp, dp, i64, ctypes.c_double, vp]),
             "fgls_hip_adjoint_block": (i32, [vp, dp, i64, i32, dp, i64, vp]),
             "fgls_hip_adjoint_end": (i32, [vp, dp, ctypes.POINTER(ctypes.c_double),
@@ -98,6 +100,7 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_solve_block_i8", "fgls_hip_to_i8", "fgls_hip_trsm",
                         "fgls_hip_trmm", "fgls_hip_snp_stage", "fgls_hip_check",
                         "fgls_hip_adjoint_begin", "fgls_hip_adjoint_block", "fgls_hip_adjoint_end",
+                        "fgls_hip_set_trait_phi", "fgls_hip_get_factor",
                         "fgls_factor_M")
             if hasattr(lib(), s)]
 
@@ -189,6 +192,26 @@ class DeviceContext:
             ldxl = XL.shape[1]
         _check(lib().fgls_hip_set_trait(self._h, _ptr(L), L.shape[1], _ptr(XL) if self.wXL else 0,
                                         ldxl, _ptr(y), _stream(stream)))
+
+    def set_trait_phi(self, Phi, h2: float, sigma2: float, XL, y, stream=None):
+        """Build M = sigma2 (h2 Phi + (1 - h2) I) and factor it on the device, then
+        the per-trait precompute.  Phi: (n, ldphi) column-major (symmetric)."""
+        _need_cuda_f64(Phi, "Phi")
+        _need_cuda_f64(y, "y")
+        ldxl = self.n
+        if self.wXL > 0:
+            _need_cuda_f64(XL, "XL")
+            ldxl = XL.shape[1]
+        _check(lib().fgls_hip_set_trait_phi(self._h, _ptr(Phi), Phi.shape[1], float(h2), float(sigma2),
+                                            _ptr(XL) if self.wXL else 0, ldxl, _ptr(y), _stream(stream)))
+
+    def factor(self, stream=None):
+        """The current trait's factor L as an (n, n) tensor holding L column-major
+        (row c = column c; entries above the diagonal are unspecified)."""
+        import torch
+        L = torch.empty((self.n, self.n), dtype=torch.float64, device="cuda")
+        _check(lib().fgls_hip_get_factor(self._h, _ptr(L), self.n, _stream(stream)))
+        return L
 
     def trait(self):
         import numpy as np

@@ -54,6 +54,10 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *                  reference's FGLS_GPU_SERIAL, fgls_chol_gpu.c:44-46)
  *   stats=1        print stage timings and rates to stderr
  *   io_threads=<k> parallel preads of each X_R block (default 4)
+ *   potrf=host     build M and its Cholesky factor on the host (OpenBLAS,
+ *                  as the reference) and upload the factor per trait; the
+ *                  default potrf=gpu uploads Phi once and builds and factors
+ *                  M on the device (fgls_hip_set_trait_phi)
  *   precision=mixed  int8 genotypes x six 7-bit slices of L^-1 on the int8
  *                  MFMA (wXR = 1; every X_R value must be an integer in
  *                  [-127, 127], else FGLS_ERR_ARG); precision=fp64 (default)

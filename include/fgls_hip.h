@@ -170,6 +170,25 @@ int fgls_hip_check(fgls_hip_ctx *ctx, void *stream, int64_t *bad_snp,
                    int *info);
 
 /*
+ * Per-trait precompute from Phi instead of a host factor: builds
+ * M = sigma2 (h2 Phi + (1 - h2) I) and its lower Cholesky factor on the
+ * device (FP64, blocked, 64-column panels) and then does what
+ * fgls_hip_set_trait does.  Replaces the host M construction and dpotrf of
+ * fgls_chol_gpu.c:406-430 and the L upload of fgls_chol_gpu.c:434-445.
+ * dPhi: device, column-major, ld >= n (only the lower triangle is read).
+ * A non-SPD M gives FGLS_ERR_NOTSPD with "dpotrf(M) failed (info: i)",
+ * i the order of the first non-positive leading minor as LAPACK reports it.
+ * Synchronizes `stream`.
+ */
+int fgls_hip_set_trait_phi(fgls_hip_ctx *ctx, const double *dPhi, int64_t ldphi,
+                           double h2, double sigma2, const double *dXL,
+                           int64_t ldxl, const double *dy, void *stream);
+
+/* Copy of the current trait's factor L (n x n, column-major, ld >= n; the
+ * lower triangle is the factor, the strict upper triangle is unspecified). */
+int fgls_hip_get_factor(fgls_hip_ctx *ctx, double *dL, int64_t ldl, void *stream);
+
+/*
  * Adjoint of the loss l = sum_i w_i beta_i^2 over the SNPs of the current
  * trait, beta_i = b_i[p-1] (the SNP coefficient of the per-SNP solve,
  * fgls_chol.c:386-439), with respect to y and h^2, where

@@ -147,3 +147,26 @@ def test_fgls_chol_gpu_mixed_rejects_dosages(pkg, gpu, tmp_path):
                           f("XL.in"), f("XR.in"), f("Y.in"), f("B.out"), f("V.out"),
                           options={"precision": "mixed"})
     assert ei.value.code == pkg.FGLS_ERR_ARG and "precision=fp64" in ei.value.msg
+
+
+def test_fgls_chol_gpu_host_and_device_factor_agree(pkg, gpu, tmp_path):
+    """potrf=host (the reference's host dpotrf) and the default device
+    factorisation give the same files to the parity bar."""
+    n, m, t, wXL = 500, 700, 2, 3
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=77, t=t)
+    d = str(tmp_path)
+    H, Sig = np.array([0.3, 0.65]), np.array([1.0, 2.0])
+    write_problem_dir(prob, d, H=H, Sig=Sig)
+    p = wXL + 1
+    f = lambda s: os.path.join(d, s)
+    out = {}
+    for mode in ("gpu", "host"):
+        pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, 128, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f("XR.in"), f("Y.in"), f(f"B_{mode}.out"), f(f"V_{mode}.out"),
+                          options={"potrf": mode})
+        out[mode] = (read_out(d, f"B_{mode}.out", t * m * p), read_out(d, f"V_{mode}.out", t * m * p * p))
+    e = orc.parity_errors(out["gpu"][0], out["gpu"][1], out["host"][0], out["host"][1], p)
+    assert e["B_scaled"] < 1e-11 and e["V_scaled"] < 1e-11 and e["V_upper_zero"], e
+    Bo, Vo = orc.fgls_chol(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y, wXR=1)
+    e = orc.parity_errors(out["gpu"][0], out["gpu"][1], Bo, Vo, p)
+    assert e["B_scaled"] < 1e-10 and e["V_scaled"] < 1e-10, e
