@@ -53,6 +53,7 @@ typedef struct {
   int mixed;         /* precision=mixed: int8 genotypes x sliced L^-1 */
   int io_threads;    /* parallel preads of X_R per block */
   int host_potrf;    /* potrf=host: M and its factor on the host (OpenBLAS) */
+  int traits_per_pass;  /* traits sharing one read of X_R (0: as many as fit) */
 } options_t;
 
 static double now_s(void) {
@@ -91,6 +92,7 @@ static void parse_options(const char *s, options_t *o) {
     else if (!strcmp(tok, "serial")) o->serial = (int)v;
     else if (!strcmp(tok, "stats")) o->stats = (int)v;
     else if (!strcmp(tok, "io_threads")) o->io_threads = (int)v;
+    else if (!strcmp(tok, "traits_per_pass")) o->traits_per_pass = (int)v;
   }
 }
 
@@ -222,13 +224,24 @@ int fgls_factor_M(int n, const double *Phi, int64_t ldphi, double h2, double sig
   return FGLS_OK;
 }
 
+/* Device bytes of one trait's context beyond the shared buffers. */
+static size_t ctx_bytes(int n, int p, int64_t block, int mixed) {
+  const size_t ldn = ((size_t)n + 127) / 128 * 128;
+  const size_t rt = mixed ? ldn / 32 : ldn / 128;
+  size_t b = 2 * ldn * ldn * sizeof(double);                    /* L, L^-1 */
+  b += rt * (size_t)(p + 1) * (size_t)block * sizeof(double);    /* partial sums */
+  b += (size_t)block * ((size_t)p + (size_t)p * p) * sizeof(double) * 2;   /* B, V slots */
+  if (mixed) b += ldn * ldn * 6;                                 /* int8 slices */
+  return b + (64u << 20);
+}
+
 int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int y_b,
                      int num_threads, const char *Phi_path, const char *h2_path,
                      const char *sigma2_path, const char *XL_path, const char *XR_path,
                      const char *Y_path, const char *B_path, const char *V_path,
                      const char *options) {
   int rc = FGLS_OK;
-  options_t opt = {0, 0, 0, 0, 0, 0, 0};
+  options_t opt = {0, 0, 0, 0, 0, 0, 0, 0};
   parse_options(options ? options : getenv("FGLS_OPTIONS"), &opt);
   if (opt.mixed < 0) return fgls_set_error(FGLS_ERR_ARG, "precision must be fp64 or mixed");
   if (opt.host_potrf < 0) return fgls_set_error(FGLS_ERR_ARG, "potrf must be gpu or host");
@@ -258,17 +271,18 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   const int threads = num_threads > 0 ? num_threads : host_cores();
 
   double *Phi = NULL, *M = NULL, *h = NULL, *sig = NULL, *XL = NULL, *y = NULL;
-  double *hB[2] = {NULL, NULL}, *hV[2] = {NULL, NULL};
-  unsigned long long *hErr = NULL;  /* pinned copies of the failure record, one per slot */
+  /* per slot s and trait k of a pass: index s * K + k */
+  double **hB = NULL, **hV = NULL, **dB = NULL, **dV = NULL;
+  unsigned long long *hErr = NULL;  /* pinned copies of the failure records */
   unsigned int *hBad = NULL;        /* pinned: genotypes of the slot that are not int8-exact */
   int8_t *dX8[2] = {NULL, NULL};
   unsigned int *dBad = NULL;
   const int ld8 = (n + 63) & ~63;
   double *dL = NULL, *dXL = NULL, *dy = NULL, *dX[2] = {NULL, NULL};
-  double *dB[2] = {NULL, NULL}, *dV[2] = {NULL, NULL};
   hipStream_t s_copy = NULL, s_comp = NULL;
   hipEvent_t ev_h2d[2] = {NULL, NULL}, ev_out[2] = {NULL, NULL}, ev_free[2] = {NULL, NULL};
-  fgls_hip_ctx *ctx = NULL;
+  fgls_hip_ctx **ctx = NULL;
+  int K = 1;                        /* traits per pass over X_R */
   reader_t R;
   memset(&R, 0, sizeof(R));
   R.fd = -1;
@@ -305,10 +319,32 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
     goto done;
   }
 
-  /* device state */
+  /* device state: K trait contexts share each X_R block (one read per pass);
+   * K is all t traits when they fit in 60 % of the free device memory */
   HCHK(hipSetDevice(opt.device));
-  RCHK(fgls_hip_ctx_create(&ctx, n, wXL, wXR, (int)(block > 0 ? block : 1), opt.device));
-  if (opt.mixed) RCHK(fgls_hip_ctx_set_mode(ctx, FGLS_MODE_MIXED_I8));
+  {
+    size_t free_b = 0, total_b = 0;
+    HCHK(hipMemGetInfo(&free_b, &total_b));
+    const size_t per = ctx_bytes(n, p, block, opt.mixed);
+    const size_t shared = (size_t)block * wXR * ldx * sizeof(double) * 2 + nn * sizeof(double);
+    long long fit = free_b > shared ? (long long)((free_b - shared) * 0.6 / per) : 1;
+    if (fit < 1) fit = 1;
+    K = opt.traits_per_pass > 0 ? opt.traits_per_pass : (int)(fit < t ? fit : t);
+    if (K > t) K = t;
+  }
+  ctx = (fgls_hip_ctx **)calloc((size_t)K, sizeof(*ctx));
+  hB = (double **)calloc((size_t)2 * K, sizeof(double *));
+  hV = (double **)calloc((size_t)2 * K, sizeof(double *));
+  dB = (double **)calloc((size_t)2 * K, sizeof(double *));
+  dV = (double **)calloc((size_t)2 * K, sizeof(double *));
+  if (!ctx || !hB || !hV || !dB || !dV) {
+    rc = fgls_set_error(FGLS_ERR_NOMEM, "host allocation failed");
+    goto done;
+  }
+  for (int k = 0; k < K; ++k) {
+    RCHK(fgls_hip_ctx_create(&ctx[k], n, wXL, wXR, (int)(block > 0 ? block : 1), opt.device));
+    if (opt.mixed) RCHK(fgls_hip_ctx_set_mode(ctx[k], FGLS_MODE_MIXED_I8));
+  }
   HCHK(hipStreamCreateWithFlags(&s_copy, hipStreamNonBlocking));
   HCHK(hipStreamCreateWithFlags(&s_comp, hipStreamNonBlocking));
   for (int s = 0; s < 2; ++s) {
@@ -323,8 +359,8 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   HCHK(hipMalloc((void **)&dXL, (size_t)n * (wXL > 0 ? wXL : 1) * sizeof(double)));
   HCHK(hipMalloc((void **)&dy, (size_t)n * sizeof(double)));
   if (wXL > 0) HCHK(hipMemcpy(dXL, XL, (size_t)n * wXL * sizeof(double), hipMemcpyHostToDevice));
-  HCHK(hipHostMalloc((void **)&hErr, 2 * sizeof(unsigned long long), hipHostMallocDefault));
-  hErr[0] = hErr[1] = ~0ull;
+  HCHK(hipHostMalloc((void **)&hErr, (size_t)2 * K * sizeof(unsigned long long), hipHostMallocDefault));
+  for (int i = 0; i < 2 * K; ++i) hErr[i] = ~0ull;
   HCHK(hipHostMalloc((void **)&hBad, 2 * sizeof(unsigned int), hipHostMallocDefault));
   hBad[0] = hBad[1] = 0;
   if (opt.mixed && nchunks > 0) {
@@ -336,16 +372,19 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
     for (int s = 0; s < 2; ++s) {
       HCHK(hipMalloc((void **)&dX[s], xbytes));
       HCHK(hipMemset(dX[s], 0, xbytes));
-      HCHK(hipMalloc((void **)&dB[s], (size_t)block * p * sizeof(double)));
-      HCHK(hipMalloc((void **)&dV[s], (size_t)block * pp * sizeof(double)));
       HCHK(hipHostMalloc((void **)&R.slot[s], (size_t)block * wXR * n * sizeof(double), hipHostMallocDefault));
-      HCHK(hipHostMalloc((void **)&hB[s], (size_t)block * p * sizeof(double), hipHostMallocDefault));
-      HCHK(hipHostMalloc((void **)&hV[s], (size_t)block * pp * sizeof(double), hipHostMallocDefault));
+      for (int k = 0; k < K; ++k) {
+        const int q = s * K + k;
+        HCHK(hipMalloc((void **)&dB[q], (size_t)block * p * sizeof(double)));
+        HCHK(hipMalloc((void **)&dV[q], (size_t)block * pp * sizeof(double)));
+        HCHK(hipHostMalloc((void **)&hB[q], (size_t)block * p * sizeof(double), hipHostMallocDefault));
+        HCHK(hipHostMalloc((void **)&hV[q], (size_t)block * pp * sizeof(double), hipHostMallocDefault));
+      }
     }
     R.path = XR_path;
     R.n = n;
     R.wXR = wXR;
-    R.t = t;
+    R.t = (t + K - 1) / K;   /* passes over X_R */
     R.m = m;
     R.block = block;
     R.nchunks = nchunks;
@@ -362,23 +401,27 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   }
   scipy_openblas_set_num_threads(threads);
 
-  int64_t it = 0;  /* global chunk counter across traits (reader order) */
-  for (int j = 0; j < t; ++j) {
-    double t0 = now_s();
-    if (opt.host_potrf) {
-      if ((rc = fgls_factor_M(n, Phi, n, h[j], sig[j], M, n, threads)) != FGLS_OK) goto done;
-      t_potrf += now_s() - t0;
-      t0 = now_s();
-      RCHK(read_exact(Y_path, y, (size_t)n, (size_t)j * n));
-      HCHK(hipMemcpyAsync(dL, M, nn * sizeof(double), hipMemcpyHostToDevice, s_comp));
-      HCHK(hipMemcpyAsync(dy, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s_comp));
-      RCHK(fgls_hip_set_trait(ctx, dL, n, dXL, n, dy, s_comp));
-    } else {
-      RCHK(read_exact(Y_path, y, (size_t)n, (size_t)j * n));
-      HCHK(hipMemcpyAsync(dy, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s_comp));
-      RCHK(fgls_hip_set_trait_phi(ctx, dL, n, h[j], sig[j], dXL, n, dy, s_comp));
+  int64_t it = 0;  /* global chunk counter across passes (reader order) */
+  for (int j0 = 0; j0 < t; j0 += K) {
+    const int kn = (t - j0) < K ? (t - j0) : K;   /* traits of this pass */
+    for (int k = 0; k < kn; ++k) {
+      const int j = j0 + k;
+      double t0 = now_s();
+      if (opt.host_potrf) {
+        if ((rc = fgls_factor_M(n, Phi, n, h[j], sig[j], M, n, threads)) != FGLS_OK) goto done;
+        t_potrf += now_s() - t0;
+        t0 = now_s();
+        RCHK(read_exact(Y_path, y, (size_t)n, (size_t)j * n));
+        HCHK(hipMemcpyAsync(dL, M, nn * sizeof(double), hipMemcpyHostToDevice, s_comp));
+        HCHK(hipMemcpyAsync(dy, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s_comp));
+        RCHK(fgls_hip_set_trait(ctx[k], dL, n, dXL, n, dy, s_comp));
+      } else {
+        RCHK(read_exact(Y_path, y, (size_t)n, (size_t)j * n));
+        HCHK(hipMemcpyAsync(dy, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s_comp));
+        RCHK(fgls_hip_set_trait_phi(ctx[k], dL, n, h[j], sig[j], dXL, n, dy, s_comp));
+      }
+      t_trait += now_s() - t0;
     }
-    t_trait += now_s() - t0;
 
     double tl0 = now_s();
     for (int64_t c = 0; c <= nchunks; ++c, ++it) {
@@ -391,7 +434,7 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
           rc = fgls_set_error(FGLS_ERR_IO, "%s", R.err[s]);
           goto done;
         }
-        /* device X_R slot s is free once the solve of chunk c-2 has run */
+        /* device X_R slot s is free once the solves of chunk c-2 have run */
         HCHK(hipStreamWaitEvent(s_copy, ev_free[s], 0));
         HCHK(hipMemcpy2DAsync(dX[s], (size_t)ldx * sizeof(double), R.slot[s],
                               (size_t)n * sizeof(double), (size_t)n * sizeof(double),
@@ -404,16 +447,21 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
           HCHK(hipMemsetAsync(dBad + s, 0, sizeof(unsigned int), s_comp));
           RCHK(fgls_hip_to_i8(dX[s], ldx, n, cnt, dX8[s], ld8, dBad + s, s_comp));
           HCHK(hipEventRecord(ev_free[s], s_comp));
-          RCHK(fgls_hip_solve_block_i8(ctx, dX8[s], ld8, cnt, dB[s], dV[s], (int64_t)j * m + i0, s_comp));
           HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
-        } else {
-          RCHK(fgls_hip_solve_block(ctx, dX[s], ldx, cnt, dB[s], dV[s], (int64_t)j * m + i0, s_comp));
-          HCHK(hipEventRecord(ev_free[s], s_comp));
         }
-        HCHK(hipMemcpyAsync(hB[s], dB[s], (size_t)cnt * p * sizeof(double), hipMemcpyDeviceToHost, s_comp));
-        HCHK(hipMemcpyAsync(hV[s], dV[s], (size_t)cnt * pp * sizeof(double), hipMemcpyDeviceToHost, s_comp));
-        HCHK(hipMemcpyAsync(&hErr[s], fgls_ctx_err_ptr(ctx), sizeof(unsigned long long),
-                            hipMemcpyDeviceToHost, s_comp));
+        for (int k = 0; k < kn; ++k) {
+          const int q = s * K + k;
+          const int64_t base = (int64_t)(j0 + k) * m + i0;
+          if (opt.mixed)
+            RCHK(fgls_hip_solve_block_i8(ctx[k], dX8[s], ld8, cnt, dB[q], dV[q], base, s_comp));
+          else
+            RCHK(fgls_hip_solve_block(ctx[k], dX[s], ldx, cnt, dB[q], dV[q], base, s_comp));
+          HCHK(hipMemcpyAsync(hB[q], dB[q], (size_t)cnt * p * sizeof(double), hipMemcpyDeviceToHost, s_comp));
+          HCHK(hipMemcpyAsync(hV[q], dV[q], (size_t)cnt * pp * sizeof(double), hipMemcpyDeviceToHost, s_comp));
+          HCHK(hipMemcpyAsync(&hErr[q], fgls_ctx_err_ptr(ctx[k]), sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, s_comp));
+        }
+        if (!opt.mixed) HCHK(hipEventRecord(ev_free[s], s_comp));
         HCHK(hipEventRecord(ev_out[s], s_comp));
         if (opt.serial) HCHK(hipDeviceSynchronize());
         /* the pinned read slot can be refilled once its upload has landed */
@@ -431,16 +479,20 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
                               "[-127, 127] (use precision=fp64)", hBad[ps], (long long)i0);
           goto done;
         }
-        if (hErr[ps] != ~0ull) {  /* first failing SNP so far (fgls_chol_gpu.c:664-670) */
-          const long long bad = (long long)(hErr[ps] >> 8) - (long long)j * m;
-          rc = fgls_set_error(FGLS_ERR_NOTSPD, "dpotrf(V) failed (info: %d) - i: %lld",
-                              (int)(hErr[ps] & 0xff), bad);
-          goto done;
+        for (int k = 0; k < kn; ++k) {
+          const int q = ps * K + k;
+          const int j = j0 + k;
+          if (hErr[q] != ~0ull) {  /* first failing SNP so far (fgls_chol_gpu.c:664-670) */
+            const long long bad = (long long)(hErr[q] >> 8) - (long long)j * m;
+            rc = fgls_set_error(FGLS_ERR_NOTSPD, "dpotrf(V) failed (info: %d) - i: %lld",
+                                (int)(hErr[q] & 0xff), bad);
+            goto done;
+          }
+          RCHK(write_at(fdB, B_path, hB[q], (size_t)cnt * p * sizeof(double),
+                        (off_t)(((int64_t)j * m + i0) * p) * (off_t)sizeof(double)));
+          RCHK(write_at(fdV, V_path, hV[q], (size_t)cnt * pp * sizeof(double),
+                        (off_t)(((int64_t)j * m + i0) * pp) * (off_t)sizeof(double)));
         }
-        RCHK(write_at(fdB, B_path, hB[ps], (size_t)cnt * p * sizeof(double),
-                      (off_t)(((int64_t)j * m + i0) * p) * (off_t)sizeof(double)));
-        RCHK(write_at(fdV, V_path, hV[ps], (size_t)cnt * pp * sizeof(double),
-                      (off_t)(((int64_t)j * m + i0) * pp) * (off_t)sizeof(double)));
       }
     }
     --it;  /* the c == nchunks pass consumed no slot */
@@ -450,10 +502,10 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   if (opt.stats) {
     const double snps = (double)m * t;
     fprintf(stderr,
-            "[fgls] n=%d p=%d m=%d t=%d block=%lld: host potrf %.3fs, trait setup %.3fs, "
+            "[fgls] n=%d p=%d m=%d t=%d block=%lld traits/pass=%d: host potrf %.3fs, trait setup %.3fs, "
             "block loop %.3fs (%.0f SNPs/s, %.2f GB/s X_R), reader busy %.3fs, threads %d\n",
-            n, p, m, t, (long long)block, t_potrf, t_trait, t_loop, t_loop > 0 ? snps / t_loop : 0.0,
-            t_loop > 0 ? snps * wXR * n * 8.0 / t_loop / 1e9 : 0.0, R.busy_s, threads);
+            n, p, m, t, (long long)block, K, t_potrf, t_trait, t_loop, t_loop > 0 ? snps / t_loop : 0.0,
+            t_loop > 0 ? (double)m * R.t * wXR * n * 8.0 / t_loop / 1e9 : 0.0, R.busy_s, threads);
   }
 
 done:
@@ -475,23 +527,32 @@ done:
     if (ev_out[s]) hipEventDestroy(ev_out[s]);
     if (ev_free[s]) hipEventDestroy(ev_free[s]);
     if (dX[s]) hipFree(dX[s]);
-    if (dB[s]) hipFree(dB[s]);
-    if (dV[s]) hipFree(dV[s]);
     if (R.slot[s]) hipHostFree(R.slot[s]);
-    if (hB[s]) hipHostFree(hB[s]);
-    if (hV[s]) hipHostFree(hV[s]);
+    if (dX8[s]) hipFree(dX8[s]);
   }
+  for (int q = 0; q < 2 * K; ++q) {
+    if (dB && dB[q]) hipFree(dB[q]);
+    if (dV && dV[q]) hipFree(dV[q]);
+    if (hB && hB[q]) hipHostFree(hB[q]);
+    if (hV && hV[q]) hipHostFree(hV[q]);
+  }
+  free(dB);
+  free(dV);
+  free(hB);
+  free(hV);
   if (hErr) hipHostFree(hErr);
   if (hBad) hipHostFree(hBad);
   if (dBad) hipFree(dBad);
-  for (int s = 0; s < 2; ++s)
-    if (dX8[s]) hipFree(dX8[s]);
   if (s_copy) hipStreamDestroy(s_copy);
   if (s_comp) hipStreamDestroy(s_comp);
   if (dL) hipFree(dL);
   if (dXL) hipFree(dXL);
   if (dy) hipFree(dy);
-  if (ctx) fgls_hip_ctx_destroy(ctx);
+  if (ctx) {
+    for (int k = 0; k < K; ++k)
+      if (ctx[k]) fgls_hip_ctx_destroy(ctx[k]);
+    free(ctx);
+  }
   if (R.fd >= 0) close(R.fd);
   if (fdB >= 0) close(fdB);
   if (fdV >= 0) close(fdV);

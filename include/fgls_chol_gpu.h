@@ -54,6 +54,10 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *                  reference's FGLS_GPU_SERIAL, fgls_chol_gpu.c:44-46)
  *   stats=1        print stage timings and rates to stderr
  *   io_threads=<k> parallel preads of each X_R block (default 4)
+ *   traits_per_pass=<k>  traits solved per read of each X_R block (default:
+ *                  all t traits when their contexts fit in 60 % of the free
+ *                  device memory; the reference re-reads X_R for every
+ *                  trait, fgls_chol.c:223 and :336-337)
  *   potrf=host     build M and its Cholesky factor on the host (OpenBLAS,
  *                  as the reference) and upload the factor per trait; the
  *                  default potrf=gpu uploads Phi once and builds and factors

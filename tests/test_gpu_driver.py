@@ -68,8 +68,10 @@ def test_fgls_chol_gpu_matches_oracle(pkg, gpu, tmp_path, n, wXL, m, t, x_b, wXR
     assert B.size == Bo.size and V.size == Vo.size
     e = orc.parity_errors(B, V, Bo, Vo, p)
     assert e["B_scaled"] < 1e-10 and e["V_scaled"] < 1e-10 and e["V_upper_zero"], e
-    # block size and the serialised debug mode do not change a single bit
-    for opts in ({"block": x_b}, {"serial": 1, "block": 3 * x_b}):
+    # block size, the serialised debug mode and the number of traits sharing
+    # one read of X_R do not change a single bit
+    for opts in ({"block": x_b}, {"serial": 1, "block": 3 * x_b}, {"traits_per_pass": 1},
+                 {"traits_per_pass": 2, "block": 2 * x_b}):
         pkg.fgls_chol_gpu(n, p, m, t, wXL, wXR, x_b, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
                           f("XL.in"), f("XR.in"), f("Y.in"), f("B2.out"), f("V2.out"), options=opts)
         assert np.array_equal(read_out(d, "B2.out", t * m * p), B)
