@@ -11,7 +11,7 @@ X_R ~ Binomial(2, f_j), f_j ~ U(0.05, 0.5), y = X_L beta + eps, eps ~ N(0, M).
 Two paths run on the same genotypes (drawn as int8 hard calls; the FP64 path
 reads their exact FP64 copy, 80 GB, the mixed path the int8 copy, 10 GB):
   fp64  -- FP64 MFMA triangular product through L^-1;
-  mixed -- int8 MFMA over six 7-bit slices of L^-1, exact int32 accumulation.
+  mixed -- int8 MFMA over six int8 digit slices of L^-1 (radix 254), exact int32 accumulation.
 A step is one pass of a path over all m SNPs: per block of SNPs the reduce
 launch (triangular product L^-1 X_R with the per-SNP reductions) then the
 finalize launch (p x p solves, b_i and V_i).  The per-trait precompute
@@ -407,7 +407,7 @@ def main():
         "config": {"workload": f"Cholesky FGLS, n={n}, p={p}, m={m}, t=1, h2={args.h2}; value = {head} path",
                    "n": n, "p": p, "m": m, "block": block, "path": head,
                    "paths": {"fp64": "FP64 genotypes, FP64 MFMA triangular product through L^-1",
-                             "mixed": "int8 genotypes x 6 signed 7-bit slices of the FP64 L^-1 on the int8 "
+                             "mixed": "int8 genotypes x 6 int8 digit slices (radix 254) of the FP64 L^-1 on the int8 "
                                       "MFMA, exact int32 accumulation, FP64 recombination and p x p solves"}},
         "roofline": roofline(head, res),
         "cpu_baseline": cpu,

@@ -9,10 +9,14 @@
 typedef double fgls_v4d __attribute__((ext_vector_type(4)));
 typedef int fgls_v4i __attribute__((ext_vector_type(4)));
 
-// Mixed mode: L^-1 as NSLICE signed 7-bit slices; the int8 product runs on
+// Mixed mode: L^-1 as NSLICE int8 digit slices; the int8 product runs on
 // workgroup tiles of I8_TM rows x I8_TN SNP columns, grouped I8_GR x I8_GC
 // per XCD (see fgls_tile_of_block).
 constexpr int FGLS_NSLICE = 6;
+// Radix of the slice digits (balanced, digits in [-127, 127]) and the
+// largest n for which the int32 sums stay exact: 127 * 127 * n < 2^31.
+constexpr int FGLS_SLICE_RADIX = 254;
+constexpr int FGLS_I8_MAX_N = 133000;
 // The grouping is a build-time tunable (-DFGLS_I8_GR=.. -DFGLS_I8_GC=..).
 #ifndef FGLS_I8_GR
 #define FGLS_I8_GR 16

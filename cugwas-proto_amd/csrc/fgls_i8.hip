@@ -1,5 +1,5 @@
 // fgls_i8.hip -- the mixed-precision hot kernel: int8 genotypes times six
-// signed 7-bit slices of L^-1 on v_mfma_i32_16x16x64_i8, exact int32
+// int8 digit slices of L^-1 on v_mfma_i32_16x16x64_i8, exact int32
 // accumulation, FP64 recombination and the per-SNP reductions.
 //
 // This translation unit is compiled with -mllvm -amdgpu-mfma-vgpr-form=1:
@@ -8,10 +8,10 @@
 // once more than a few dozen of them are live).
 //
 // Slice representation (built by k_slice_linv in fgls_kernels.hip): row i of
-// L^-1 times sc_i = 2^(6-e_i) is sum_s a_s 2^(-7s), a_s in [-64, 64], stored
-// in 1 KB blocks (fgls_a8_offset).  For integer genotypes x, (A_s x)_i is an
-// exact int32 (|a x| <= 64*127 per term, exact up to n ~ 2.6e5), and
-// z_i = sum_s 2^(-7s) (A_s x)_i / sc_i.
+// L^-1 times sc_i = 127 / (row maximum) is sum_s a_s R^(-s), R = 254,
+// a_s in [-127, 127], stored in 1 KB blocks (fgls_a8_offset).  For integer
+// genotypes x, (A_s x)_i is an exact int32 (|a x| <= 127*127 per term, exact
+// for n <= 133,000), and z_i = sum_s R^(-s) (A_s x)_i / sc_i.
 
 #include "fgls_device.h"
 
@@ -178,7 +178,8 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
         const int row = r0 + 16 * ti + 4 * lg + v;
         double t = (double)acc[S - 1][ti][tj][v];
 #pragma unroll
-        for (int s2 = S - 2; s2 >= 0; --s2) t = fma(t, 1.0 / 128.0, (double)acc[s2][ti][tj][v]);
+        for (int s2 = S - 2; s2 >= 0; --s2)
+          t = fma(t, 1.0 / FGLS_SLICE_RADIX, (double)acc[s2][ti][tj][v]);
         const double z = t * inv_scale[row];
         q[0] = fma(z, z, q[0]);
 #pragma unroll

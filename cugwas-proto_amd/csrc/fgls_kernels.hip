@@ -379,15 +379,18 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
 }
 
 // ---------------------------------------------------------------------------
-// Mixed precision: L^-1 as S signed 7-bit slices, genotypes as int8.
+// Mixed precision: L^-1 as S int8 digit slices, genotypes as int8.
 //
-// Row i of L^-1 is scaled by sc_i = 2^(6-e_i) (e_i = frexp exponent of the
-// row's largest magnitude, so |L^-1_ik| sc_i < 64) and written as
-//   L^-1_ik sc_i = sum_{s<S} a_s,ik 2^(-7s) + O(2^(-7S+6)),  a_s,ik in [-64, 64]
-// (a_s = rint(r_s), r_{s+1} = 128 (r_s - a_s); every step is exact in FP64).
-// z_i = L^-1 x for integer genotypes x is then sum_s 2^(-7s) (A_s x)_i / sc_i,
+// Row i of L^-1 is scaled by sc_i = 127 / max_k |L^-1_ik| and written in
+// balanced radix R = FGLS_SLICE_RADIX = 254:
+//   L^-1_ik sc_i = sum_{s<S} a_s,ik R^(-s) + r_S R^(-S),  a_s,ik in [-127, 127],
+// a_s = rint(r_s), r_{s+1} = R (r_s - a_s), so |r_s| <= 127 at every level
+// and the digits use the whole int8 range (6 digits: 127 R^-6 / 127 = 3.7e-15
+// of the row maximum; the FP64 rounding of the scaling and of R (r - a) is
+// 2^-52 relative, far below).
+// z_i = L^-1 x for integer genotypes x is then (sum_s R^(-s) (A_s x)_i) / sc_i,
 // where every A_s x is an exact int32 product on the int8 MFMA
-// (|a x| <= 64*127 per term, so the sum stays exact up to n ~ 2.6e5).
+// (|a x| <= 127*127 per term, exact for n <= 133,000, checked at set_mode).
 //
 // Slice layout: fgls_a8_offset (1 KB blocks in MFMA A-operand lane order, so
 // one 64-k step of a 16-row tile and one slice is a single contiguous load).
@@ -413,10 +416,8 @@ __global__ __launch_bounds__(256) void k_slice_linv(int ldn, const double* __res
   if ((tid & 63) == 0) red[tid >> 6] = mx;
   __syncthreads();
   mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-  int e = 0;
-  if (mx > 0.0) frexp(mx, &e);         // mx in [2^(e-1), 2^e)
-  const double sc = ldexp(1.0, 6 - e); // mx * sc in [32, 64)
-  if (tid == 0) inv_scale[row] = ldexp(1.0, e - 6);
+  const double sc = (mx > 0.0) ? 127.0 / mx : 1.0;   // |L^-1_ik| sc <= 127
+  if (tid == 0) inv_scale[row] = (mx > 0.0) ? mx / 127.0 : 0.0;
   const int KC = ldn / 16;
   for (int kc = tid; kc < KC; kc += 256) {
     int8_t out[S][16];
@@ -425,9 +426,9 @@ __global__ __launch_bounds__(256) void k_slice_linv(int ldn, const double* __res
       double r = Lr[kc * 16 + b] * sc;
 #pragma unroll
       for (int s2 = 0; s2 < S; ++s2) {
-        const double a = rint(r);
+        const double a = fmin(fmax(rint(r), -127.0), 127.0);
         out[s2][b] = (int8_t)(int)a;
-        r = (r - a) * 128.0;
+        r = (r - a) * (double)FGLS_SLICE_RADIX;
       }
     }
 #pragma unroll
@@ -801,7 +802,7 @@ void launch_trmm_reduce(const fgls_hip_ctx* c, const double* X, int64_t ldx, int
                      (int64_t)c->ldn, c->part, (int64_t)c->max_block);
 }
 
-constexpr int NSLICE = FGLS_NSLICE;   // 7-bit slices of L^-1 in the mixed mode
+constexpr int NSLICE = FGLS_NSLICE;   // int8 digit slices of L^-1 in the mixed mode
 
 int build_slices(fgls_hip_ctx* c, hipStream_t st) {
   hipLaunchKernelGGL((k_slice_linv<NSLICE>), dim3(c->ldn), dim3(256), 0, st, c->ldn, c->Linv, c->A8,
@@ -1108,6 +1109,8 @@ int fgls_hip_ctx_set_mode(fgls_hip_ctx* c, int mode) {
   if (mode != FGLS_MODE_FP64 && mode != FGLS_MODE_MIXED_I8) return fail(FGLS_ERR_ARG, "unknown mode %d", mode);
   if (mode == FGLS_MODE_MIXED_I8 && c->wXR != 1)
     return fail(FGLS_ERR_ARG, "the mixed mode needs wXR = 1 (got %d)", c->wXR);
+  if (mode == FGLS_MODE_MIXED_I8 && c->n > FGLS_I8_MAX_N)
+    return fail(FGLS_ERR_ARG, "the mixed mode is exact for n <= %d (got %d)", FGLS_I8_MAX_N, c->n);
   if ((rc = set_device(c))) return rc;
   if (mode == FGLS_MODE_MIXED_I8) {
     if (!c->A8) {

@@ -62,7 +62,7 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *                  as the reference) and upload the factor per trait; the
  *                  default potrf=gpu uploads Phi once and builds and factors
  *                  M on the device (fgls_hip_set_trait_phi)
- *   precision=mixed  int8 genotypes x six 7-bit slices of L^-1 on the int8
+ *   precision=mixed  int8 genotypes x six int8 digit slices of L^-1 on the int8
  *                  MFMA (wXR = 1; every X_R value must be an integer in
  *                  [-127, 127], else FGLS_ERR_ARG); precision=fp64 (default)
  *                  is the reference's arithmetic

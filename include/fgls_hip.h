@@ -33,9 +33,10 @@ extern "C" {
 
 /* Arithmetic of the per-SNP hot path (fgls_hip_ctx_set_mode). */
 #define FGLS_MODE_FP64     0   /* FP64 genotypes, FP64 MFMA                    */
-#define FGLS_MODE_MIXED_I8 1   /* int8 genotypes; L^-1 as 6 signed 7-bit slices;
-                                  exact int32 accumulation on the int8 MFMA;
-                                  FP64 per-row recombination and per-SNP solve */
+#define FGLS_MODE_MIXED_I8 1   /* int8 genotypes; L^-1 as 6 int8 digit slices
+                                  (balanced radix 254); exact int32 accumulation
+                                  on the int8 MFMA; FP64 per-row recombination
+                                  and per-SNP solve; n <= 133,000 */
 
 typedef struct fgls_hip_ctx fgls_hip_ctx;
 
@@ -119,7 +120,7 @@ int fgls_hip_block_finalize(fgls_hip_ctx *ctx, int count, double *dB,
  * The genotypes of a block are int8, column-major, ld ldx8 >=
  * ceil(n/64)*64 with ldx8 % 16 == 0 (rows n..ld-1 are ignored).  b and V
  * have the layout of fgls_hip_solve_block; their error against the FP64
- * path is set by the slice count (<= 1e-9 relative at n = 1e4, see the tests).
+ * path is set by the slice count (b within 1e-9 of max(|b|, se), see the tests).
  */
 int fgls_hip_ctx_set_mode(fgls_hip_ctx *ctx, int mode);
 int fgls_hip_block_reduce_i8(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ldx8,
