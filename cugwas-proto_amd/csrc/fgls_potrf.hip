@@ -2,15 +2,17 @@
 //   M = sigma2 (h2 Phi + (1 - h2) I) = L L'      (FP64, lower)
 // replacing the host M construction and dpotrf of fgls_chol_gpu.c:406-430.
 //
-// Blocked right-looking factorisation with 64-column panels.  For the panel
-// at column j:
+// Two-level right-looking factorisation.  Outer panels of OB = 256 columns;
+// inside a panel, 64-column steps:
 //   k_chol_diag   factors the 64 x 64 diagonal block in LDS and forms the
 //                 inverse of its factor (one workgroup);
 //   k_chol_panel  L21 = A21 L11^-T for the rows below (64-row workgroups);
-//                 the panel is also written row-major to a scratch buffer
-//                 so that the update reads 4 consecutive k per lane;
-//   k_chol_update A22 -= L21 L21' on the lower tiles of the trailing matrix
-//                 (128 x 128 tiles, FP64 MFMA, K = 64).
+//                 the rows are also written row-major to a panel buffer P
+//                 (ld 256) so that updates read 4 consecutive k per lane;
+//   k_chol_update A -= P P' on lower tiles: first the rest of the outer
+//                 panel (K = 64), then, once the outer panel is done, the
+//                 whole trailing matrix (K = 256: 4x fewer passes over it
+//                 than 64-column steps would make).
 // The first non-positive pivot is recorded as LAPACK's info (1-based order
 // of the failing leading minor); later panels still run on harmless values.
 
@@ -25,7 +27,8 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int PB = 64;   // panel width
+constexpr int PB = 64;   // inner panel width
+constexpr int OB = 256;  // outer panel width (ld of P)
 constexpr int UT = 128;  // update tile (2 x 2 waves of 64 x 64)
 
 using fgls::fail;
@@ -115,10 +118,10 @@ __global__ __launch_bounds__(256) void k_chol_diag(int n, int j, double* __restr
 }
 
 // L21 = A21 L11^-T for rows [i0, i0 + 64) below the diagonal block at j
-// (j + 64 < n, so the panel is full); also P[i * 64 + c] = L21(i, c).
+// (j + 64 < n, so the panel is full); also P[i * OB + poff + c] = L21(i, c).
 __global__ __launch_bounds__(256) void k_chol_panel(int n, int j, double* __restrict__ L, int64_t ldl,
                                                     const double* __restrict__ Dinv_j,
-                                                    double* __restrict__ P) {
+                                                    double* __restrict__ P, int poff) {
   __shared__ double A[PB][PB + 1];
   __shared__ double Di[PB][PB + 1];
   const int tid = threadIdx.x;
@@ -143,38 +146,36 @@ __global__ __launch_bounds__(256) void k_chol_panel(int n, int j, double* __rest
   if (r < rows) {
 #pragma unroll
     for (int q = 0; q < 16; ++q) L[(size_t)(j + c0 + q) * ldl + i0 + r] = x[q];
-    v4d* prow = reinterpret_cast<v4d*>(P + (size_t)(i0 + r) * PB + c0);
+    v4d* prow = reinterpret_cast<v4d*>(P + (size_t)(i0 + r) * OB + poff + c0);
 #pragma unroll
     for (int q = 0; q < 4; ++q) prow[q] = v4d{x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]};
   }
 }
 
-// A22 -= L21 L21' on the lower 128 x 128 tiles of the trailing matrix
-// (rows and columns [j + 64, n)), K = 64 from the row-major panel P.
-// Workgroup b -> tile (ti, tj), ti >= tj, b = ti (ti + 1) / 2 + tj.
-// MFMA operands as in the FP64 product: lane group g supplies k = 4g + s,
-// one 32-byte load per lane and operand; swapped operands put 16 lanes on 16
-// consecutive rows of one column for the read-modify-write.
-__global__ __launch_bounds__(256) void k_chol_update(int n, int j, double* __restrict__ L, int64_t ldl,
-                                                     const double* __restrict__ P) {
-  const int b = blockIdx.x;
-  int ti = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
-  while ((ti + 1) * (ti + 2) / 2 <= b) ++ti;
-  while (ti * (ti + 1) / 2 > b) --ti;
-  const int tj = b - ti * (ti + 1) / 2;
-  const int base = j + PB;
+// A(row, col) -= sum_{k<K} P(row, k) P(col, k) for col in [c0, c1), row in
+// [c0, n), row >= col; P row-major with ld OB (the columns of the current
+// panel).  Grid: (row tiles, column tiles) of 128 from c0; tiles entirely
+// above the diagonal exit.  MFMA operands as in the FP64 product: lane group
+// g supplies k = 4g + s, one 32-byte load per lane and operand; swapped
+// operands put 16 lanes on 16 consecutive rows of one column for the
+// read-modify-write.
+template <int K>
+__global__ __launch_bounds__(256) void k_chol_update(int n, int c0g, int c1g, double* __restrict__ L,
+                                                     int64_t ldl, const double* __restrict__ P) {
+  const int ti = blockIdx.x, tj = blockIdx.y;
+  if (tj > ti) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wr = w >> 1, wc = w & 1;
   const int li = lane & 15, lg = lane >> 4;
-  const int r0 = base + ti * UT + wr * 64;
-  const int c0 = base + tj * UT + wc * 64;
-  if (r0 >= n || c0 >= n || c0 > r0 + 63) return;   // nothing of the lower triangle here
+  const int r0 = c0g + ti * UT + wr * 64;
+  const int c0 = c0g + tj * UT + wc * 64;
+  if (r0 >= n || c0 >= c1g || c0 > r0 + 63) return;   // nothing of the region here
   const double* Ap[4];
   const double* Bp[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    Ap[q] = P + (size_t)(r0 + 16 * q + li) * PB + 4 * lg;
-    Bp[q] = P + (size_t)(c0 + 16 * q + li) * PB + 4 * lg;
+    Ap[q] = P + (size_t)(r0 + 16 * q + li) * OB + 4 * lg;
+    Bp[q] = P + (size_t)(c0 + 16 * q + li) * OB + 4 * lg;
   }
   v4d acc[4][4];
 #pragma unroll
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(256) void k_chol_update(int n, int j, double* __res
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[a][c] = v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 2
-  for (int k0 = 0; k0 < PB; k0 += 16) {
+  for (int k0 = 0; k0 < K; k0 += 16) {
     v4d av[4], bv[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -190,11 +191,11 @@ __global__ __launch_bounds__(256) void k_chol_update(int n, int j, double* __res
       bv[q] = *reinterpret_cast<const v4d*>(Bp[q] + k0);
     }
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+    for (int s2 = 0; s2 < 4; ++s2)
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[a][c] = mfma_f64(bv[c][s], av[a][s], acc[a][c]);
+        for (int c = 0; c < 4; ++c) acc[a][c] = mfma_f64(bv[c][s2], av[a][s2], acc[a][c]);
   }
   // element v of acc[a][c] is (row r0 + 16a + li, column c0 + 16c + lg + 4v)
 #pragma unroll
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(256) void k_chol_update(int n, int j, double* __res
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int col = c0 + 16 * c + lg + 4 * v;
-      if (col >= n) continue;
+      if (col >= c1g) continue;
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         const int row = r0 + 16 * a + li;
@@ -212,11 +213,18 @@ __global__ __launch_bounds__(256) void k_chol_update(int n, int j, double* __res
   }
 }
 
+template <int K>
+void launch_update(int n, int c0, int c1, double* L, int64_t ldl, const double* P, hipStream_t st) {
+  if (c0 >= n || c1 <= c0) return;
+  const int tr = (n - c0 + UT - 1) / UT, tc = (c1 - c0 + UT - 1) / UT;
+  hipLaunchKernelGGL(k_chol_update<K>, dim3(tr, tc), dim3(256), 0, st, n, c0, c1, L, ldl, P);
+}
+
 int factor_on_device(fgls_hip_ctx* c, const double* dPhi, int64_t ldphi, double h2, double sigma2,
                      hipStream_t st) {
   const int n = c->n, ldn = c->ldn;
   if (!c->panel) {
-    const size_t bytes = (size_t)(ldn + UT) * PB * sizeof(double);
+    const size_t bytes = (size_t)(ldn + UT) * OB * sizeof(double);
     if (hipMalloc((void**)&c->panel, bytes) != hipSuccess)
       return fail(FGLS_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
     HIP_TRY(hipMemset(c->panel, 0, bytes));
@@ -224,17 +232,21 @@ int factor_on_device(fgls_hip_ctx* c, const double* dPhi, int64_t ldphi, double 
   HIP_TRY(hipMemsetAsync(c->status + 1, 0, sizeof(int), st));
   hipLaunchKernelGGL(k_build_M, dim3((ldn + 255) / 256, ldn), dim3(256), 0, st, n, ldn, dPhi, ldphi,
                      sigma2 * h2, sigma2 * (1.0 - h2), c->L);
-  for (int j = 0; j < n; j += PB) {
-    double* Dj = c->Dinv + (size_t)(j / PB) * PB * PB;
-    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, st, n, j, c->L, (int64_t)ldn, Dj,
-                       c->status + 1);
-    const int below = n - j - PB;
-    if (below <= 0) break;
-    hipLaunchKernelGGL(k_chol_panel, dim3((below + PB - 1) / PB), dim3(256), 0, st, n, j, c->L,
-                       (int64_t)ldn, (const double*)Dj, c->panel);
-    const int T = (below + UT - 1) / UT;
-    hipLaunchKernelGGL(k_chol_update, dim3(T * (T + 1) / 2), dim3(256), 0, st, n, j, c->L,
-                       (int64_t)ldn, (const double*)c->panel);
+  for (int j = 0; j < n; j += OB) {
+    const int jend = min(j + OB, n);
+    for (int jj = j; jj < jend; jj += PB) {
+      double* Dj = c->Dinv + (size_t)(jj / PB) * PB * PB;
+      hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, st, n, jj, c->L, (int64_t)ldn, Dj,
+                         c->status + 1);
+      const int below = n - jj - PB;
+      if (below <= 0) break;
+      hipLaunchKernelGGL(k_chol_panel, dim3((below + PB - 1) / PB), dim3(256), 0, st, n, jj, c->L,
+                         (int64_t)ldn, (const double*)Dj, c->panel, jj - j);
+      // the rest of this outer panel, from this step's 64 columns
+      launch_update<PB>(n, jj + PB, jend, c->L, (int64_t)ldn, c->panel + (jj - j), st);
+    }
+    // the trailing matrix, from the whole outer panel
+    if (jend - j == OB) launch_update<OB>(n, jend, n, c->L, (int64_t)ldn, c->panel, st);
   }
   HIP_TRY(hipGetLastError());
   return FGLS_OK;
