@@ -98,35 +98,39 @@ __global__ __launch_bounds__(64) void k_diag_inv(int n, const double* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Blocked left-looking TRSM, L Z = B, one workgroup per 64-column panel.
+// Blocked left-looking TRSM, L Z = B, one workgroup per panel of NC (64 or
+// 32) columns.
 // For every 64-row block I:   R = B_I - sum_{J<I} L_IJ Z_J   (FP64 MFMA)
 //                             Z_I = Dinv_I R                 (FP64 MFMA, R in LDS)
-// IDENT: B is the identity, panel P's rows above block P are zero (L^-1 is
-// lower triangular), so the sweep starts at block P.
+// IDENT: B is the identity, panel P's rows above its first column are zero
+// (L^-1 is lower triangular), so the sweep starts at the block holding it.
+// The sweep over row blocks is sequential inside a panel: narrower panels
+// (NC = 32) give more workgroups and a shorter sweep per workgroup.
 // ROWMAJOR: Z(r, c) is stored at Z[r*ldz + c] (used for Linv), else at
 // Z[c*ldz + r]; B is always column-major and may alias Z (in place) when
 // !ROWMAJOR.
 // ---------------------------------------------------------------------------
-template <bool IDENT, bool ROWMAJOR>
+template <bool IDENT, bool ROWMAJOR, int NC = NB>
 __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const double* __restrict__ L,
                                                     int64_t ldl, const double* __restrict__ Dinv,
                                                     const double* B, int64_t ldb, double* Z,
                                                     int64_t ldz) {
-  __shared__ double Rs[NB][NB + 2];
+  constexpr int TJ = NC / 16;
+  __shared__ double Rs[NB][NC + 2];
   const int P = blockIdx.x;
-  const int c0 = P * NB;
+  const int c0 = P * NC;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int nblk = (n + NB - 1) / NB;
-  const int Istart = IDENT ? P : 0;
-  int colc[4];
+  const int Istart = IDENT ? c0 / NB : 0;
+  int colc[TJ];
 #pragma unroll
-  for (int tj = 0; tj < 4; ++tj) colc[tj] = min(c0 + 16 * tj + li, nrhs - 1);
+  for (int tj = 0; tj < TJ; ++tj) colc[tj] = min(c0 + 16 * tj + li, nrhs - 1);
 
   for (int I = Istart; I < nblk; ++I) {
-    v4d acc[4];
+    v4d acc[TJ];
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj) acc[tj] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int tj = 0; tj < TJ; ++tj) acc[tj] = v4d{0.0, 0.0, 0.0, 0.0};
     const int rowA = I * NB + 16 * w + li;
     const bool rowok = rowA < n;
     for (int k0 = Istart * NB; k0 < I * NB; k0 += 16) {
@@ -134,9 +138,9 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const doubl
 #pragma unroll
       for (int s = 0; s < 4; ++s)
         a[s] = rowok ? L[(size_t)(k0 + 4 * lg + s) * ldl + rowA] : 0.0;
-      double b[4][4];
+      double b[TJ][4];
 #pragma unroll
-      for (int tj = 0; tj < 4; ++tj) {
+      for (int tj = 0; tj < TJ; ++tj) {
         if (ROWMAJOR) {
 #pragma unroll
           for (int s = 0; s < 4; ++s) b[tj][s] = Z[(size_t)(k0 + 4 * lg + s) * ldz + colc[tj]];
@@ -148,11 +152,11 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const doubl
 #pragma unroll
       for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int tj = 0; tj < 4; ++tj) acc[tj] = mfma_f64(a[s], b[tj][s], acc[tj]);
+        for (int tj = 0; tj < TJ; ++tj) acc[tj] = mfma_f64(a[s], b[tj][s], acc[tj]);
     }
-    // R = B_I - acc into LDS (natural 64x64 layout)
+    // R = B_I - acc into LDS (natural 64 x NC layout)
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj) {
+    for (int tj = 0; tj < TJ; ++tj) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int r = 16 * w + lg + 4 * v, c = 16 * tj + li;
@@ -165,9 +169,9 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const doubl
     }
     __syncthreads();
     // Z_I = Dinv_I * R
-    v4d zc[4];
+    v4d zc[TJ];
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj) zc[tj] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int tj = 0; tj < TJ; ++tj) zc[tj] = v4d{0.0, 0.0, 0.0, 0.0};
     const double* D = Dinv + (size_t)I * NB * NB;
 #pragma unroll
     for (int k0 = 0; k0 < NB; k0 += 16) {
@@ -176,11 +180,11 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const doubl
         const int k = k0 + 4 * lg + s;
         const double a = D[(size_t)k * NB + 16 * w + li];
 #pragma unroll
-        for (int tj = 0; tj < 4; ++tj) zc[tj] = mfma_f64(a, Rs[k][16 * tj + li], zc[tj]);
+        for (int tj = 0; tj < TJ; ++tj) zc[tj] = mfma_f64(a, Rs[k][16 * tj + li], zc[tj]);
       }
     }
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj) {
+    for (int tj = 0; tj < TJ; ++tj) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int grow = I * NB + 16 * w + lg + 4 * v, gcol = c0 + 16 * tj + li;
@@ -880,8 +884,8 @@ int finish_trait(fgls_hip_ctx* c, const double* dXL, int64_t ldxl, const double*
   HIP_TRY(hipMemcpyAsync(c->Wtmp + (size_t)wXL * ldn, dy, (size_t)n * sizeof(double),
                          hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_diag_inv, dim3(c->nblk), dim3(64), 0, st, n, c->L, (int64_t)ldn, c->Dinv);
-  hipLaunchKernelGGL((k_trsm_panel<true, true>), dim3(c->nblk), dim3(256), 0, st, n, n, c->L,
-                     (int64_t)ldn, c->Dinv, (const double*)nullptr, (int64_t)0, c->Linv,
+  hipLaunchKernelGGL((k_trsm_panel<true, true, 32>), dim3((n + 31) / 32), dim3(256), 0, st, n, n,
+                     c->L, (int64_t)ldn, c->Dinv, (const double*)nullptr, (int64_t)0, c->Linv,
                      (int64_t)ldn);
   // [X~_L | y~] = L^-1 [X_L | y] through the explicit inverse: wXL + 1 columns
   // of the triangular product spread over all row tiles (a TRSM would run
