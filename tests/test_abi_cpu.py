@@ -84,3 +84,31 @@ def test_driver_usage_and_exit_codes(pkg, tmp_path):
     assert r.returncode == 1 and "fgls_eigen" in r.stderr
     r = subprocess.run([drv, "g", str(tmp_path), "1"], input="10 4 5 1 3 5 1 1", capture_output=True, text=True)
     assert r.returncode == 1 and r.stderr.startswith("[Error]: ") and "Phi.in" in r.stderr
+
+
+def test_new_entry_points_reject_bad_arguments(pkg):
+    """NULL contexts and pointers are defined errors (no device needed)."""
+    lib = pkg.lib()
+    assert lib.fgls_hip_set_trait_phi(None, None, 0, 0.5, 1.0, None, 0, None, None) == pkg.FGLS_ERR_ARG
+    assert lib.fgls_hip_get_factor(None, None, 0, None) == pkg.FGLS_ERR_ARG
+    assert lib.fgls_hip_adjoint_begin(None, None, 0, 1.0, None) == pkg.FGLS_ERR_ARG
+    assert lib.fgls_hip_adjoint_block(None, None, 0, 1, None, 0, None) == pkg.FGLS_ERR_ARG
+    assert lib.fgls_hip_adjoint_end(None, None, None, None, None) == pkg.FGLS_ERR_ARG
+    assert b"NULL context" in lib.fgls_hip_last_error()
+
+
+def test_host_factor_matches_lapack(pkg):
+    """fgls_factor_M (host M construction + dpotrf, fgls_chol_gpu.c:406-430)
+    against scipy's Cholesky of the oracle's M, and its info on a non-SPD M."""
+    from oracle import fgls_oracle as orc
+    from tests.fgls_data import make_phi
+    rng = np.random.default_rng(3)
+    Phi = make_phi(rng, 150)
+    L = pkg.factor_M(Phi, 0.35, 1.7, threads=2)
+    Lref = orc.cholesky_lower(orc.build_M(Phi, 0.35, 1.7))
+    assert np.linalg.norm(L - Lref) / np.linalg.norm(Lref) < 1e-14
+    Phi_bad = np.eye(50)
+    Phi_bad[20, 20] = -5.0
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.factor_M(Phi_bad, 0.9, 1.0)
+    assert ei.value.code == pkg.FGLS_ERR_NOTSPD and "info: 21" in ei.value.msg
