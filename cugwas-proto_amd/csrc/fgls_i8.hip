@@ -163,38 +163,92 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
     __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
   }
 
-  // z(row r0+16ti+4lg+v, col c0+16tj+li) = inv_scale[row] * sum_s 2^(-7s) acc[s];
-  // one 16-column fragment at a time, so that the accumulators of finished
-  // fragments are free and the FP64 sums stay within the register budget
-#pragma unroll
-  for (int tj = 0; tj < 4; ++tj) {
-    double q[NQ];
-#pragma unroll
-    for (int qi = 0; qi < NQ; ++qi) q[qi] = 0.0;
-#pragma unroll
-    for (int ti = 0; ti < 2; ++ti) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = r0 + 16 * ti + 4 * lg + v;
-        double t = (double)acc[S - 1][ti][tj][v];
-#pragma unroll
-        for (int s2 = S - 2; s2 >= 0; --s2)
-          t = fma(t, 1.0 / FGLS_SLICE_RADIX, (double)acc[s2][ti][tj][v]);
-        const double z = t * inv_scale[row];
-        q[0] = fma(z, z, q[0]);
-#pragma unroll
-        for (int k = 0; k < NQ - 1; ++k) q[1 + k] = fma(z, W[(size_t)k * ldw + row], q[1 + k]);
+  if constexpr (NQ <= 5) {   // p <= 4: sums interleaved with the conversion (fastest here)
+    // z(row r0+16ti+4lg+v, col c0+16tj+li) = inv_scale[row] * sum_s R^(-s) acc[s];
+    // one 16-column fragment at a time, so that the accumulators of finished
+    // fragments are free and the FP64 sums stay within the register budget
+  #pragma unroll
+    for (int tj = 0; tj < 4; ++tj) {
+      double q[NQ];
+  #pragma unroll
+      for (int qi = 0; qi < NQ; ++qi) q[qi] = 0.0;
+  #pragma unroll
+      for (int ti = 0; ti < 2; ++ti) {
+  #pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + 16 * ti + 4 * lg + v;
+          double t = (double)acc[S - 1][ti][tj][v];
+  #pragma unroll
+          for (int s2 = S - 2; s2 >= 0; --s2)
+            t = fma(t, 1.0 / FGLS_SLICE_RADIX, (double)acc[s2][ti][tj][v]);
+          const double z = t * inv_scale[row];
+          q[0] = fma(z, z, q[0]);
+  #pragma unroll
+          for (int k = 0; k < NQ - 1; ++k) q[1 + k] = fma(z, W[(size_t)k * ldw + row], q[1 + k]);
+        }
+      }
+  #pragma unroll
+      for (int qi = 0; qi < NQ; ++qi) {
+        q[qi] += __shfl_xor(q[qi], 16, 64);
+        q[qi] += __shfl_xor(q[qi], 32, 64);
+      }
+      const int col = c0 + 16 * tj + li;
+      if (lg == 0 && col < count) {
+  #pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) part[((size_t)rt * NQ + qi) * ldp + col] = q[qi];
       }
     }
-#pragma unroll
-    for (int qi = 0; qi < NQ; ++qi) {
-      q[qi] += __shfl_xor(q[qi], 16, 64);
-      q[qi] += __shfl_xor(q[qi], 32, 64);
-    }
-    const int col = c0 + 16 * tj + li;
-    if (lg == 0 && col < count) {
-#pragma unroll
-      for (int qi = 0; qi < NQ; ++qi) part[((size_t)rt * NQ + qi) * ldp + col] = q[qi];
+  } else {
+    // z(row r0+16ti+4lg+v, col c0+16tj+li) = inv_scale[row] * sum_s R^(-s) acc[s].
+    // All 32 z of a lane are formed first (the int32 accumulators die as they
+    // are read: 192 registers become 64), then the per-column sums; with the
+    // sums interleaved with the conversion, p >= 8 spilled inside the loop.
+    double z[4][2][4];
+  #pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+  #pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double isc = inv_scale[r0 + 16 * ti + 4 * lg + v];
+  #pragma unroll
+        for (int tj = 0; tj < 4; ++tj) {
+          double t = (double)acc[S - 1][ti][tj][v];
+  #pragma unroll
+          for (int s2 = S - 2; s2 >= 0; --s2)
+            t = fma(t, 1.0 / FGLS_SLICE_RADIX, (double)acc[s2][ti][tj][v]);
+          z[tj][ti][v] = t * isc;
+        }
+      }
+  #pragma unroll
+    for (int tj = 0; tj < 4; ++tj) {
+      // W is re-read (from cache) for every fragment: an opaque zero in the
+      // row index keeps the compiler from holding 8 (p + 1) values of W live
+      // across the four fragments
+      int oz;
+      asm volatile("v_mov_b32 %0, 0" : "=v"(oz));
+      double q[NQ];
+  #pragma unroll
+      for (int qi = 0; qi < NQ; ++qi) q[qi] = 0.0;
+  #pragma unroll
+      for (int ti = 0; ti < 2; ++ti) {
+  #pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + 16 * ti + 4 * lg + v + oz;
+          const double zz = z[tj][ti][v];
+          q[0] = fma(zz, zz, q[0]);
+  #pragma unroll
+          for (int k = 0; k < NQ - 1; ++k) q[1 + k] = fma(zz, W[(size_t)k * ldw + row], q[1 + k]);
+        }
+      }
+  #pragma unroll
+      for (int qi = 0; qi < NQ; ++qi) {
+        q[qi] += __shfl_xor(q[qi], 16, 64);
+        q[qi] += __shfl_xor(q[qi], 32, 64);
+      }
+      const int col = c0 + 16 * tj + li;
+      if (lg == 0 && col < count) {
+  #pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) part[((size_t)rt * NQ + qi) * ldp + col] = q[qi];
+      }
     }
   }
 }
