@@ -54,6 +54,7 @@ typedef struct {
   int io_threads;    /* parallel preads of X_R per block */
   int host_potrf;    /* potrf=host: M and its factor on the host (OpenBLAS) */
   int traits_per_pass;  /* traits sharing one read of X_R (0: as many as fit) */
+  int xr_i8;         /* xr_format=i8: X_R file of int8 values (n bytes per column) */
 } options_t;
 
 static double now_s(void) {
@@ -78,6 +79,10 @@ static void parse_options(const char *s, options_t *o) {
     char *eq = strchr(tok, '=');
     if (!eq) continue;
     *eq = 0;
+    if (!strcmp(tok, "xr_format")) {
+      o->xr_i8 = !strcmp(eq + 1, "i8") ? 1 : (!strcmp(eq + 1, "f64") ? 0 : -1);
+      continue;
+    }
     if (!strcmp(tok, "potrf")) {
       o->host_potrf = !strcmp(eq + 1, "host") ? 1 : (!strcmp(eq + 1, "gpu") ? 0 : -1);
       continue;
@@ -147,8 +152,9 @@ typedef struct {
   int fd;
   const char *path;
   int n, wXR, t;
+  int esize;                /* bytes per X_R value: 8 (FP64 file) or 1 (int8 file) */
   int64_t m, block, nchunks;
-  double *slot[2];          /* pinned */
+  void *slot[2];            /* pinned */
   sem_t free_[2], full[2];
   int status[2];            /* FGLS_OK or FGLS_ERR_IO per filled slot */
   char err[2][256];
@@ -159,7 +165,7 @@ typedef struct {
 
 static void *reader_main(void *arg) {
   reader_t *R = (reader_t *)arg;
-  const size_t col_bytes = (size_t)R->n * sizeof(double);
+  const size_t col_bytes = (size_t)R->n * (size_t)R->esize;
   for (int64_t it = 0; it < (int64_t)R->t * R->nchunks; ++it) {
     const int s = (int)(it & 1);
     sem_wait(&R->free_[s]);
@@ -241,10 +247,11 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
                      const char *Y_path, const char *B_path, const char *V_path,
                      const char *options) {
   int rc = FGLS_OK;
-  options_t opt = {0, 0, 0, 0, 0, 0, 0, 0};
+  options_t opt = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   parse_options(options ? options : getenv("FGLS_OPTIONS"), &opt);
   if (opt.mixed < 0) return fgls_set_error(FGLS_ERR_ARG, "precision must be fp64 or mixed");
   if (opt.host_potrf < 0) return fgls_set_error(FGLS_ERR_ARG, "potrf must be gpu or host");
+  if (opt.xr_i8 < 0) return fgls_set_error(FGLS_ERR_ARG, "xr_format must be f64 or i8");
   if (opt.mixed && wXR != 1)
     return fgls_set_error(FGLS_ERR_ARG, "precision=mixed needs wXR = 1 (got %d)", wXR);
 
@@ -363,16 +370,21 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   for (int i = 0; i < 2 * K; ++i) hErr[i] = ~0ull;
   HCHK(hipHostMalloc((void **)&hBad, 2 * sizeof(unsigned int), hipHostMallocDefault));
   hBad[0] = hBad[1] = 0;
-  if (opt.mixed && nchunks > 0) {
+  if ((opt.mixed || opt.xr_i8) && nchunks > 0) {
     HCHK(hipMalloc((void **)&dBad, 2 * sizeof(unsigned int)));
-    for (int s = 0; s < 2; ++s) HCHK(hipMalloc((void **)&dX8[s], (size_t)block * ld8));
+    for (int s = 0; s < 2; ++s) {
+      /* rows n..ld8-1 stay zero: uploads and conversions write rows < n */
+      HCHK(hipMalloc((void **)&dX8[s], (size_t)block * wXR * ld8));
+      HCHK(hipMemset(dX8[s], 0, (size_t)block * wXR * ld8));
+    }
   }
   if (nchunks > 0) {
     const size_t xbytes = (size_t)block * wXR * ldx * sizeof(double);
     for (int s = 0; s < 2; ++s) {
       HCHK(hipMalloc((void **)&dX[s], xbytes));
       HCHK(hipMemset(dX[s], 0, xbytes));
-      HCHK(hipHostMalloc((void **)&R.slot[s], (size_t)block * wXR * n * sizeof(double), hipHostMallocDefault));
+      HCHK(hipHostMalloc((void **)&R.slot[s], (size_t)block * wXR * n * (opt.xr_i8 ? 1 : sizeof(double)),
+                         hipHostMallocDefault));
       for (int k = 0; k < K; ++k) {
         const int q = s * K + k;
         HCHK(hipMalloc((void **)&dB[q], (size_t)block * p * sizeof(double)));
@@ -383,6 +395,7 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
     }
     R.path = XR_path;
     R.n = n;
+    R.esize = opt.xr_i8 ? 1 : (int)sizeof(double);
     R.wXR = wXR;
     R.t = (t + K - 1) / K;   /* passes over X_R */
     R.m = m;
@@ -436,18 +449,23 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
         }
         /* device X_R slot s is free once the solves of chunk c-2 have run */
         HCHK(hipStreamWaitEvent(s_copy, ev_free[s], 0));
-        HCHK(hipMemcpy2DAsync(dX[s], (size_t)ldx * sizeof(double), R.slot[s],
-                              (size_t)n * sizeof(double), (size_t)n * sizeof(double),
-                              (size_t)cnt * wXR, hipMemcpyHostToDevice, s_copy));
+        if (opt.xr_i8)
+          HCHK(hipMemcpy2DAsync(dX8[s], (size_t)ld8, R.slot[s], (size_t)n, (size_t)n,
+                                (size_t)cnt * wXR, hipMemcpyHostToDevice, s_copy));
+        else
+          HCHK(hipMemcpy2DAsync(dX[s], (size_t)ldx * sizeof(double), R.slot[s],
+                                (size_t)n * sizeof(double), (size_t)n * sizeof(double),
+                                (size_t)cnt * wXR, hipMemcpyHostToDevice, s_copy));
         HCHK(hipEventRecord(ev_h2d[s], s_copy));
         HCHK(hipStreamWaitEvent(s_comp, ev_h2d[s], 0));
-        if (opt.mixed) {
+        if (opt.mixed && !opt.xr_i8) {
           /* exact FP64 -> int8 on the device; a value that is not an integer
            * in [-127, 127] is counted and fails the run at write-back */
           HCHK(hipMemsetAsync(dBad + s, 0, sizeof(unsigned int), s_comp));
           RCHK(fgls_hip_to_i8(dX[s], ldx, n, cnt, dX8[s], ld8, dBad + s, s_comp));
-          HCHK(hipEventRecord(ev_free[s], s_comp));
           HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
+        } else if (!opt.mixed && opt.xr_i8) {
+          RCHK(fgls_hip_i8_to_f64(dX8[s], ld8, n, cnt * wXR, dX[s], ldx, s_comp));
         }
         for (int k = 0; k < kn; ++k) {
           const int q = s * K + k;
@@ -461,7 +479,7 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
           HCHK(hipMemcpyAsync(&hErr[q], fgls_ctx_err_ptr(ctx[k]), sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, s_comp));
         }
-        if (!opt.mixed) HCHK(hipEventRecord(ev_free[s], s_comp));
+        HCHK(hipEventRecord(ev_free[s], s_comp));
         HCHK(hipEventRecord(ev_out[s], s_comp));
         if (opt.serial) HCHK(hipDeviceSynchronize());
         /* the pinned read slot can be refilled once its upload has landed */
@@ -505,7 +523,7 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
             "[fgls] n=%d p=%d m=%d t=%d block=%lld traits/pass=%d: host potrf %.3fs, trait setup %.3fs, "
             "block loop %.3fs (%.0f SNPs/s, %.2f GB/s X_R), reader busy %.3fs, threads %d\n",
             n, p, m, t, (long long)block, K, t_potrf, t_trait, t_loop, t_loop > 0 ? snps / t_loop : 0.0,
-            t_loop > 0 ? (double)m * R.t * wXR * n * 8.0 / t_loop / 1e9 : 0.0, R.busy_s, threads);
+            t_loop > 0 ? (double)m * R.t * wXR * n * R.esize / t_loop / 1e9 : 0.0, R.busy_s, threads);
   }
 
 done:

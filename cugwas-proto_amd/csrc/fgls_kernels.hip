@@ -462,6 +462,14 @@ __global__ __launch_bounds__(256) void k_to_i8(int n, const double* __restrict__
   if (nb) atomicAdd(bad, nb);
 }
 
+// int8 genotypes -> FP64 (column-major, ld ldx; rows n..ldx-1 written as 0).
+__global__ __launch_bounds__(256) void k_i8_to_f64(int n, const int8_t* __restrict__ X8, int64_t ldx8,
+                                                   double* __restrict__ X, int64_t ldx) {
+  const int col = blockIdx.x;   // one workgroup per column
+  for (int r = threadIdx.x; r < ldx; r += 256)
+    X[(size_t)col * ldx + r] = (r < n) ? (double)X8[(size_t)col * ldx8 + r] : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // p x p tail of the per-SNP stage (fgls_chol.c:401-438): Cholesky of the
 // lower triangle of S, two triangular solves for b, inverse through the
@@ -1150,6 +1158,16 @@ int fgls_hip_to_i8(const double* dX, int64_t ldx, int n, int ncols, int8_t* dX8,
   if (ncols == 0) return FGLS_OK;
   hipLaunchKernelGGL(k_to_i8, dim3(ncols), dim3(256), 0, (hipStream_t)stream, n, dX, ldx, dX8, ldx8,
                      dbad);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_i8_to_f64(const int8_t* dX8, int64_t ldx8, int n, int ncols, double* dX, int64_t ldx,
+                       void* stream) {
+  if (n < 1 || ncols < 0 || ldx < n || ldx8 < n || !dX || !dX8)
+    return fail(FGLS_ERR_ARG, "bad arguments to fgls_hip_i8_to_f64");
+  if (ncols == 0) return FGLS_OK;
+  hipLaunchKernelGGL(k_i8_to_f64, dim3(ncols), dim3(256), 0, (hipStream_t)stream, n, dX8, ldx8, dX, ldx);
   HIP_TRY(hipGetLastError());
   return FGLS_OK;
 }

@@ -54,6 +54,10 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *                  reference's FGLS_GPU_SERIAL, fgls_chol_gpu.c:44-46)
  *   stats=1        print stage timings and rates to stderr
  *   io_threads=<k> parallel preads of each X_R block (default 4)
+ *   xr_format=i8   XR_path holds int8 values (m*wXR*n bytes, the same order
+ *                  as the FP64 file): 8x fewer bytes from disk and over
+ *                  PCIe; used as is by precision=mixed, converted to FP64 on
+ *                  the device otherwise (default xr_format=f64, io.c:35-55)
  *   traits_per_pass=<k>  traits solved per read of each X_R block (default:
  *                  all t traits when their contexts fit in 60 % of the free
  *                  device memory; the reference re-reads X_R for every

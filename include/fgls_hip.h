@@ -133,6 +133,11 @@ int fgls_hip_solve_block_i8(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ldx8,
  * caller-cleared) and written as 0.  Asynchronous. */
 int fgls_hip_to_i8(const double *dX, int64_t ldx, int n, int ncols,
                    int8_t *dX8, int64_t ldx8, unsigned int *dbad, void *stream);
+/* int8 genotypes (ld ldx8) -> FP64 (ld ldx, rows n..ldx-1 zeroed), for
+ * compact genotype input on the FP64 path.  No counterpart in the
+ * reference (its X_R is FP64 only, io.c:35-55).  Asynchronous. */
+int fgls_hip_i8_to_f64(const int8_t *dX8, int64_t ldx8, int n, int ncols,
+                       double *dX, int64_t ldx, void *stream);
 
 /*
  * In-place X <- L^-1 X for ncols columns (ld ldx), with the factor of the

@@ -172,3 +172,33 @@ def test_fgls_chol_gpu_host_and_device_factor_agree(pkg, gpu, tmp_path):
     Bo, Vo = orc.fgls_chol(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y, wXR=1)
     e = orc.parity_errors(out["gpu"][0], out["gpu"][1], Bo, Vo, p)
     assert e["B_scaled"] < 1e-10 and e["V_scaled"] < 1e-10, e
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_int8_genotype_file(pkg, gpu, tmp_path, precision):
+    """xr_format=i8 (compact X_R file, 1 byte per value) gives exactly the
+    files of the FP64 X_R file with the same integer genotypes."""
+    print(f"int8 X_R file, precision={precision}", flush=True)
+    n, m, t, wXL = 700, 1100, 2, 3
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=21, t=t)
+    d = str(tmp_path)
+    H, Sig = np.array([0.4, 0.6]), np.array([1.0, 1.5])
+    write_problem_dir(prob, d, H=H, Sig=Sig)
+    f = lambda s: os.path.join(d, s)
+    prob.XR.astype(np.int8).ravel(order="F").tofile(f("XR8.in"))
+    p = wXL + 1
+    out = {}
+    for fmt, xr in (("f64", "XR.in"), ("i8", "XR8.in")):
+        pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, 256, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f(xr), f("Y.in"), f(f"B_{fmt}.out"), f(f"V_{fmt}.out"),
+                          options={"precision": precision, "xr_format": fmt, "block": 512})
+        out[fmt] = (read_out(d, f"B_{fmt}.out", t * m * p), read_out(d, f"V_{fmt}.out", t * m * p * p))
+    assert np.array_equal(out["i8"][0], out["f64"][0])
+    assert np.array_equal(out["i8"][1], out["f64"][1])
+    # a short int8 file is a defined error
+    np.zeros(n * (m - 1), dtype=np.int8).tofile(f("XR8s.in"))
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.fgls_chol_gpu(n, p, m, 1, wXL, 1, 256, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f("XR8s.in"), f("Y.in"), f("B_s.out"), f("V_s.out"),
+                          options={"precision": precision, "xr_format": "i8"})
+    assert ei.value.code == pkg.FGLS_ERR_IO and "not read completely" in ei.value.msg
