@@ -106,12 +106,20 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i
   __builtin_amdgcn_sched_barrier(0);
 }
 
-template <int NQ, int S>
+// DOS (dosages): every SNP has two int8 digit columns, lo in [0, 127] and hi
+// in [-127, 127], with x * scale = 128 hi + lo; the lo plane holds columns
+// 0..count-1 of X8 and the hi plane starts plane_off bytes later.  A wave then
+// covers 32 SNPs: fragments 0, 1 read their lo digits and fragments 2, 3 the
+// hi digits of the same SNPs (the same MFMA work per fragment as four
+// genotype fragments), and the epilogue forms acc_lo + 128 acc_hi in FP64.
+template <int NQ, int S, bool DOS = false>
 __global__ __launch_bounds__(256, 2) void k_trmm_i8(
     int n, int count, int RT32, int CT, int KS, const int8_t* __restrict__ A8,
     const double* __restrict__ inv_scale, const int8_t* __restrict__ X8, int64_t ldx8,
-    const double* __restrict__ W, int64_t ldw, double* __restrict__ part, int64_t ldp) {
+    const double* __restrict__ W, int64_t ldw, double* __restrict__ part, int64_t ldp,
+    int64_t plane_off = 0, double inv_xscale = 1.0) {
   static_assert(S % 2 == 0, "the 2 x S slice blocks are spread over 4 waves");
+  constexpr int WN = DOS ? 32 : 64;   // SNP columns per wave
   __shared__ SliceStage<S> st0, st1;
   int rt, ct;
   if (!fgls_tile_of_block<I8_GR, I8_GC>(blockIdx.x, RT32, CT, rt, ct)) return;
@@ -119,7 +127,7 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lg = lane >> 4;
   const int r0 = rt * I8_TM;
-  const int c0 = ct * I8_TN + w * 64;
+  const int c0 = ct * (4 * WN) + w * WN;
   const int n64 = (n + 63) & ~63;
   const int nsteps = (r0 < n64) ? min((r0 + I8_TM + 63) & ~63, n64) / 64 : 0;
 
@@ -137,7 +145,11 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   const int8_t* Xw = X8 + (size_t)cb * ldx8;
   int boff[4];
 #pragma unroll
-  for (int tj = 0; tj < 4; ++tj) boff[tj] = (min(c0 + 16 * tj + li, count - 1) - cb) * (int)ldx8 + 16 * lg;
+  for (int tj = 0; tj < 4; ++tj) {
+    const int cj = DOS ? (tj & 1) : tj;   // DOS: 0, 1 lo digits; 2, 3 hi digits
+    boff[tj] = (min(c0 + 16 * cj + li, count - 1) - cb) * (int)ldx8 + 16 * lg +
+               ((DOS && tj >= 2) ? (int)plane_off : 0);
+  }
 
   fgls_v4i acc[S][2][4];
 #pragma unroll
@@ -163,7 +175,55 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
     __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
   }
 
-  if constexpr (NQ <= 5) {   // p <= 4: sums interleaved with the conversion (fastest here)
+  if constexpr (DOS) {
+    // z(row r0+16ti+4lg+v, SNP c0+16tj+li), tj < 2:
+    //   inv_scale[row] / scale * sum_s R^(-s) (acc[s][tj] + 128 acc[s][tj+2])
+    double z[2][2][4];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double isc = inv_scale[r0 + 16 * ti + 4 * lg + v] * inv_xscale;
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) {
+          double t = fma(128.0, (double)acc[S - 1][ti][tj + 2][v], (double)acc[S - 1][ti][tj][v]);
+#pragma unroll
+          for (int s2 = S - 2; s2 >= 0; --s2)
+            t = fma(t, 1.0 / FGLS_SLICE_RADIX,
+                    fma(128.0, (double)acc[s2][ti][tj + 2][v], (double)acc[s2][ti][tj][v]));
+          z[tj][ti][v] = t * isc;
+        }
+      }
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) {
+      int oz;
+      asm volatile("v_mov_b32 %0, 0" : "=v"(oz));
+      double q[NQ];
+#pragma unroll
+      for (int qi = 0; qi < NQ; ++qi) q[qi] = 0.0;
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + 16 * ti + 4 * lg + v + oz;
+          const double zz = z[tj][ti][v];
+          q[0] = fma(zz, zz, q[0]);
+#pragma unroll
+          for (int k = 0; k < NQ - 1; ++k) q[1 + k] = fma(zz, W[(size_t)k * ldw + row], q[1 + k]);
+        }
+      }
+#pragma unroll
+      for (int qi = 0; qi < NQ; ++qi) {
+        q[qi] += __shfl_xor(q[qi], 16, 64);
+        q[qi] += __shfl_xor(q[qi], 32, 64);
+      }
+      const int col = c0 + 16 * tj + li;
+      if (lg == 0 && col < count) {
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) part[((size_t)rt * NQ + qi) * ldp + col] = q[qi];
+      }
+    }
+  } else if constexpr (NQ <= 5) {   // p <= 4: sums interleaved with the conversion (fastest here)
     // z(row r0+16ti+4lg+v, col c0+16tj+li) = inv_scale[row] * sum_s R^(-s) acc[s];
     // one 16-column fragment at a time, so that the accumulators of finished
     // fragments are free and the FP64 sums stay within the register budget
@@ -256,21 +316,30 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
 template <int P>
 void launch(int n, int count, int RT32, int KS, const int8_t* A8, const double* isc,
             const int8_t* X8, int64_t ldx8, const double* W, int64_t ldw, double* part, int64_t ldp,
-            hipStream_t st) {
-  const int CT = (count + I8_TN - 1) / I8_TN;
-  const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
-  hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE>), dim3(G), dim3(256), 0, st, n, count, RT32, CT,
-                     KS, A8, isc, X8, ldx8, W, ldw, part, ldp);
+            int64_t plane_off, double inv_xscale, hipStream_t st) {
+  if (plane_off > 0) {
+    const int CT = (count + 127) / 128;
+    const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
+    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, true>), dim3(G), dim3(256), 0, st, n, count,
+                       RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off, inv_xscale);
+  } else {
+    const int CT = (count + I8_TN - 1) / I8_TN;
+    const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
+    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE>), dim3(G), dim3(256), 0, st, n, count, RT32,
+                       CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, (int64_t)0, 1.0);
+  }
 }
 
 }  // namespace
 
 void fgls_launch_trmm_i8(int p, int n, int count, int RT32, int KS, const int8_t* A8,
                          const double* inv_scale, const int8_t* X8, int64_t ldx8, const double* W,
-                         int64_t ldw, double* part, int64_t ldp, hipStream_t st) {
+                         int64_t ldw, double* part, int64_t ldp, int64_t plane_off,
+                         double inv_xscale, hipStream_t st) {
 #define FGLS_I8_CASE(P)                                                                      \
   case P:                                                                                    \
-    launch<P>(n, count, RT32, KS, A8, inv_scale, X8, ldx8, W, ldw, part, ldp, st);           \
+    launch<P>(n, count, RT32, KS, A8, inv_scale, X8, ldx8, W, ldw, part, ldp, plane_off,     \
+              inv_xscale, st);                                                               \
     break;
   switch (p) {
     FGLS_I8_CASE(1) FGLS_I8_CASE(2) FGLS_I8_CASE(3) FGLS_I8_CASE(4) FGLS_I8_CASE(5)

@@ -462,6 +462,35 @@ __global__ __launch_bounds__(256) void k_to_i8(int n, const double* __restrict__
   if (nb) atomicAdd(bad, nb);
 }
 
+// FP64 dosages -> two int8 digit planes: v = rint(x * scale) = 128 hi + lo,
+// lo in [0, 127] at X8, hi in [-127, 127] at X8 + plane_off (rows n..ldx8-1
+// zeroed).  A value with |x * scale - v| > 1e-6 or v outside
+// [-16256, 16383] is counted in *bad and written as 0.
+__global__ __launch_bounds__(256) void k_to_dosage_i8(int n, const double* __restrict__ X, int64_t ldx,
+                                                      double scale, int8_t* __restrict__ X8,
+                                                      int64_t ldx8, int64_t plane_off,
+                                                      unsigned int* __restrict__ bad) {
+  const int col = blockIdx.x;   // one workgroup per column
+  unsigned int nb = 0;
+  for (int r = threadIdx.x; r < ldx8; r += 256) {
+    int lo = 0, hi = 0;
+    if (r < n) {
+      const double y = X[(size_t)col * ldx + r] * scale;
+      const double v = rint(y);
+      if (!(fabs(y - v) <= 1e-6 && v >= -16256.0 && v <= 16383.0)) {
+        ++nb;
+      } else {
+        const int iv = (int)v;
+        hi = iv >> 7;     // floor(v / 128)
+        lo = iv & 127;
+      }
+    }
+    X8[(size_t)col * ldx8 + r] = (int8_t)lo;
+    X8[plane_off + (size_t)col * ldx8 + r] = (int8_t)hi;
+  }
+  if (nb) atomicAdd(bad, nb);
+}
+
 // int8 genotypes -> FP64 (column-major, ld ldx; rows n..ldx-1 written as 0).
 __global__ __launch_bounds__(256) void k_i8_to_f64(int n, const int8_t* __restrict__ X8, int64_t ldx8,
                                                    double* __restrict__ X, int64_t ldx) {
@@ -1172,8 +1201,14 @@ int fgls_hip_i8_to_f64(const int8_t* dX8, int64_t ldx8, int n, int ncols, double
   return FGLS_OK;
 }
 
-int fgls_hip_block_reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count,
-                             void* stream) {
+}  // extern "C"
+
+namespace {
+
+// The int8 product of a block: genotypes (planes = 1) or dosage digits
+// (planes = 2: lo at dX8, hi at dX8 + count * ldx8, x = (128 hi + lo) / scale).
+int reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count, int planes, double scale,
+              void* stream) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!c->have_trait) return fail(FGLS_ERR_STATE, "no trait set");
@@ -1184,16 +1219,29 @@ int fgls_hip_block_reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, i
   if (!dX8) return fail(FGLS_ERR_ARG, "NULL pointer");
   if (ldx8 < (int64_t)round_up(c->n, 64) || (ldx8 & 15) || ((uintptr_t)dX8 & 15))
     return fail(FGLS_ERR_ARG, "int8 X_R needs ld >= ceil(n/64)*64, ld %% 16 == 0 and 16-byte alignment");
-  if (ldx8 > ((int64_t)1 << 31) / 64)  // 32-bit offsets within a wave's 64 columns
-    return fail(FGLS_ERR_ARG, "int8 X_R ld %lld too large", (long long)ldx8);
+  // 32-bit offsets within a wave's columns (and, for dosages, to the hi plane)
+  if (ldx8 * (int64_t)(planes == 2 ? count + 64 : 64) > ((int64_t)1 << 31) - 1)
+    return fail(FGLS_ERR_ARG, "int8 X_R block of %d columns with ld %lld too large", count,
+                (long long)ldx8);
+  if (planes == 2 && !(scale > 0.0)) return fail(FGLS_ERR_ARG, "dosage scale must be positive");
   if ((rc = set_device(c))) return rc;
   hipStream_t st = (hipStream_t)stream;
   fgls_launch_trmm_i8(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, dX8, ldx8, c->W,
-                      (int64_t)c->ldn, c->part, (int64_t)c->max_block, st);
+                      (int64_t)c->ldn, c->part, (int64_t)c->max_block,
+                      planes == 2 ? (int64_t)count * ldx8 : 0, planes == 2 ? 1.0 / scale : 1.0, st);
   HIP_TRY(hipGetLastError());
   c->reduced = count;
   c->last_rt = c->RT8;
   return FGLS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgls_hip_block_reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count,
+                             void* stream) {
+  return reduce_i8(c, dX8, ldx8, count, 1, 1.0, stream);
 }
 
 int fgls_hip_solve_block_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count, double* dB,
@@ -1204,6 +1252,32 @@ int fgls_hip_solve_block_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, in
   rc = fgls_hip_block_reduce_i8(c, dX8, ldx8, count, stream);
   if (rc) return rc;
   return fgls_hip_block_finalize(c, count, dB, dV, snp_base, stream);
+}
+
+int fgls_hip_block_reduce_dosage(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count,
+                                 double scale, void* stream) {
+  return reduce_i8(c, dX8, ldx8, count, 2, scale, stream);
+}
+
+int fgls_hip_solve_block_dosage(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count,
+                                double scale, double* dB, double* dV, int64_t snp_base, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (count > 0 && (!dB || !dV)) return fail(FGLS_ERR_ARG, "NULL pointer");
+  rc = fgls_hip_block_reduce_dosage(c, dX8, ldx8, count, scale, stream);
+  if (rc) return rc;
+  return fgls_hip_block_finalize(c, count, dB, dV, snp_base, stream);
+}
+
+int fgls_hip_to_dosage_i8(const double* dX, int64_t ldx, int n, int ncols, double scale,
+                          int8_t* dX8, int64_t ldx8, unsigned int* dbad, void* stream) {
+  if (n < 1 || ncols < 0 || ldx < n || ldx8 < n || !dX || !dX8 || !dbad || !(scale > 0.0))
+    return fail(FGLS_ERR_ARG, "bad arguments to fgls_hip_to_dosage_i8");
+  if (ncols == 0) return FGLS_OK;
+  hipLaunchKernelGGL(k_to_dosage_i8, dim3(ncols), dim3(256), 0, (hipStream_t)stream, n, dX, ldx,
+                     scale, dX8, ldx8, (int64_t)ncols * ldx8, dbad);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
 }
 
 int fgls_hip_block_finalize(fgls_hip_ctx* c, int count, double* dB, double* dV, int64_t snp_base,

@@ -72,6 +72,10 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_trmm": (i32, [vp, dp, i64, i32, dp, i64, vp]),
             "fgls_hip_snp_stage": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
             "fgls_hip_check": (i32, [vp, vp, ctypes.POINTER(i64), ctypes.POINTER(i32)]),
+            "fgls_hip_to_dosage_i8": (i32, [dp, i64, i32, i32, ctypes.c_double, dp, i64, dp, vp]),
+            "fgls_hip_block_reduce_dosage": (i32, [vp, dp, i64, i32, ctypes.c_double, vp]),
+            "fgls_hip_solve_block_dosage": (i32, [vp, dp, i64, i32, ctypes.c_double, dp, dp, i64, vp]),
+            "fgls_hip_i8_to_f64": (i32, [dp, i64, i32, i32, dp, i64, vp]),
             "fgls_hip_set_trait_phi": (i32, [vp, dp, i64, ctypes.c_double, ctypes.c_double, dp, i64, dp, vp]),
             "fgls_hip_get_factor": (i32, [vp, dp, i64, vp]),
             "fgls_hip_adjoint_begin": (i32, [vp, dp, i64, ctypes.c_double, vp]),
@@ -100,7 +104,9 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_solve_block_i8", "fgls_hip_to_i8", "fgls_hip_trsm",
                         "fgls_hip_trmm", "fgls_hip_snp_stage", "fgls_hip_check",
                         "fgls_hip_adjoint_begin", "fgls_hip_adjoint_block", "fgls_hip_adjoint_end",
-                        "fgls_hip_set_trait_phi", "fgls_hip_get_factor",
+                        "fgls_hip_set_trait_phi", "fgls_hip_get_factor", "fgls_hip_to_dosage_i8",
+                        "fgls_hip_block_reduce_dosage", "fgls_hip_solve_block_dosage",
+                        "fgls_hip_i8_to_f64",
                         "fgls_factor_M")
             if hasattr(lib(), s)]
 
@@ -271,6 +277,37 @@ class DeviceContext:
         if nb:
             raise FglsError(FGLS_ERR_ARG, f"{nb} genotype values are not integers in [-127, 127]")
         return X8
+
+    def to_dosage_i8(self, X, scale: float = 1000.0, ncols: int | None = None, X8=None, stream=None):
+        """FP64 dosages (ncols, ldx) -> two int8 digit planes, a (2 * ncols, ld8)
+        tensor (lo plane first); raises if a value times scale is not an integer
+        in [-16256, 16383]."""
+        import torch
+        _need_cuda_f64(X, "X")
+        ncols = X.shape[0] if ncols is None else ncols
+        if X8 is None:
+            X8 = torch.empty((2 * ncols, self.ld8), dtype=torch.int8, device=X.device)
+        bad = torch.zeros(1, dtype=torch.int32, device=X.device)
+        _check(lib().fgls_hip_to_dosage_i8(_ptr(X), X.shape[1], self.n, ncols, float(scale), _ptr(X8),
+                                           X8.shape[1], _ptr(bad), _stream(stream)))
+        nb = int(bad.item())
+        if nb:
+            raise FglsError(FGLS_ERR_ARG, f"{nb} dosage values are not multiples of 1/{scale:g} in range")
+        return X8
+
+    def solve_block_dosage(self, X8, scale: float = 1000.0, count: int | None = None, B=None, V=None,
+                           snp_base: int = 0, stream=None):
+        """X8: (2 * count, ld8) digit planes from to_dosage_i8."""
+        import torch
+        if count is None:
+            count = X8.shape[0] // 2
+        if B is None:
+            B = torch.empty((count, self.p), dtype=torch.float64, device=X8.device)
+        if V is None:
+            V = torch.empty((count, self.p * self.p), dtype=torch.float64, device=X8.device)
+        _check(lib().fgls_hip_solve_block_dosage(self._h, _ptr(X8), X8.shape[1], count, float(scale),
+                                                 _ptr(B), _ptr(V), snp_base, _stream(stream)))
+        return B, V
 
     def block_reduce_i8(self, X8, count: int | None = None, stream=None):
         count = X8.shape[0] if count is None else count

@@ -133,6 +133,25 @@ int fgls_hip_solve_block_i8(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ldx8,
  * caller-cleared) and written as 0.  Asynchronous. */
 int fgls_hip_to_i8(const double *dX, int64_t ldx, int n, int ncols,
                    int8_t *dX8, int64_t ldx8, unsigned int *dbad, void *stream);
+/*
+ * Mixed mode for dosages (no counterpart in the reference): values with a
+ * fixed number of decimals, e.g. imputed dosages rounded to 0.001, as
+ * v = x * scale = 128 hi + lo with two int8 digits lo in [0, 127] and
+ * hi in [-127, 127] (|v| <= 16,256).  dX8 holds two planes of count columns
+ * (ld ldx8): lo at dX8, hi at dX8 + count * ldx8.  The product costs twice
+ * the int8 work of genotypes; b and V as fgls_hip_solve_block_i8.
+ * to_dosage_i8 builds the planes from FP64 values (ncols columns, planes
+ * ncols * ldx8 apart), counting into *dbad the values that are not within
+ * 1e-6 of an integer after scaling or out of range.  Asynchronous.
+ */
+int fgls_hip_to_dosage_i8(const double *dX, int64_t ldx, int n, int ncols, double scale,
+                          int8_t *dX8, int64_t ldx8, unsigned int *dbad, void *stream);
+int fgls_hip_block_reduce_dosage(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ldx8,
+                                 int count, double scale, void *stream);
+int fgls_hip_solve_block_dosage(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ldx8,
+                                int count, double scale, double *dB, double *dV,
+                                int64_t snp_base, void *stream);
+
 /* int8 genotypes (ld ldx8) -> FP64 (ld ldx, rows n..ldx-1 zeroed), for
  * compact genotype input on the FP64 path.  No counterpart in the
  * reference (its X_R is FP64 only, io.c:35-55).  Asynchronous. */

@@ -89,3 +89,34 @@ def test_mixed_singular_snp_reports_index(pkg, gpu):
         ctx.check()
     assert ei.value.bad_snp == 51 and ei.value.info == 4
     ctx.close()
+
+
+@pytest.mark.parametrize("n,m,wXL,block", [(1000, 700, 3, 700), (333, 300, 7, 128), (2500, 257, 5, 257)])
+def test_mixed_dosages_match_oracle(pkg, gpu, n, m, wXL, block):
+    """Imputed dosages (3 decimals in [0, 2]) on the int8 path as two digit
+    planes: same bound as genotypes against the FP64 oracle."""
+    print(f"mixed dosages n={n} m={m} wXL={wXL}", flush=True)
+    import torch
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=3 * n + m, h2=0.45, dosage="imputed")
+    tr, ctx = setup(pkg, prob, max_block=block)
+    ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)
+    b_ref, V_ref = orc.solve_snps(prob.XR, tr, 1)
+    XR = pkg.colmajor(prob.XR)
+    Bs, Vs = [], []
+    for i0 in range(0, m, block):
+        cnt = min(block, m - i0)
+        X8 = ctx.to_dosage_i8(XR[i0:i0 + cnt].contiguous(), 1000.0)
+        B, V = ctx.solve_block_dosage(X8, 1000.0, snp_base=i0)
+        Bs.append(B.cpu().numpy())
+        Vs.append(V.cpu().numpy())
+    ctx.check()
+    e = orc.parity_errors(np.concatenate(Bs), np.concatenate(Vs), b_ref, v_flat(V_ref), wXL + 1)
+    assert e["V_upper_zero"]
+    assert e["B_scaled"] < TOL_SCALED and e["V_scaled"] < TOL_SCALED, e
+    assert e["var_rel"] < TOL_VAR, e
+    # values that are not multiples of 1/1000 are rejected
+    bad = XR[:4].clone()
+    bad[0, 5] += 0.0004
+    with pytest.raises(pkg.FglsError):
+        ctx.to_dosage_i8(bad, 1000.0)
+    ctx.close()
