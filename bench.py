@@ -217,7 +217,9 @@ def main():
     ap.add_argument("--check-snps", type=int, default=512)
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "none"], default="auto")
     ap.add_argument("--cpu-snps", type=int, default=16384)
-    ap.add_argument("--paths", default="fp64,mixed", help="comma-separated: fp64, mixed")
+    ap.add_argument("--paths", default="fp64,mixed,dosage",
+                    help="comma-separated: fp64, mixed (int8 genotypes), dosage (3-decimal dosages "
+                         "as two int8 digit planes)")
     ap.add_argument("--adjoint-m", type=int, default=200_000,
                     help="SNPs of the adjoint measurement at n=5000, p=4, blocks of 2048 (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -270,9 +272,35 @@ def main():
     log(f"trait setup on the device (potrf of M + precompute): {trait_setup * 1e3:.1f} ms")
     starts = list(range(0, m, block))
 
+    D8 = None
+    if "dosage" in args.paths.split(","):
+        # dosages with 3 decimals, round(U(0, 2), 3), as per-block pairs of
+        # int8 digit planes (lo, hi) of 1000 x; the FP64 values of a sample
+        # are rebuilt for the parity check
+        g = torch.Generator(device=dev)
+        g.manual_seed(args.seed + 7)
+        ld8 = (n + 63) // 64 * 64
+        D8 = []
+        for i0 in range(0, m, block):
+            cnt = min(block, m - i0)
+            v = torch.randint(0, 2001, (cnt, n), dtype=torch.int32, device=dev, generator=g)
+            t8 = torch.zeros((2 * cnt, ld8), dtype=torch.int8, device=dev)
+            t8[:cnt, :n] = (v & 127).to(torch.int8)
+            t8[cnt:, :n] = (v >> 7).to(torch.int8)
+            D8.append(t8)
+            del v
+        torch.cuda.synchronize()
+        log(f"dosage planes ready: {sum(t.numel() for t in D8) / 1e9:.1f} GB int8")
+
     def run_path(mode):
         """Time `args.steps` passes of one path; returns outputs and timings."""
-        if mode == "mixed":
+        if mode == "dosage":
+            t0 = time.perf_counter()
+            ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)
+            torch.cuda.synchronize()
+            setup_s = time.perf_counter() - t0
+            reduce = lambda i0, cnt: ctx.block_reduce_dosage(D8[i0 // block], cnt, 1000.0)
+        elif mode == "mixed":
             t0 = time.perf_counter()
             ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)      # slices L^-1 of the current trait
             torch.cuda.synchronize()
@@ -338,7 +366,15 @@ def main():
         Vr = V_ref.transpose(0, 2, 1).reshape(idx.size, -1)
         sel = torch.as_tensor(idx, device=dev)
         for mode, res in paths.items():
-            e = orc.parity_errors(res["B"][sel].cpu().numpy(), res["V"][sel].cpu().numpy(), b_ref, Vr, p)
+            if mode == "dosage":
+                Xd = np.stack([(D8[i // block][i % block, :n].cpu().numpy().astype(np.int64) +
+                                128 * D8[i // block][min(block, m - (i // block) * block) + i % block, :n]
+                                .cpu().numpy().astype(np.int64)) / 1000.0 for i in idx], axis=1)
+                bd, Vd = orc.solve_snps(Xd, tr, 1)
+                e = orc.parity_errors(res["B"][sel].cpu().numpy(), res["V"][sel].cpu().numpy(), bd,
+                                      Vd.transpose(0, 2, 1).reshape(idx.size, -1), p)
+            else:
+                e = orc.parity_errors(res["B"][sel].cpu().numpy(), res["V"][sel].cpu().numpy(), b_ref, Vr, p)
             check[mode] = dict(e, snps=int(idx.size))
             log(f"{mode} parity vs oracle on {idx.size} SNPs: {e}")
 
@@ -363,13 +399,15 @@ def main():
                     "frac": ach / PEAK_FP64_TFLOPS, "traffic": traffic(mode),
                     "kernel": "k_trmm<EPI_REDUCE> (fgls_hip_block_reduce)",
                     "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block, "flop_per_snp": float(n) * n}
-        ops = NSLICE * flop                         # int8 ops the 6-slice product executes
+        planes = 2 if mode == "dosage" else 1
+        ops = planes * NSLICE * flop                # int8 ops the 6-slice product executes
         ach = ops / (res["reduce_ms"] * 1e-3) / 1e12
         return {"bound": "mfma", "achieved": ach, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_INT8_TOPS, "traffic": traffic(mode),
-                "kernel": "k_trmm_i8 (fgls_hip_block_reduce_i8)", "achieved_is": "int8 TOP/s",
+                "kernel": ("k_trmm_i8 dosage planes (fgls_hip_block_reduce_dosage)" if planes == 2
+                           else "k_trmm_i8 (fgls_hip_block_reduce_i8)"), "achieved_is": "int8 TOP/s",
                 "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block,
-                "int8_ops_per_snp": NSLICE * float(n) * n,
+                "int8_ops_per_snp": planes * NSLICE * float(n) * n,
                 "fp64_equivalent_tflops": flop / (res["reduce_ms"] * 1e-3) / 1e12}
 
     head = "mixed" if "mixed" in paths else "fp64"
@@ -386,7 +424,7 @@ def main():
         log(f"mixed vs fp64 over all {m} SNPs: {e}; speedup {summary['speedup_mixed_over_fp64']:.2f}x")
     adjoint = None
     if args.adjoint_m > 0:
-        del X, X8, paths
+        del X, X8, D8, paths
         torch.cuda.empty_cache()
         adjoint = run_adjoint(torch, pkg, dev, args.adjoint_m, args.steps)
         log(f"adjoint (n=5000, m={args.adjoint_m}, block 2048): {adjoint['snps_per_s']:.0f} SNPs/s, "
@@ -408,7 +446,9 @@ def main():
                    "n": n, "p": p, "m": m, "block": block, "path": head,
                    "paths": {"fp64": "FP64 genotypes, FP64 MFMA triangular product through L^-1",
                              "mixed": "int8 genotypes x 6 int8 digit slices (radix 254) of the FP64 L^-1 on the int8 "
-                                      "MFMA, exact int32 accumulation, FP64 recombination and p x p solves"}},
+                                      "MFMA, exact int32 accumulation, FP64 recombination and p x p solves",
+                             "dosage": "dosages round(U(0,2),3) as two int8 digit planes (1000 x = 128 hi + lo) "
+                                       "through the same int8 product, twice its int8 work"}},
         "roofline": roofline(head, res),
         "cpu_baseline": cpu,
         "trait_setup_s": trait_setup,

@@ -55,6 +55,7 @@ typedef struct {
   int host_potrf;    /* potrf=host: M and its factor on the host (OpenBLAS) */
   int traits_per_pass;  /* traits sharing one read of X_R (0: as many as fit) */
   int xr_i8;         /* xr_format=i8: X_R file of int8 values (n bytes per column) */
+  double dosage_scale;  /* precision=mixed for dosages: x * scale = 128 hi + lo (0: genotypes) */
 } options_t;
 
 static double now_s(void) {
@@ -89,6 +90,11 @@ static void parse_options(const char *s, options_t *o) {
     }
     if (!strcmp(tok, "precision")) {
       o->mixed = !strcmp(eq + 1, "mixed") ? 1 : (!strcmp(eq + 1, "fp64") ? 0 : -1);
+      continue;
+    }
+    if (!strcmp(tok, "dosage_scale")) {
+      o->dosage_scale = atof(eq + 1);
+      if (!(o->dosage_scale > 0.0)) o->dosage_scale = -1.0;
       continue;
     }
     long long v = atoll(eq + 1);
@@ -247,11 +253,15 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
                      const char *Y_path, const char *B_path, const char *V_path,
                      const char *options) {
   int rc = FGLS_OK;
-  options_t opt = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  options_t opt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0.0};
   parse_options(options ? options : getenv("FGLS_OPTIONS"), &opt);
   if (opt.mixed < 0) return fgls_set_error(FGLS_ERR_ARG, "precision must be fp64 or mixed");
   if (opt.host_potrf < 0) return fgls_set_error(FGLS_ERR_ARG, "potrf must be gpu or host");
   if (opt.xr_i8 < 0) return fgls_set_error(FGLS_ERR_ARG, "xr_format must be f64 or i8");
+  if (opt.dosage_scale < 0.0) return fgls_set_error(FGLS_ERR_ARG, "dosage_scale must be positive");
+  if (opt.dosage_scale > 0.0 && (!opt.mixed || opt.xr_i8))
+    return fgls_set_error(FGLS_ERR_ARG, "dosage_scale needs precision=mixed and an FP64 X_R file");
+  const int dos = opt.dosage_scale > 0.0;
   if (opt.mixed && wXR != 1)
     return fgls_set_error(FGLS_ERR_ARG, "precision=mixed needs wXR = 1 (got %d)", wXR);
 
@@ -373,9 +383,11 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   if ((opt.mixed || opt.xr_i8) && nchunks > 0) {
     HCHK(hipMalloc((void **)&dBad, 2 * sizeof(unsigned int)));
     for (int s = 0; s < 2; ++s) {
-      /* rows n..ld8-1 stay zero: uploads and conversions write rows < n */
-      HCHK(hipMalloc((void **)&dX8[s], (size_t)block * wXR * ld8));
-      HCHK(hipMemset(dX8[s], 0, (size_t)block * wXR * ld8));
+      /* rows n..ld8-1 stay zero: uploads and conversions write rows < n;
+       * dosages take two digit planes */
+      const size_t b8 = (size_t)block * wXR * ld8 * (dos ? 2 : 1);
+      HCHK(hipMalloc((void **)&dX8[s], b8));
+      HCHK(hipMemset(dX8[s], 0, b8));
     }
   }
   if (nchunks > 0) {
@@ -462,7 +474,10 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
           /* exact FP64 -> int8 on the device; a value that is not an integer
            * in [-127, 127] is counted and fails the run at write-back */
           HCHK(hipMemsetAsync(dBad + s, 0, sizeof(unsigned int), s_comp));
-          RCHK(fgls_hip_to_i8(dX[s], ldx, n, cnt, dX8[s], ld8, dBad + s, s_comp));
+          if (dos)
+            RCHK(fgls_hip_to_dosage_i8(dX[s], ldx, n, cnt, opt.dosage_scale, dX8[s], ld8, dBad + s, s_comp));
+          else
+            RCHK(fgls_hip_to_i8(dX[s], ldx, n, cnt, dX8[s], ld8, dBad + s, s_comp));
           HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
         } else if (!opt.mixed && opt.xr_i8) {
           RCHK(fgls_hip_i8_to_f64(dX8[s], ld8, n, cnt * wXR, dX[s], ldx, s_comp));
@@ -470,7 +485,10 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
         for (int k = 0; k < kn; ++k) {
           const int q = s * K + k;
           const int64_t base = (int64_t)(j0 + k) * m + i0;
-          if (opt.mixed)
+          if (dos)
+            RCHK(fgls_hip_solve_block_dosage(ctx[k], dX8[s], ld8, cnt, opt.dosage_scale, dB[q], dV[q],
+                                             base, s_comp));
+          else if (opt.mixed)
             RCHK(fgls_hip_solve_block_i8(ctx[k], dX8[s], ld8, cnt, dB[q], dV[q], base, s_comp));
           else
             RCHK(fgls_hip_solve_block(ctx[k], dX[s], ldx, cnt, dB[q], dV[q], base, s_comp));
@@ -492,9 +510,15 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
         const int cnt = (int)((m - i0) < block ? (m - i0) : block);
         HCHK(hipEventSynchronize(ev_out[ps]));
         if (hBad[ps] != 0) {
-          rc = fgls_set_error(FGLS_ERR_ARG,
-                              "precision=mixed: %u genotype values of SNPs %lld.. are not integers in "
-                              "[-127, 127] (use precision=fp64)", hBad[ps], (long long)i0);
+          if (dos)
+            rc = fgls_set_error(FGLS_ERR_ARG,
+                                "precision=mixed: %u dosage values of SNPs %lld.. are not multiples of "
+                                "1/%g in range (use precision=fp64)", hBad[ps], (long long)i0,
+                                opt.dosage_scale);
+          else
+            rc = fgls_set_error(FGLS_ERR_ARG,
+                                "precision=mixed: %u genotype values of SNPs %lld.. are not integers in "
+                                "[-127, 127] (use precision=fp64)", hBad[ps], (long long)i0);
           goto done;
         }
         for (int k = 0; k < kn; ++k) {

@@ -295,6 +295,12 @@ class DeviceContext:
             raise FglsError(FGLS_ERR_ARG, f"{nb} dosage values are not multiples of 1/{scale:g} in range")
         return X8
 
+    def block_reduce_dosage(self, X8, count: int | None = None, scale: float = 1000.0, stream=None):
+        count = X8.shape[0] // 2 if count is None else count
+        _check(lib().fgls_hip_block_reduce_dosage(self._h, _ptr(X8), X8.shape[1], count, float(scale),
+                                                  _stream(stream)))
+        return count
+
     def solve_block_dosage(self, X8, scale: float = 1000.0, count: int | None = None, B=None, V=None,
                            snp_base: int = 0, stream=None):
         """X8: (2 * count, ld8) digit planes from to_dosage_i8."""

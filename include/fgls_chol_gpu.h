@@ -54,6 +54,11 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *                  reference's FGLS_GPU_SERIAL, fgls_chol_gpu.c:44-46)
  *   stats=1        print stage timings and rates to stderr
  *   io_threads=<k> parallel preads of each X_R block (default 4)
+ *   dosage_scale=<s>  with precision=mixed: X_R values are dosages with a
+ *                  fixed number of decimals (s = 1000 for 3 decimals), run as
+ *                  two int8 digits per value (twice the int8 work of
+ *                  genotypes); a value that is not a multiple of 1/s is an
+ *                  FGLS_ERR_ARG
  *   xr_format=i8   XR_path holds int8 values (m*wXR*n bytes, the same order
  *                  as the FP64 file): 8x fewer bytes from disk and over
  *                  PCIe; used as is by precision=mixed, converted to FP64 on

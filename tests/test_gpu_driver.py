@@ -202,3 +202,27 @@ def test_int8_genotype_file(pkg, gpu, tmp_path, precision):
                           f("XL.in"), f("XR8s.in"), f("Y.in"), f("B_s.out"), f("V_s.out"),
                           options={"precision": precision, "xr_format": "i8"})
     assert ei.value.code == pkg.FGLS_ERR_IO and "not read completely" in ei.value.msg
+
+
+def test_fgls_chol_gpu_mixed_dosages(pkg, gpu, tmp_path):
+    """precision=mixed with dosage_scale=1000 on imputed dosages (3 decimals)
+    through the file pipeline, against the oracle."""
+    print("fgls_chol_gpu precision=mixed dosage_scale=1000", flush=True)
+    n, m, t, wXL = 900, 1300, 2, 3
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=99, t=t, dosage="imputed")
+    d = str(tmp_path)
+    H, Sig = np.array([0.3, 0.55]), np.array([1.0, 0.8])
+    write_problem_dir(prob, d, H=H, Sig=Sig)
+    p = wXL + 1
+    f = lambda s: os.path.join(d, s)
+    pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, 256, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                      f("XL.in"), f("XR.in"), f("Y.in"), f("B.out"), f("V.out"),
+                      options={"precision": "mixed", "dosage_scale": 1000, "block": 512})
+    Bo, Vo = orc.fgls_chol(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y, wXR=1)
+    e = orc.parity_errors(read_out(d, "B.out", t * m * p), read_out(d, "V.out", t * m * p * p), Bo, Vo, p)
+    assert e["B_scaled"] < 2e-11 and e["var_rel"] < 1e-12 and e["V_upper_zero"], e
+    with pytest.raises(pkg.FglsError) as ei:      # the same data as genotypes: not int8-exact
+        pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, 256, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f("XR.in"), f("Y.in"), f("B2.out"), f("V2.out"),
+                          options={"precision": "mixed"})
+    assert ei.value.code == pkg.FGLS_ERR_ARG
