@@ -5,12 +5,15 @@ Import name: ``cugwas_proto_amd`` (the directory name carries a hyphen, so
 """
 from .fgls import (  # noqa: F401
     DeviceContext,
+    EigenContext,
     FglsError,
     colmajor,
     device_count,
     exported_symbols,
+    eigen_decompose,
     factor_M,
     fgls_chol_gpu,
+    fgls_eigen,
     last_error,
     lib,
     LIB_PATH,

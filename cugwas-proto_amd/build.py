@@ -7,6 +7,7 @@ Outputs (git-ignored, shipped with the tree):
 """
 from __future__ import annotations
 
+import concurrent.futures
 import glob
 import os
 import shutil
@@ -61,19 +62,25 @@ def build(verbose: bool = False) -> dict:
     blasdir = os.path.dirname(blas)
     blasname = os.path.basename(blas)
 
-    kobj = os.path.join(objdir, "fgls_kernels.o")
-    run([hipcc(), *HIP_FLAGS, "-I", INCLUDE, "-c", os.path.join(CSRC, "fgls_kernels.hip"), "-o", kobj], verbose)
-    # the int8 kernel keeps its MFMA accumulators in VGPRs (see fgls_i8.hip)
-    i8obj = os.path.join(objdir, "fgls_i8.o")
-    run([hipcc(), *HIP_FLAGS, "-mllvm", "-amdgpu-mfma-vgpr-form=1", "-I", INCLUDE, "-c",
-         os.path.join(CSRC, "fgls_i8.hip"), "-o", i8obj], verbose)
-
-    extra_objs = []
-    for name in ("fgls_adjoint", "fgls_potrf"):
+    # one hipcc per translation unit, run concurrently
+    hip_units = {
+        "fgls_kernels": [],
+        # the int8 kernel keeps its MFMA accumulators in VGPRs (see fgls_i8.hip)
+        "fgls_i8": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+        "fgls_adjoint": [],
+        "fgls_potrf": [],
+        "fgls_eigen": [],
+    }
+    hip_objs = []
+    cmds = []
+    for name, extra in hip_units.items():
         obj = os.path.join(objdir, name + ".o")
-        run([hipcc(), *HIP_FLAGS, "-I", INCLUDE, "-c", os.path.join(CSRC, name + ".hip"), "-o", obj],
-            verbose)
-        extra_objs.append(obj)
+        cmds.append([hipcc(), *HIP_FLAGS, *extra, "-I", INCLUDE, "-c", os.path.join(CSRC, name + ".hip"),
+                     "-o", obj])
+        hip_objs.append(obj)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(cmds)) as pool:
+        for f in [pool.submit(run, c, verbose) for c in cmds]:
+            f.result()
 
     host_objs = []
     for src in sorted(glob.glob(os.path.join(CSRC, "*.c"))):
@@ -87,7 +94,7 @@ def build(verbose: bool = False) -> dict:
 
     so = os.path.join(LIBDIR, "libfgls_amd.so")
     tmp = so + ".tmp"
-    run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, kobj, i8obj, *extra_objs, *host_objs,
+    run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *hip_objs, *host_objs,
          "-L", blasdir, f"-l:{blasname}", f"-Wl,-rpath,{blasdir}", "-fopenmp", "-lpthread", "-lrt"],
         verbose)
     os.replace(tmp, so)

@@ -10,7 +10,7 @@
  * the last stdout line (driver.c:98-100, get_diff_ms rounding).
  *
  * Variants: 'g' (the GPU solver, driver.c:78-86) and 'c' run fgls_chol_gpu;
- * 'e' (fgls_eigen) is not part of this build and exits with an error.
+ * 'e' runs fgls_eigen (driver.c:70-77), the eigenbasis variant on the GPU.
  * --check reproduces the -DDEBUG comparison (driver.c:102-154): the max
  * element-wise relative error of B.out against B_exp.out and of V.out
  * against V_exp.out when those files exist (V over its lower triangle).
@@ -22,6 +22,7 @@
 #include <sys/time.h>
 
 #include "fgls_chol_gpu.h"
+#include "fgls_eigen.h"
 
 #define PATH_LEN 4096
 
@@ -62,7 +63,7 @@ static double compare(const double *a, const double *b, size_t size, int p) {
 
 int main(int argc, char *argv[]) {
   if (argc != 4 && !(argc == 5 && strcmp(argv[4], "--check") == 0)) {
-    fprintf(stderr, "Usage: %s <variant - [g|c]> <data_set_directory> <#repetitions> [--check]\n", argv[0]);
+    fprintf(stderr, "Usage: %s <variant - [g|c|e]> <data_set_directory> <#repetitions> [--check]\n", argv[0]);
     fprintf(stderr, "\tnote: data_set_directory must contain files:\n");
     fprintf(stderr, "- Phi.in\n- H.in\n- Sig.in\n- XL.in\n- XR.in\n- Y.in\n");
     return EXIT_FAILURE;
@@ -76,10 +77,6 @@ int main(int argc, char *argv[]) {
     return EXIT_FAILURE;
   }
   const int wXR = p - wXL;
-  if (var == 'e') {
-    fprintf(stderr, "[Error]: variant 'e' (fgls_eigen) is not part of this build\n");
-    return EXIT_FAILURE;
-  }
   char Phi[PATH_LEN], H[PATH_LEN], Sig[PATH_LEN], XL[PATH_LEN], XR[PATH_LEN], Y[PATH_LEN],
       B[PATH_LEN], V[PATH_LEN];
   snprintf(XL, PATH_LEN, "%s/XL.in", dir);
@@ -93,8 +90,12 @@ int main(int argc, char *argv[]) {
 
   struct timeval start, end;
   gettimeofday(&start, NULL);
-  for (int i = 0; i < nrep; ++i)
-    fgls_chol_gpu(n, p, m, t, wXL, wXR, x_b, y_b, nths, Phi, H, Sig, XL, XR, Y, B, V);
+  for (int i = 0; i < nrep; ++i) {
+    if (var == 'e')
+      fgls_eigen(n, p, m, t, wXL, wXR, x_b, y_b, nths, Phi, H, Sig, XL, XR, Y, B, V);
+    else
+      fgls_chol_gpu(n, p, m, t, wXL, wXR, x_b, y_b, nths, Phi, H, Sig, XL, XR, Y, B, V);
+  }
   gettimeofday(&end, NULL);
   long total = nrep > 0 ? diff_ms(&start, &end) / nrep : 0;
 

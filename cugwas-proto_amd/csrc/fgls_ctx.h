@@ -86,6 +86,34 @@ void launch_product_store(const fgls_hip_ctx* c, const double* A, bool dense, co
 // y * X[row, col] and y * ctx->W[:, k] go to ctx->part (p + 1 per column).
 void launch_product_dot(const fgls_hip_ctx* c, const double* A, const double* X, int64_t ldx,
                         int count, hipStream_t st);
+// Dense Z = A X (A: full row-major ldn x ldn, rows >= n zero), column-major
+// out (ld ldz >= ldn).
+void launch_dense_store(int n, int ldn, const double* A, const double* X, int64_t ldx, int ncols,
+                        double* Z, int64_t ldz, hipStream_t st);
+// Dense FP64 product of a block with ntr groups of p + 1 weight columns
+// (group k at W + k (p+1) ldw: column 0 weighs z*z, the others z); per
+// 128-row tile, trait and SNP the p + 1 sums go to
+// part[((rt ntr + k)(p+1) + q) ldp + col].
+void launch_eig_reduce(int p, int n, int ldn, const double* A, const double* X, int64_t ldx,
+                       int count, const double* W, int64_t ldw, int ntr, double* part, int64_t ldp,
+                       hipStream_t st);
+// b and V of every SNP and trait from partials laid out as launch_eig_reduce's
+// (RT row tiles; trait k: scalars at trait + k trait_ld, outputs at dB + k ldb
+// and dV + k ldv, failure record err[k]).
+void launch_finalize_multi(int p, int count, int RT, const double* part, int64_t ldp, int ntr,
+                           const double* trait, int64_t trait_ld, double* dB, int64_t ldb,
+                           double* dV, int64_t ldv, unsigned long long* err, int64_t base,
+                           hipStream_t st);
+// B_t, V_tl, b_L and res_sigma of one trait from W = [X~_L | y~] (n x (wXL+1),
+// ld ldw) into trait (TRAIT_LD doubles); status[0] = 0 or the LAPACK-style
+// info of the Cholesky of V_tl.
+void launch_trait_stats(int n, int wXL, const double* W, int64_t ldw, double* trait, int* status,
+                        hipStream_t st);
+// int8 digit slices and row scales of a row-major ldn x ldn matrix.
+void launch_slices(int ldn, const double* A, int8_t* A8, double* invsc, hipStream_t st);
+// Doubles per trait of the per-trait scalars.
+constexpr int TRAIT_LD = 2 * PMAX + PMAX * PMAX + 8;
+
 // b and V of every SNP of a block from its transformed columns Z.
 void launch_snp_stage_any(const fgls_hip_ctx* c, const double* Z, int64_t ldz, int count,
                           double* dB, double* dV, int64_t base, hipStream_t st);

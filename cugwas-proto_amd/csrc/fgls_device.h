@@ -61,10 +61,12 @@ __host__ __device__ constexpr size_t fgls_a8_offset(int row, int s, int kc, int 
 // Launch of the int8 triangular product with fused reductions (fgls_i8.hip),
 // for p = 1..16.  part receives (p+1) sums per SNP per 32-row tile.  With
 // plane_off > 0 the genotypes are dosages as two int8 digit planes (lo at X8,
-// hi at X8 + plane_off, x = (128 hi + lo) * inv_xscale).
+// hi at X8 + plane_off, x = (128 hi + lo) * inv_xscale).  eig_ntr > 0: A8
+// holds a full matrix and W has eig_ntr groups of p+1 weight columns (the
+// eigenbasis variant; part then holds eig_ntr (p+1) sums per SNP and tile).
 void fgls_launch_trmm_i8(int p, int n, int count, int RT32, int KS, const int8_t* A8,
                          const double* inv_scale, const int8_t* X8, int64_t ldx8, const double* W,
                          int64_t ldw, double* part, int64_t ldp, int64_t plane_off,
-                         double inv_xscale, hipStream_t st);
+                         double inv_xscale, int eig_ntr, hipStream_t st);
 
 #endif

@@ -17,6 +17,12 @@
  * (LAPACK dpotrf, as fgls_chol_gpu.c:406-430), L is uploaded once and the
  * device precompute runs (fgls_hip_set_trait).  X_R is streamed once per
  * trait (the CPU twin's trait loop, fgls_chol.c:223 and :336-337).
+ *
+ * fgls_eigen() (include/fgls_eigen.h, the reference's src/fgls_eigen.c:37-236)
+ * runs the same pipeline in the eigenbasis of Phi: one host dsyevr of Phi
+ * (fgls_eigen.c:780-806), Z and its eigenvalues uploaded once, and one dense
+ * product Z' x_R per block serving every trait of a pass
+ * (fgls_hip_eig_solve_block).
  */
 #define _GNU_SOURCE
 #include <errno.h>
@@ -38,12 +44,18 @@
 #include <hip/hip_runtime_api.h>
 
 #include "fgls_chol_gpu.h"
+#include "fgls_eigen.h"
 #include "fgls_hip.h"
 #include "fgls_internal.h"
 
 /* LP64 LAPACK of the OpenBLAS build the library is linked with. */
 extern void scipy_dpotrf_(const char *uplo, const int *n, double *a, const int *lda, int *info);
 extern void scipy_openblas_set_num_threads(int n);
+extern void scipy_dsyevr_(const char *jobz, const char *range, const char *uplo, const int *n,
+                          double *a, const int *lda, const double *vl, const double *vu,
+                          const int *il, const int *iu, const double *abstol, int *m, double *w,
+                          double *z, const int *ldz, int *isuppz, double *work, const int *lwork,
+                          int *iwork, const int *liwork, int *info);
 
 typedef struct {
   int64_t block;     /* SNPs per device launch */
@@ -236,6 +248,62 @@ int fgls_factor_M(int n, const double *Phi, int64_t ldphi, double h2, double sig
   return FGLS_OK;
 }
 
+int fgls_eigen_decompose(int n, const double *Phi, int64_t ldphi, double *Z, int64_t ldz, double *W,
+                         int threads) {
+  if (n < 1 || !Phi || !Z || !W || ldphi < n || ldz < n)
+    return fgls_set_error(FGLS_ERR_ARG, "fgls_eigen_decompose: bad arguments");
+  if (threads < 1) threads = host_cores();
+  /* dsyevr destroys its input: work on a copy of the lower triangle */
+  double *A = (double *)malloc((size_t)n * n * sizeof(double));
+  int *isuppz = (int *)malloc(2 * (size_t)n * sizeof(int));
+  double *work = NULL;
+  int *iwork = NULL;
+  int rc = FGLS_OK;
+  if (!A || !isuppz) {
+    rc = fgls_set_error(FGLS_ERR_NOMEM, "host allocation for dsyevr failed");
+    goto out;
+  }
+  for (int64_t c = 0; c < n; ++c) memcpy(A + (size_t)c * n, Phi + (size_t)c * ldphi, (size_t)n * sizeof(double));
+  scipy_openblas_set_num_threads(threads);
+  {
+    /* eigenDec, fgls_eigen.c:780-806: dsyevr("V", "A", "L"); workspace by query */
+    const double dd = -1.0;
+    const int id = 0, ld = (int)ldz;
+    int nfound = 0, info = 0, lwork = -1, liwork = -1, iw = 0;
+    double wq = 0.0;
+    scipy_dsyevr_("V", "A", "L", &n, A, &n, &dd, &dd, &id, &id, &dd, &nfound, W, Z, &ld, isuppz, &wq,
+                  &lwork, &iw, &liwork, &info);
+    lwork = (int)wq + 1;
+    liwork = iw > 0 ? iw : 10 * n;
+    work = (double *)malloc((size_t)lwork * sizeof(double));
+    iwork = (int *)malloc((size_t)liwork * sizeof(int));
+    if (!work || !iwork) {
+      rc = fgls_set_error(FGLS_ERR_NOMEM, "host allocation for dsyevr failed");
+      goto out;
+    }
+    scipy_dsyevr_("V", "A", "L", &n, A, &n, &dd, &dd, &id, &id, &dd, &nfound, W, Z, &ld, isuppz, work,
+                  &lwork, iwork, &liwork, &info);
+    if (info != 0 || nfound != n)
+      rc = fgls_set_error(FGLS_ERR_NOTSPD, "dsyevr failed (info: %d)", info);
+  }
+out:
+  free(A);
+  free(isuppz);
+  free(work);
+  free(iwork);
+  return rc;
+}
+
+/* Device bytes of one eigen-context trait beyond the shared buffers. */
+static size_t eig_trait_bytes(int n, int p, int64_t block, int mixed) {
+  const size_t ldn = ((size_t)n + 127) / 128 * 128;
+  const size_t rt = mixed ? ldn / 32 : ldn / 128;
+  size_t b = rt * (size_t)(p + 1) * (size_t)block * sizeof(double);   /* partial sums */
+  b += ldn * (size_t)(2 * p + 3) * sizeof(double);                   /* weights, Y, Z'Y */
+  b += (size_t)block * ((size_t)p + (size_t)p * p) * sizeof(double) * 2;   /* B, V slots */
+  return b;
+}
+
 /* Device bytes of one trait's context beyond the shared buffers. */
 static size_t ctx_bytes(int n, int p, int64_t block, int mixed) {
   const size_t ldn = ((size_t)n + 127) / 128 * 128;
@@ -247,11 +315,12 @@ static size_t ctx_bytes(int n, int p, int64_t block, int mixed) {
   return b + (64u << 20);
 }
 
-int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int y_b,
-                     int num_threads, const char *Phi_path, const char *h2_path,
-                     const char *sigma2_path, const char *XL_path, const char *XR_path,
-                     const char *Y_path, const char *B_path, const char *V_path,
-                     const char *options) {
+/* The streaming solver; eig = 0: Cholesky variant, 1: eigenbasis variant. */
+static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x_b, int y_b,
+                    int num_threads, const char *Phi_path, const char *h2_path,
+                    const char *sigma2_path, const char *XL_path, const char *XR_path,
+                    const char *Y_path, const char *B_path, const char *V_path,
+                    const char *options) {
   int rc = FGLS_OK;
   options_t opt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0.0};
   parse_options(options ? options : getenv("FGLS_OPTIONS"), &opt);
@@ -264,6 +333,8 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   const int dos = opt.dosage_scale > 0.0;
   if (opt.mixed && wXR != 1)
     return fgls_set_error(FGLS_ERR_ARG, "precision=mixed needs wXR = 1 (got %d)", wXR);
+  if (eig && wXR != 1)
+    return fgls_set_error(FGLS_ERR_ARG, "the eigen variant needs wXR = 1 (got %d)", wXR);
 
   if (n < 1 || m < 0 || t < 1 || wXL < 0 || wXR < 1 || p != wXL + wXR || x_b < 1)
     return fgls_set_error(FGLS_ERR_ARG,
@@ -288,8 +359,13 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   const int threads = num_threads > 0 ? num_threads : host_cores();
 
   double *Phi = NULL, *M = NULL, *h = NULL, *sig = NULL, *XL = NULL, *y = NULL;
-  /* per slot s and trait k of a pass: index s * K + k */
+  /* per slot s and trait k of a pass: index s * K + k, views into one
+   * buffer per slot (dBs[s] + k * block * p, ...) */
   double **hB = NULL, **hV = NULL, **dB = NULL, **dV = NULL;
+  double *hBs[2] = {NULL, NULL}, *hVs[2] = {NULL, NULL}, *dBs[2] = {NULL, NULL}, *dVs[2] = {NULL, NULL};
+  fgls_hip_eig *ectx = NULL;       /* eigen variant: one context for the K traits of a pass */
+  double *Zh = NULL, *Wh = NULL, *dW = NULL, *yK = NULL;
+  double t_eig = 0.0;
   unsigned long long *hErr = NULL;  /* pinned copies of the failure records */
   unsigned int *hBad = NULL;        /* pinned: genotypes of the slot that are not int8-exact */
   int8_t *dX8[2] = {NULL, NULL};
@@ -310,7 +386,7 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
 
   const size_t nn = (size_t)n * n;
   Phi = (double *)malloc(nn * sizeof(double));
-  M = opt.host_potrf ? (double *)malloc(nn * sizeof(double)) : Phi;
+  M = (opt.host_potrf || eig) ? (double *)malloc(nn * sizeof(double)) : Phi;
   h = (double *)calloc((size_t)t, sizeof(double));
   sig = (double *)calloc((size_t)t, sizeof(double));
   XL = (double *)calloc((size_t)n * (wXL > 0 ? wXL : 1), sizeof(double));
@@ -342,8 +418,10 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   {
     size_t free_b = 0, total_b = 0;
     HCHK(hipMemGetInfo(&free_b, &total_b));
-    const size_t per = ctx_bytes(n, p, block, opt.mixed);
-    const size_t shared = (size_t)block * wXR * ldx * sizeof(double) * 2 + nn * sizeof(double);
+    const size_t ldn = ((size_t)n + 127) / 128 * 128;
+    const size_t per = eig ? eig_trait_bytes(n, p, block, opt.mixed) : ctx_bytes(n, p, block, opt.mixed);
+    size_t shared = (size_t)block * wXR * ldx * sizeof(double) * 2 + nn * sizeof(double);
+    if (eig) shared += ldn * ldn * (sizeof(double) + (opt.mixed ? 6 : 0)) + (64u << 20);
     long long fit = free_b > shared ? (long long)((free_b - shared) * 0.6 / per) : 1;
     if (fit < 1) fit = 1;
     K = opt.traits_per_pass > 0 ? opt.traits_per_pass : (int)(fit < t ? fit : t);
@@ -358,9 +436,14 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
     rc = fgls_set_error(FGLS_ERR_NOMEM, "host allocation failed");
     goto done;
   }
-  for (int k = 0; k < K; ++k) {
-    RCHK(fgls_hip_ctx_create(&ctx[k], n, wXL, wXR, (int)(block > 0 ? block : 1), opt.device));
-    if (opt.mixed) RCHK(fgls_hip_ctx_set_mode(ctx[k], FGLS_MODE_MIXED_I8));
+  if (eig) {
+    RCHK(fgls_hip_eig_create(&ectx, n, wXL, K, (int)(block > 0 ? block : 1), opt.device));
+    if (opt.mixed) RCHK(fgls_hip_eig_set_mode(ectx, FGLS_MODE_MIXED_I8));
+  } else {
+    for (int k = 0; k < K; ++k) {
+      RCHK(fgls_hip_ctx_create(&ctx[k], n, wXL, wXR, (int)(block > 0 ? block : 1), opt.device));
+      if (opt.mixed) RCHK(fgls_hip_ctx_set_mode(ctx[k], FGLS_MODE_MIXED_I8));
+    }
   }
   HCHK(hipStreamCreateWithFlags(&s_copy, hipStreamNonBlocking));
   HCHK(hipStreamCreateWithFlags(&s_comp, hipStreamNonBlocking));
@@ -372,10 +455,28 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   /* device potrf: Phi is uploaded once and every trait's M is built and
    * factored on the device; host potrf: each trait's factor is uploaded */
   HCHK(hipMalloc((void **)&dL, nn * sizeof(double)));
-  if (!opt.host_potrf) HCHK(hipMemcpy(dL, Phi, nn * sizeof(double), hipMemcpyHostToDevice));
+  if (!opt.host_potrf && !eig) HCHK(hipMemcpy(dL, Phi, nn * sizeof(double), hipMemcpyHostToDevice));
   HCHK(hipMalloc((void **)&dXL, (size_t)n * (wXL > 0 ? wXL : 1) * sizeof(double)));
-  HCHK(hipMalloc((void **)&dy, (size_t)n * sizeof(double)));
+  HCHK(hipMalloc((void **)&dy, (size_t)n * K * sizeof(double)));
   if (wXL > 0) HCHK(hipMemcpy(dXL, XL, (size_t)n * wXL * sizeof(double), hipMemcpyHostToDevice));
+  if (eig) {
+    /* Phi = Z diag(W) Z' on the host (eigenDec, fgls_eigen.c:780-806); Z and
+     * W stay on the device for every trait */
+    double te = now_s();
+    Zh = M;
+    Wh = (double *)malloc((size_t)n * sizeof(double));
+    yK = (double *)malloc((size_t)n * K * sizeof(double));
+    if (!Wh || !yK) {
+      rc = fgls_set_error(FGLS_ERR_NOMEM, "host allocation failed");
+      goto done;
+    }
+    RCHK(fgls_eigen_decompose(n, Phi, n, Zh, n, Wh, threads));
+    t_eig = now_s() - te;
+    HCHK(hipMalloc((void **)&dW, (size_t)n * sizeof(double)));
+    HCHK(hipMemcpy(dL, Zh, nn * sizeof(double), hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(dW, Wh, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    RCHK(fgls_hip_eig_set_basis(ectx, dL, n, dW, dXL, n, s_comp));
+  }
   HCHK(hipHostMalloc((void **)&hErr, (size_t)2 * K * sizeof(unsigned long long), hipHostMallocDefault));
   for (int i = 0; i < 2 * K; ++i) hErr[i] = ~0ull;
   HCHK(hipHostMalloc((void **)&hBad, 2 * sizeof(unsigned int), hipHostMallocDefault));
@@ -397,12 +498,16 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
       HCHK(hipMemset(dX[s], 0, xbytes));
       HCHK(hipHostMalloc((void **)&R.slot[s], (size_t)block * wXR * n * (opt.xr_i8 ? 1 : sizeof(double)),
                          hipHostMallocDefault));
+      HCHK(hipMalloc((void **)&dBs[s], (size_t)K * block * p * sizeof(double)));
+      HCHK(hipMalloc((void **)&dVs[s], (size_t)K * block * pp * sizeof(double)));
+      HCHK(hipHostMalloc((void **)&hBs[s], (size_t)K * block * p * sizeof(double), hipHostMallocDefault));
+      HCHK(hipHostMalloc((void **)&hVs[s], (size_t)K * block * pp * sizeof(double), hipHostMallocDefault));
       for (int k = 0; k < K; ++k) {
         const int q = s * K + k;
-        HCHK(hipMalloc((void **)&dB[q], (size_t)block * p * sizeof(double)));
-        HCHK(hipMalloc((void **)&dV[q], (size_t)block * pp * sizeof(double)));
-        HCHK(hipHostMalloc((void **)&hB[q], (size_t)block * p * sizeof(double), hipHostMallocDefault));
-        HCHK(hipHostMalloc((void **)&hV[q], (size_t)block * pp * sizeof(double), hipHostMallocDefault));
+        dB[q] = dBs[s] + (size_t)k * block * p;
+        dV[q] = dVs[s] + (size_t)k * block * pp;
+        hB[q] = hBs[s] + (size_t)k * block * p;
+        hV[q] = hVs[s] + (size_t)k * block * pp;
       }
     }
     R.path = XR_path;
@@ -429,7 +534,15 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   int64_t it = 0;  /* global chunk counter across passes (reader order) */
   for (int j0 = 0; j0 < t; j0 += K) {
     const int kn = (t - j0) < K ? (t - j0) : K;   /* traits of this pass */
-    for (int k = 0; k < kn; ++k) {
+    if (eig) {
+      /* Z' Y, weights, B_t, V_tl and res_sigma of the pass's traits */
+      double t0 = now_s();
+      RCHK(read_exact(Y_path, yK, (size_t)n * kn, (size_t)j0 * n));
+      HCHK(hipMemcpyAsync(dy, yK, (size_t)n * kn * sizeof(double), hipMemcpyHostToDevice, s_comp));
+      RCHK(fgls_hip_eig_set_traits(ectx, kn, h + j0, sig + j0, dy, n, s_comp));
+      t_trait += now_s() - t0;
+    }
+    for (int k = 0; k < kn && !eig; ++k) {
       const int j = j0 + k;
       double t0 = now_s();
       if (opt.host_potrf) {
@@ -482,9 +595,26 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
         } else if (!opt.mixed && opt.xr_i8) {
           RCHK(fgls_hip_i8_to_f64(dX8[s], ld8, n, cnt * wXR, dX[s], ldx, s_comp));
         }
+        if (eig) {
+          const int64_t ldb = block * p, ldv = block * pp;
+          if (dos)
+            RCHK(fgls_hip_eig_solve_block_dosage(ectx, dX8[s], ld8, cnt, opt.dosage_scale, dBs[s], ldb,
+                                                 dVs[s], ldv, i0, s_comp));
+          else if (opt.mixed)
+            RCHK(fgls_hip_eig_solve_block_i8(ectx, dX8[s], ld8, cnt, dBs[s], ldb, dVs[s], ldv, i0, s_comp));
+          else
+            RCHK(fgls_hip_eig_solve_block(ectx, dX[s], ldx, cnt, dBs[s], ldb, dVs[s], ldv, i0, s_comp));
+          HCHK(hipMemcpyAsync(&hErr[s * K], fgls_eig_err_ptr(ectx), (size_t)kn * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, s_comp));
+        }
         for (int k = 0; k < kn; ++k) {
           const int q = s * K + k;
           const int64_t base = (int64_t)(j0 + k) * m + i0;
+          if (eig) {
+            HCHK(hipMemcpyAsync(hB[q], dB[q], (size_t)cnt * p * sizeof(double), hipMemcpyDeviceToHost, s_comp));
+            HCHK(hipMemcpyAsync(hV[q], dV[q], (size_t)cnt * pp * sizeof(double), hipMemcpyDeviceToHost, s_comp));
+            continue;
+          }
           if (dos)
             RCHK(fgls_hip_solve_block_dosage(ctx[k], dX8[s], ld8, cnt, opt.dosage_scale, dB[q], dV[q],
                                              base, s_comp));
@@ -525,7 +655,7 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
           const int q = ps * K + k;
           const int j = j0 + k;
           if (hErr[q] != ~0ull) {  /* first failing SNP so far (fgls_chol_gpu.c:664-670) */
-            const long long bad = (long long)(hErr[q] >> 8) - (long long)j * m;
+            const long long bad = (long long)(hErr[q] >> 8) - (eig ? 0 : (long long)j * m);
             rc = fgls_set_error(FGLS_ERR_NOTSPD, "dpotrf(V) failed (info: %d) - i: %lld",
                                 (int)(hErr[q] & 0xff), bad);
             goto done;
@@ -544,9 +674,10 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int 
   if (opt.stats) {
     const double snps = (double)m * t;
     fprintf(stderr,
-            "[fgls] n=%d p=%d m=%d t=%d block=%lld traits/pass=%d: host potrf %.3fs, trait setup %.3fs, "
+            "[fgls] %s n=%d p=%d m=%d t=%d block=%lld traits/pass=%d: host %s %.3fs, trait setup %.3fs, "
             "block loop %.3fs (%.0f SNPs/s, %.2f GB/s X_R), reader busy %.3fs, threads %d\n",
-            n, p, m, t, (long long)block, K, t_potrf, t_trait, t_loop, t_loop > 0 ? snps / t_loop : 0.0,
+            eig ? "eigen" : "chol", n, p, m, t, (long long)block, K, eig ? "dsyevr" : "potrf",
+            eig ? t_eig : t_potrf, t_trait, t_loop, t_loop > 0 ? snps / t_loop : 0.0,
             t_loop > 0 ? (double)m * R.t * wXR * n * R.esize / t_loop / 1e9 : 0.0, R.busy_s, threads);
   }
 
@@ -572,11 +703,11 @@ done:
     if (R.slot[s]) hipHostFree(R.slot[s]);
     if (dX8[s]) hipFree(dX8[s]);
   }
-  for (int q = 0; q < 2 * K; ++q) {
-    if (dB && dB[q]) hipFree(dB[q]);
-    if (dV && dV[q]) hipFree(dV[q]);
-    if (hB && hB[q]) hipHostFree(hB[q]);
-    if (hV && hV[q]) hipHostFree(hV[q]);
+  for (int s = 0; s < 2; ++s) {
+    if (dBs[s]) hipFree(dBs[s]);
+    if (dVs[s]) hipFree(dVs[s]);
+    if (hBs[s]) hipHostFree(hBs[s]);
+    if (hVs[s]) hipHostFree(hVs[s]);
   }
   free(dB);
   free(dV);
@@ -595,6 +726,10 @@ done:
       if (ctx[k]) fgls_hip_ctx_destroy(ctx[k]);
     free(ctx);
   }
+  if (ectx) fgls_hip_eig_destroy(ectx);
+  if (dW) hipFree(dW);
+  free(Wh);
+  free(yK);
   if (R.fd >= 0) close(R.fd);
   if (fdB >= 0) close(fdB);
   if (fdV >= 0) close(fdV);
@@ -605,6 +740,36 @@ done:
   free(XL);
   free(y);
   return rc;
+}
+
+int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int y_b,
+                     int num_threads, const char *Phi_path, const char *h2_path,
+                     const char *sigma2_path, const char *XL_path, const char *XR_path,
+                     const char *Y_path, const char *B_path, const char *V_path,
+                     const char *options) {
+  return fgls_run(0, n, p, m, t, wXL, wXR, x_b, y_b, num_threads, Phi_path, h2_path, sigma2_path,
+                  XL_path, XR_path, Y_path, B_path, V_path, options);
+}
+
+int fgls_eigen_gpu_ex(int n, int p, int m, int t, int wXL, int wXR, int x_b, int y_b,
+                      int num_threads, const char *Phi_path, const char *h2_path,
+                      const char *sigma2_path, const char *XL_path, const char *XR_path,
+                      const char *Y_path, const char *B_path, const char *V_path,
+                      const char *options) {
+  return fgls_run(1, n, p, m, t, wXL, wXR, x_b, y_b, num_threads, Phi_path, h2_path, sigma2_path,
+                  XL_path, XR_path, Y_path, B_path, V_path, options);
+}
+
+int fgls_eigen(int n, int p, int m, int t, int wXL, int wXR, int x_b, int y_b, int num_threads,
+               char *Phi_path, char *h2_path, char *sigma2_path, char *XL_path, char *XR_path,
+               char *Y_path, char *B_path, char *V_path) {
+  int rc = fgls_eigen_gpu_ex(n, p, m, t, wXL, wXR, x_b, y_b, num_threads, Phi_path, h2_path,
+                             sigma2_path, XL_path, XR_path, Y_path, B_path, V_path, NULL);
+  if (rc != FGLS_OK) {
+    fprintf(stderr, "[Error]: %s\n", fgls_hip_last_error());   /* common.c:11-17 */
+    exit(EXIT_FAILURE);
+  }
+  return 0;
 }
 
 int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR, int x_b, int y_b, int num_threads,

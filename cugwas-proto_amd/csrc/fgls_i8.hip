@@ -112,12 +112,17 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i
 // covers 32 SNPs: fragments 0, 1 read their lo digits and fragments 2, 3 the
 // hi digits of the same SNPs (the same MFMA work per fragment as four
 // genotype fragments), and the epilogue forms acc_lo + 128 acc_hi in FP64.
-template <int NQ, int S, bool DOS = false>
+//
+// EIG (the eigenbasis variant): A8 holds the slices of a full matrix (every
+// row tile runs all K steps), and the epilogue reduces z against ntr groups
+// of NQ weight columns (group k at W + k*NQ*ldw: column 0 weighs z*z,
+// columns 1..NQ-1 weigh z), writing part[((rt*ntr + k)*NQ + q)*ldp + col].
+template <int NQ, int S, bool DOS = false, bool EIG = false>
 __global__ __launch_bounds__(256, 2) void k_trmm_i8(
     int n, int count, int RT32, int CT, int KS, const int8_t* __restrict__ A8,
     const double* __restrict__ inv_scale, const int8_t* __restrict__ X8, int64_t ldx8,
     const double* __restrict__ W, int64_t ldw, double* __restrict__ part, int64_t ldp,
-    int64_t plane_off = 0, double inv_xscale = 1.0) {
+    int64_t plane_off = 0, double inv_xscale = 1.0, int ntr = 1) {
   static_assert(S % 2 == 0, "the 2 x S slice blocks are spread over 4 waves");
   constexpr int WN = DOS ? 32 : 64;   // SNP columns per wave
   __shared__ SliceStage<S> st0, st1;
@@ -129,7 +134,7 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   const int r0 = rt * I8_TM;
   const int c0 = ct * (4 * WN) + w * WN;
   const int n64 = (n + 63) & ~63;
-  const int nsteps = (r0 < n64) ? min((r0 + I8_TM + 63) & ~63, n64) / 64 : 0;
+  const int nsteps = (r0 < n64) ? (EIG ? n64 : min((r0 + I8_TM + 63) & ~63, n64)) / 64 : 0;
 
   // wave w moves blocks w, w + 4, w + 8 of each step: block = ti * S + s
   const int8_t* asrc[S / 2];
@@ -175,7 +180,68 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
     __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
   }
 
-  if constexpr (DOS) {
+  if constexpr (EIG) {
+    // z(row r0+16ti+4lg+v, SNP c0+16tj+li) for the NF fragments of the wave
+    // (dosages: lo and hi digits combined), then one pass per weight group
+    constexpr int NF = DOS ? 2 : 4;
+    double z[NF][2][4];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double isc = inv_scale[r0 + 16 * ti + 4 * lg + v] * inv_xscale;
+#pragma unroll
+        for (int tj = 0; tj < NF; ++tj) {
+          double t;
+          if constexpr (DOS) {
+            t = fma(128.0, (double)acc[S - 1][ti][tj + 2][v], (double)acc[S - 1][ti][tj][v]);
+#pragma unroll
+            for (int s2 = S - 2; s2 >= 0; --s2)
+              t = fma(t, 1.0 / FGLS_SLICE_RADIX,
+                      fma(128.0, (double)acc[s2][ti][tj + 2][v], (double)acc[s2][ti][tj][v]));
+          } else {
+            t = (double)acc[S - 1][ti][tj][v];
+#pragma unroll
+            for (int s2 = S - 2; s2 >= 0; --s2)
+              t = fma(t, 1.0 / FGLS_SLICE_RADIX, (double)acc[s2][ti][tj][v]);
+          }
+          z[tj][ti][v] = t * isc;
+        }
+      }
+    for (int k = 0; k < ntr; ++k) {
+      const double* Wk = W + (size_t)k * NQ * ldw;
+#pragma unroll
+      for (int tj = 0; tj < NF; ++tj) {
+        int oz;   // opaque zero: W is re-read per fragment instead of held live
+        asm volatile("v_mov_b32 %0, 0" : "=v"(oz));
+        double q[NQ];
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) q[qi] = 0.0;
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = r0 + 16 * ti + 4 * lg + v + oz;
+            const double zz = z[tj][ti][v];
+            q[0] = fma(zz * Wk[row], zz, q[0]);
+#pragma unroll
+            for (int j = 1; j < NQ; ++j) q[j] = fma(zz, Wk[(size_t)j * ldw + row], q[j]);
+          }
+        }
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) {
+          q[qi] += __shfl_xor(q[qi], 16, 64);
+          q[qi] += __shfl_xor(q[qi], 32, 64);
+        }
+        const int col = c0 + 16 * tj + li;
+        if (lg == 0 && col < count) {
+#pragma unroll
+          for (int qi = 0; qi < NQ; ++qi)
+            part[(((size_t)rt * ntr + k) * NQ + qi) * ldp + col] = q[qi];
+        }
+      }
+    }
+  } else if constexpr (DOS) {
     // z(row r0+16ti+4lg+v, SNP c0+16tj+li), tj < 2:
     //   inv_scale[row] / scale * sum_s R^(-s) (acc[s][tj] + 128 acc[s][tj+2])
     double z[2][2][4];
@@ -313,21 +379,35 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   }
 }
 
-template <int P>
+template <int P, bool EIG>
 void launch(int n, int count, int RT32, int KS, const int8_t* A8, const double* isc,
             const int8_t* X8, int64_t ldx8, const double* W, int64_t ldw, double* part, int64_t ldp,
-            int64_t plane_off, double inv_xscale, hipStream_t st) {
+            int64_t plane_off, double inv_xscale, int ntr, hipStream_t st) {
   if (plane_off > 0) {
     const int CT = (count + 127) / 128;
     const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
-    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, true>), dim3(G), dim3(256), 0, st, n, count,
-                       RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off, inv_xscale);
+    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, true, EIG>), dim3(G), dim3(256), 0, st, n,
+                       count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off,
+                       inv_xscale, ntr);
   } else {
     const int CT = (count + I8_TN - 1) / I8_TN;
     const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
-    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE>), dim3(G), dim3(256), 0, st, n, count, RT32,
-                       CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, (int64_t)0, 1.0);
+    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, EIG>), dim3(G), dim3(256), 0, st, n,
+                       count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, (int64_t)0, 1.0,
+                       ntr);
   }
+}
+
+template <int P>
+void launch_any(int n, int count, int RT32, int KS, const int8_t* A8, const double* isc,
+                const int8_t* X8, int64_t ldx8, const double* W, int64_t ldw, double* part,
+                int64_t ldp, int64_t plane_off, double inv_xscale, int eig_ntr, hipStream_t st) {
+  if (eig_ntr > 0)
+    launch<P, true>(n, count, RT32, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off,
+                    inv_xscale, eig_ntr, st);
+  else
+    launch<P, false>(n, count, RT32, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off,
+                     inv_xscale, 1, st);
 }
 
 }  // namespace
@@ -335,11 +415,11 @@ void launch(int n, int count, int RT32, int KS, const int8_t* A8, const double* 
 void fgls_launch_trmm_i8(int p, int n, int count, int RT32, int KS, const int8_t* A8,
                          const double* inv_scale, const int8_t* X8, int64_t ldx8, const double* W,
                          int64_t ldw, double* part, int64_t ldp, int64_t plane_off,
-                         double inv_xscale, hipStream_t st) {
+                         double inv_xscale, int eig_ntr, hipStream_t st) {
 #define FGLS_I8_CASE(P)                                                                      \
   case P:                                                                                    \
-    launch<P>(n, count, RT32, KS, A8, inv_scale, X8, ldx8, W, ldw, part, ldp, plane_off,     \
-              inv_xscale, st);                                                               \
+    launch_any<P>(n, count, RT32, KS, A8, inv_scale, X8, ldx8, W, ldw, part, ldp, plane_off, \
+                  inv_xscale, eig_ntr, st);                                                  \
     break;
   switch (p) {
     FGLS_I8_CASE(1) FGLS_I8_CASE(2) FGLS_I8_CASE(3) FGLS_I8_CASE(4) FGLS_I8_CASE(5)

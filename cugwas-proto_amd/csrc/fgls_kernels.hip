@@ -211,9 +211,12 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const doubl
 //   EPI_REDUCE: per column, the NQ sums over the tile's rows of
 //               z*z and z*W[:,k] (k < NQ-1) go to part[(rt*NQ+q)*ldp + col]
 //   EPI_DOT   : as EPI_REDUCE with z*X[row, col] (rows < n) in place of z*z
+//   EPI_MULTI : ntr groups of NQ weight columns (group k at W + k*NQ*ldw):
+//               column 0 weighs z*z, columns 1..NQ-1 weigh z; the sums of
+//               group k go to part[((rt*ntr + k)*NQ + q)*ldp + col]
 // Grid: (column tiles, row tiles), heaviest row tiles first.
 // ---------------------------------------------------------------------------
-enum { EPI_STORE = 0, EPI_REDUCE = 1, EPI_DOT = 2 };
+enum { EPI_STORE = 0, EPI_REDUCE = 1, EPI_DOT = 2, EPI_MULTI = 3 };
 
 template <int EPI, int NQ, bool DENSE = false>
 __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
@@ -221,7 +224,8 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
                                               const double* __restrict__ X, int64_t ldx,
                                               double* __restrict__ Zout, int64_t ldz,
                                               const double* __restrict__ W, int64_t ldw,
-                                              double* __restrict__ part, int64_t ldp) {
+                                              double* __restrict__ part, int64_t ldp,
+                                              int ntr = 1) {
   const int ct = blockIdx.x;
   const int rt = RT - 1 - (int)blockIdx.y;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
                                            : mfma_f64(a[ti][s], b[tj][s], acc[ti][tj]);
   }
 
-  if (EPI == EPI_STORE) {
+  if constexpr (EPI == EPI_STORE) {
     // operands swapped: element v of acc[ti][tj] is z(row r0+16ti+li, column
     // c0+16tj+lg+4v), so 16 lanes store 16 consecutive rows of one column.
 #pragma unroll
@@ -321,6 +325,63 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
           for (int ti = 0; ti < 4; ++ti) Zout[(size_t)col * ldz + r0 + 16 * ti + li] = acc[ti][tj][v];
         }
       }
+    }
+  } else if constexpr (EPI == EPI_MULTI) {
+    // element v of acc[ti][tj] is z(row r0+16ti+lg+4v, column c0+16tj+li);
+    // one pass over the accumulators per weight group
+    __shared__ double red[2][64][NQ];
+    for (int k = 0; k < ntr; ++k) {
+      const double* Wk = W + (size_t)k * NQ * ldw;
+      double q[4][NQ];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) q[tj][qi] = 0.0;
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + 16 * ti + lg + 4 * v;
+          double wv[NQ];
+#pragma unroll
+          for (int j = 0; j < NQ; ++j) wv[j] = Wk[(size_t)j * ldw + row];
+#pragma unroll
+          for (int tj = 0; tj < 4; ++tj) {
+            const double z = acc[ti][tj][v];
+            q[tj][0] = fma(z * wv[0], z, q[tj][0]);
+#pragma unroll
+            for (int j = 1; j < NQ; ++j) q[tj][j] = fma(z, wv[j], q[tj][j]);
+          }
+        }
+      }
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) {
+          double x = q[tj][qi];
+          x += __shfl_xor(x, 16, 64);
+          x += __shfl_xor(x, 32, 64);
+          q[tj][qi] = x;
+        }
+      if (wr == 1 && lg == 0) {
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+          for (int qi = 0; qi < NQ; ++qi) red[wc][16 * tj + li][qi] = q[tj][qi];
+      }
+      __syncthreads();
+      if (wr == 0 && lg == 0) {
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) {
+          const int col = c0 + 16 * tj + li;
+          if (col < count) {
+#pragma unroll
+            for (int qi = 0; qi < NQ; ++qi)
+              part[(((size_t)rt * ntr + k) * NQ + qi) * ldp + col] = q[tj][qi] + red[wc][16 * tj + li][qi];
+          }
+        }
+      }
+      __syncthreads();   // red is refilled by the next group
     }
   } else {
     // element v of acc[ti][tj] is z(row r0+16ti+lg+4v, column c0+16tj+li)
@@ -608,22 +669,30 @@ __device__ __forceinline__ void record_failure(unsigned long long* err, int64_t 
 //   [wXL + wXL^2]          res_sigma
 //   [wXL + wXL^2 + 1, +wXL) b_L
 
-// Finalize from the TRMM partials (wXR = 1, p = P): thread per SNP.
+// Finalize from the TRMM partials (wXR = 1, p = P): thread per SNP, one
+// grid row per trait k of ntr (partials of trait k at
+// part[((rt*ntr + k)*NQ + q)*ldp + i]; its scalars at trait + k*trait_ld,
+// its outputs at Bout + k*ldb, Vout + k*ldv, its failure record at err[k]).
 template <int P>
 __global__ __launch_bounds__(128) void k_finalize(int count, int RT, const double* __restrict__ part,
-                                                  int64_t ldp, const double* __restrict__ trait,
-                                                  double* __restrict__ Bout, double* __restrict__ Vout,
+                                                  int64_t ldp, int ntr, const double* __restrict__ trait,
+                                                  int64_t trait_ld, double* __restrict__ Bout, int64_t ldb,
+                                                  double* __restrict__ Vout, int64_t ldv,
                                                   unsigned long long* __restrict__ err, int64_t snp_base) {
   constexpr int NQ = P + 1;
   constexpr int WXL = P - 1;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
   if (i >= count) return;
+  trait += (size_t)k * trait_ld;
+  Bout += (size_t)k * ldb;
+  Vout += (size_t)k * ldv;
   double sums[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) sums[q] = 0.0;
   for (int rt = 0; rt < RT; ++rt) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) sums[q] += part[((size_t)rt * NQ + q) * ldp + i];
+    for (int q = 0; q < NQ; ++q) sums[q] += part[(((size_t)rt * ntr + k) * NQ + q) * ldp + i];
   }
   double S[P][P], b[P];
 #pragma unroll
@@ -642,7 +711,7 @@ __global__ __launch_bounds__(128) void k_finalize(int count, int RT, const doubl
   b[WXL] = sums[NQ - 1];
   const double res = trait[trait_res_off(WXL)];
   const int info = finalize_snp<P>(S, b, res, Bout + (size_t)i * P, Vout + (size_t)i * P * P);
-  if (info) record_failure(err, snp_base + i, info);
+  if (info) record_failure(err + k, snp_base + i, info);
 }
 
 // Per-SNP stage from transformed columns Z (any wXR): one wave per SNP,
@@ -831,7 +900,8 @@ void launch_finalize(const fgls_hip_ctx* c, int count, double* dB, double* dV, i
                      hipStream_t st) {
   dim3 grid((count + 127) / 128);
   hipLaunchKernelGGL(k_finalize<P>, grid, dim3(128), 0, st, count, c->last_rt, c->part,
-                     (int64_t)c->max_block, c->trait, dB, dV, c->err, base);
+                     (int64_t)c->max_block, 1, c->trait, (int64_t)0, dB, (int64_t)0, dV, (int64_t)0,
+                     c->err, base);
 }
 
 template <int P>
@@ -901,6 +971,56 @@ void launch_product_store(const fgls_hip_ctx* c, const double* A, bool dense, co
 void launch_product_dot(const fgls_hip_ctx* c, const double* A, const double* X, int64_t ldx,
                         int count, hipStream_t st) {
   FGLS_P_SWITCH(c->p, launch_trmm_dot, c, A, X, ldx, count, st);
+}
+
+void launch_dense_store(int n, int ldn, const double* A, const double* X, int64_t ldx, int ncols,
+                        double* Z, int64_t ldz, hipStream_t st) {
+  dim3 grid((ncols + TN - 1) / TN, ldn / TM);
+  hipLaunchKernelGGL((k_trmm<EPI_STORE, 2, true>), grid, dim3(256), 0, st, n, ncols, ldn / TM, A,
+                     (int64_t)ldn, X, ldx, Z, ldz, (const double*)nullptr, (int64_t)0,
+                     (double*)nullptr, (int64_t)0, 1);
+}
+
+template <int P>
+void launch_eig_reduce_p(int n, int ldn, const double* A, const double* X, int64_t ldx, int count,
+                         const double* W, int64_t ldw, int ntr, double* part, int64_t ldp,
+                         hipStream_t st) {
+  dim3 grid((count + TN - 1) / TN, ldn / TM);
+  hipLaunchKernelGGL((k_trmm<EPI_MULTI, P + 1, true>), grid, dim3(256), 0, st, n, count, ldn / TM,
+                     A, (int64_t)ldn, X, ldx, (double*)nullptr, (int64_t)0, W, ldw, part, ldp, ntr);
+}
+
+void launch_eig_reduce(int p, int n, int ldn, const double* A, const double* X, int64_t ldx,
+                       int count, const double* W, int64_t ldw, int ntr, double* part, int64_t ldp,
+                       hipStream_t st) {
+  FGLS_P_SWITCH(p, launch_eig_reduce_p, n, ldn, A, X, ldx, count, W, ldw, ntr, part, ldp, st);
+}
+
+template <int P>
+void launch_finalize_multi_p(int count, int RT, const double* part, int64_t ldp, int ntr,
+                             const double* trait, int64_t trait_ld, double* dB, int64_t ldb,
+                             double* dV, int64_t ldv, unsigned long long* err, int64_t base,
+                             hipStream_t st) {
+  dim3 grid((count + 127) / 128, ntr);
+  hipLaunchKernelGGL(k_finalize<P>, grid, dim3(128), 0, st, count, RT, part, ldp, ntr, trait,
+                     trait_ld, dB, ldb, dV, ldv, err, base);
+}
+
+void launch_finalize_multi(int p, int count, int RT, const double* part, int64_t ldp, int ntr,
+                           const double* trait, int64_t trait_ld, double* dB, int64_t ldb,
+                           double* dV, int64_t ldv, unsigned long long* err, int64_t base,
+                           hipStream_t st) {
+  FGLS_P_SWITCH(p, launch_finalize_multi_p, count, RT, part, ldp, ntr, trait, trait_ld, dB, ldb,
+                dV, ldv, err, base, st);
+}
+
+void launch_trait_stats(int n, int wXL, const double* W, int64_t ldw, double* trait, int* status,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_trait_stats, dim3(1), dim3(256), 0, st, n, wXL, W, ldw, trait, status);
+}
+
+void launch_slices(int ldn, const double* A, int8_t* A8, double* invsc, hipStream_t st) {
+  hipLaunchKernelGGL((k_slice_linv<NSLICE>), dim3(ldn), dim3(256), 0, st, ldn, A, A8, invsc);
 }
 
 void launch_snp_stage_any(const fgls_hip_ctx* c, const double* Z, int64_t ldz, int count,
@@ -1228,7 +1348,8 @@ int reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count, int p
   hipStream_t st = (hipStream_t)stream;
   fgls_launch_trmm_i8(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, dX8, ldx8, c->W,
                       (int64_t)c->ldn, c->part, (int64_t)c->max_block,
-                      planes == 2 ? (int64_t)count * ldx8 : 0, planes == 2 ? 1.0 / scale : 1.0, st);
+                      planes == 2 ? (int64_t)count * ldx8 : 0, planes == 2 ? 1.0 / scale : 1.0, 0,
+                      st);
   HIP_TRY(hipGetLastError());
   c->reduced = count;
   c->last_rt = c->RT8;

@@ -12,6 +12,9 @@ Two levels are exposed:
 * ``DeviceContext`` -- the device layer (include/fgls_hip.h) on torch CUDA
   tensors: per-trait precompute, fused block solve, TRSM, TRMM and the
   per-SNP stage.
+* ``fgls_eigen(...)`` and ``EigenContext`` -- the same for the eigenbasis
+  variant (fgls_eigen.h:68-74): one eigendecomposition of Phi, one product
+  Z' X_R per block for all traits.
 
 Matrix convention on tensors: a column-major n x k matrix with leading
 dimension ld is a contiguous tensor of shape (k, ld) (row c = column c).
@@ -85,6 +88,22 @@ def lib() -> ctypes.CDLL:
             "fgls_factor_M": (i32, [i32, dp, i64, ctypes.c_double, ctypes.c_double, dp, i64, i32]),
             "fgls_chol_gpu": (i32, [i32] * 9 + [ctypes.c_char_p] * 8),
             "fgls_chol_gpu_ex": (i32, [i32] * 9 + [ctypes.c_char_p] * 8 + [ctypes.c_void_p]),
+            "fgls_eigen": (i32, [i32] * 9 + [ctypes.c_char_p] * 8),
+            "fgls_eigen_gpu_ex": (i32, [i32] * 9 + [ctypes.c_char_p] * 8 + [ctypes.c_void_p]),
+            "fgls_eigen_decompose": (i32, [i32, dp, i64, dp, i64, dp, i32]),
+            "fgls_hip_eig_create": (i32, [ctypes.POINTER(vp), i32, i32, i32, i32, i32]),
+            "fgls_hip_eig_destroy": (i32, [vp]),
+            "fgls_hip_eig_ldn": (i32, [vp]),
+            "fgls_hip_eig_set_mode": (i32, [vp, i32]),
+            "fgls_hip_eig_set_basis": (i32, [vp, dp, i64, dp, dp, i64, vp]),
+            "fgls_hip_eig_set_traits": (i32, [vp, i32, dp, dp, dp, i64, vp]),
+            "fgls_hip_eig_get_trait": (i32, [vp, i32, dp, dp, dp]),
+            "fgls_hip_eig_solve_block": (i32, [vp, dp, i64, i32, dp, i64, dp, i64, i64, vp]),
+            "fgls_hip_eig_solve_block_i8": (i32, [vp, dp, i64, i32, dp, i64, dp, i64, i64, vp]),
+            "fgls_hip_eig_solve_block_dosage": (i32, [vp, dp, i64, i32, ctypes.c_double, dp, i64, dp, i64,
+                                                      i64, vp]),
+            "fgls_hip_eig_check": (i32, [vp, vp, ctypes.POINTER(i32), ctypes.POINTER(i64),
+                                         ctypes.POINTER(i32)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name, None)
@@ -107,7 +126,12 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_set_trait_phi", "fgls_hip_get_factor", "fgls_hip_to_dosage_i8",
                         "fgls_hip_block_reduce_dosage", "fgls_hip_solve_block_dosage",
                         "fgls_hip_i8_to_f64",
-                        "fgls_factor_M")
+                        "fgls_factor_M", "fgls_eigen", "fgls_eigen_gpu_ex", "fgls_eigen_decompose",
+                        "fgls_hip_eig_create", "fgls_hip_eig_destroy", "fgls_hip_eig_ldn",
+                        "fgls_hip_eig_set_mode", "fgls_hip_eig_set_basis", "fgls_hip_eig_set_traits",
+                        "fgls_hip_eig_get_trait", "fgls_hip_eig_solve_block",
+                        "fgls_hip_eig_solve_block_i8", "fgls_hip_eig_solve_block_dosage",
+                        "fgls_hip_eig_check")
             if hasattr(lib(), s)]
 
 
@@ -404,6 +428,138 @@ class DeviceContext:
             raise err
 
 
+class EigenContext:
+    """The eigenbasis variant on the device (include/fgls_hip.h, fgls_hip_eig_*):
+    one eigendecomposition Phi = Z diag(W) Z' serves every trait; a block of
+    SNPs is solved for all the traits of the last set_traits at once."""
+
+    def __init__(self, n: int, wXL: int, max_traits: int = 1, max_block: int = 4096, device: int = 0):
+        self.n, self.wXL, self.p = n, wXL, wXL + 1
+        self.max_traits, self.max_block, self.device = max_traits, max_block, device
+        self.ntr = 0
+        h = ctypes.c_void_p()
+        _check(lib().fgls_hip_eig_create(ctypes.byref(h), n, wXL, max_traits, max_block, device))
+        self._h = h
+        self.ldn = int(lib().fgls_hip_eig_ldn(h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fgls_hip_eig_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ld8(self) -> int:
+        return (self.n + 63) // 64 * 64
+
+    def set_mode(self, mode: int):
+        _check(lib().fgls_hip_eig_set_mode(self._h, mode))
+
+    def set_basis(self, Z, W, XL, stream=None):
+        """Z: (n, ldz) tensor holding the eigenvectors column-major (row r =
+        eigenvector r); W: (n,) eigenvalues; XL: (wXL, ldxl) or None."""
+        _need_cuda_f64(Z, "Z")
+        _need_cuda_f64(W, "W")
+        ldxl = self.n
+        if self.wXL > 0:
+            _need_cuda_f64(XL, "XL")
+            ldxl = XL.shape[1]
+        _check(lib().fgls_hip_eig_set_basis(self._h, _ptr(Z), Z.shape[1], _ptr(W),
+                                            _ptr(XL) if self.wXL else 0, ldxl, _stream(stream)))
+
+    def set_traits(self, h2, sigma2, Y, stream=None):
+        """h2, sigma2: sequences of ntr floats; Y: (ntr, ldy) tensor (row k = trait k)."""
+        import numpy as np
+        _need_cuda_f64(Y, "Y")
+        h = np.ascontiguousarray(np.atleast_1d(np.asarray(h2, dtype=np.float64)))
+        sg = np.ascontiguousarray(np.atleast_1d(np.asarray(sigma2, dtype=np.float64)))
+        if sg.size == 1 and h.size > 1:
+            sg = np.full(h.size, float(sg[0]))
+        ntr = int(h.size)
+        if sg.size != ntr or Y.shape[0] < ntr:
+            raise FglsError(FGLS_ERR_ARG, "h2, sigma2 and Y disagree on the number of traits")
+        _check(lib().fgls_hip_eig_set_traits(self._h, ntr, h.ctypes.data, sg.ctypes.data, _ptr(Y),
+                                             Y.shape[1], _stream(stream)))
+        self.ntr = ntr
+
+    def trait(self, k: int = 0):
+        import numpy as np
+        Bt = np.zeros(max(self.wXL, 1))
+        Vtl = np.zeros(max(self.wXL * self.wXL, 1))
+        res = np.zeros(1)
+        _check(lib().fgls_hip_eig_get_trait(self._h, k, Bt.ctypes.data, Vtl.ctypes.data, res.ctypes.data))
+        return Bt[:self.wXL], Vtl[:self.wXL * self.wXL].reshape(self.wXL, self.wXL, order="F"), float(res[0])
+
+    def _outputs(self, count, B, V, device):
+        import torch
+        if B is None:
+            B = torch.empty((self.ntr, count, self.p), dtype=torch.float64, device=device)
+        if V is None:
+            V = torch.empty((self.ntr, count, self.p * self.p), dtype=torch.float64, device=device)
+        return B, V
+
+    def solve_block(self, XR, count: int | None = None, B=None, V=None, snp_base: int = 0, stream=None):
+        """XR: (count, ldx).  Returns B (ntr, count, p) and V (ntr, count, p*p)."""
+        _need_cuda_f64(XR, "XR")
+        count = XR.shape[0] if count is None else count
+        B, V = self._outputs(count, B, V, XR.device)
+        _check(lib().fgls_hip_eig_solve_block(self._h, _ptr(XR), XR.shape[1], count, _ptr(B),
+                                              B.stride(0), _ptr(V), V.stride(0), snp_base, _stream(stream)))
+        return B, V
+
+    def solve_block_i8(self, X8, count: int | None = None, B=None, V=None, snp_base: int = 0, stream=None):
+        count = X8.shape[0] if count is None else count
+        B, V = self._outputs(count, B, V, X8.device)
+        _check(lib().fgls_hip_eig_solve_block_i8(self._h, _ptr(X8), X8.shape[1], count, _ptr(B),
+                                                 B.stride(0), _ptr(V), V.stride(0), snp_base,
+                                                 _stream(stream)))
+        return B, V
+
+    def solve_block_dosage(self, X8, scale: float = 1000.0, count: int | None = None, B=None, V=None,
+                           snp_base: int = 0, stream=None):
+        """X8: (2 * count, ld8) digit planes (DeviceContext.to_dosage_i8)."""
+        count = X8.shape[0] // 2 if count is None else count
+        B, V = self._outputs(count, B, V, X8.device)
+        _check(lib().fgls_hip_eig_solve_block_dosage(self._h, _ptr(X8), X8.shape[1], count, float(scale),
+                                                     _ptr(B), B.stride(0), _ptr(V), V.stride(0), snp_base,
+                                                     _stream(stream)))
+        return B, V
+
+    def check(self, stream=None):
+        tr = ctypes.c_int(-1)
+        snp = ctypes.c_int64(-1)
+        info = ctypes.c_int(0)
+        rc = lib().fgls_hip_eig_check(self._h, _stream(stream), ctypes.byref(tr), ctypes.byref(snp),
+                                      ctypes.byref(info))
+        if rc != FGLS_OK:
+            err = FglsError(rc, last_error())
+            err.trait, err.bad_snp, err.info = int(tr.value), int(snp.value), int(info.value)
+            raise err
+
+
+def eigen_decompose(Phi, threads: int = 0):
+    """Host Phi = Z diag(W) Z' by LAPACK dsyevr (fgls_eigen.c:780-806): numpy in,
+    (Z as an n x n Fortran-order array, W ascending) out."""
+    import numpy as np
+    Phi = np.asfortranarray(Phi, dtype=np.float64)
+    n = Phi.shape[0]
+    Z = np.empty((n, n), dtype=np.float64, order="F")
+    W = np.empty(n, dtype=np.float64)
+    _check(lib().fgls_eigen_decompose(n, Phi.ctypes.data, n, Z.ctypes.data, n, W.ctypes.data, int(threads)))
+    return Z, W
+
+
 def factor_M(Phi, h2: float, sigma2: float, threads: int = 0):
     """Host Cholesky factor of M = sigma2 (h2 Phi + (1 - h2) I) (fgls_chol_gpu.c:406-430):
     numpy n x n in, numpy lower factor (column-major, Fortran order) out."""
@@ -432,5 +588,22 @@ def fgls_chol_gpu(n, p, m, t, wXL, wXR, x_b, y_b, num_threads,
     if options:
         opts = ",".join(f"{k}={v}" for k, v in options.items()).encode()
     rc = L.fgls_chol_gpu_ex(*args, *paths, opts)
+    _check(rc)
+    return rc
+
+
+def fgls_eigen(n, p, m, t, wXL, wXR, x_b, y_b, num_threads,
+               Phi_path, h2_path, sigma2_path, XL_path, XR_path, Y_path, B_path, V_path,
+               options: dict | None = None) -> int:
+    """The reference's eigenbasis entry point (fgls_eigen.h:68-74) on the GPU,
+    files in and out; raises FglsError where the C symbol exits."""
+    L = lib()
+    args = [int(v) for v in (n, p, m, t, wXL, wXR, x_b, y_b, num_threads)]
+    paths = [os.fsencode(s) for s in (Phi_path, h2_path, sigma2_path, XL_path, XR_path, Y_path,
+                                      B_path, V_path)]
+    opts = None
+    if options:
+        opts = ",".join(f"{k}={v}" for k, v in options.items()).encode()
+    rc = L.fgls_eigen_gpu_ex(*args, *paths, opts)
     _check(rc)
     return rc

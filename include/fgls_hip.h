@@ -239,6 +239,67 @@ int fgls_hip_adjoint_block(fgls_hip_ctx *ctx, const double *dXR, int64_t ldxr,
 int fgls_hip_adjoint_end(fgls_hip_ctx *ctx, double *dgrad_y, double *hloss,
                          double *hgrad_h2, void *stream);
 
+/*
+ * Eigenbasis variant (src/fgls_eigen.c): Phi = Z diag(W) Z' once, then for
+ * trait k M_k = sigma2_k (h2_k Phi + (1 - h2_k) I) = Z diag(a_k W + b_k) Z'
+ * (fgls_eigen.c:455-468), so that one dense product x' = Z' x_R per SNP
+ * serves every trait and only weighted sums over the rows of x' differ per
+ * trait.  b and V are those of the Cholesky path (same layout, same
+ * per-SNP stage, fgls_eigen.c:510-563).  wXR = 1.
+ *
+ * create:     n individuals, wXL covariates (p = wXL + 1), up to max_traits
+ *             traits per set_traits, up to max_block SNPs per solve call.
+ * set_mode:   FGLS_MODE_FP64 (default) or FGLS_MODE_MIXED_I8 (Z' as six
+ *             int8 digit slices, int8 genotypes or dosage digit planes as in
+ *             the Cholesky context; n <= 133,000).
+ * set_basis:  dZ (n x n eigenvectors, column-major, ld ldz), dW (n
+ *             eigenvalues), dXL (n x wXL, ld ldxl): keeps Z' and computes
+ *             Z' X_L on the device.  Replaces the out-of-core gemms of XL
+ *             (fgls_eigen.c:693-724) and keeps Z resident instead of
+ *             writing Z'X to temporary files.  Synchronizes `stream`.
+ * set_traits: ntr traits with h2[k], sigma2[k] (host arrays) and dY (n x ntr,
+ *             column-major, ld ldy): Z' Y, the per-trait weights, B_t, V_tl
+ *             and res_sigma (fgls_eigen.c:455-491 and residual_sigma,
+ *             :997-1152).  FGLS_ERR_NOTSPD if some a_k W_l + b_k <= 0 or a
+ *             V_tl is not SPD.  Synchronizes `stream`.
+ * solve_block(_i8, _dosage): count SNPs (X_R as for fgls_hip_solve_block,
+ *             fgls_hip_solve_block_i8, fgls_hip_solve_block_dosage); trait k
+ *             writes b to dB + k*ldb (count*p) and V to dV + k*ldv
+ *             (count*p*p); ldb, ldv are ignored for one trait.  Replaces the
+ *             ooc_loops per-SNP code, fgls_eigen.c:494-563.  Asynchronous;
+ *             per-SNP dpotrf failures are recorded per trait.
+ * check:      synchronizes `stream`; FGLS_ERR_NOTSPD with the first recorded
+ *             failure (*trait, *bad_snp counted from snp_base, *info), then
+ *             clears the records.
+ */
+typedef struct fgls_hip_eig fgls_hip_eig;
+int fgls_hip_eig_create(fgls_hip_eig **out, int n, int wXL, int max_traits,
+                        int max_block, int device);
+int fgls_hip_eig_destroy(fgls_hip_eig *e);
+int fgls_hip_eig_ldn(const fgls_hip_eig *e);
+int fgls_hip_eig_set_mode(fgls_hip_eig *e, int mode);
+int fgls_hip_eig_set_basis(fgls_hip_eig *e, const double *dZ, int64_t ldz,
+                           const double *dW, const double *dXL, int64_t ldxl,
+                           void *stream);
+int fgls_hip_eig_set_traits(fgls_hip_eig *e, int ntr, const double *h2,
+                            const double *sigma2, const double *dY, int64_t ldy,
+                            void *stream);
+/* B_t (wXL), V_tl (wXL*wXL, column-major, upper 0) and res_sigma of trait k. */
+int fgls_hip_eig_get_trait(fgls_hip_eig *e, int k, double *hBt, double *hVtl,
+                           double *hres_sigma);
+int fgls_hip_eig_solve_block(fgls_hip_eig *e, const double *dXR, int64_t ldxr,
+                             int count, double *dB, int64_t ldb, double *dV,
+                             int64_t ldv, int64_t snp_base, void *stream);
+int fgls_hip_eig_solve_block_i8(fgls_hip_eig *e, const int8_t *dX8, int64_t ldx8,
+                                int count, double *dB, int64_t ldb, double *dV,
+                                int64_t ldv, int64_t snp_base, void *stream);
+int fgls_hip_eig_solve_block_dosage(fgls_hip_eig *e, const int8_t *dX8,
+                                    int64_t ldx8, int count, double scale,
+                                    double *dB, int64_t ldb, double *dV,
+                                    int64_t ldv, int64_t snp_base, void *stream);
+int fgls_hip_eig_check(fgls_hip_eig *e, void *stream, int *trait,
+                       int64_t *bad_snp, int *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,12 @@ variant (src/fgls_chol_gpu.c):
 * ``fgls_chol``         fgls_chol.c:223-491  (trait loop, output layout
                                               fgls_chol.c:458-474)
 * ``read_inputs`` / ``write_outputs``  io.c:7-33 raw little-endian doubles.
+* ``eigen_decompose``, ``eigen_trait``, ``eigen_solve_snps``, ``fgls_eigen``
+                        the eigenbasis solver src/fgls_eigen.c: dsyevr of Phi
+                        (:780-806), Winv = sqrt(1/(sigma h W + sigma (1-h)))
+                        (:455-468), the scaled Z'X_L, Z'y, B_t, V_tl (:470-491),
+                        residual_sigma (:997-1152) and the per-SNP loop
+                        (:494-563); the per-SNP stage is per_snp_from_z.
 * ``snp_loss_adjoint``  no reference counterpart: gradients of
                         l = sum_i w_i b_i[p-1]^2 w.r.t. y and h (an addition
                         on top of the per-SNP stage); pinned only by
@@ -147,6 +153,76 @@ def fgls_chol(Phi, H, Sig, XL, XR, Y, wXR: int = 1, x_b: int | None = None):
         for ib in range(0, m, x_b):
             cnt = min(x_b, m - ib)
             b, V = solve_snps(XR[:, ib * wXR:(ib + cnt) * wXR], tr, wXR)
+            o = (j * m + ib) * p
+            Bout[o:o + cnt * p] = b.ravel()
+            ov = (j * m + ib) * p * p
+            Vout[ov:ov + cnt * p * p] = V.transpose(0, 2, 1).ravel()
+    return Bout, Vout
+
+
+def eigen_decompose(Phi: np.ndarray):
+    """eigenDec (fgls_eigen.c:780-806): dsyevr("V", "A", "L") of Phi.
+    Returns (Z, W) with Phi = Z diag(W) Z', W ascending."""
+    W, Z = sla.eigh(np.asarray(Phi, dtype=np.float64), lower=True, driver="evr", check_finite=False)
+    return Z, W
+
+
+def eigen_trait(Z: np.ndarray, W: np.ndarray, XL: np.ndarray, y: np.ndarray, h: float,
+                sigma: float) -> Trait:
+    """Per-trait quantities of the eigenbasis solver for one trait.
+
+    Winv = sqrt(1 / (sigma h W + sigma (1 - h)))          fgls_eigen.c:455-468
+    X~_L = Winv * Z'X_L, y~ = Winv * Z'y                   :476-487
+    B_t = X~_L' y~, V_tl = lower(X~_L' X~_L)               :490-500
+    res_sigma: b_L from V_tl, r = y - X_L b_L,
+               |Winv * Z'r|^2 / (n - wXL)                  residual_sigma, :1119-1152
+    The returned Trait has L = None; XLt / yt hold the scaled X~_L, y~."""
+    n = Z.shape[0]
+    wXL = XL.shape[1]
+    alpha, beta = sigma * h, sigma * (1.0 - h)
+    Winv = np.sqrt(1.0 / (alpha * W + beta))
+    XLt = Winv[:, None] * (Z.T @ XL)
+    yt = Winv * (Z.T @ y)
+    B_t = XLt.T @ yt
+    V_tl = np.tril(XLt.T @ XLt)
+    if wXL > 0:
+        C = np.linalg.cholesky(V_tl + np.tril(V_tl, -1).T)
+        bL = sla.solve_triangular(C, B_t, lower=True)
+        bL = sla.solve_triangular(C.T, bL, lower=False)
+        r = y - XL @ bL
+    else:
+        r = np.array(y, dtype=np.float64, copy=True)
+    rt = Winv * (Z.T @ r)
+    res = float(rt @ rt) / (n - wXL)
+    tr = Trait(L=None, XLt=XLt, yt=yt, B_t=B_t, V_tl=V_tl, res_sigma=res)
+    tr.Winv = Winv
+    return tr
+
+
+def eigen_solve_snps(Z: np.ndarray, tr: Trait, XR_block: np.ndarray, wXR: int):
+    """x~ = Winv * Z'x for the columns of one block (the out-of-core gemm,
+    fgls_eigen.c:727-757, and the Winv scaling, :506-509), then the per-SNP
+    stage (:510-563, the same arithmetic as fgls_chol.c:386-439)."""
+    return per_snp_from_z(tr.Winv[:, None] * (Z.T @ XR_block), tr, wXR)
+
+
+def fgls_eigen(Phi, H, Sig, XL, XR, Y, wXR: int = 1, x_b: int | None = None):
+    """The eigenbasis solver over t traits; arguments and output layout as
+    fgls_chol (fgls_eigen.c:591-611 writes the same offsets)."""
+    n = Phi.shape[0]
+    wXL = XL.shape[1]
+    p = wXL + wXR
+    m = XR.shape[1] // wXR
+    t = Y.shape[1]
+    x_b = m if x_b is None else x_b
+    Z, W = eigen_decompose(Phi)
+    Bout = np.empty(t * m * p)
+    Vout = np.empty(t * m * p * p)
+    for j in range(t):
+        tr = eigen_trait(Z, W, XL, Y[:, j], float(H[j]), float(Sig[j]))
+        for ib in range(0, m, x_b):
+            cnt = min(x_b, m - ib)
+            b, V = eigen_solve_snps(Z, tr, XR[:, ib * wXR:(ib + cnt) * wXR], wXR)
             o = (j * m + ib) * p
             Bout[o:o + cnt * p] = b.ravel()
             ov = (j * m + ib) * p * p

@@ -55,3 +55,19 @@ def fgls_chol_dir(d: str, n: int, p: int, m: int, t: int, wXL: int, x_b: int, th
     lib.fgls_chol(n, p, m, t, wXL, p - wXL, x_b, 1, threads, f("Phi.in"), f("H.in"), f("Sig.in"),
                   f("XL.in"), f("XR.in"), f("Y.in"), f(out_prefix + "B.out"), f(out_prefix + "V.out"))
     return time.perf_counter() - t0
+
+
+def fgls_eigen_dir(d: str, n: int, p: int, m: int, t: int, wXL: int, x_b: int, threads: int = 1,
+                   out_prefix: str = "") -> float:
+    """Run the reference's eigenbasis solver fgls_eigen (src/fgls_eigen.c:37-236) on
+    directory d.  It writes its temporaries XL.in.tmp, XR.in.tmp and Y.in.tmp
+    next to the inputs (common.c:98-100).  Returns the wall time in seconds."""
+    lib = _load()
+    lib.fgls_eigen.restype = ctypes.c_int
+    lib.fgls_eigen.argtypes = [ctypes.c_int] * 9 + [ctypes.c_char_p] * 8
+    set_blas_threads(threads)
+    f = lambda s: os.path.join(d, s).encode()
+    t0 = time.perf_counter()
+    lib.fgls_eigen(n, p, m, t, wXL, p - wXL, x_b, 1, threads, f("Phi.in"), f("H.in"), f("Sig.in"),
+                   f("XL.in"), f("XR.in"), f("Y.in"), f(out_prefix + "B.out"), f(out_prefix + "V.out"))
+    return time.perf_counter() - t0

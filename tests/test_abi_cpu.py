@@ -25,8 +25,23 @@ def declared_functions():
 def test_headers_declare_the_boundary():
     names = declared_functions()
     for required in ("fgls_chol_gpu", "fgls_chol_gpu_ex", "fgls_hip_ctx_create", "fgls_hip_set_trait",
-                     "fgls_hip_solve_block", "fgls_hip_trsm", "fgls_hip_snp_stage", "fgls_hip_check"):
+                     "fgls_hip_solve_block", "fgls_hip_trsm", "fgls_hip_snp_stage", "fgls_hip_check",
+                     "fgls_eigen", "fgls_eigen_gpu_ex", "fgls_hip_eig_create", "fgls_hip_eig_solve_block"):
         assert required in names
+
+
+def test_host_eigendecomposition(pkg):
+    """fgls_eigen_decompose (dsyevr, fgls_eigen.c:780-806) against numpy: the
+    reconstruction, orthogonality and ascending order; only the lower triangle
+    of Phi is read."""
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((90, 180))
+    Phi = A @ A.T / 180
+    Z, W = pkg.eigen_decompose(np.tril(Phi) + np.triu(np.full_like(Phi, 7.0), 1), threads=2)
+    assert np.abs(Z @ np.diag(W) @ Z.T - Phi).max() < 1e-13
+    assert np.abs(Z.T @ Z - np.eye(90)).max() < 1e-13
+    assert np.allclose(W, np.linalg.eigvalsh(Phi), rtol=0, atol=1e-13)
+    assert np.all(np.diff(W) >= 0)
 
 
 def test_library_exports_every_declared_symbol(pkg):
@@ -81,7 +96,7 @@ def test_driver_usage_and_exit_codes(pkg, tmp_path):
     r = subprocess.run([drv], capture_output=True, text=True)
     assert r.returncode == 1 and "Usage" in r.stderr
     r = subprocess.run([drv, "e", str(tmp_path), "1"], input="10 4 5 1 3 5 1 1", capture_output=True, text=True)
-    assert r.returncode == 1 and "fgls_eigen" in r.stderr
+    assert r.returncode == 1 and "[Error]" in r.stderr and "Phi.in" in r.stderr   # variant e: fgls_eigen
     r = subprocess.run([drv, "g", str(tmp_path), "1"], input="10 4 5 1 3 5 1 1", capture_output=True, text=True)
     assert r.returncode == 1 and r.stderr.startswith("[Error]: ") and "Phi.in" in r.stderr
 

@@ -134,6 +134,67 @@ def test_reference_build_reproduces_fixture(tmp_path):
     assert orc.max_rel_err(B, np.fromfile(os.path.join(d, "B_exp.out"))) < 1e-12
 
 
+def test_eigen_oracle_reproduces_reference_fixture(txl):
+    """The eigenbasis restatement (fgls_eigen.c) against test_XL/B_exp.out."""
+    prm, Phi, H, Sig, XL, XR, Y, Bexp = txl
+    sub = slice(0, 400)
+    B, _ = orc.fgls_eigen(Phi, H[:4], Sig[:4], XL, XR[:, sub], Y[:, :4], wXR=1)
+    assert orc.max_rel_err(B, Bexp[:4, sub].ravel()) < 1e-10
+
+
+@pytest.mark.parametrize("n,wXL,m,t", [(200, 3, 300, 3), (97, 0, 50, 2), (150, 6, 80, 1)])
+def test_eigen_oracle_matches_cholesky_oracle(n, wXL, m, t):
+    """M^-1 = Z diag(1/(a W + b)) Z' = L^-T L^-1: both restatements give the same
+    b and V (two different factorisations, so agreement is not by construction)."""
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=n + t, t=t)
+    H = np.linspace(0.15, 0.85, t)
+    Sig = np.linspace(0.5, 2.0, t)
+    Be, Ve = orc.fgls_eigen(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y)
+    Bc, Vc = orc.fgls_chol(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y)
+    e = orc.parity_errors(Be, Ve, Bc, Vc, wXL + 1)
+    assert e["B_scaled"] < 1e-11 and e["V_scaled"] < 1e-11 and e["V_upper_zero"], e
+
+
+def test_eigen_oracle_catches_planted_mistakes():
+    """Plausible mistakes in the eigenbasis weights fail the Cholesky pin:
+    a and b swapped, Winv without the square root, Z in place of Z'."""
+    prob = make_problem(n=120, wXL=3, m=40, seed=4, t=1, h2=0.3)
+    L = orc.cholesky_lower(orc.build_M(prob.Phi, 0.3, 1.2))
+    bc, Vc = orc.solve_snps(prob.XR, orc.trait_precompute(L, prob.XL, prob.y), 1)
+    Z, W = orc.eigen_decompose(prob.Phi)
+
+    def err(Zm, Wm, h, sigma, sqrt=True):
+        tr = orc.eigen_trait(Zm, Wm, prob.XL, prob.y, h, sigma)
+        if not sqrt:
+            tr.Winv = tr.Winv ** 2
+        b, V = orc.eigen_solve_snps(Zm, tr, prob.XR, 1)
+        return orc.parity_errors(b, V.transpose(0, 2, 1).reshape(len(b), -1), bc,
+                                 Vc.transpose(0, 2, 1).reshape(len(bc), -1), 4)["B_scaled"]
+
+    assert err(Z, W, 0.3, 1.2) < 1e-11
+    assert err(Z, W, 0.7, 1.2) > 1e-4                 # a and b swapped (h used as 1 - h)
+    assert err(Z, W, 0.3, 1.2, sqrt=False) > 1e-4     # Winv^2 in place of Winv
+    assert err(Z.T.copy(), W, 0.3, 1.2) > 1e-4        # rows of Z in place of columns
+
+
+@pytest.mark.skipif(not ref_runner.available(), reason="reference sources were not built (oracle/Makefile)")
+def test_eigen_oracle_matches_reference_build(tmp_path):
+    """B and V of the eigenbasis restatement against the reference's own
+    fgls_eigen (3 traits with their own h2 and sigma2)."""
+    n, wXL, m, t, x_b = 300, 3, 500, 3, 128
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=3, t=t)
+    H = np.array([0.4, 0.7, 0.2])
+    Sig = np.array([1.0, 2.0, 0.5])
+    d = str(tmp_path)
+    write_problem_dir(prob, d, H=H, Sig=Sig)
+    ref_runner.fgls_eigen_dir(d, n, wXL + 1, m, t, wXL, x_b, threads=2)
+    B = np.fromfile(os.path.join(d, "B.out"))
+    V = np.fromfile(os.path.join(d, "V.out"))
+    Bo, Vo = orc.fgls_eigen(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y, wXR=1, x_b=x_b)
+    e = orc.parity_errors(Bo, Vo, B, V, wXL + 1)
+    assert e["B_scaled"] < 1e-12 and e["V_scaled"] < 1e-12 and e["V_upper_zero"], e
+
+
 def test_fixture_files_match_driver_stdin():
     prm, Phi, H, Sig, XL, Y, Bexp = load_fixture()
     assert (prm["n"], prm["p"], prm["m"], prm["t"], prm["wXL"]) == (500, 4, 2000, 20, 3)
