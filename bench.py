@@ -20,6 +20,11 @@ outside the steps.  `value` is the mixed path's SNPs/s; both paths, their
 speed ratio and the mixed path's error against the FP64 path over all m SNPs
 are in `paths`.
 
+Afterwards the eigenbasis variant (fgls_eigen: Phi = Z diag(W) Z' once on
+the device, then one dense product Z' X_R per block for all traits) runs over
+the same genotypes for --eigen-traits traits with their own h2; it is
+reported under `eigen` (SNPs/s and trait-SNPs/s), not in `value`.
+
 Prints one JSON line (rank 0).  The CPU baseline is the reference's own
 fgls_chol (compiled from its sources by oracle/Makefile) on a bounded sample
 of the same workload, or the FP64 oracle when that build is absent.
@@ -203,6 +208,127 @@ def run_adjoint(torch, pkg, dev, m, steps):
             "grad_y_norm": float(torch.linalg.vector_norm(gy))}
 
 
+def run_eigen(torch, pkg, args, Phi, XL, y, X, X8, chol_fp64, dev):
+    """The eigenbasis variant (fgls_eigen) over the same genotypes for
+    args.eigen_traits traits with their own h2 (the first is the Cholesky
+    runs' h2): one dense product Z' X_R per block serves all of them.  The
+    device eigendecomposition and the per-trait setup are one-offs outside the
+    steps.  Parity: the first trait against the Cholesky FP64 run over all m
+    SNPs; every trait against the FP64 oracle (Cholesky form) on a sample."""
+    import numpy as np
+    n, m, wXL = args.n, args.m, args.wXL
+    p = wXL + 1
+    T = args.eigen_traits
+    H = [args.h2] + [float(v) for v in np.linspace(0.2, 0.8, T - 1)] if T > 1 else [args.h2]
+    block = min(args.block, m)
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed + 11)
+    # extra traits: y_k ~ N(0, 1) (b and V of any y are pinned the same way)
+    Y = torch.empty((T, n), dtype=torch.float64, device=dev)
+    Y[0] = y
+    if T > 1:
+        Y[1:] = torch.randn((T - 1, n), dtype=torch.float64, device=dev, generator=g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Z, W = pkg.eigen_decompose_device(Phi)
+    torch.cuda.synchronize()
+    eig_s = time.perf_counter() - t0
+    log(f"eigen: device eigendecomposition of Phi (n={n}): {eig_s:.2f} s")
+    out = {"traits": T, "h2": H, "eigendecomposition_s": eig_s, "paths": {}}
+    e = pkg.EigenContext(n, wXL, max_traits=T, max_block=block)
+    for mode in args.eigen_paths.split(","):
+        t0 = time.perf_counter()
+        e.set_mode(pkg.FGLS_MODE_MIXED_I8 if mode == "mixed" else pkg.FGLS_MODE_FP64)
+        e.set_basis(Z, W, XL.T.contiguous())
+        e.set_traits(H, [1.0] * T, Y)
+        torch.cuda.synchronize()
+        setup_s = time.perf_counter() - t0
+        Bo = torch.empty((T, m, p), dtype=torch.float64, device=dev)
+        Vo = torch.empty((T, m, p * p), dtype=torch.float64, device=dev)
+        starts = list(range(0, m, block))
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in starts]
+
+        def step(record):
+            for bi, i0 in enumerate(starts):
+                cnt = min(block, m - i0)
+                if record:
+                    evs[bi][0].record()
+                if mode == "mixed":
+                    e.solve_block_i8(X8[i0:i0 + cnt], cnt, Bo[:, i0:i0 + cnt], Vo[:, i0:i0 + cnt], snp_base=i0)
+                else:
+                    e.solve_block(X[i0:i0 + cnt], cnt, Bo[:, i0:i0 + cnt], Vo[:, i0:i0 + cnt], snp_base=i0)
+                if record:
+                    evs[bi][1].record()
+
+        for _ in range(args.warmup):
+            step(False)
+        e.check()
+        torch.cuda.synchronize()
+        blk_ms, nfull = 0.0, 0
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step(True)
+            torch.cuda.synchronize()
+            for bi, i0 in enumerate(starts):
+                if min(block, m - i0) == block:
+                    blk_ms += evs[bi][0].elapsed_time(evs[bi][1])
+                    nfull += 1
+            log(f"eigen {mode} step {k + 1}/{args.steps} done")
+        elapsed = time.perf_counter() - t0
+        e.check()
+        blk_ms /= max(nfull, 1)
+        flop = 2.0 * n * n * block                  # dense Z' X_R: 2 n^2 per SNP
+        if mode == "mixed":
+            ach = NSLICE * flop / (blk_ms * 1e-3) / 1e12
+            roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
+                    "frac": ach / PEAK_INT8_TOPS, "achieved_is": "int8 TOP/s (product + finalize per block)",
+                    "int8_ops_per_snp": NSLICE * 2.0 * n * n}
+        else:
+            ach = flop / (blk_ms * 1e-3) / 1e12
+            roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / PEAK_FP64_TFLOPS, "achieved_is": "FP64 TFLOP/s (product + finalize per block)",
+                    "flop_per_snp": 2.0 * n * n}
+        res = {"snps_per_s": m * args.steps / elapsed, "trait_snps_per_s": T * m * args.steps / elapsed,
+               "ms_per_step": elapsed / args.steps * 1e3, "block_ms": blk_ms, "setup_s": setup_s,
+               "roofline": roof}
+        if chol_fp64 is not None:
+            res["trait0_vs_cholesky_fp64_all_snps"] = orc_parity({"B": Bo[0], "V": Vo[0]}, chol_fp64, p)
+        if args.check_snps > 0:
+            res["parity_vs_oracle"] = eigen_oracle_check(torch, args, Phi, XL, X, Y, H, Bo, Vo, dev)
+        log(f"eigen {mode}: {res['snps_per_s']:.0f} SNPs/s x {T} traits = {res['trait_snps_per_s']:.0f} "
+            f"trait-SNPs/s, {blk_ms:.2f} ms per {block}-SNP block ({roof['frac'] * 100:.1f}% of peak); "
+            f"setup {setup_s:.2f} s; parity {res.get('parity_vs_oracle')}; "
+            f"vs chol {res.get('trait0_vs_cholesky_fp64_all_snps')}")
+        out["paths"][mode] = res
+        del Bo, Vo
+    e.close()
+    return out
+
+
+def eigen_oracle_check(torch, args, Phi, XL, X, Y, H, Bo, Vo, dev):
+    """Max errors over the traits of a sample of SNPs against the FP64 oracle
+    (Cholesky form of each trait: the same b and V as the eigenbasis)."""
+    import numpy as np
+    from oracle import fgls_oracle as orc
+    n, m, wXL = args.n, args.m, args.wXL
+    p = wXL + 1
+    idx = np.unique(np.random.default_rng(1).integers(0, m, max(args.check_snps // 4, 16)))
+    sel = torch.as_tensor(idx, device=dev)
+    Xs = X[sel, :n].cpu().numpy().T.copy()
+    Phin, XLn = Phi.cpu().numpy(), XL.cpu().numpy()
+    worst = {}
+    for k, h in enumerate(H):
+        Lc = orc.cholesky_lower(orc.build_M(Phin, h, 1.0))
+        tr = orc.trait_precompute(Lc, XLn, Y[k].cpu().numpy())
+        b_ref, V_ref = orc.solve_snps(Xs, tr, 1)
+        err = orc.parity_errors(Bo[k][sel].cpu().numpy(), Vo[k][sel].cpu().numpy(), b_ref,
+                                V_ref.transpose(0, 2, 1).reshape(idx.size, -1), p)
+        for key, v in err.items():
+            worst[key] = (worst.get(key, True) and v) if isinstance(v, bool) else max(worst.get(key, 0.0), v)
+    worst["snps"], worst["traits"] = int(idx.size), len(H)
+    return worst
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +346,9 @@ def main():
     ap.add_argument("--paths", default="fp64,mixed,dosage",
                     help="comma-separated: fp64, mixed (int8 genotypes), dosage (3-decimal dosages "
                          "as two int8 digit planes)")
+    ap.add_argument("--eigen-traits", type=int, default=4,
+                    help="traits of the eigenbasis variant's run over the same genotypes (0: skip)")
+    ap.add_argument("--eigen-paths", default="fp64,mixed")
     ap.add_argument("--adjoint-m", type=int, default=200_000,
                     help="SNPs of the adjoint measurement at n=5000, p=4, blocks of 2048 (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -422,6 +551,11 @@ def main():
         summary["mixed_vs_fp64_all_snps"] = e
         summary["speedup_mixed_over_fp64"] = paths["mixed"]["snps_per_s"] / paths["fp64"]["snps_per_s"]
         log(f"mixed vs fp64 over all {m} SNPs: {e}; speedup {summary['speedup_mixed_over_fp64']:.2f}x")
+    eigen = None
+    if args.eigen_traits > 0:
+        ctx.close()
+        torch.cuda.empty_cache()
+        eigen = run_eigen(torch, pkg, args, Phi, XL, y, X, X8, paths.get("fp64"), dev)
     adjoint = None
     if args.adjoint_m > 0:
         del X, X8, D8, paths
@@ -454,6 +588,7 @@ def main():
         "trait_setup_s": trait_setup,
         "paths": summary,
         "parity_vs_oracle": check,
+        "eigen": eigen,
         "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)

@@ -11,6 +11,7 @@ from .fgls import (  # noqa: F401
     device_count,
     exported_symbols,
     eigen_decompose,
+    eigen_decompose_device,
     factor_M,
     fgls_chol_gpu,
     fgls_eigen,

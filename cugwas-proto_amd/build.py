@@ -95,7 +95,7 @@ def build(verbose: bool = False) -> dict:
     so = os.path.join(LIBDIR, "libfgls_amd.so")
     tmp = so + ".tmp"
     run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *hip_objs, *host_objs,
-         "-L", blasdir, f"-l:{blasname}", f"-Wl,-rpath,{blasdir}", "-fopenmp", "-lpthread", "-lrt"],
+         "-L", blasdir, f"-l:{blasname}", f"-Wl,-rpath,{blasdir}", "-fopenmp", "-lpthread", "-lrt", "-ldl"],
         verbose)
     os.replace(tmp, so)
 

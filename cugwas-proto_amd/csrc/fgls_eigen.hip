@@ -20,11 +20,14 @@
 //   Wsc   ldn x ntr p      per trait: [sqrt(d_k) Z'X_L, sqrt(d_k) Z'y_k]
 //   part  tiles x ntr x (p+1) x max_block  per-row-tile partial sums
 
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <rocsolver/rocsolver.h>
 #include <stdint.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "fgls_ctx.h"
@@ -181,9 +184,73 @@ int pre_solve(fgls_hip_eig* e, int count) {
   return FGLS_OK;
 }
 
+// rocSOLVER's dsyevd for the device eigendecomposition, loaded on first use
+// (the library is large and only this one-off setup needs it).
+struct RocSolver {
+  decltype(&rocblas_create_handle) create = nullptr;
+  decltype(&rocblas_destroy_handle) destroy = nullptr;
+  decltype(&rocblas_set_stream) set_stream = nullptr;
+  decltype(&rocsolver_dsyevd) dsyevd = nullptr;
+  bool ok = false;
+};
+
+const RocSolver& rocsolver() {
+  static RocSolver r;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    void* h = dlopen("librocsolver.so.0", RTLD_NOW | RTLD_LOCAL);
+    if (!h) h = dlopen("/opt/rocm/lib/librocsolver.so.0", RTLD_NOW | RTLD_LOCAL);
+    if (!h) return;
+    r.create = (decltype(r.create))dlsym(h, "rocblas_create_handle");
+    r.destroy = (decltype(r.destroy))dlsym(h, "rocblas_destroy_handle");
+    r.set_stream = (decltype(r.set_stream))dlsym(h, "rocblas_set_stream");
+    r.dsyevd = (decltype(r.dsyevd))dlsym(h, "rocsolver_dsyevd");
+    r.ok = r.create && r.destroy && r.set_stream && r.dsyevd;
+  });
+  return r;
+}
+
 }  // namespace
 
 extern "C" {
+
+int fgls_hip_eigen_decompose(int n, const double* dPhi, int64_t ldphi, double* dZ, int64_t ldz,
+                             double* dW, int device, void* stream) {
+  if (n < 1 || !dPhi || !dZ || !dW || ldphi < n || ldz < n || ldz > INT32_MAX)
+    return fail(FGLS_ERR_ARG, "fgls_hip_eigen_decompose: bad arguments");
+  const RocSolver& rs = rocsolver();
+  if (!rs.ok) return fail(FGLS_ERR_STATE, "rocSOLVER (librocsolver.so.0) is not available");
+  HIP_TRY(hipSetDevice(device));
+  hipStream_t st = (hipStream_t)stream;
+  // dsyevd overwrites its input with the eigenvectors: Z starts as Phi
+  HIP_TRY(hipMemcpy2DAsync(dZ, (size_t)ldz * sizeof(double), dPhi, (size_t)ldphi * sizeof(double),
+                           (size_t)n * sizeof(double), (size_t)n, hipMemcpyDeviceToDevice, st));
+  double* E = nullptr;
+  rocblas_int* dinfo = nullptr;
+  if (hipMalloc((void**)&E, (size_t)n * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&dinfo, sizeof(rocblas_int)) != hipSuccess) {
+    if (E) hipFree(E);
+    return fail(FGLS_ERR_NOMEM, "hipMalloc of the dsyevd workspace failed");
+  }
+  rocblas_handle h = nullptr;
+  int rc = FGLS_OK;
+  rocblas_int info = 0;
+  if (rs.create(&h) != rocblas_status_success || rs.set_stream(h, st) != rocblas_status_success) {
+    rc = fail(FGLS_ERR_HIP, "rocblas handle creation failed");
+  } else if (rs.dsyevd(h, rocblas_evect_original, rocblas_fill_lower, n, dZ, (rocblas_int)ldz, dW, E,
+                       dinfo) != rocblas_status_success) {
+    rc = fail(FGLS_ERR_HIP, "rocsolver_dsyevd failed to launch");
+  } else if (hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st) != hipSuccess ||
+             hipStreamSynchronize(st) != hipSuccess) {
+    rc = fail(FGLS_ERR_HIP, "dsyevd: synchronisation failed");
+  } else if (info != 0) {
+    rc = fail(FGLS_ERR_NOTSPD, "dsyevd failed (info: %d)", (int)info);
+  }
+  if (h) rs.destroy(h);
+  hipFree(E);
+  hipFree(dinfo);
+  return rc;
+}
 
 int fgls_hip_eig_create(fgls_hip_eig** out, int n, int wXL, int max_traits, int max_block,
                         int device) {

@@ -68,6 +68,7 @@ typedef struct {
   int traits_per_pass;  /* traits sharing one read of X_R (0: as many as fit) */
   int xr_i8;         /* xr_format=i8: X_R file of int8 values (n bytes per column) */
   double dosage_scale;  /* precision=mixed for dosages: x * scale = 128 hi + lo (0: genotypes) */
+  int eig_host;      /* eig=host: eigendecomposition of Phi by host LAPACK dsyevr (default: device) */
 } options_t;
 
 static double now_s(void) {
@@ -98,6 +99,10 @@ static void parse_options(const char *s, options_t *o) {
     }
     if (!strcmp(tok, "potrf")) {
       o->host_potrf = !strcmp(eq + 1, "host") ? 1 : (!strcmp(eq + 1, "gpu") ? 0 : -1);
+      continue;
+    }
+    if (!strcmp(tok, "eig")) {
+      o->eig_host = !strcmp(eq + 1, "host") ? 1 : (!strcmp(eq + 1, "gpu") ? 0 : -1);
       continue;
     }
     if (!strcmp(tok, "precision")) {
@@ -322,8 +327,9 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
                     const char *Y_path, const char *B_path, const char *V_path,
                     const char *options) {
   int rc = FGLS_OK;
-  options_t opt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0.0};
+  options_t opt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0.0, 0};
   parse_options(options ? options : getenv("FGLS_OPTIONS"), &opt);
+  if (opt.eig_host < 0) return fgls_set_error(FGLS_ERR_ARG, "eig must be gpu or host");
   if (opt.mixed < 0) return fgls_set_error(FGLS_ERR_ARG, "precision must be fp64 or mixed");
   if (opt.host_potrf < 0) return fgls_set_error(FGLS_ERR_ARG, "potrf must be gpu or host");
   if (opt.xr_i8 < 0) return fgls_set_error(FGLS_ERR_ARG, "xr_format must be f64 or i8");
@@ -460,8 +466,9 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   HCHK(hipMalloc((void **)&dy, (size_t)n * K * sizeof(double)));
   if (wXL > 0) HCHK(hipMemcpy(dXL, XL, (size_t)n * wXL * sizeof(double), hipMemcpyHostToDevice));
   if (eig) {
-    /* Phi = Z diag(W) Z' on the host (eigenDec, fgls_eigen.c:780-806); Z and
-     * W stay on the device for every trait */
+    /* Phi = Z diag(W) Z' once (eigenDec, fgls_eigen.c:780-806): by default on
+     * the device (rocSOLVER dsyevd), with eig=host by LAPACK dsyevr on the
+     * host as the reference; Z and W stay on the device for every trait */
     double te = now_s();
     Zh = M;
     Wh = (double *)malloc((size_t)n * sizeof(double));
@@ -470,11 +477,21 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
       rc = fgls_set_error(FGLS_ERR_NOMEM, "host allocation failed");
       goto done;
     }
-    RCHK(fgls_eigen_decompose(n, Phi, n, Zh, n, Wh, threads));
-    t_eig = now_s() - te;
     HCHK(hipMalloc((void **)&dW, (size_t)n * sizeof(double)));
-    HCHK(hipMemcpy(dL, Zh, nn * sizeof(double), hipMemcpyHostToDevice));
-    HCHK(hipMemcpy(dW, Wh, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    if (opt.eig_host) {
+      RCHK(fgls_eigen_decompose(n, Phi, n, Zh, n, Wh, threads));
+      HCHK(hipMemcpy(dL, Zh, nn * sizeof(double), hipMemcpyHostToDevice));
+      HCHK(hipMemcpy(dW, Wh, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+      double *dPhi = NULL;
+      HCHK(hipMalloc((void **)&dPhi, nn * sizeof(double)));
+      rc = hipMemcpy(dPhi, Phi, nn * sizeof(double), hipMemcpyHostToDevice) == hipSuccess
+               ? fgls_hip_eigen_decompose(n, dPhi, n, dL, n, dW, opt.device, s_comp)
+               : fgls_set_error(FGLS_ERR_HIP, "upload of Phi failed");
+      hipFree(dPhi);
+      if (rc != FGLS_OK) goto done;
+    }
+    t_eig = now_s() - te;
     RCHK(fgls_hip_eig_set_basis(ectx, dL, n, dW, dXL, n, s_comp));
   }
   HCHK(hipHostMalloc((void **)&hErr, (size_t)2 * K * sizeof(unsigned long long), hipHostMallocDefault));
@@ -674,9 +691,10 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   if (opt.stats) {
     const double snps = (double)m * t;
     fprintf(stderr,
-            "[fgls] %s n=%d p=%d m=%d t=%d block=%lld traits/pass=%d: host %s %.3fs, trait setup %.3fs, "
+            "[fgls] %s n=%d p=%d m=%d t=%d block=%lld traits/pass=%d: %s %.3fs, trait setup %.3fs, "
             "block loop %.3fs (%.0f SNPs/s, %.2f GB/s X_R), reader busy %.3fs, threads %d\n",
-            eig ? "eigen" : "chol", n, p, m, t, (long long)block, K, eig ? "dsyevr" : "potrf",
+            eig ? "eigen" : "chol", n, p, m, t, (long long)block, K,
+            eig ? (opt.eig_host ? "host dsyevr" : "device dsyevd") : "host potrf",
             eig ? t_eig : t_potrf, t_trait, t_loop, t_loop > 0 ? snps / t_loop : 0.0,
             t_loop > 0 ? (double)m * R.t * wXR * n * R.esize / t_loop / 1e9 : 0.0, R.busy_s, threads);
   }

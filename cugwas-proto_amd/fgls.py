@@ -91,6 +91,7 @@ def lib() -> ctypes.CDLL:
             "fgls_eigen": (i32, [i32] * 9 + [ctypes.c_char_p] * 8),
             "fgls_eigen_gpu_ex": (i32, [i32] * 9 + [ctypes.c_char_p] * 8 + [ctypes.c_void_p]),
             "fgls_eigen_decompose": (i32, [i32, dp, i64, dp, i64, dp, i32]),
+            "fgls_hip_eigen_decompose": (i32, [i32, dp, i64, dp, i64, dp, i32, vp]),
             "fgls_hip_eig_create": (i32, [ctypes.POINTER(vp), i32, i32, i32, i32, i32]),
             "fgls_hip_eig_destroy": (i32, [vp]),
             "fgls_hip_eig_ldn": (i32, [vp]),
@@ -127,6 +128,7 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_block_reduce_dosage", "fgls_hip_solve_block_dosage",
                         "fgls_hip_i8_to_f64",
                         "fgls_factor_M", "fgls_eigen", "fgls_eigen_gpu_ex", "fgls_eigen_decompose",
+                        "fgls_hip_eigen_decompose",
                         "fgls_hip_eig_create", "fgls_hip_eig_destroy", "fgls_hip_eig_ldn",
                         "fgls_hip_eig_set_mode", "fgls_hip_eig_set_basis", "fgls_hip_eig_set_traits",
                         "fgls_hip_eig_get_trait", "fgls_hip_eig_solve_block",
@@ -557,6 +559,20 @@ def eigen_decompose(Phi, threads: int = 0):
     Z = np.empty((n, n), dtype=np.float64, order="F")
     W = np.empty(n, dtype=np.float64)
     _check(lib().fgls_eigen_decompose(n, Phi.ctypes.data, n, Z.ctypes.data, n, W.ctypes.data, int(threads)))
+    return Z, W
+
+
+def eigen_decompose_device(Phi, stream=None, device: int = 0):
+    """Device Phi = Z diag(W) Z' (rocSOLVER dsyevd): Phi an (n, ldphi) CUDA
+    tensor holding Phi column-major; returns (Z as an (n, n) tensor whose row r
+    is eigenvector r, i.e. Z column-major, W ascending)."""
+    import torch
+    _need_cuda_f64(Phi, "Phi")
+    n = Phi.shape[0]
+    Z = torch.empty((n, n), dtype=torch.float64, device=Phi.device)
+    W = torch.empty(n, dtype=torch.float64, device=Phi.device)
+    _check(lib().fgls_hip_eigen_decompose(n, _ptr(Phi), Phi.shape[1], _ptr(Z), n, _ptr(W), device,
+                                          _stream(stream)))
     return Z, W
 
 

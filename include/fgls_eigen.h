@@ -5,8 +5,9 @@
  * int fgls_eigen(...) has the reference's signature and file contract
  * (src/fgls_eigen.h:68-74, identical to fgls_chol_gpu): the same input files,
  * the same B.out / V.out layout (see fgls_chol_gpu.h), the same results up to
- * rounding.  It computes Phi = Z diag(W) Z' once on the host (LAPACK dsyevr,
- * as eigenDec, src/fgls_eigen.c:780-806), keeps Z resident in device memory,
+ * rounding.  It computes Phi = Z diag(W) Z' once (on the device by default;
+ * host LAPACK dsyevr as eigenDec, src/fgls_eigen.c:780-806, with eig=host),
+ * keeps Z resident in device memory,
  * and runs one dense product Z' x_R per SNP on the device that serves every
  * trait of a pass (the per-trait work is a weighted reduction over the rows
  * of Z' x_R, fused into that product).
@@ -20,7 +21,7 @@
  *     (common.c:11-17), including M_k not positive definite
  *     (h2 W_l + (1 - h2) <= 0 for some eigenvalue W_l of Phi).
  * y_b is ignored; num_threads sets the host LAPACK threads of the
- * eigendecomposition.
+ * eigendecomposition with eig=host.
  */
 #ifndef FGLS_EIGEN_H
 #define FGLS_EIGEN_H
@@ -45,7 +46,9 @@ int fgls_eigen(int n, int p, int m, int t, int wXL, int wXR,
  * fgls_chol_gpu_ex (block, device, serial, stats, io_threads, precision,
  * dosage_scale, xr_format, traits_per_pass); potrf is not used.  With
  * traits_per_pass (default: all traits that fit) the traits of a pass share
- * one product Z' x_R per block.
+ * one product Z' x_R per block.  eig=host decomposes Phi with host LAPACK
+ * dsyevr as the reference; the default eig=gpu uses the device
+ * (fgls_hip_eigen_decompose).
  */
 int fgls_eigen_gpu_ex(int n, int p, int m, int t, int wXL, int wXR,
                       int x_b, int y_b, int num_threads,

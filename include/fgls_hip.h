@@ -273,6 +273,13 @@ int fgls_hip_adjoint_end(fgls_hip_ctx *ctx, double *dgrad_y, double *hloss,
  *             clears the records.
  */
 typedef struct fgls_hip_eig fgls_hip_eig;
+/* Device eigendecomposition Phi = Z diag(W) Z' (dPhi column-major, lower
+ * triangle read; dZ n x n column-major, ld ldz; dW n eigenvalues, ascending)
+ * through rocSOLVER's dsyevd, loaded on first use: the device counterpart of
+ * eigenDec's dsyevr (src/fgls_eigen.c:780-806).  FGLS_ERR_STATE when
+ * rocSOLVER cannot be loaded.  Synchronizes `stream`. */
+int fgls_hip_eigen_decompose(int n, const double *dPhi, int64_t ldphi, double *dZ,
+                             int64_t ldz, double *dW, int device, void *stream);
 int fgls_hip_eig_create(fgls_hip_eig **out, int n, int wXL, int max_traits,
                         int max_block, int device);
 int fgls_hip_eig_destroy(fgls_hip_eig *e);

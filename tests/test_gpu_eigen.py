@@ -135,12 +135,37 @@ def test_eigen_rejects_non_positive_M_and_bad_strides(pkg, gpu):
     e.close()
 
 
+def test_device_eigendecomposition(pkg, gpu):
+    """fgls_hip_eigen_decompose against the host dsyevr: eigenvalues, the
+    reconstruction of Phi and orthogonality of Z; and the eigen context built
+    from it solves as from the host decomposition."""
+    import torch
+    prob = make_problem(n=700, wXL=3, m=200, seed=31, t=1)
+    Phi_d = pkg.colmajor(prob.Phi)
+    Z, W = pkg.eigen_decompose_device(Phi_d)
+    Zc = Z.cpu().numpy().T            # column-major rows -> numpy columns
+    Wc = W.cpu().numpy()
+    Zh, Wh = pkg.eigen_decompose(prob.Phi, threads=4)
+    assert np.abs(Wc - Wh).max() < 1e-12 * np.abs(Wh).max()
+    assert np.abs(Zc @ np.diag(Wc) @ Zc.T - prob.Phi).max() < 1e-12
+    assert np.abs(Zc.T @ Zc - np.eye(prob.n)).max() < 1e-12
+    e = pkg.EigenContext(prob.n, 3, max_traits=1, max_block=256)
+    e.set_basis(Z, W, pkg.colmajor(prob.XL))
+    e.set_traits([0.4], [1.0], pkg.colmajor(prob.Y))
+    B, V = e.solve_block(pkg.colmajor(prob.XR))
+    e.check()
+    (tr, b_ref, V_ref), = oracle_traits(prob, [0.4], [1.0])
+    err = orc.parity_errors(B[0].cpu().numpy(), V[0].cpu().numpy(), b_ref, v_flat(V_ref), 4)
+    assert err["B_scaled"] < TOL_FP64 and err["V_scaled"] < TOL_FP64, err
+    e.close()
+
+
 def write_dir(prob, d, H, Sig):
     write_problem_dir(prob, d, H=H, Sig=Sig)
 
 
 @pytest.mark.parametrize("opts", [None, {"precision": "mixed"}, {"traits_per_pass": 2, "block": 100},
-                                  {"precision": "mixed", "xr_format": "i8"}])
+                                  {"precision": "mixed", "xr_format": "i8"}, {"eig": "host"}])
 def test_fgls_eigen_files_match_oracle(pkg, gpu, tmp_path, opts):
     n, wXL, m, t = 500, 3, 777, 3
     prob = make_problem(n=n, wXL=wXL, m=m, seed=99, t=t)
