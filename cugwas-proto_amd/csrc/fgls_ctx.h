@@ -28,6 +28,8 @@ struct fgls_hip_ctx {
   int RT8;        // 32-row tiles of the int8 product
   int part_rt;    // row-tile capacity of `part`
   int8_t* A8;     // slices of L^-1 (mixed mode)
+  int8_t* X8t;    // tiled genotypes of the current block (mixed mode)
+  size_t X8t_cap;
   double* invsc;  // per-row 2^(e_i - 6)
   unsigned int* bad;  // count of genotypes that are not int8-exact
   double* L;
@@ -111,6 +113,9 @@ void launch_trait_stats(int n, int wXL, const double* W, int64_t ldw, double* tr
                         hipStream_t st);
 // int8 digit slices and row scales of a row-major ldn x ldn matrix.
 void launch_slices(int ldn, const double* A, int8_t* A8, double* invsc, hipStream_t st);
+// Grows a device buffer to at least `need` bytes (synchronizes the device
+// when it reallocates).
+int ensure_buffer(int8_t** buf, size_t* cap, size_t need);
 // Doubles per trait of the per-trait scalars.
 constexpr int TRAIT_LD = 2 * PMAX + PMAX * PMAX + 8;
 

@@ -58,10 +58,24 @@ __host__ __device__ constexpr size_t fgls_a8_offset(int row, int s, int kc, int 
   return (((((size_t)(row >> 4) * KS + (kc >> 2)) * S + s) * 64) + 16 * (kc & 3) + (row & 15)) * 16;
 }
 
+// Bytes of the tiled genotypes of `count` columns and `planes` planes.
+__host__ __device__ constexpr size_t fgls_x8t_bytes(int n, int count, int planes) {
+  return (size_t)planes * (size_t)((count + 15) / 16) * 16 * (size_t)((n + 63) / 64) * 64;
+}
+// Offset of the hi plane in the tiled genotypes of `count` columns.
+__host__ __device__ constexpr int64_t fgls_x8t_plane(int n, int count) {
+  return (int64_t)fgls_x8t_bytes(n, count, 1);
+}
+// Column-major genotypes (ld ldx8 >= ceil(n/64)*64, ldx8 % 16 == 0, 16-byte
+// aligned; `planes` planes of count columns, count ldx8 bytes apart) -> the
+// tiled layout of the int8 product (fgls_i8.hip).
+void fgls_launch_tile_i8(int n, int count, int planes, const int8_t* X8, int64_t ldx8, int8_t* X8T,
+                         hipStream_t st);
 // Launch of the int8 triangular product with fused reductions (fgls_i8.hip),
-// for p = 1..16.  part receives (p+1) sums per SNP per 32-row tile.  With
-// plane_off > 0 the genotypes are dosages as two int8 digit planes (lo at X8,
-// hi at X8 + plane_off, x = (128 hi + lo) * inv_xscale).  eig_ntr > 0: A8
+// for p = 1..16, on tiled genotypes X8T (ld8 = ceil(n/64)*64).  part receives
+// (p+1) sums per SNP per 32-row tile.  With plane_off > 0 the genotypes are
+// dosages as two int8 digit planes (lo at X8T, hi at X8T + plane_off,
+// x = (128 hi + lo) * inv_xscale).  eig_ntr > 0: A8
 // holds a full matrix and W has eig_ntr groups of p+1 weight columns (the
 // eigenbasis variant; part then holds eig_ntr (p+1) sums per SNP and tile).
 void fgls_launch_trmm_i8(int p, int n, int count, int RT32, int KS, const int8_t* A8,

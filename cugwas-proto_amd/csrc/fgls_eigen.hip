@@ -57,6 +57,8 @@ struct fgls_hip_eig {
   double* trait;     // ntr x TRAIT_LD
   double* part;
   int8_t* A8;        // slices of Z' (mixed mode)
+  int8_t* X8t;       // tiled genotypes of the current block (mixed mode)
+  size_t X8t_cap;
   double* invsc;
   unsigned long long* err;   // per trait: (first failing SNP << 8) | info, all ones if none
   int* status;       // [0, max_traits): V_tl Cholesky info per trait; [max_traits]: bad weights
@@ -154,13 +156,15 @@ int reduce_i8(fgls_hip_eig* e, const int8_t* dX8, int64_t ldx8, int count, int p
   if (!dX8) return fail(FGLS_ERR_ARG, "NULL pointer");
   if (ldx8 < round_up(e->n, 64) || (ldx8 & 15) || ((uintptr_t)dX8 & 15))
     return fail(FGLS_ERR_ARG, "int8 X_R needs ld >= ceil(n/64)*64, ld %% 16 == 0 and 16-byte alignment");
-  if (ldx8 * (int64_t)(planes == 2 ? count + 64 : 64) > ((int64_t)1 << 31) - 1)
-    return fail(FGLS_ERR_ARG, "int8 X_R block of %d columns with ld %lld too large", count,
-                (long long)ldx8);
+  if (planes == 2 && fgls_x8t_plane(e->n, count) > ((int64_t)1 << 31) - 1)
+    return fail(FGLS_ERR_ARG, "int8 X_R block of %d columns too large", count);
   if (planes == 2 && !(scale > 0.0)) return fail(FGLS_ERR_ARG, "dosage scale must be positive");
-  fgls_launch_trmm_i8(e->p, e->n, count, e->RT8, e->ldn / 64, e->A8, e->invsc, dX8, ldx8, e->Wg,
-                      (int64_t)e->ldn, e->part, (int64_t)e->max_block,
-                      planes == 2 ? (int64_t)count * ldx8 : 0, planes == 2 ? 1.0 / scale : 1.0,
+  int rc = fgls::ensure_buffer(&e->X8t, &e->X8t_cap, fgls_x8t_bytes(e->n, count, planes));
+  if (rc) return rc;
+  fgls_launch_tile_i8(e->n, count, planes, dX8, ldx8, e->X8t, st);
+  fgls_launch_trmm_i8(e->p, e->n, count, e->RT8, e->ldn / 64, e->A8, e->invsc, e->X8t,
+                      round_up(e->n, 64), e->Wg, (int64_t)e->ldn, e->part, (int64_t)e->max_block,
+                      planes == 2 ? fgls_x8t_plane(e->n, count) : 0, planes == 2 ? 1.0 / scale : 1.0,
                       e->ntr, st);
   HIP_TRY(hipGetLastError());
   e->last_rt = e->RT8;
@@ -319,7 +323,7 @@ int fgls_hip_eig_destroy(fgls_hip_eig* e) {
   hipSetDevice(e->device);
   hipDeviceSynchronize();
   void* ptrs[] = {e->Zt, e->evals, e->XLin, e->XLp, e->Yin, e->Yp, e->Wg, e->Wsc, e->hyp,
-                  e->trait, e->part, e->A8, e->invsc, e->err, e->status};
+                  e->trait, e->part, e->A8, e->invsc, e->err, e->status, e->X8t};
   for (void* q : ptrs)
     if (q) hipFree(q);
   delete e;
@@ -341,6 +345,8 @@ int fgls_hip_eig_set_mode(fgls_hip_eig* e, int mode) {
         return fail(FGLS_ERR_NOMEM, "hipMalloc of the row scales failed");
     }
     if ((rc = ensure_part(e, e->RT8))) return rc;
+    if ((rc = fgls::ensure_buffer(&e->X8t, &e->X8t_cap, fgls_x8t_bytes(e->n, e->max_block, 1))))
+      return rc;
     if (e->have_basis) {
       fgls::launch_slices(e->ldn, e->Zt, e->A8, e->invsc, nullptr);
       HIP_TRY(hipGetLastError());

@@ -52,8 +52,14 @@ constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); }
 //
 // MFMA v_mfma_i32_16x16x64_i8: A lane l holds 16 consecutive k of row l%16
 // (k chunk l/16 of the 64), B lane l the same 16 k of column l%16; D element
-// v of lane l is D[4*(l/16)+v][l%16].  X8: column-major int8, ld ldx8 >=
-// ceil(n/64)*64, ldx8 % 16 == 0; its rows >= n meet all-zero slice columns.
+// v of lane l is D[4*(l/16)+v][l%16].
+//
+// Genotypes X8T are tiled like the slices (fgls_x8t_offset): one 1 KB block
+// per (16 SNP columns, 64-k step) in B-operand lane order, so a wave's
+// fragment is one contiguous 1 KB load (column-major columns would make it 16
+// half cache lines ld apart); KSX = ld8 / 64 steps per column group, rows
+// >= n meet all-zero slice columns.  fgls_launch_tile_i8 builds X8T from the
+// column-major genotypes of the C-ABI.
 template <int S>
 struct SliceStage {
   fgls_v4i blk[2 * S][64];
@@ -79,7 +85,7 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i
   const int k1 = min(k + 1, last);
 #pragma unroll
   for (int tj = 0; tj < 4; ++tj)
-    bn[tj] = *reinterpret_cast<const fgls_v4i*>(Xw + 64 * k1 + boff[tj]);
+    bn[tj] = *reinterpret_cast<const fgls_v4i*>(Xw + 1024 * k1 + boff[tj]);
   __builtin_amdgcn_sched_barrier(0);
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_sched_barrier(0);
@@ -108,7 +114,8 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i
 
 // DOS (dosages): every SNP has two int8 digit columns, lo in [0, 127] and hi
 // in [-127, 127], with x * scale = 128 hi + lo; the lo plane holds columns
-// 0..count-1 of X8 and the hi plane starts plane_off bytes later.  A wave then
+// 0..count-1 of X8T and the hi plane (tiled the same way) starts plane_off
+// bytes later.  A wave then
 // covers 32 SNPs: fragments 0, 1 read their lo digits and fragments 2, 3 the
 // hi digits of the same SNPs (the same MFMA work per fragment as four
 // genotype fragments), and the epilogue forms acc_lo + 128 acc_hi in FP64.
@@ -120,7 +127,7 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i
 template <int NQ, int S, bool DOS = false, bool EIG = false>
 __global__ __launch_bounds__(256, 2) void k_trmm_i8(
     int n, int count, int RT32, int CT, int KS, const int8_t* __restrict__ A8,
-    const double* __restrict__ inv_scale, const int8_t* __restrict__ X8, int64_t ldx8,
+    const double* __restrict__ inv_scale, const int8_t* __restrict__ X8T, int64_t ld8,
     const double* __restrict__ W, int64_t ldw, double* __restrict__ part, int64_t ldp,
     int64_t plane_off = 0, double inv_xscale = 1.0, int ntr = 1) {
   static_assert(S % 2 == 0, "the 2 x S slice blocks are spread over 4 waves");
@@ -143,16 +150,18 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
     const int blk = w + 4 * j, ti = blk / S, s2 = blk % S;
     asrc[j] = A8 + fgls_a8_offset(r0 + 16 * ti, s2, 0, S, KS) + 16 * lane;
   }
-  // genotype columns of this wave: Xw + boff[tj] (+ 64 k) for fragment tj;
-  // columns past count re-read the last one (64 ldx8 < 2^31 is checked by
-  // the host)
-  const int cb = min(c0, count - 1);
-  const int8_t* Xw = X8 + (size_t)cb * ldx8;
+  // genotype fragments of this wave: Xw + boff[tj] (+ 1024 k) for fragment tj;
+  // column groups past the last one re-read it (plane_off < 2^31 is checked
+  // by the host)
+  const int KSX = (int)(ld8 / 64);
+  const int CG = (count + 15) / 16;
+  const int cgb = min(c0 / 16, CG - 1);
+  const int8_t* Xw = X8T + (size_t)cgb * KSX * 1024;
   int boff[4];
 #pragma unroll
   for (int tj = 0; tj < 4; ++tj) {
     const int cj = DOS ? (tj & 1) : tj;   // DOS: 0, 1 lo digits; 2, 3 hi digits
-    boff[tj] = (min(c0 + 16 * cj + li, count - 1) - cb) * (int)ldx8 + 16 * lg +
+    boff[tj] = (min(c0 / 16 + cj, CG - 1) - cgb) * KSX * 1024 + 16 * lane +
                ((DOS && tj >= 2) ? (int)plane_off : 0);
   }
 
@@ -379,6 +388,24 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   }
 }
 
+// Column-major genotypes (ld ldx8; `planes` planes of count columns, plane q
+// at X8 + q count ldx8) -> tiled X8T (plane q at X8T + q CG KSX 1024, CG =
+// ceil(count/16), KSX = ld8/64): one workgroup per (column group, plane); its
+// output is contiguous, each thread writes one 16-byte lane chunk.  Columns
+// past count repeat the last column.
+__global__ __launch_bounds__(256) void k_tile_i8(int count, int KSX, const int8_t* __restrict__ X8,
+                                                 int64_t ldx8, int8_t* __restrict__ X8T) {
+  const int cg = blockIdx.x, pl = blockIdx.y, CG = gridDim.x;
+  const int8_t* src = X8 + (size_t)pl * count * ldx8;
+  int8_t* dst = X8T + ((size_t)pl * CG + cg) * KSX * 1024;
+  for (int t = threadIdx.x; t < KSX * 64; t += 256) {
+    const int ks = t >> 6, g = (t >> 4) & 3, i = t & 15;
+    const int col = min(cg * 16 + i, count - 1);
+    *reinterpret_cast<fgls_v4i*>(dst + (size_t)t * 16) =
+        *reinterpret_cast<const fgls_v4i*>(src + (size_t)col * ldx8 + 64 * ks + 16 * g);
+  }
+}
+
 template <int P, bool EIG>
 void launch(int n, int count, int RT32, int KS, const int8_t* A8, const double* isc,
             const int8_t* X8, int64_t ldx8, const double* W, int64_t ldw, double* part, int64_t ldp,
@@ -411,6 +438,13 @@ void launch_any(int n, int count, int RT32, int KS, const int8_t* A8, const doub
 }
 
 }  // namespace
+
+void fgls_launch_tile_i8(int n, int count, int planes, const int8_t* X8, int64_t ldx8, int8_t* X8T,
+                         hipStream_t st) {
+  const int KSX = (n + 63) / 64;
+  hipLaunchKernelGGL(k_tile_i8, dim3((count + 15) / 16, planes), dim3(256), 0, st, count, KSX, X8,
+                     ldx8, X8T);
+}
 
 void fgls_launch_trmm_i8(int p, int n, int count, int RT32, int KS, const int8_t* A8,
                          const double* inv_scale, const int8_t* X8, int64_t ldx8, const double* W,

@@ -1019,6 +1019,20 @@ void launch_trait_stats(int n, int wXL, const double* W, int64_t ldw, double* tr
   hipLaunchKernelGGL(k_trait_stats, dim3(1), dim3(256), 0, st, n, wXL, W, ldw, trait, status);
 }
 
+int ensure_buffer(int8_t** buf, size_t* cap, size_t need) {
+  if (*cap >= need && *buf) return FGLS_OK;
+  HIP_TRY(hipDeviceSynchronize());
+  if (*buf) HIP_TRY(hipFree(*buf));
+  *buf = nullptr;
+  *cap = 0;
+  if (hipMalloc((void**)buf, need) != hipSuccess) {
+    *buf = nullptr;
+    return fail(FGLS_ERR_NOMEM, "hipMalloc of %zu bytes failed", need);
+  }
+  *cap = need;
+  return FGLS_OK;
+}
+
 void launch_slices(int ldn, const double* A, int8_t* A8, double* invsc, hipStream_t st) {
   hipLaunchKernelGGL((k_slice_linv<NSLICE>), dim3(ldn), dim3(256), 0, st, ldn, A, A8, invsc);
 }
@@ -1142,7 +1156,7 @@ int fgls_hip_ctx_destroy(fgls_hip_ctx* c) {
   hipSetDevice(c->device);
   hipDeviceSynchronize();
   void* ptrs[] = {c->L, c->Linv, c->Dinv, c->W, c->Wtmp, c->trait, c->part, c->z, c->err, c->status,
-                  c->A8, c->invsc, c->bad, c->panel};
+                  c->A8, c->invsc, c->bad, c->panel, c->X8t};
   for (void* p : ptrs)
     if (p) hipFree(p);
   fgls_adjoint_free(c->adj);
@@ -1281,6 +1295,8 @@ int fgls_hip_ctx_set_mode(fgls_hip_ctx* c, int mode) {
       if (hipMalloc((void**)&c->invsc, (size_t)c->ldn * sizeof(double)) != hipSuccess)
         return fail(FGLS_ERR_NOMEM, "hipMalloc of the row scales failed");
     }
+    if ((rc = fgls::ensure_buffer(&c->X8t, &c->X8t_cap, fgls_x8t_bytes(c->n, c->max_block, 1))))
+      return rc;
     if (c->part_rt < c->RT8) {
       double* np = nullptr;
       const size_t bytes = (size_t)c->RT8 * (c->p + 1) * (size_t)c->max_block * sizeof(double);
@@ -1339,17 +1355,18 @@ int reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count, int p
   if (!dX8) return fail(FGLS_ERR_ARG, "NULL pointer");
   if (ldx8 < (int64_t)round_up(c->n, 64) || (ldx8 & 15) || ((uintptr_t)dX8 & 15))
     return fail(FGLS_ERR_ARG, "int8 X_R needs ld >= ceil(n/64)*64, ld %% 16 == 0 and 16-byte alignment");
-  // 32-bit offsets within a wave's columns (and, for dosages, to the hi plane)
-  if (ldx8 * (int64_t)(planes == 2 ? count + 64 : 64) > ((int64_t)1 << 31) - 1)
-    return fail(FGLS_ERR_ARG, "int8 X_R block of %d columns with ld %lld too large", count,
-                (long long)ldx8);
+  // 32-bit offsets to the hi plane of the tiled dosage digits
+  if (planes == 2 && fgls_x8t_plane(c->n, count) > ((int64_t)1 << 31) - 1)
+    return fail(FGLS_ERR_ARG, "int8 X_R block of %d columns too large", count);
   if (planes == 2 && !(scale > 0.0)) return fail(FGLS_ERR_ARG, "dosage scale must be positive");
   if ((rc = set_device(c))) return rc;
+  if ((rc = fgls::ensure_buffer(&c->X8t, &c->X8t_cap, fgls_x8t_bytes(c->n, count, planes)))) return rc;
   hipStream_t st = (hipStream_t)stream;
-  fgls_launch_trmm_i8(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, dX8, ldx8, c->W,
-                      (int64_t)c->ldn, c->part, (int64_t)c->max_block,
-                      planes == 2 ? (int64_t)count * ldx8 : 0, planes == 2 ? 1.0 / scale : 1.0, 0,
-                      st);
+  fgls_launch_tile_i8(c->n, count, planes, dX8, ldx8, c->X8t, st);
+  fgls_launch_trmm_i8(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, c->X8t,
+                      round_up(c->n, 64), c->W, (int64_t)c->ldn, c->part, (int64_t)c->max_block,
+                      planes == 2 ? fgls_x8t_plane(c->n, count) : 0, planes == 2 ? 1.0 / scale : 1.0,
+                      0, st);
   HIP_TRY(hipGetLastError());
   c->reduced = count;
   c->last_rt = c->RT8;
