@@ -18,6 +18,8 @@ from .fgls import (  # noqa: F401
     last_error,
     lib,
     LIB_PATH,
+    pack_u2,
+    u2_to_i8,
     FGLS_OK,
     FGLS_ERR_ARG,
     FGLS_ERR_HIP,

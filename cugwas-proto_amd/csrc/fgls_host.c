@@ -66,7 +66,7 @@ typedef struct {
   int io_threads;    /* parallel preads of X_R per block */
   int host_potrf;    /* potrf=host: M and its factor on the host (OpenBLAS) */
   int traits_per_pass;  /* traits sharing one read of X_R (0: as many as fit) */
-  int xr_i8;         /* xr_format=i8: X_R file of int8 values (n bytes per column) */
+  int xr_i8;         /* xr_format: 0 f64, 1 i8 (n bytes per column), 2 u2 (ceil(n/4) bytes per column) */
   double dosage_scale;  /* precision=mixed for dosages: x * scale = 128 hi + lo (0: genotypes) */
   int eig_host;      /* eig=host: eigendecomposition of Phi by host LAPACK dsyevr (default: device) */
 } options_t;
@@ -94,7 +94,7 @@ static void parse_options(const char *s, options_t *o) {
     if (!eq) continue;
     *eq = 0;
     if (!strcmp(tok, "xr_format")) {
-      o->xr_i8 = !strcmp(eq + 1, "i8") ? 1 : (!strcmp(eq + 1, "f64") ? 0 : -1);
+      o->xr_i8 = !strcmp(eq + 1, "i8") ? 1 : !strcmp(eq + 1, "u2") ? 2 : (!strcmp(eq + 1, "f64") ? 0 : -1);
       continue;
     }
     if (!strcmp(tok, "potrf")) {
@@ -175,7 +175,7 @@ typedef struct {
   int fd;
   const char *path;
   int n, wXR, t;
-  int esize;                /* bytes per X_R value: 8 (FP64 file) or 1 (int8 file) */
+  size_t col_bytes;         /* bytes per X_R column: 8 n (FP64), n (int8) or ceil(n/4) (2-bit) */
   int64_t m, block, nchunks;
   void *slot[2];            /* pinned */
   sem_t free_[2], full[2];
@@ -188,7 +188,7 @@ typedef struct {
 
 static void *reader_main(void *arg) {
   reader_t *R = (reader_t *)arg;
-  const size_t col_bytes = (size_t)R->n * (size_t)R->esize;
+  const size_t col_bytes = R->col_bytes;
   for (int64_t it = 0; it < (int64_t)R->t * R->nchunks; ++it) {
     const int s = (int)(it & 1);
     sem_wait(&R->free_[s]);
@@ -332,7 +332,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   if (opt.eig_host < 0) return fgls_set_error(FGLS_ERR_ARG, "eig must be gpu or host");
   if (opt.mixed < 0) return fgls_set_error(FGLS_ERR_ARG, "precision must be fp64 or mixed");
   if (opt.host_potrf < 0) return fgls_set_error(FGLS_ERR_ARG, "potrf must be gpu or host");
-  if (opt.xr_i8 < 0) return fgls_set_error(FGLS_ERR_ARG, "xr_format must be f64 or i8");
+  if (opt.xr_i8 < 0) return fgls_set_error(FGLS_ERR_ARG, "xr_format must be f64, i8 or u2");
   if (opt.dosage_scale < 0.0) return fgls_set_error(FGLS_ERR_ARG, "dosage_scale must be positive");
   if (opt.dosage_scale > 0.0 && (!opt.mixed || opt.xr_i8))
     return fgls_set_error(FGLS_ERR_ARG, "dosage_scale needs precision=mixed and an FP64 X_R file");
@@ -375,6 +375,9 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   unsigned long long *hErr = NULL;  /* pinned copies of the failure records */
   unsigned int *hBad = NULL;        /* pinned: genotypes of the slot that are not int8-exact */
   int8_t *dX8[2] = {NULL, NULL};
+  uint8_t *dP2[2] = {NULL, NULL};   /* xr_format=u2: packed columns of the slot */
+  const size_t col_bytes = opt.xr_i8 == 2 ? (size_t)(n + 3) / 4 : opt.xr_i8 == 1 ? (size_t)n
+                                                                               : (size_t)n * sizeof(double);
   unsigned int *dBad = NULL;
   const int ld8 = (n + 63) & ~63;
   double *dL = NULL, *dXL = NULL, *dy = NULL, *dX[2] = {NULL, NULL};
@@ -506,6 +509,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
       const size_t b8 = (size_t)block * wXR * ld8 * (dos ? 2 : 1);
       HCHK(hipMalloc((void **)&dX8[s], b8));
       HCHK(hipMemset(dX8[s], 0, b8));
+      if (opt.xr_i8 == 2) HCHK(hipMalloc((void **)&dP2[s], (size_t)block * wXR * col_bytes));
     }
   }
   if (nchunks > 0) {
@@ -513,8 +517,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     for (int s = 0; s < 2; ++s) {
       HCHK(hipMalloc((void **)&dX[s], xbytes));
       HCHK(hipMemset(dX[s], 0, xbytes));
-      HCHK(hipHostMalloc((void **)&R.slot[s], (size_t)block * wXR * n * (opt.xr_i8 ? 1 : sizeof(double)),
-                         hipHostMallocDefault));
+      HCHK(hipHostMalloc((void **)&R.slot[s], (size_t)block * wXR * col_bytes, hipHostMallocDefault));
       HCHK(hipMalloc((void **)&dBs[s], (size_t)K * block * p * sizeof(double)));
       HCHK(hipMalloc((void **)&dVs[s], (size_t)K * block * pp * sizeof(double)));
       HCHK(hipHostMalloc((void **)&hBs[s], (size_t)K * block * p * sizeof(double), hipHostMallocDefault));
@@ -529,7 +532,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     }
     R.path = XR_path;
     R.n = n;
-    R.esize = opt.xr_i8 ? 1 : (int)sizeof(double);
+    R.col_bytes = col_bytes;
     R.wXR = wXR;
     R.t = (t + K - 1) / K;   /* passes over X_R */
     R.m = m;
@@ -591,7 +594,9 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
         }
         /* device X_R slot s is free once the solves of chunk c-2 have run */
         HCHK(hipStreamWaitEvent(s_copy, ev_free[s], 0));
-        if (opt.xr_i8)
+        if (opt.xr_i8 == 2)
+          HCHK(hipMemcpyAsync(dP2[s], R.slot[s], (size_t)cnt * wXR * col_bytes, hipMemcpyHostToDevice, s_copy));
+        else if (opt.xr_i8)
           HCHK(hipMemcpy2DAsync(dX8[s], (size_t)ld8, R.slot[s], (size_t)n, (size_t)n,
                                 (size_t)cnt * wXR, hipMemcpyHostToDevice, s_copy));
         else
@@ -600,6 +605,13 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
                                 (size_t)cnt * wXR, hipMemcpyHostToDevice, s_copy));
         HCHK(hipEventRecord(ev_h2d[s], s_copy));
         HCHK(hipStreamWaitEvent(s_comp, ev_h2d[s], 0));
+        if (opt.xr_i8 == 2) {
+          /* 2-bit codes -> int8 on the device; code 3 is counted and fails
+           * the run at write-back */
+          HCHK(hipMemsetAsync(dBad + s, 0, sizeof(unsigned int), s_comp));
+          RCHK(fgls_hip_u2_to_i8(dP2[s], (int64_t)col_bytes, n, cnt * wXR, dX8[s], ld8, dBad + s, s_comp));
+          HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
+        }
         if (opt.mixed && !opt.xr_i8) {
           /* exact FP64 -> int8 on the device; a value that is not an integer
            * in [-127, 127] is counted and fails the run at write-back */
@@ -657,7 +669,10 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
         const int cnt = (int)((m - i0) < block ? (m - i0) : block);
         HCHK(hipEventSynchronize(ev_out[ps]));
         if (hBad[ps] != 0) {
-          if (dos)
+          if (opt.xr_i8 == 2)
+            rc = fgls_set_error(FGLS_ERR_ARG, "%s: %u 2-bit codes of SNPs %lld.. are 3 (not a genotype)",
+                                XR_path, hBad[ps], (long long)i0);
+          else if (dos)
             rc = fgls_set_error(FGLS_ERR_ARG,
                                 "precision=mixed: %u dosage values of SNPs %lld.. are not multiples of "
                                 "1/%g in range (use precision=fp64)", hBad[ps], (long long)i0,
@@ -696,7 +711,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
             eig ? "eigen" : "chol", n, p, m, t, (long long)block, K,
             eig ? (opt.eig_host ? "host dsyevr" : "device dsyevd") : "host potrf",
             eig ? t_eig : t_potrf, t_trait, t_loop, t_loop > 0 ? snps / t_loop : 0.0,
-            t_loop > 0 ? (double)m * R.t * wXR * n * R.esize / t_loop / 1e9 : 0.0, R.busy_s, threads);
+            t_loop > 0 ? (double)m * R.t * wXR * R.col_bytes / t_loop / 1e9 : 0.0, R.busy_s, threads);
   }
 
 done:
@@ -720,6 +735,7 @@ done:
     if (dX[s]) hipFree(dX[s]);
     if (R.slot[s]) hipHostFree(R.slot[s]);
     if (dX8[s]) hipFree(dX8[s]);
+    if (dP2[s]) hipFree(dP2[s]);
   }
   for (int s = 0; s < 2; ++s) {
     if (dBs[s]) hipFree(dBs[s]);

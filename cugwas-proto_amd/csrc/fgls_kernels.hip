@@ -552,6 +552,32 @@ __global__ __launch_bounds__(256) void k_to_dosage_i8(int n, const double* __res
   if (nb) atomicAdd(bad, nb);
 }
 
+// 2-bit genotypes -> int8 (column-major, ld ldx8, rows n..ldx8-1 zeroed):
+// column col is ceil(n/4) bytes at P + col*ldp, row r in bits 2(r%4)+1..2(r%4)
+// of byte r/4; codes 0, 1, 2 are the genotype, code 3 is counted in *bad and
+// written as 0.  One workgroup per column, one packed byte per thread.
+__global__ __launch_bounds__(256) void k_u2_to_i8(int n, const uint8_t* __restrict__ P, int64_t ldp,
+                                                  int8_t* __restrict__ X8, int64_t ldx8,
+                                                  unsigned int* __restrict__ bad) {
+  const int col = blockIdx.x;
+  const int nb = (n + 3) / 4;
+  unsigned int nbad = 0;
+  for (int b = threadIdx.x; b < (int)(ldx8 / 4); b += 256) {
+    const unsigned int byte = (b < nb) ? P[(size_t)col * ldp + b] : 0u;
+    unsigned int out = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const unsigned int c = (byte >> (2 * j)) & 3u;
+      if (4 * b + j < n) {
+        if (c == 3u) ++nbad;
+        else out |= c << (8 * j);
+      }
+    }
+    *reinterpret_cast<unsigned int*>(X8 + (size_t)col * ldx8 + 4 * b) = out;
+  }
+  if (nbad) atomicAdd(bad, nbad);
+}
+
 // int8 genotypes -> FP64 (column-major, ld ldx; rows n..ldx-1 written as 0).
 __global__ __launch_bounds__(256) void k_i8_to_f64(int n, const int8_t* __restrict__ X8, int64_t ldx8,
                                                    double* __restrict__ X, int64_t ldx) {
@@ -1322,6 +1348,18 @@ int fgls_hip_to_i8(const double* dX, int64_t ldx, int n, int ncols, int8_t* dX8,
     return fail(FGLS_ERR_ARG, "bad arguments to fgls_hip_to_i8");
   if (ncols == 0) return FGLS_OK;
   hipLaunchKernelGGL(k_to_i8, dim3(ncols), dim3(256), 0, (hipStream_t)stream, n, dX, ldx, dX8, ldx8,
+                     dbad);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_u2_to_i8(const uint8_t* dP, int64_t ldp, int n, int ncols, int8_t* dX8, int64_t ldx8,
+                      unsigned int* dbad, void* stream) {
+  if (n < 1 || ncols < 0 || !dP || !dX8 || !dbad || ldp < (n + 3) / 4 || ldx8 < n || (ldx8 & 3) ||
+      ((uintptr_t)dX8 & 3))
+    return fail(FGLS_ERR_ARG, "bad arguments to fgls_hip_u2_to_i8");
+  if (ncols == 0) return FGLS_OK;
+  hipLaunchKernelGGL(k_u2_to_i8, dim3(ncols), dim3(256), 0, (hipStream_t)stream, n, dP, ldp, dX8, ldx8,
                      dbad);
   HIP_TRY(hipGetLastError());
   return FGLS_OK;

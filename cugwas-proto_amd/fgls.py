@@ -79,6 +79,7 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_block_reduce_dosage": (i32, [vp, dp, i64, i32, ctypes.c_double, vp]),
             "fgls_hip_solve_block_dosage": (i32, [vp, dp, i64, i32, ctypes.c_double, dp, dp, i64, vp]),
             "fgls_hip_i8_to_f64": (i32, [dp, i64, i32, i32, dp, i64, vp]),
+            "fgls_hip_u2_to_i8": (i32, [dp, i64, i32, i32, dp, i64, dp, vp]),
             "fgls_hip_set_trait_phi": (i32, [vp, dp, i64, ctypes.c_double, ctypes.c_double, dp, i64, dp, vp]),
             "fgls_hip_get_factor": (i32, [vp, dp, i64, vp]),
             "fgls_hip_adjoint_begin": (i32, [vp, dp, i64, ctypes.c_double, vp]),
@@ -126,7 +127,7 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_adjoint_begin", "fgls_hip_adjoint_block", "fgls_hip_adjoint_end",
                         "fgls_hip_set_trait_phi", "fgls_hip_get_factor", "fgls_hip_to_dosage_i8",
                         "fgls_hip_block_reduce_dosage", "fgls_hip_solve_block_dosage",
-                        "fgls_hip_i8_to_f64",
+                        "fgls_hip_i8_to_f64", "fgls_hip_u2_to_i8",
                         "fgls_factor_M", "fgls_eigen", "fgls_eigen_gpu_ex", "fgls_eigen_decompose",
                         "fgls_hip_eigen_decompose",
                         "fgls_hip_eig_create", "fgls_hip_eig_destroy", "fgls_hip_eig_ldn",
@@ -560,6 +561,44 @@ def eigen_decompose(Phi, threads: int = 0):
     W = np.empty(n, dtype=np.float64)
     _check(lib().fgls_eigen_decompose(n, Phi.ctypes.data, n, Z.ctypes.data, n, W.ctypes.data, int(threads)))
     return Z, W
+
+
+def pack_u2(X):
+    """Genotypes X (n x m, values 0, 1, 2; column j = SNP j) -> the 2-bit
+    layout of xr_format=u2 / fgls_hip_u2_to_i8: an (m, ceil(n/4)) uint8 array,
+    individual r in bits 2(r%4)..2(r%4)+1 of byte r/4.  Values outside
+    {0, 1, 2} raise."""
+    import numpy as np
+    X = np.asarray(X)
+    if X.ndim == 1:
+        X = X[:, None]
+    n, m = X.shape
+    c = np.rint(X).astype(np.int64)
+    if not np.array_equal(c, X) or c.min(initial=0) < 0 or c.max(initial=0) > 2:
+        raise FglsError(FGLS_ERR_ARG, "2-bit genotypes must be 0, 1 or 2")
+    nb = (n + 3) // 4
+    pad = np.zeros((nb * 4, m), dtype=np.uint8)
+    pad[:n] = c.astype(np.uint8)
+    q = pad.reshape(nb, 4, m)
+    packed = q[:, 0] | (q[:, 1] << 2) | (q[:, 2] << 4) | (q[:, 3] << 6)
+    return np.ascontiguousarray(packed.T)
+
+
+def u2_to_i8(P, n: int, X8=None, stream=None):
+    """Device 2-bit genotypes P ((ncols, ldp) uint8 tensor) -> int8 (ncols, ld8)
+    column-major block; raises if a code is 3."""
+    import torch
+    ncols = P.shape[0]
+    ld8 = (n + 63) // 64 * 64
+    if X8 is None:
+        X8 = torch.empty((ncols, ld8), dtype=torch.int8, device=P.device)
+    bad = torch.zeros(1, dtype=torch.int32, device=P.device)
+    _check(lib().fgls_hip_u2_to_i8(_ptr(P), P.shape[1], n, ncols, _ptr(X8), X8.shape[1], _ptr(bad),
+                                   _stream(stream)))
+    nb = int(bad.item())
+    if nb:
+        raise FglsError(FGLS_ERR_ARG, f"{nb} 2-bit codes are 3 (not a genotype)")
+    return X8
 
 
 def eigen_decompose_device(Phi, stream=None, device: int = 0):

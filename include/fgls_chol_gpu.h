@@ -63,6 +63,11 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *                  as the FP64 file): 8x fewer bytes from disk and over
  *                  PCIe; used as is by precision=mixed, converted to FP64 on
  *                  the device otherwise (default xr_format=f64, io.c:35-55)
+ *   xr_format=u2   XR_path holds 2-bit genotypes, ceil(n/4) bytes per column
+ *                  (m*wXR columns in the FP64 file's order; individual r in
+ *                  bits 2(r%4)..2(r%4)+1 of byte r/4, codes 0/1/2 = the
+ *                  genotype, code 3 is an FGLS_ERR_ARG): 32x fewer bytes than
+ *                  FP64; unpacked to int8 on the device (fgls_hip_u2_to_i8)
  *   traits_per_pass=<k>  traits solved per read of each X_R block (default:
  *                  all t traits when their contexts fit in 60 % of the free
  *                  device memory; the reference re-reads X_R for every

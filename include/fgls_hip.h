@@ -152,6 +152,14 @@ int fgls_hip_solve_block_dosage(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ld
                                 int count, double scale, double *dB, double *dV,
                                 int64_t snp_base, void *stream);
 
+/* 2-bit genotypes -> int8 (column-major, ld ldx8 >= n, ldx8 % 4 == 0, rows
+ * n..ldx8-1 zeroed): column c is ceil(n/4) bytes at dP + c*ldp, individual r
+ * in bits 2(r%4)..2(r%4)+1 of byte r/4 (the first individual in the low
+ * bits); codes 0, 1, 2 are the genotype, code 3 is counted into *dbad
+ * (device counter, caller-cleared) and written as 0.  No counterpart in the
+ * reference (its X_R is FP64 only, io.c:35-55).  Asynchronous. */
+int fgls_hip_u2_to_i8(const uint8_t *dP, int64_t ldp, int n, int ncols,
+                      int8_t *dX8, int64_t ldx8, unsigned int *dbad, void *stream);
 /* int8 genotypes (ld ldx8) -> FP64 (ld ldx, rows n..ldx-1 zeroed), for
  * compact genotype input on the FP64 path.  No counterpart in the
  * reference (its X_R is FP64 only, io.c:35-55).  Asynchronous. */

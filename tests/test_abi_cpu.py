@@ -127,3 +127,17 @@ def test_host_factor_matches_lapack(pkg):
     with pytest.raises(pkg.FglsError) as ei:
         pkg.factor_M(Phi_bad, 0.9, 1.0)
     assert ei.value.code == pkg.FGLS_ERR_NOTSPD and "info: 21" in ei.value.msg
+
+
+def test_pack_u2_layout(pkg):
+    """The 2-bit layout of xr_format=u2: individual r in bits 2(r%4)..2(r%4)+1
+    of byte r/4, first individual in the low bits, columns padded to whole
+    bytes; anything but 0, 1, 2 is rejected."""
+    X = np.array([[0, 2], [1, 2], [2, 0], [1, 1], [2, 0]])          # n = 5, m = 2
+    P = pkg.pack_u2(X)
+    assert P.dtype == np.uint8 and P.shape == (2, 2)
+    assert P[0, 0] == (0 | 1 << 2 | 2 << 4 | 1 << 6) and P[0, 1] == 2
+    assert P[1, 0] == (2 | 2 << 2 | 0 << 4 | 1 << 6) and P[1, 1] == 0
+    for bad in (np.array([[3]]), np.array([[-1]]), np.array([[0.5]])):
+        with pytest.raises(pkg.FglsError):
+            pkg.pack_u2(bad)

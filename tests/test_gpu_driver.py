@@ -226,3 +226,57 @@ def test_fgls_chol_gpu_mixed_dosages(pkg, gpu, tmp_path):
                           f("XL.in"), f("XR.in"), f("Y.in"), f("B2.out"), f("V2.out"),
                           options={"precision": "mixed"})
     assert ei.value.code == pkg.FGLS_ERR_ARG
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_u2_genotype_file(pkg, gpu, tmp_path, precision):
+    """xr_format=u2 (2-bit X_R file, ceil(n/4) bytes per SNP) gives exactly the
+    files of the int8 X_R file with the same genotypes (n not a multiple of 4,
+    ragged last block); a code 3 is a defined error."""
+    print(f"2-bit X_R file, precision={precision}", flush=True)
+    n, m, t, wXL = 701, 1100, 2, 3
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=23, t=t)
+    d = str(tmp_path)
+    H, Sig = np.array([0.4, 0.6]), np.array([1.0, 1.5])
+    write_problem_dir(prob, d, H=H, Sig=Sig)
+    f = lambda s: os.path.join(d, s)
+    prob.XR.astype(np.int8).ravel(order="F").tofile(f("XR8.in"))
+    P = pkg.pack_u2(prob.XR)
+    assert P.shape == (m, (n + 3) // 4)
+    P.tofile(f("XR2.in"))
+    p = wXL + 1
+    out = {}
+    for fmt, xr in (("i8", "XR8.in"), ("u2", "XR2.in")):
+        pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, 256, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f(xr), f("Y.in"), f(f"B_{fmt}.out"), f(f"V_{fmt}.out"),
+                          options={"precision": precision, "xr_format": fmt, "block": 512})
+        out[fmt] = (read_out(d, f"B_{fmt}.out", t * m * p), read_out(d, f"V_{fmt}.out", t * m * p * p))
+    assert np.array_equal(out["u2"][0], out["i8"][0])
+    assert np.array_equal(out["u2"][1], out["i8"][1])
+    # the eigen variant reads the same file
+    pkg.fgls_eigen(n, p, m, t, wXL, 1, 256, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"), f("XL.in"),
+                   f("XR2.in"), f("Y.in"), f("Be.out"), f("Ve.out"),
+                   options={"precision": precision, "xr_format": "u2"})
+    Bo, Vo = orc.fgls_eigen(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y)
+    e = orc.parity_errors(read_out(d, "Be.out", t * m * p), read_out(d, "Ve.out", t * m * p * p), Bo, Vo, p)
+    assert e["B_scaled"] < 2e-11 and e["V_scaled"] < 2e-11, e
+    # a code 3 (SNP 900, individual 10: byte 2, bits 4-5) fails the run with a defined error
+    P[900, 2] |= 0x30
+    P.tofile(f("XR2bad.in"))
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, 256, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f("XR2bad.in"), f("Y.in"), f("B.out"), f("V.out"),
+                          options={"precision": precision, "xr_format": "u2", "block": 512})
+    assert ei.value.code == pkg.FGLS_ERR_ARG and "1 2-bit codes" in ei.value.msg
+
+
+def test_u2_unpack_kernel(pkg, gpu):
+    """fgls_hip_u2_to_i8 against the packed values, every padding case of n % 4."""
+    import torch
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 3, 4, 5, 63, 64, 65, 1001):
+        X = rng.integers(0, 3, size=(n, 37))
+        P = torch.from_numpy(pkg.pack_u2(X)).cuda()
+        X8 = pkg.u2_to_i8(P, n)
+        got = X8.cpu().numpy()
+        assert np.array_equal(got[:, :n].T, X.astype(np.int8))
+        assert not got[:, n:].any()
