@@ -248,13 +248,18 @@ def run_eigen(torch, pkg, args, Phi, XL, y, X, X8, chol_fp64, dev):
         starts = list(range(0, m, block))
         evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in starts]
 
+        XT = (torch.empty(pkg.lib().fgls_hip_x8t_bytes(n, block, 1), dtype=torch.int8, device=dev)
+              if mode == "mixed" else None)
+
         def step(record):
             for bi, i0 in enumerate(starts):
                 cnt = min(block, m - i0)
+                if mode == "mixed":
+                    pkg.tile_i8(X8[i0:i0 + cnt], n, cnt, X8T=XT)
                 if record:
                     evs[bi][0].record()
                 if mode == "mixed":
-                    e.solve_block_i8(X8[i0:i0 + cnt], cnt, Bo[:, i0:i0 + cnt], Vo[:, i0:i0 + cnt], snp_base=i0)
+                    e.solve_block_i8t(XT, cnt, 1, 1.0, Bo[:, i0:i0 + cnt], Vo[:, i0:i0 + cnt], snp_base=i0)
                 else:
                     e.solve_block(X[i0:i0 + cnt], cnt, Bo[:, i0:i0 + cnt], Vo[:, i0:i0 + cnt], snp_base=i0)
                 if record:
@@ -300,7 +305,7 @@ def run_eigen(torch, pkg, args, Phi, XL, y, X, X8, chol_fp64, dev):
             f"setup {setup_s:.2f} s; parity {res.get('parity_vs_oracle')}; "
             f"vs chol {res.get('trait0_vs_cholesky_fp64_all_snps')}")
         out["paths"][mode] = res
-        del Bo, Vo
+        del Bo, Vo, XT
     e.close()
     return out
 
@@ -421,31 +426,46 @@ def main():
         torch.cuda.synchronize()
         log(f"dosage planes ready: {sum(t.numel() for t in D8) / 1e9:.1f} GB int8")
 
+    X8T = None
+
     def run_path(mode):
-        """Time `args.steps` passes of one path; returns outputs and timings."""
-        if mode == "dosage":
-            t0 = time.perf_counter()
-            ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)
-            torch.cuda.synchronize()
-            setup_s = time.perf_counter() - t0
-            reduce = lambda i0, cnt: ctx.block_reduce_dosage(D8[i0 // block], cnt, 1000.0)
-        elif mode == "mixed":
+        """Time `args.steps` passes of one path; returns outputs and timings.
+        The int8 paths tile each block's genotypes (fgls_hip_tile_i8) and then
+        run the product on the tiles; both launches are inside the step and
+        timed apart."""
+        nonlocal X8T
+        tile = None
+        if mode in ("dosage", "mixed"):
             t0 = time.perf_counter()
             ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)      # slices L^-1 of the current trait
             torch.cuda.synchronize()
             setup_s = time.perf_counter() - t0
-            reduce = lambda i0, cnt: ctx.block_reduce_i8(X8[i0:i0 + cnt], cnt)
+            planes = 2 if mode == "dosage" else 1
+            if X8T is None or X8T.numel() < pkg.lib().fgls_hip_x8t_bytes(n, block, planes):
+                X8T = None
+                torch.cuda.empty_cache()
+                X8T = torch.empty(pkg.lib().fgls_hip_x8t_bytes(n, block, planes), dtype=torch.int8, device=dev)
+            if mode == "dosage":
+                tile = lambda i0, cnt: pkg.tile_i8(D8[i0 // block], n, cnt, planes=2, X8T=X8T)
+                reduce = lambda i0, cnt: ctx.block_reduce_i8t(X8T, cnt, 2, 1000.0)
+            else:
+                tile = lambda i0, cnt: pkg.tile_i8(X8[i0:i0 + cnt], n, cnt, X8T=X8T)
+                reduce = lambda i0, cnt: ctx.block_reduce_i8t(X8T, cnt)
         else:
             ctx.set_mode(pkg.FGLS_MODE_FP64)
             setup_s = 0.0
             reduce = lambda i0, cnt: ctx.block_reduce(X[i0:i0 + cnt], cnt)
         Bo = torch.empty((m, p), dtype=torch.float64, device=dev)
         Vo = torch.empty((m, p * p), dtype=torch.float64, device=dev)
-        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in starts]
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in starts]
 
         def step(record):
             for bi, i0 in enumerate(starts):
                 cnt = min(block, m - i0)
+                if record:
+                    evs[bi][3].record()
+                if tile is not None:
+                    tile(i0, cnt)
                 if record:
                     evs[bi][0].record()
                 reduce(i0, cnt)
@@ -459,7 +479,7 @@ def main():
             step(False)
         ctx.check()
         torch.cuda.synchronize()
-        red_ms, fin_ms, nfull = 0.0, 0.0, 0
+        red_ms, fin_ms, tile_ms, nfull = 0.0, 0.0, 0.0, 0
         t0 = time.perf_counter()
         for k in range(args.steps):
             step(True)
@@ -468,13 +488,14 @@ def main():
                 if min(block, m - i0) == block:
                     red_ms += evs[bi][0].elapsed_time(evs[bi][1])
                     fin_ms += evs[bi][1].elapsed_time(evs[bi][2])
+                    tile_ms += evs[bi][3].elapsed_time(evs[bi][0])
                     nfull += 1
             log(f"{mode} step {k + 1}/{args.steps} done")
         elapsed = time.perf_counter() - t0
         ctx.check()
         return {"B": Bo, "V": Vo, "elapsed": elapsed, "snps_per_s": m * args.steps / elapsed,
                 "reduce_ms": red_ms / max(nfull, 1), "finalize_ms": fin_ms / max(nfull, 1),
-                "setup_s": setup_s}
+                "tile_ms": tile_ms / max(nfull, 1), "setup_s": setup_s}
 
     paths = {}
     for mode in args.paths.split(","):
@@ -533,8 +554,8 @@ def main():
         ach = ops / (res["reduce_ms"] * 1e-3) / 1e12
         return {"bound": "mfma", "achieved": ach, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_INT8_TOPS, "traffic": traffic(mode),
-                "kernel": ("k_trmm_i8 dosage planes (fgls_hip_block_reduce_dosage)" if planes == 2
-                           else "k_trmm_i8 (fgls_hip_block_reduce_i8)"), "achieved_is": "int8 TOP/s",
+                "kernel": ("k_trmm_i8 dosage planes (fgls_hip_block_reduce_i8t, planes 2)" if planes == 2
+                           else "k_trmm_i8 (fgls_hip_block_reduce_i8t)"), "achieved_is": "int8 TOP/s",
                 "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block,
                 "int8_ops_per_snp": planes * NSLICE * float(n) * n,
                 "fp64_equivalent_tflops": flop / (res["reduce_ms"] * 1e-3) / 1e12}
@@ -545,6 +566,7 @@ def main():
     for mode, r in paths.items():
         summary[mode] = {"snps_per_s": r["snps_per_s"], "ms_per_step": r["elapsed"] / args.steps * 1e3,
                          "reduce_kernel_ms": r["reduce_ms"], "finalize_kernel_ms": r["finalize_ms"],
+                         "tile_kernel_ms": r["tile_ms"],
                          "mode_setup_s": r["setup_s"], "roofline": roofline(mode, r)}
     if "mixed" in paths and "fp64" in paths:
         e = orc_parity(paths["mixed"], paths["fp64"], p)

@@ -19,6 +19,7 @@ from .fgls import (  # noqa: F401
     lib,
     LIB_PATH,
     pack_u2,
+    tile_i8,
     u2_to_i8,
     FGLS_OK,
     FGLS_ERR_ARG,

@@ -149,26 +149,37 @@ int ensure_part(fgls_hip_eig* e, int tiles) {
   return FGLS_OK;
 }
 
-// The int8 product of a block (planes = 1: genotypes; 2: dosage digits).
-int reduce_i8(fgls_hip_eig* e, const int8_t* dX8, int64_t ldx8, int count, int planes, double scale,
-              hipStream_t st) {
+// The int8 product of a block of tiled genotypes (planes = 1) or dosage
+// digits (planes = 2).
+int reduce_i8t(fgls_hip_eig* e, const int8_t* dX8T, int count, int planes, double scale,
+               hipStream_t st) {
   if (e->mode != FGLS_MODE_MIXED_I8) return fail(FGLS_ERR_STATE, "eigen context is not in the mixed mode");
-  if (!dX8) return fail(FGLS_ERR_ARG, "NULL pointer");
-  if (ldx8 < round_up(e->n, 64) || (ldx8 & 15) || ((uintptr_t)dX8 & 15))
-    return fail(FGLS_ERR_ARG, "int8 X_R needs ld >= ceil(n/64)*64, ld %% 16 == 0 and 16-byte alignment");
+  if (planes != 1 && planes != 2) return fail(FGLS_ERR_ARG, "planes must be 1 or 2");
+  if (!dX8T || ((uintptr_t)dX8T & 15)) return fail(FGLS_ERR_ARG, "tiled genotypes need 16-byte alignment");
   if (planes == 2 && fgls_x8t_plane(e->n, count) > ((int64_t)1 << 31) - 1)
     return fail(FGLS_ERR_ARG, "int8 X_R block of %d columns too large", count);
   if (planes == 2 && !(scale > 0.0)) return fail(FGLS_ERR_ARG, "dosage scale must be positive");
-  int rc = fgls::ensure_buffer(&e->X8t, &e->X8t_cap, fgls_x8t_bytes(e->n, count, planes));
-  if (rc) return rc;
-  fgls_launch_tile_i8(e->n, count, planes, dX8, ldx8, e->X8t, st);
-  fgls_launch_trmm_i8(e->p, e->n, count, e->RT8, e->ldn / 64, e->A8, e->invsc, e->X8t,
+  fgls_launch_trmm_i8(e->p, e->n, count, e->RT8, e->ldn / 64, e->A8, e->invsc, dX8T,
                       round_up(e->n, 64), e->Wg, (int64_t)e->ldn, e->part, (int64_t)e->max_block,
                       planes == 2 ? fgls_x8t_plane(e->n, count) : 0, planes == 2 ? 1.0 / scale : 1.0,
                       e->ntr, st);
   HIP_TRY(hipGetLastError());
   e->last_rt = e->RT8;
   return FGLS_OK;
+}
+
+// Column-major genotypes: tiled into the context's buffer, then the product.
+int reduce_i8(fgls_hip_eig* e, const int8_t* dX8, int64_t ldx8, int count, int planes, double scale,
+              hipStream_t st) {
+  if (e->mode != FGLS_MODE_MIXED_I8) return fail(FGLS_ERR_STATE, "eigen context is not in the mixed mode");
+  if (!dX8) return fail(FGLS_ERR_ARG, "NULL pointer");
+  if (ldx8 < round_up(e->n, 64) || (ldx8 & 15) || ((uintptr_t)dX8 & 15))
+    return fail(FGLS_ERR_ARG, "int8 X_R needs ld >= ceil(n/64)*64, ld %% 16 == 0 and 16-byte alignment");
+  int rc = fgls::ensure_buffer(&e->X8t, &e->X8t_cap, fgls_x8t_bytes(e->n, count, planes));
+  if (rc) return rc;
+  fgls_launch_tile_i8(e->n, count, planes, dX8, ldx8, e->X8t, st);
+  HIP_TRY(hipGetLastError());
+  return reduce_i8t(e, e->X8t, count, planes, scale, st);
 }
 
 int solve_common(fgls_hip_eig* e, int count, double* dB, int64_t ldb, double* dV, int64_t ldv,
@@ -483,6 +494,17 @@ int fgls_hip_eig_solve_block_dosage(fgls_hip_eig* e, const int8_t* dX8, int64_t 
   if ((rc = check_outputs(e, count, dB, ldb, dV, ldv))) return rc;
   hipStream_t st = (hipStream_t)stream;
   if ((rc = reduce_i8(e, dX8, ldx8, count, 2, scale, st))) return rc;
+  return solve_common(e, count, dB, ldb, dV, ldv, snp_base, st);
+}
+
+int fgls_hip_eig_solve_block_i8t(fgls_hip_eig* e, const int8_t* dX8T, int count, int planes,
+                                 double scale, double* dB, int64_t ldb, double* dV, int64_t ldv,
+                                 int64_t snp_base, void* stream) {
+  int rc = pre_solve(e, count);
+  if (rc || count == 0) return rc;
+  if ((rc = check_outputs(e, count, dB, ldb, dV, ldv))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if ((rc = reduce_i8t(e, dX8T, count, planes, scale, st))) return rc;
   return solve_common(e, count, dB, ldb, dV, ldv, snp_base, st);
 }
 

@@ -376,6 +376,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   unsigned int *hBad = NULL;        /* pinned: genotypes of the slot that are not int8-exact */
   int8_t *dX8[2] = {NULL, NULL};
   uint8_t *dP2[2] = {NULL, NULL};   /* xr_format=u2: packed columns of the slot */
+  int8_t *dX8T[2] = {NULL, NULL};   /* mixed: the slot's genotypes tiled once for every trait */
   const size_t col_bytes = opt.xr_i8 == 2 ? (size_t)(n + 3) / 4 : opt.xr_i8 == 1 ? (size_t)n
                                                                                : (size_t)n * sizeof(double);
   unsigned int *dBad = NULL;
@@ -510,6 +511,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
       HCHK(hipMalloc((void **)&dX8[s], b8));
       HCHK(hipMemset(dX8[s], 0, b8));
       if (opt.xr_i8 == 2) HCHK(hipMalloc((void **)&dP2[s], (size_t)block * wXR * col_bytes));
+      if (opt.mixed) HCHK(hipMalloc((void **)&dX8T[s], fgls_hip_x8t_bytes(n, (int)block, dos ? 2 : 1)));
     }
   }
   if (nchunks > 0) {
@@ -624,13 +626,13 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
         } else if (!opt.mixed && opt.xr_i8) {
           RCHK(fgls_hip_i8_to_f64(dX8[s], ld8, n, cnt * wXR, dX[s], ldx, s_comp));
         }
+        const int planes = dos ? 2 : 1;
+        if (opt.mixed) RCHK(fgls_hip_tile_i8(dX8[s], ld8, n, cnt, planes, dX8T[s], s_comp));
         if (eig) {
           const int64_t ldb = block * p, ldv = block * pp;
-          if (dos)
-            RCHK(fgls_hip_eig_solve_block_dosage(ectx, dX8[s], ld8, cnt, opt.dosage_scale, dBs[s], ldb,
-                                                 dVs[s], ldv, i0, s_comp));
-          else if (opt.mixed)
-            RCHK(fgls_hip_eig_solve_block_i8(ectx, dX8[s], ld8, cnt, dBs[s], ldb, dVs[s], ldv, i0, s_comp));
+          if (opt.mixed)
+            RCHK(fgls_hip_eig_solve_block_i8t(ectx, dX8T[s], cnt, planes, dos ? opt.dosage_scale : 1.0, dBs[s],
+                                              ldb, dVs[s], ldv, i0, s_comp));
           else
             RCHK(fgls_hip_eig_solve_block(ectx, dX[s], ldx, cnt, dBs[s], ldb, dVs[s], ldv, i0, s_comp));
           HCHK(hipMemcpyAsync(&hErr[s * K], fgls_eig_err_ptr(ectx), (size_t)kn * sizeof(unsigned long long),
@@ -644,11 +646,9 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
             HCHK(hipMemcpyAsync(hV[q], dV[q], (size_t)cnt * pp * sizeof(double), hipMemcpyDeviceToHost, s_comp));
             continue;
           }
-          if (dos)
-            RCHK(fgls_hip_solve_block_dosage(ctx[k], dX8[s], ld8, cnt, opt.dosage_scale, dB[q], dV[q],
-                                             base, s_comp));
-          else if (opt.mixed)
-            RCHK(fgls_hip_solve_block_i8(ctx[k], dX8[s], ld8, cnt, dB[q], dV[q], base, s_comp));
+          if (opt.mixed)
+            RCHK(fgls_hip_solve_block_i8t(ctx[k], dX8T[s], cnt, planes, dos ? opt.dosage_scale : 1.0, dB[q],
+                                          dV[q], base, s_comp));
           else
             RCHK(fgls_hip_solve_block(ctx[k], dX[s], ldx, cnt, dB[q], dV[q], base, s_comp));
           HCHK(hipMemcpyAsync(hB[q], dB[q], (size_t)cnt * p * sizeof(double), hipMemcpyDeviceToHost, s_comp));
@@ -736,6 +736,7 @@ done:
     if (R.slot[s]) hipHostFree(R.slot[s]);
     if (dX8[s]) hipFree(dX8[s]);
     if (dP2[s]) hipFree(dP2[s]);
+    if (dX8T[s]) hipFree(dX8T[s]);
   }
   for (int s = 0; s < 2; ++s) {
     if (dBs[s]) hipFree(dBs[s]);

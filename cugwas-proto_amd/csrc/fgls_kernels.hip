@@ -1381,34 +1381,56 @@ namespace {
 
 // The int8 product of a block: genotypes (planes = 1) or dosage digits
 // (planes = 2: lo at dX8, hi at dX8 + count * ldx8, x = (128 hi + lo) / scale).
-int reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count, int planes, double scale,
-              void* stream) {
+// The int8 product of a block of tiled genotypes (planes = 1) or dosage
+// digits (planes = 2, x = (128 hi + lo) / scale).
+int reduce_i8t(fgls_hip_ctx* c, const int8_t* dX8T, int count, int planes, double scale,
+               void* stream) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!c->have_trait) return fail(FGLS_ERR_STATE, "no trait set");
   if (c->mode != FGLS_MODE_MIXED_I8) return fail(FGLS_ERR_STATE, "context is not in the mixed mode");
   if (count < 0 || count > c->max_block)
     return fail(FGLS_ERR_ARG, "count=%d outside [0, %d]", count, c->max_block);
+  if (planes != 1 && planes != 2) return fail(FGLS_ERR_ARG, "planes must be 1 or 2");
   if (count == 0) return FGLS_OK;
-  if (!dX8) return fail(FGLS_ERR_ARG, "NULL pointer");
-  if (ldx8 < (int64_t)round_up(c->n, 64) || (ldx8 & 15) || ((uintptr_t)dX8 & 15))
-    return fail(FGLS_ERR_ARG, "int8 X_R needs ld >= ceil(n/64)*64, ld %% 16 == 0 and 16-byte alignment");
+  if (!dX8T || ((uintptr_t)dX8T & 15)) return fail(FGLS_ERR_ARG, "tiled genotypes need 16-byte alignment");
   // 32-bit offsets to the hi plane of the tiled dosage digits
   if (planes == 2 && fgls_x8t_plane(c->n, count) > ((int64_t)1 << 31) - 1)
     return fail(FGLS_ERR_ARG, "int8 X_R block of %d columns too large", count);
   if (planes == 2 && !(scale > 0.0)) return fail(FGLS_ERR_ARG, "dosage scale must be positive");
   if ((rc = set_device(c))) return rc;
-  if ((rc = fgls::ensure_buffer(&c->X8t, &c->X8t_cap, fgls_x8t_bytes(c->n, count, planes)))) return rc;
-  hipStream_t st = (hipStream_t)stream;
-  fgls_launch_tile_i8(c->n, count, planes, dX8, ldx8, c->X8t, st);
-  fgls_launch_trmm_i8(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, c->X8t,
+  fgls_launch_trmm_i8(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, dX8T,
                       round_up(c->n, 64), c->W, (int64_t)c->ldn, c->part, (int64_t)c->max_block,
                       planes == 2 ? fgls_x8t_plane(c->n, count) : 0, planes == 2 ? 1.0 / scale : 1.0,
-                      0, st);
+                      0, (hipStream_t)stream);
   HIP_TRY(hipGetLastError());
   c->reduced = count;
   c->last_rt = c->RT8;
   return FGLS_OK;
+}
+
+int check_x8(int n, const int8_t* dX8, int64_t ldx8) {
+  if (!dX8) return fail(FGLS_ERR_ARG, "NULL pointer");
+  if (ldx8 < (int64_t)round_up(n, 64) || (ldx8 & 15) || ((uintptr_t)dX8 & 15))
+    return fail(FGLS_ERR_ARG, "int8 X_R needs ld >= ceil(n/64)*64, ld %% 16 == 0 and 16-byte alignment");
+  return FGLS_OK;
+}
+
+// Column-major genotypes: tiled into the context's buffer, then the product.
+int reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count, int planes, double scale,
+              void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (count < 0 || count > c->max_block)
+    return fail(FGLS_ERR_ARG, "count=%d outside [0, %d]", count, c->max_block);
+  if (count == 0 || !c->have_trait || c->mode != FGLS_MODE_MIXED_I8)
+    return reduce_i8t(c, nullptr, count, planes, scale, stream);   // the state errors
+  if ((rc = check_x8(c->n, dX8, ldx8))) return rc;
+  if ((rc = set_device(c))) return rc;
+  if ((rc = fgls::ensure_buffer(&c->X8t, &c->X8t_cap, fgls_x8t_bytes(c->n, count, planes)))) return rc;
+  fgls_launch_tile_i8(c->n, count, planes, dX8, ldx8, c->X8t, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return reduce_i8t(c, c->X8t, count, planes, scale, stream);
 }
 
 }  // namespace
@@ -1418,6 +1440,38 @@ extern "C" {
 int fgls_hip_block_reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count,
                              void* stream) {
   return reduce_i8(c, dX8, ldx8, count, 1, 1.0, stream);
+}
+
+size_t fgls_hip_x8t_bytes(int n, int count, int planes) {
+  if (n < 1 || count < 0 || planes < 1 || planes > 2) return 0;
+  return fgls_x8t_bytes(n, count, planes);
+}
+
+int fgls_hip_tile_i8(const int8_t* dX8, int64_t ldx8, int n, int count, int planes, int8_t* dX8T,
+                     void* stream) {
+  if (n < 1 || count < 0 || (planes != 1 && planes != 2) || !dX8T || ((uintptr_t)dX8T & 15))
+    return fail(FGLS_ERR_ARG, "bad arguments to fgls_hip_tile_i8");
+  int rc = check_x8(n, dX8, ldx8);
+  if (rc) return rc;
+  if (count == 0) return FGLS_OK;
+  fgls_launch_tile_i8(n, count, planes, dX8, ldx8, dX8T, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_block_reduce_i8t(fgls_hip_ctx* c, const int8_t* dX8T, int count, int planes,
+                              double scale, void* stream) {
+  return reduce_i8t(c, dX8T, count, planes, scale, stream);
+}
+
+int fgls_hip_solve_block_i8t(fgls_hip_ctx* c, const int8_t* dX8T, int count, int planes, double scale,
+                             double* dB, double* dV, int64_t snp_base, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (count > 0 && (!dB || !dV)) return fail(FGLS_ERR_ARG, "NULL pointer");
+  rc = reduce_i8t(c, dX8T, count, planes, scale, stream);
+  if (rc) return rc;
+  return fgls_hip_block_finalize(c, count, dB, dV, snp_base, stream);
 }
 
 int fgls_hip_solve_block_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count, double* dB,

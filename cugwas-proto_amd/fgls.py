@@ -80,6 +80,12 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_solve_block_dosage": (i32, [vp, dp, i64, i32, ctypes.c_double, dp, dp, i64, vp]),
             "fgls_hip_i8_to_f64": (i32, [dp, i64, i32, i32, dp, i64, vp]),
             "fgls_hip_u2_to_i8": (i32, [dp, i64, i32, i32, dp, i64, dp, vp]),
+            "fgls_hip_x8t_bytes": (ctypes.c_size_t, [i32, i32, i32]),
+            "fgls_hip_tile_i8": (i32, [dp, i64, i32, i32, i32, dp, vp]),
+            "fgls_hip_block_reduce_i8t": (i32, [vp, dp, i32, i32, ctypes.c_double, vp]),
+            "fgls_hip_solve_block_i8t": (i32, [vp, dp, i32, i32, ctypes.c_double, dp, dp, i64, vp]),
+            "fgls_hip_eig_solve_block_i8t": (i32, [vp, dp, i32, i32, ctypes.c_double, dp, i64, dp, i64,
+                                                   i64, vp]),
             "fgls_hip_set_trait_phi": (i32, [vp, dp, i64, ctypes.c_double, ctypes.c_double, dp, i64, dp, vp]),
             "fgls_hip_get_factor": (i32, [vp, dp, i64, vp]),
             "fgls_hip_adjoint_begin": (i32, [vp, dp, i64, ctypes.c_double, vp]),
@@ -127,7 +133,9 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_adjoint_begin", "fgls_hip_adjoint_block", "fgls_hip_adjoint_end",
                         "fgls_hip_set_trait_phi", "fgls_hip_get_factor", "fgls_hip_to_dosage_i8",
                         "fgls_hip_block_reduce_dosage", "fgls_hip_solve_block_dosage",
-                        "fgls_hip_i8_to_f64", "fgls_hip_u2_to_i8",
+                        "fgls_hip_i8_to_f64", "fgls_hip_u2_to_i8", "fgls_hip_x8t_bytes",
+                        "fgls_hip_tile_i8", "fgls_hip_block_reduce_i8t", "fgls_hip_solve_block_i8t",
+                        "fgls_hip_eig_solve_block_i8t",
                         "fgls_factor_M", "fgls_eigen", "fgls_eigen_gpu_ex", "fgls_eigen_decompose",
                         "fgls_hip_eigen_decompose",
                         "fgls_hip_eig_create", "fgls_hip_eig_destroy", "fgls_hip_eig_ldn",
@@ -342,6 +350,23 @@ class DeviceContext:
                                                  _ptr(B), _ptr(V), snp_base, _stream(stream)))
         return B, V
 
+    def block_reduce_i8t(self, X8T, count: int, planes: int = 1, scale: float = 1.0, stream=None):
+        """The int8 product of tiled genotypes (tile_i8)."""
+        _check(lib().fgls_hip_block_reduce_i8t(self._h, _ptr(X8T), count, planes, float(scale),
+                                               _stream(stream)))
+        return count
+
+    def solve_block_i8t(self, X8T, count: int, planes: int = 1, scale: float = 1.0, B=None, V=None,
+                        snp_base: int = 0, stream=None):
+        import torch
+        if B is None:
+            B = torch.empty((count, self.p), dtype=torch.float64, device=X8T.device)
+        if V is None:
+            V = torch.empty((count, self.p * self.p), dtype=torch.float64, device=X8T.device)
+        _check(lib().fgls_hip_solve_block_i8t(self._h, _ptr(X8T), count, planes, float(scale), _ptr(B),
+                                              _ptr(V), snp_base, _stream(stream)))
+        return B, V
+
     def block_reduce_i8(self, X8, count: int | None = None, stream=None):
         count = X8.shape[0] if count is None else count
         _check(lib().fgls_hip_block_reduce_i8(self._h, _ptr(X8), X8.shape[1], count, _stream(stream)))
@@ -539,6 +564,15 @@ class EigenContext:
                                                      _stream(stream)))
         return B, V
 
+    def solve_block_i8t(self, X8T, count: int, planes: int = 1, scale: float = 1.0, B=None, V=None,
+                        snp_base: int = 0, stream=None):
+        """Tiled genotypes (tile_i8)."""
+        B, V = self._outputs(count, B, V, X8T.device)
+        _check(lib().fgls_hip_eig_solve_block_i8t(self._h, _ptr(X8T), count, planes, float(scale), _ptr(B),
+                                                  B.stride(0), _ptr(V), V.stride(0), snp_base,
+                                                  _stream(stream)))
+        return B, V
+
     def check(self, stream=None):
         tr = ctypes.c_int(-1)
         snp = ctypes.c_int64(-1)
@@ -582,6 +616,21 @@ def pack_u2(X):
     q = pad.reshape(nb, 4, m)
     packed = q[:, 0] | (q[:, 1] << 2) | (q[:, 2] << 4) | (q[:, 3] << 6)
     return np.ascontiguousarray(packed.T)
+
+
+def tile_i8(X8, n: int, count: int | None = None, planes: int = 1, X8T=None, stream=None):
+    """Column-major int8 genotypes X8 ((planes * count, ld8) tensor; dosage digit
+    planes as from DeviceContext.to_dosage_i8) -> the tiled layout of the int8
+    product, a flat int8 tensor of x8t_bytes(n, count, planes) bytes."""
+    import torch
+    count = X8.shape[0] // planes if count is None else count
+    nbytes = int(lib().fgls_hip_x8t_bytes(n, count, planes))
+    if X8T is None:
+        X8T = torch.empty(nbytes, dtype=torch.int8, device=X8.device)
+    if X8T.numel() < nbytes:
+        raise FglsError(FGLS_ERR_ARG, f"X8T needs {nbytes} bytes")
+    _check(lib().fgls_hip_tile_i8(_ptr(X8), X8.shape[1], n, count, planes, _ptr(X8T), _stream(stream)))
+    return X8T
 
 
 def u2_to_i8(P, n: int, X8=None, stream=None):

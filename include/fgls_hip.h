@@ -17,6 +17,7 @@
 #ifndef FGLS_HIP_H
 #define FGLS_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -151,6 +152,27 @@ int fgls_hip_block_reduce_dosage(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t l
 int fgls_hip_solve_block_dosage(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ldx8,
                                 int count, double scale, double *dB, double *dV,
                                 int64_t snp_base, void *stream);
+
+/*
+ * Tiled genotypes: the layout the int8 product reads (1 KB blocks, one per 16
+ * SNP columns and 64 individuals, in the MFMA operand's lane order).  The
+ * column-major entry points above tile each block into the context's own
+ * buffer first; callers that keep genotypes resident, or solve several
+ * traits per block, tile once with fgls_hip_tile_i8 (dX8T of
+ * fgls_hip_x8t_bytes(n, count, planes) bytes, 16-byte aligned; planes = 2 for
+ * dosage digit planes laid out as for fgls_hip_solve_block_dosage) and pass
+ * dX8T to the *_i8t entry points (planes = 1 genotypes, 2 dosages with
+ * x = (128 hi + lo) / scale; scale is ignored for planes = 1).  No
+ * counterpart in the reference.
+ */
+size_t fgls_hip_x8t_bytes(int n, int count, int planes);
+int fgls_hip_tile_i8(const int8_t *dX8, int64_t ldx8, int n, int count, int planes,
+                     int8_t *dX8T, void *stream);
+int fgls_hip_block_reduce_i8t(fgls_hip_ctx *ctx, const int8_t *dX8T, int count, int planes,
+                              double scale, void *stream);
+int fgls_hip_solve_block_i8t(fgls_hip_ctx *ctx, const int8_t *dX8T, int count, int planes,
+                             double scale, double *dB, double *dV, int64_t snp_base,
+                             void *stream);
 
 /* 2-bit genotypes -> int8 (column-major, ld ldx8 >= n, ldx8 % 4 == 0, rows
  * n..ldx8-1 zeroed): column c is ceil(n/4) bytes at dP + c*ldp, individual r
@@ -312,6 +334,10 @@ int fgls_hip_eig_solve_block_dosage(fgls_hip_eig *e, const int8_t *dX8,
                                     int64_t ldx8, int count, double scale,
                                     double *dB, int64_t ldb, double *dV,
                                     int64_t ldv, int64_t snp_base, void *stream);
+/* The same from tiled genotypes (fgls_hip_tile_i8). */
+int fgls_hip_eig_solve_block_i8t(fgls_hip_eig *e, const int8_t *dX8T, int count,
+                                 int planes, double scale, double *dB, int64_t ldb,
+                                 double *dV, int64_t ldv, int64_t snp_base, void *stream);
 int fgls_hip_eig_check(fgls_hip_eig *e, void *stream, int *trait,
                        int64_t *bad_snp, int *info);
 

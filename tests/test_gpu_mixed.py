@@ -120,3 +120,37 @@ def test_mixed_dosages_match_oracle(pkg, gpu, n, m, wXL, block):
     with pytest.raises(pkg.FglsError):
         ctx.to_dosage_i8(bad, 1000.0)
     ctx.close()
+
+
+def test_tiled_genotype_entry_points(pkg, gpu):
+    """tile_i8 + the *_i8t entry points give bitwise the results of the
+    column-major entry points (Cholesky and eigen contexts, genotypes and
+    dosage digit planes, a count that is not a multiple of 16)."""
+    import torch
+    prob = make_problem(n=650, wXL=3, m=203, seed=9, t=1)
+    _, ctx = setup(pkg, prob, max_block=256)
+    ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)
+    X8 = ctx.to_i8(pkg.colmajor(prob.XR))
+    B1, V1 = ctx.solve_block_i8(X8)
+    X8T = pkg.tile_i8(X8, prob.n)
+    assert X8T.numel() == pkg.lib().fgls_hip_x8t_bytes(prob.n, prob.m, 1)
+    B2, V2 = ctx.solve_block_i8t(X8T, prob.m)
+    assert torch.equal(B1, B2) and torch.equal(V1, V2)
+    D = np.round(np.random.default_rng(1).uniform(0, 2, size=(prob.n, prob.m)), 3)
+    D8 = ctx.to_dosage_i8(pkg.colmajor(D), 1000.0)
+    B3, V3 = ctx.solve_block_dosage(D8, 1000.0)
+    B4, V4 = ctx.solve_block_i8t(pkg.tile_i8(D8, prob.n, prob.m, planes=2), prob.m, planes=2, scale=1000.0)
+    assert torch.equal(B3, B4) and torch.equal(V3, V4)
+    ctx.check()
+    Z, W = pkg.eigen_decompose(prob.Phi, threads=4)
+    e = pkg.EigenContext(prob.n, 3, max_traits=2, max_block=256)
+    e.set_mode(pkg.FGLS_MODE_MIXED_I8)
+    e.set_basis(torch.from_numpy(np.ascontiguousarray(Z.T)).cuda(), torch.from_numpy(W).cuda(),
+                pkg.colmajor(prob.XL))
+    e.set_traits([0.4, 0.7], [1.0, 1.0], pkg.colmajor(np.column_stack([prob.y, prob.y[::-1]])))
+    E1 = e.solve_block_i8(X8)
+    E2 = e.solve_block_i8t(X8T, prob.m)
+    assert torch.equal(E1[0], E2[0]) and torch.equal(E1[1], E2[1])
+    e.check()
+    e.close()
+    ctx.close()
