@@ -213,3 +213,39 @@ def test_driver_variant_e(pkg, gpu, tmp_path):
     V = np.fromfile(os.path.join(d, "V.out"), dtype="<f8")
     err = orc.parity_errors(B, V, Bo, Vo, p)
     assert err["B_scaled"] < TOL_FP64 and err["V_scaled"] < TOL_FP64 and err["V_upper_zero"], err
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_eigen_singular_snp_reports_trait_and_index(pkg, gpu, tmp_path, mixed):
+    """An all-zero SNP column makes S_i singular for every trait (zero last
+    pivot, fgls_eigen.c:540-547): each trait records its first failing SNP;
+    check() reports trait 0 and clears; the file entry point fails with the
+    reference's message."""
+    prob = make_problem(n=300, wXL=3, m=100, seed=19, t=2)
+    prob.XR[:, 37] = 0.0
+    prob.XR[:, 80] = 0.0
+    H, Sig = np.array([0.3, 0.6]), np.array([1.0, 1.0])
+    e = eig_setup(pkg, prob, H, Sig, max_block=128, mixed=mixed)
+    if mixed:
+        dc = pkg.DeviceContext(prob.n, 3, 1, max_block=16)
+        B, V = e.solve_block_i8(dc.to_i8(pkg.colmajor(prob.XR)), snp_base=1000)
+        dc.close()
+    else:
+        B, V = e.solve_block(pkg.colmajor(prob.XR), snp_base=1000)
+    with pytest.raises(pkg.FglsError) as ei:
+        e.check()
+    assert ei.value.code == pkg.FGLS_ERR_NOTSPD
+    assert (ei.value.trait, ei.value.bad_snp, ei.value.info) == (0, 1037, 4)
+    e.check()   # cleared
+    assert np.isnan(B[1, 37].cpu().numpy()).all() and np.isfinite(B[1, 36].cpu().numpy()).all()
+    # an empty block is a no-op
+    e.solve_block(pkg.colmajor(prob.XR[:, :1]), count=0)
+    e.close()
+    d = str(tmp_path)
+    write_problem_dir(prob, d, H=H, Sig=Sig)
+    f = lambda s: os.path.join(d, s)
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.fgls_eigen(prob.n, 4, prob.m, 2, 3, 1, 50, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                       f("XL.in"), f("XR.in"), f("Y.in"), f("B.out"), f("V.out"),
+                       options={"precision": "mixed" if mixed else "fp64"})
+    assert ei.value.code == pkg.FGLS_ERR_NOTSPD and "i: 37" in ei.value.msg
