@@ -325,7 +325,10 @@ def parity_errors(B, V, Bref, Vref, p: int) -> dict:
       covariances as |V - V_ref| / max(|V_ref|, se_r * se_c).
     * ``var_rel``: element-wise relative error of the variances se^2, i.e.
       of var(b_ik), never near zero.
-    The scaled forms are the parity bars; both are reported."""
+    * ``B_norm``: per SNP the normwise relative error of its coefficient
+      vector, max_i |b_i - b_ref_i|_2 / |b_ref_i|_2; ``V_norm`` the same for
+      the lower triangle of V_i (Frobenius norm).
+    The scaled and normwise forms are the parity bars; all are reported."""
     B = np.asarray(B, dtype=np.float64).reshape(-1, p)
     Bref = np.asarray(Bref, dtype=np.float64).reshape(-1, p)
     V = np.asarray(V, dtype=np.float64).reshape(-1, p, p)        # [i][c][r]
@@ -337,8 +340,13 @@ def parity_errors(B, V, Bref, Vref, p: int) -> dict:
     r, c = np.tril_indices(p)
     se_new = np.einsum("icc->ic", V)
     var_rel = np.abs(se_new ** 2 - se ** 2) / (se ** 2)
+    bn = np.linalg.norm(B - Bref, axis=1) / np.linalg.norm(Bref, axis=1)
+    VL, VLr = V[:, c, r], Vref[:, c, r]
+    vn = np.linalg.norm(VL - VLr, axis=1) / np.linalg.norm(VLr, axis=1)
     return {"B_rel": max_rel_err(B, Bref), "V_rel": max_rel_err(V, Vref, p=p, lower_only=True),
             "var_rel": float(np.max(var_rel)) if var_rel.size else 0.0,
             "B_scaled": float(np.max(eb)) if eb.size else 0.0,
             "V_scaled": float(np.max(ev[:, c, r])) if ev.size else 0.0,
+            "B_norm": float(np.max(bn)) if bn.size else 0.0,
+            "V_norm": float(np.max(vn)) if vn.size else 0.0,
             "V_upper_zero": bool(np.all(V[:, r[r != c], c[r != c]] == 0.0)) if p > 1 else True}

@@ -37,6 +37,7 @@ def test_mixed_matches_oracle(pkg, gpu, n, m, wXL, h2):
     assert e["V_upper_zero"]
     assert e["B_scaled"] < TOL_SCALED and e["V_scaled"] < TOL_SCALED, e
     assert e["var_rel"] < TOL_VAR, e
+    assert e["B_norm"] < TOL_VAR and e["V_norm"] < TOL_VAR, e
     # the FP64 path of the same context agrees too (mode switch keeps the trait)
     ctx.set_mode(pkg.FGLS_MODE_FP64)
     B64, V64 = ctx.solve_block(pkg.colmajor(prob.XR, ld=(n + 3) // 4 * 4))

@@ -230,6 +230,11 @@ def test_parity_metric_catches_planted_errors():
     assert not orc.parity_errors(b, bad, b, Vf, 4)["V_upper_zero"]
     bb = b.copy(); bb[7, 3] += 1e-6 * Vf[7, 15]                # SNP coefficient off by 1e-6 se
     assert orc.parity_errors(bb, Vf, b, Vf, 4)["B_scaled"] > 1e-7
+    bb = b.copy(); bb[7, 0] *= 1 + 1e-6                         # the intercept of one SNP
+    e = orc.parity_errors(bb, Vf, b, Vf, 4)
+    assert 1e-7 < e["B_norm"] < 1e-6 and e["V_norm"] == 0.0
+    bad = Vf.copy(); bad[2, 0] *= 1 + 1e-6                      # V_00 of one SNP
+    assert orc.parity_errors(b, bad, b, Vf, 4)["V_norm"] > 1e-8
 
 
 # --- adjoint of l = sum_i w_i b_i[p-1]^2 (no reference counterpart) ---------
