@@ -165,7 +165,6 @@ def run_adjoint(torch, pkg, dev, m, steps):
     n, wXL, block = 5000, 3, 2048
     ldx = (n + 3) // 4 * 4
     Phi, L, XL, y, X, X8 = make_inputs(torch, n, m, wXL, 0.6, 3, ldx, dev)
-    del X8
     g = torch.Generator(device=dev)
     g.manual_seed(4)
     w = torch.rand(m, dtype=torch.float64, device=dev, generator=g)
@@ -200,12 +199,34 @@ def run_adjoint(torch, pkg, dev, m, steps):
     torch.cuda.synchronize()
     t_fwd = (time.perf_counter() - t0) / steps
     ctx.check()
+    # mixed mode: the same adjoint with both per-block products on the int8 MFMA
+    ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)
+
+    def adjoint_pass_i8():
+        ctx.adjoint_begin(Phi, 1.0)
+        for i0 in range(0, m, block):
+            cnt = min(block, m - i0)
+            ctx.adjoint_block_i8(X8[i0:i0 + cnt], w[i0:i0 + cnt], cnt, snp_base=i0)
+        return ctx.adjoint_end()
+
+    adjoint_pass_i8()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss8, gy8, gh8 = adjoint_pass_i8()
+    torch.cuda.synchronize()
+    t_adj8 = (time.perf_counter() - t0) / steps
+    ctx.check()
     ctx.close()
+    rel_gy = float(torch.linalg.vector_norm(gy8 - gy) / torch.linalg.vector_norm(gy))
     return {"workload": f"adjoint of sum_i w_i b_i[p-1]^2: n={n}, p={wXL + 1}, m={m}, h2=0.6, "
                         f"blocks of {block}, L^-1 X_R recomputed per block",
             "snps_per_s": m / t_adj, "seconds": t_adj, "forward_snps_per_s": m / t_fwd,
             "forward_seconds": t_fwd, "loss": loss, "grad_h2": gh,
-            "grad_y_norm": float(torch.linalg.vector_norm(gy))}
+            "grad_y_norm": float(torch.linalg.vector_norm(gy)),
+            "mixed": {"snps_per_s": m / t_adj8, "seconds": t_adj8,
+                      "loss_rel_vs_fp64": abs(loss8 - loss) / abs(loss),
+                      "grad_h2_rel_vs_fp64": abs(gh8 - gh) / abs(gh), "grad_y_rel_vs_fp64": rel_gy}}
 
 
 def run_eigen(torch, pkg, args, Phi, XL, y, X, X8, chol_fp64, dev):
@@ -584,7 +605,8 @@ def main():
         torch.cuda.empty_cache()
         adjoint = run_adjoint(torch, pkg, dev, args.adjoint_m, args.steps)
         log(f"adjoint (n=5000, m={args.adjoint_m}, block 2048): {adjoint['snps_per_s']:.0f} SNPs/s, "
-            f"forward {adjoint['forward_snps_per_s']:.0f} SNPs/s")
+            f"forward {adjoint['forward_snps_per_s']:.0f} SNPs/s; mixed {adjoint['mixed']['snps_per_s']:.0f} "
+            f"SNPs/s (dl/dy vs FP64 {adjoint['mixed']['grad_y_rel_vs_fp64']:.1e})")
     out = {
         "metric": METRIC,
         "value": res["snps_per_s"],

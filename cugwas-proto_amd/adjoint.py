@@ -22,7 +22,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from .fgls import DeviceContext, FglsError, FGLS_ERR_ARG, factor_M
+from .fgls import DeviceContext, FglsError, FGLS_ERR_ARG, FGLS_MODE_MIXED_I8, factor_M
 
 
 def _as_f64_cuda(t, name):
@@ -118,23 +118,34 @@ def snp_loss(y, h2, XR, w, Phi, XL, sigma2: float = 1.0, block: int = 2048):
     return SnpLoss.apply(y, h2, XR, w, Phi, XL, sigma2, block)
 
 
-def snp_loss_grad(y, h2: float, XR, w, Phi, XL, sigma2: float = 1.0, block: int = 2048):
+def snp_loss_grad(y, h2: float, XR, w, Phi, XL, sigma2: float = 1.0, block: int = 2048,
+                  precision: str = "fp64"):
     """Loss and both gradients in one pass without autograd bookkeeping:
-    returns (l, dl/dy (n,) CUDA tensor, dl/dh2)."""
+    returns (l, dl/dy (n,) CUDA tensor, dl/dh2).  precision="mixed" runs the
+    per-block products on int8 genotypes (XR must hold integers in
+    [-127, 127])."""
+    if precision not in ("fp64", "mixed"):
+        raise FglsError(FGLS_ERR_ARG, "precision must be fp64 or mixed")
     n = int(y.shape[0])
     Phi_h = Phi.detach().cpu().numpy() if isinstance(Phi, torch.Tensor) else np.asarray(Phi)
     L = factor_M(Phi_h, float(h2), float(sigma2))
     m = int(XR.shape[0])
     dc = DeviceContext(n, int(XL.shape[0]), 1, max_block=min(block, m))
     try:
+        if precision == "mixed":
+            dc.set_mode(FGLS_MODE_MIXED_I8)
         dc.set_trait(torch.from_numpy(np.ascontiguousarray(L.T)).cuda(), _as_f64_cuda(XL, "XL"),
                      _as_f64_cuda(y, "y"))
         dc.adjoint_begin(torch.from_numpy(np.ascontiguousarray(Phi_h.T)).cuda(), float(sigma2))
         XRd = _genotypes(XR, n)
         wd = _as_f64_cuda(w, "w")
+        X8 = dc.to_i8(XRd) if precision == "mixed" else None
         for i0 in range(0, m, dc.max_block):
             cnt = min(dc.max_block, m - i0)
-            dc.adjoint_block(XRd[i0:i0 + cnt], wd[i0:i0 + cnt], cnt, snp_base=i0)
+            if X8 is not None:
+                dc.adjoint_block_i8(X8[i0:i0 + cnt], wd[i0:i0 + cnt], cnt, snp_base=i0)
+            else:
+                dc.adjoint_block(XRd[i0:i0 + cnt], wd[i0:i0 + cnt], cnt, snp_base=i0)
         loss, gy, gh = dc.adjoint_end()
         dc.check()
         return loss, gy, gh

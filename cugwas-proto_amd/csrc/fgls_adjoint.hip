@@ -19,6 +19,11 @@
 // the forward pass), b_i and S_i^-1 e_p (the forward per-SNP stage on Z),
 // one dense product G Z whose epilogue reduces t_i and s_i, and the sum
 // Z gamma.  G, G W and W' G W are built once per trait.
+//
+// Mixed mode (int8 genotypes): G Z = G L^-1 X_R = H X_R with H = G L^-1
+// built once per trait and sliced like L^-1, so both per-block products are
+// exact int8 products against the genotypes: Z = L^-1 X_R written out, and
+// Y = H X_R with t_i = y_i' z_i and s_i = W' y_i fused into its epilogue.
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -27,6 +32,7 @@
 #include <string.h>
 
 #include "fgls_ctx.h"
+#include "fgls_device.h"
 #include "fgls_hip.h"
 
 struct fgls_adjoint_state {
@@ -44,6 +50,10 @@ struct fgls_adjoint_state {
   double* gz;      // ldn: sum_i gamma_i z_i
   double* acc;     // p + 1: loss, sum p~'Gq~, sum g_L
   double* gyt;     // ldn: X~_L sum g_L + gz
+  double* H;       // ldn x ldn row-major G L^-1 (mixed mode)
+  int8_t* H8;      // its int8 digit slices
+  double* hinvsc;  // their row scales
+  bool h8_ready;   // H8 built for the current trait
 };
 
 namespace {
@@ -238,9 +248,9 @@ __global__ __launch_bounds__(256) void k_adj_grad_y(int n, const double* __restr
 }
 
 template <int P>
-void launch_adj_snp(const fgls_hip_ctx* c, int count, hipStream_t st, const double* dw) {
+void launch_adj_snp(const fgls_hip_ctx* c, int count, hipStream_t st, const double* dw, int RT) {
   const fgls_adjoint_state* a = c->adj;
-  hipLaunchKernelGGL(k_adj_snp<P>, dim3((count + 127) / 128), dim3(128), 0, st, count, c->RT,
+  hipLaunchKernelGGL(k_adj_snp<P>, dim3((count + 127) / 128), dim3(128), 0, st, count, RT,
                      c->part, (int64_t)c->max_block, a->Bs, a->Vs, c->trait, dw, a->WGW, a->gam,
                      a->rec);
 }
@@ -284,14 +294,15 @@ int ensure_state(fgls_hip_ctx* c) {
 void fgls_adjoint_free(fgls_adjoint_state* a) {
   if (!a) return;
   double* ptrs[] = {a->G, a->T, a->GW, a->WGW, a->Bs, a->Vs, a->gam, a->rec, a->gzp, a->gz,
-                    a->acc, a->gyt};
+                    a->acc, a->gyt, a->H, a->hinvsc};
   for (double* p : ptrs)
     if (p) hipFree(p);
+  if (a->H8) hipFree(a->H8);
   delete a;
 }
 
 void fgls_adjoint_invalidate(fgls_adjoint_state* a) {
-  if (a) a->ready = false;
+  if (a) a->ready = a->h8_ready = false;
 }
 
 extern "C" {
@@ -301,14 +312,13 @@ int fgls_hip_adjoint_begin(fgls_hip_ctx* c, const double* dPhi, int64_t ldphi, d
   if (!c) return fail(FGLS_ERR_ARG, "NULL context");
   if (!c->have_trait) return fail(FGLS_ERR_STATE, "no trait set");
   if (c->wXR != 1) return fail(FGLS_ERR_ARG, "the adjoint needs wXR = 1 (got %d)", c->wXR);
-  if (c->mode != FGLS_MODE_FP64) return fail(FGLS_ERR_STATE, "the adjoint runs in the FP64 mode");
   if (!dPhi || ldphi < c->n) return fail(FGLS_ERR_ARG, "Phi needs a device pointer and ld >= n");
   if (!(sigma2 > 0.0)) return fail(FGLS_ERR_ARG, "sigma2 must be positive");
   int rc = fgls::set_device(c);
   if (rc) return rc;
   if ((rc = ensure_state(c))) return rc;
   fgls_adjoint_state* a = c->adj;
-  a->ready = false;
+  a->ready = a->h8_ready = false;
   a->sigma2 = sigma2;
   hipStream_t st = (hipStream_t)stream;
   const int n = c->n, ldn = c->ldn, p = c->p;
@@ -325,6 +335,24 @@ int fgls_hip_adjoint_begin(fgls_hip_ctx* c, const double* dPhi, int64_t ldphi, d
   fgls::launch_product_store(c, a->G, true, c->W, ldn, p, a->GW, ldn, st);
   hipLaunchKernelGGL(k_wgw, dim3(1), dim3(256), 0, st, n, p, c->W, a->GW, (int64_t)ldn, a->WGW);
   HIP_TRY(hipGetLastError());
+  if (c->mode == FGLS_MODE_MIXED_I8) {
+    // H = G L^-1 row-major = (L^-T G)' : T = (L^-1)' as a row-major L^-T,
+    // then the dense product T G (G symmetric) stored column-major
+    const size_t nn = (size_t)ldn * ldn;
+    if (!a->H) {
+      if (hipMalloc((void**)&a->H, nn * sizeof(double)) != hipSuccess ||
+          hipMalloc((void**)&a->H8, nn * FGLS_NSLICE) != hipSuccess ||
+          hipMalloc((void**)&a->hinvsc, (size_t)ldn * sizeof(double)) != hipSuccess)
+        return fail(FGLS_ERR_NOMEM, "hipMalloc of the mixed adjoint's %zu bytes failed",
+                    nn * (sizeof(double) + FGLS_NSLICE));
+      if ((rc = fill(a->H, nn, 0.0, st))) return rc;
+    }
+    hipLaunchKernelGGL(k_transpose, dim3(ldn / 32, ldn / 32), dim3(256), 0, st, ldn, c->Linv, a->T);
+    fgls::launch_product_store(c, a->T, true, a->G, ldn, n, a->H, ldn, st);
+    fgls::launch_slices(ldn, a->H, a->H8, a->hinvsc, st);
+    HIP_TRY(hipGetLastError());
+    a->h8_ready = true;
+  }
   if ((rc = fill(a->gz, (size_t)ldn, 0.0, st))) return rc;
   if ((rc = fill(a->acc, (size_t)p + 1, 0.0, st))) return rc;
   a->ready = true;
@@ -351,7 +379,44 @@ int fgls_hip_adjoint_block(fgls_hip_ctx* c, const double* dXR, int64_t ldxr, int
   fgls::launch_snp_stage_any(c, c->z, ldn, count, a->Bs, a->Vs, snp_base, st);
   // t_i = z' G z and s_i = W' G z from one dense product
   fgls::launch_product_dot(c, a->G, c->z, ldn, count, st);
-  FGLS_P_SWITCH(p, launch_adj_snp, c, count, st, dw);
+  FGLS_P_SWITCH(p, launch_adj_snp, c, count, st, dw, c->RT);
+  const int nchunk = (count + CHUNK - 1) / CHUNK;
+  hipLaunchKernelGGL(k_adj_gz_partial, dim3((n + 255) / 256, nchunk), dim3(256), 0, st, n, count,
+                     c->z, (int64_t)ldn, a->gam, a->gzp, (int64_t)ldn);
+  hipLaunchKernelGGL(k_adj_gz_reduce, dim3((n + 255) / 256), dim3(256), 0, st, n, nchunk, a->gzp,
+                     (int64_t)ldn, a->gz);
+  hipLaunchKernelGGL(k_adj_accum, dim3(p + 1), dim3(256), 0, st, count, p + 1, a->rec, a->acc);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_adjoint_block_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count,
+                              const double* dw, int64_t snp_base, void* stream) {
+  if (!c) return fail(FGLS_ERR_ARG, "NULL context");
+  if (!c->adj || !c->adj->ready) return fail(FGLS_ERR_STATE, "fgls_hip_adjoint_begin not called");
+  if (c->mode != FGLS_MODE_MIXED_I8 || !c->adj->h8_ready)
+    return fail(FGLS_ERR_STATE, "fgls_hip_adjoint_begin was not called in the mixed mode");
+  if (count < 0 || count > c->max_block)
+    return fail(FGLS_ERR_ARG, "count=%d outside [0, %d]", count, c->max_block);
+  if (count == 0) return FGLS_OK;
+  if (!dX8 || !dw) return fail(FGLS_ERR_ARG, "NULL pointer");
+  const int64_t ld8 = (c->n + 63) / 64 * 64;
+  if (ldx8 < ld8 || (ldx8 & 15) || ((uintptr_t)dX8 & 15))
+    return fail(FGLS_ERR_ARG, "int8 X_R needs ld >= ceil(n/64)*64, ld %% 16 == 0 and 16-byte alignment");
+  int rc = fgls::set_device(c);
+  if (rc) return rc;
+  if ((rc = fgls::ensure_buffer(&c->X8t, &c->X8t_cap, fgls_x8t_bytes(c->n, count, 1)))) return rc;
+  fgls_adjoint_state* a = c->adj;
+  hipStream_t st = (hipStream_t)stream;
+  const int n = c->n, ldn = c->ldn, p = c->p;
+  fgls_launch_tile_i8(n, count, 1, dX8, ldx8, c->X8t, st);
+  // Z = L^-1 X_R on the int8 slices, then b and V of every SNP of the block
+  fgls_launch_i8_store(n, count, c->RT8, ldn / 64, c->A8, c->invsc, c->X8t, ld8, c->z, ldn, st);
+  fgls::launch_snp_stage_any(c, c->z, ldn, count, a->Bs, a->Vs, snp_base, st);
+  // Y = H X_R = G Z on the slices of H: t_i = y_i' z_i, s_i = W' y_i
+  fgls_launch_i8_dot(p, n, count, c->RT8, ldn / 64, a->H8, a->hinvsc, c->X8t, ld8, c->W, ldn, c->z,
+                     ldn, c->part, (int64_t)c->max_block, st);
+  FGLS_P_SWITCH(p, launch_adj_snp, c, count, st, dw, c->RT8);
   const int nchunk = (count + CHUNK - 1) / CHUNK;
   hipLaunchKernelGGL(k_adj_gz_partial, dim3((n + 255) / 256, nchunk), dim3(256), 0, st, n, count,
                      c->z, (int64_t)ldn, a->gam, a->gzp, (int64_t)ldn);

@@ -83,4 +83,15 @@ void fgls_launch_trmm_i8(int p, int n, int count, int RT32, int KS, const int8_t
                          int64_t ldw, double* part, int64_t ldp, int64_t plane_off,
                          double inv_xscale, int eig_ntr, hipStream_t st);
 
+// The int8 product z = A x (triangular A, as for the reductions) written to
+// Z (column-major, ld ldz >= RT32 * 32, 32-byte aligned); and, for a full A,
+// y = A x reduced per SNP against Zin and W: part[(rt (p+1) + 0) ldp + col] =
+// y' Zin[:, col], part[(rt (p+1) + 1 + k) ldp + col] = y' W[:, k] (k < p).
+void fgls_launch_i8_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
+                          const int8_t* X8T, int64_t ld8, double* Z, int64_t ldz, hipStream_t st);
+void fgls_launch_i8_dot(int p, int n, int count, int RT32, int KS, const int8_t* A8,
+                        const double* inv_scale, const int8_t* X8T, int64_t ld8, const double* W,
+                        int64_t ldw, const double* Zin, int64_t ldz, double* part, int64_t ldp,
+                        hipStream_t st);
+
 #endif

@@ -17,6 +17,19 @@
 
 namespace {
 
+#define FGLS_I8_P_SWITCH(p, FN, ...)                                                         \
+  switch (p) {                                                                               \
+    case 1: FN<1>(__VA_ARGS__); break;  case 2: FN<2>(__VA_ARGS__); break;                   \
+    case 3: FN<3>(__VA_ARGS__); break;  case 4: FN<4>(__VA_ARGS__); break;                   \
+    case 5: FN<5>(__VA_ARGS__); break;  case 6: FN<6>(__VA_ARGS__); break;                   \
+    case 7: FN<7>(__VA_ARGS__); break;  case 8: FN<8>(__VA_ARGS__); break;                   \
+    case 9: FN<9>(__VA_ARGS__); break;  case 10: FN<10>(__VA_ARGS__); break;                 \
+    case 11: FN<11>(__VA_ARGS__); break; case 12: FN<12>(__VA_ARGS__); break;                \
+    case 13: FN<13>(__VA_ARGS__); break; case 14: FN<14>(__VA_ARGS__); break;                \
+    case 15: FN<15>(__VA_ARGS__); break; case 16: FN<16>(__VA_ARGS__); break;                \
+    default: break;                                                                          \
+  }
+
 #define FGLS_LDS(p) ((__attribute__((address_space(3))) void*)(p))
 
 __device__ __forceinline__ fgls_v4i mfma_i8(fgls_v4i a, fgls_v4i b, fgls_v4i c) {
@@ -124,12 +137,20 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i
 // row tile runs all K steps), and the epilogue reduces z against ntr groups
 // of NQ weight columns (group k at W + k*NQ*ldw: column 0 weighs z*z,
 // columns 1..NQ-1 weigh z), writing part[((rt*ntr + k)*NQ + q)*ldp + col].
-template <int NQ, int S, bool DOS = false, bool EIG = false>
+//
+// MODE (genotypes only): I8_RED the fused reductions above; I8_STORE writes
+// z (column-major, ld ldz) instead, for the triangular A; I8_DOT, for a full
+// A, reduces per SNP y'Zin[:, col] and y'W[:, k] (Zin column-major, ld ldz)
+// into part as I8_RED does z'z and z'W[:, k].
+enum { I8_RED = 0, I8_STORE = 1, I8_DOT = 2 };
+
+template <int NQ, int S, bool DOS = false, bool EIG = false, int MODE = I8_RED>
 __global__ __launch_bounds__(256, 2) void k_trmm_i8(
     int n, int count, int RT32, int CT, int KS, const int8_t* __restrict__ A8,
     const double* __restrict__ inv_scale, const int8_t* __restrict__ X8T, int64_t ld8,
     const double* __restrict__ W, int64_t ldw, double* __restrict__ part, int64_t ldp,
-    int64_t plane_off = 0, double inv_xscale = 1.0, int ntr = 1) {
+    int64_t plane_off = 0, double inv_xscale = 1.0, int ntr = 1, double* __restrict__ Zio = nullptr,
+    int64_t ldz = 0) {
   static_assert(S % 2 == 0, "the 2 x S slice blocks are spread over 4 waves");
   constexpr int WN = DOS ? 32 : 64;   // SNP columns per wave
   __shared__ SliceStage<S> st0, st1;
@@ -141,7 +162,8 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   const int r0 = rt * I8_TM;
   const int c0 = ct * (4 * WN) + w * WN;
   const int n64 = (n + 63) & ~63;
-  const int nsteps = (r0 < n64) ? (EIG ? n64 : min((r0 + I8_TM + 63) & ~63, n64)) / 64 : 0;
+  constexpr bool FULL = EIG || MODE == I8_DOT;   // every row tile runs all K steps
+  const int nsteps = (r0 < n64) ? (FULL ? n64 : min((r0 + I8_TM + 63) & ~63, n64)) / 64 : 0;
 
   // wave w moves blocks w, w + 4, w + 8 of each step: block = ti * S + s
   const int8_t* asrc[S / 2];
@@ -189,7 +211,70 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
     __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
   }
 
-  if constexpr (EIG) {
+  if constexpr (MODE != I8_RED) {
+    static_assert(!DOS && !EIG, "store / dot epilogues take genotypes");
+    // z(row r0+16ti+4lg+v, SNP c0+16tj+li): each lane holds 4 consecutive
+    // rows of a column per (ti, tj), moved as one 32-byte access
+    double z[4][2][4];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double isc = inv_scale[r0 + 16 * ti + 4 * lg + v];
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) {
+          double t = (double)acc[S - 1][ti][tj][v];
+#pragma unroll
+          for (int s2 = S - 2; s2 >= 0; --s2)
+            t = fma(t, 1.0 / FGLS_SLICE_RADIX, (double)acc[s2][ti][tj][v]);
+          z[tj][ti][v] = t * isc;
+        }
+      }
+    if constexpr (MODE == I8_STORE) {
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        const int col = c0 + 16 * tj + li;
+        if (col < count) {
+#pragma unroll
+          for (int ti = 0; ti < 2; ++ti)
+            *reinterpret_cast<fgls_v4d*>(Zio + (size_t)col * ldz + r0 + 16 * ti + 4 * lg) =
+                fgls_v4d{z[tj][ti][0], z[tj][ti][1], z[tj][ti][2], z[tj][ti][3]};
+        }
+      }
+    } else {
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        int oz;   // opaque zero: W is re-read per fragment instead of held live
+        asm volatile("v_mov_b32 %0, 0" : "=v"(oz));
+        const int col = c0 + 16 * tj + li;
+        const double* zc = Zio + (size_t)min(col, count - 1) * ldz;
+        double q[NQ];
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) q[qi] = 0.0;
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti) {
+          const fgls_v4d zin = *reinterpret_cast<const fgls_v4d*>(zc + r0 + 16 * ti + 4 * lg);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = r0 + 16 * ti + 4 * lg + v + oz;
+            const double yy = z[tj][ti][v];
+            q[0] = fma(yy, zin[v], q[0]);
+#pragma unroll
+            for (int k = 0; k < NQ - 1; ++k) q[1 + k] = fma(yy, W[(size_t)k * ldw + row], q[1 + k]);
+          }
+        }
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) {
+          q[qi] += __shfl_xor(q[qi], 16, 64);
+          q[qi] += __shfl_xor(q[qi], 32, 64);
+        }
+        if (lg == 0 && col < count) {
+#pragma unroll
+          for (int qi = 0; qi < NQ; ++qi) part[((size_t)rt * NQ + qi) * ldp + col] = q[qi];
+        }
+      }
+    }
+  } else if constexpr (EIG) {
     // z(row r0+16ti+4lg+v, SNP c0+16tj+li) for the NF fragments of the wave
     // (dosages: lo and hi digits combined), then one pass per weight group
     constexpr int NF = DOS ? 2 : 4;
@@ -438,6 +523,38 @@ void launch_any(int n, int count, int RT32, int KS, const int8_t* A8, const doub
 }
 
 }  // namespace
+
+namespace {
+
+template <int P>
+void launch_dot(int n, int count, int RT32, int KS, const int8_t* A8, const double* isc,
+                const int8_t* X8T, int64_t ld8, const double* W, int64_t ldw, const double* Zin,
+                int64_t ldz, double* part, int64_t ldp, hipStream_t st) {
+  const int CT = (count + I8_TN - 1) / I8_TN;
+  const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
+  hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, false, I8_DOT>), dim3(G), dim3(256), 0, st,
+                     n, count, RT32, CT, KS, A8, isc, X8T, ld8, W, ldw, part, ldp, (int64_t)0, 1.0, 1,
+                     const_cast<double*>(Zin), ldz);
+}
+
+}  // namespace
+
+void fgls_launch_i8_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
+                          const int8_t* X8T, int64_t ld8, double* Z, int64_t ldz, hipStream_t st) {
+  const int CT = (count + I8_TN - 1) / I8_TN;
+  const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
+  hipLaunchKernelGGL((k_trmm_i8<2, FGLS_NSLICE, false, false, I8_STORE>), dim3(G), dim3(256), 0, st, n,
+                     count, RT32, CT, KS, A8, inv_scale, X8T, ld8, (const double*)nullptr, (int64_t)0,
+                     (double*)nullptr, (int64_t)0, (int64_t)0, 1.0, 1, Z, ldz);
+}
+
+void fgls_launch_i8_dot(int p, int n, int count, int RT32, int KS, const int8_t* A8,
+                        const double* inv_scale, const int8_t* X8T, int64_t ld8, const double* W,
+                        int64_t ldw, const double* Zin, int64_t ldz, double* part, int64_t ldp,
+                        hipStream_t st) {
+  FGLS_I8_P_SWITCH(p, launch_dot, n, count, RT32, KS, A8, inv_scale, X8T, ld8, W, ldw, Zin, ldz, part,
+                   ldp, st);
+}
 
 void fgls_launch_tile_i8(int n, int count, int planes, const int8_t* X8, int64_t ldx8, int8_t* X8T,
                          hipStream_t st) {

@@ -90,6 +90,7 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_get_factor": (i32, [vp, dp, i64, vp]),
             "fgls_hip_adjoint_begin": (i32, [vp, dp, i64, ctypes.c_double, vp]),
             "fgls_hip_adjoint_block": (i32, [vp, dp, i64, i32, dp, i64, vp]),
+            "fgls_hip_adjoint_block_i8": (i32, [vp, dp, i64, i32, dp, i64, vp]),
             "fgls_hip_adjoint_end": (i32, [vp, dp, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_double), vp]),
             "fgls_factor_M": (i32, [i32, dp, i64, ctypes.c_double, ctypes.c_double, dp, i64, i32]),
@@ -131,6 +132,7 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_solve_block_i8", "fgls_hip_to_i8", "fgls_hip_trsm",
                         "fgls_hip_trmm", "fgls_hip_snp_stage", "fgls_hip_check",
                         "fgls_hip_adjoint_begin", "fgls_hip_adjoint_block", "fgls_hip_adjoint_end",
+                        "fgls_hip_adjoint_block_i8",
                         "fgls_hip_set_trait_phi", "fgls_hip_get_factor", "fgls_hip_to_dosage_i8",
                         "fgls_hip_block_reduce_dosage", "fgls_hip_solve_block_dosage",
                         "fgls_hip_i8_to_f64", "fgls_hip_u2_to_i8", "fgls_hip_x8t_bytes",
@@ -430,6 +432,15 @@ class DeviceContext:
             raise FglsError(FGLS_ERR_ARG, "w has fewer than count weights")
         _check(lib().fgls_hip_adjoint_block(self._h, _ptr(XR), XR.shape[1], count, _ptr(w), snp_base,
                                             _stream(stream)))
+
+    def adjoint_block_i8(self, X8, w, count: int | None = None, snp_base: int = 0, stream=None):
+        """Mixed mode: X8 (count, ld8) int8 genotypes (to_i8); w: (count,) weights."""
+        _need_cuda_f64(w, "w")
+        count = X8.shape[0] if count is None else count
+        if w.numel() < count:
+            raise FglsError(FGLS_ERR_ARG, "w has fewer than count weights")
+        _check(lib().fgls_hip_adjoint_block_i8(self._h, _ptr(X8), X8.shape[1], count, _ptr(w), snp_base,
+                                               _stream(stream)))
 
     def adjoint_end(self, grad_y=None, stream=None):
         """Returns (loss, dl/dy as a (n,) tensor, dl/dh2)."""

@@ -250,7 +250,13 @@ int fgls_hip_get_factor(fgls_hip_ctx *ctx, double *dL, int64_t ldl, void *stream
  * M = sigma2 (h^2 Phi + (1 - h^2) I) is the matrix whose factor L was given
  * to fgls_hip_set_trait (fgls_chol_gpu.c:406-414).  The reference has no
  * counterpart: this is an addition on top of its forward path.
- * FP64 mode and wXR = 1 only.
+ * wXR = 1.  In the mixed mode (fgls_hip_ctx_set_mode before begin) begin
+ * also builds H = G L^-1 and its int8 digit slices (ldn*ldn*14 bytes more),
+ * and fgls_hip_adjoint_block_i8 takes int8 genotypes (column-major, as for
+ * fgls_hip_solve_block_i8): both per-block products then run on the int8
+ * MFMA (L^-1 X_R and H X_R = G L^-1 X_R, exact int32 sums, the slices'
+ * 3.7e-15 relative representation error); fgls_hip_adjoint_block works in
+ * either mode.
  *
  * begin: Phi (device, column-major, ld ldphi >= n) and sigma2 of the trait;
  *        builds G = L^-1 (Phi - I) L^-T once and clears the accumulators.
@@ -266,6 +272,9 @@ int fgls_hip_adjoint_begin(fgls_hip_ctx *ctx, const double *dPhi, int64_t ldphi,
 int fgls_hip_adjoint_block(fgls_hip_ctx *ctx, const double *dXR, int64_t ldxr,
                            int count, const double *dw, int64_t snp_base,
                            void *stream);
+int fgls_hip_adjoint_block_i8(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ldx8,
+                              int count, const double *dw, int64_t snp_base,
+                              void *stream);
 int fgls_hip_adjoint_end(fgls_hip_ctx *ctx, double *dgrad_y, double *hloss,
                          double *hgrad_h2, void *stream);
 

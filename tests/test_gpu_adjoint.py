@@ -52,6 +52,41 @@ def test_adjoint_matches_oracle(gpu, pkg, n, m, block, wXL):
     assert abs(gh - gh_o) <= TOL * abs(gh_o)
 
 
+@pytest.mark.parametrize("n,m,block,wXL", [(300, 2000, 2000, 3), (257, 700, 128, 3), (1000, 600, 256, 5)])
+def test_mixed_adjoint_matches_oracle(gpu, pkg, n, m, block, wXL):
+    """The mixed-mode adjoint (int8 products against L^-1 and H = G L^-1)
+    against the FP64 oracle, to the same bar as the FP64 adjoint, and against
+    the device's FP64 adjoint."""
+    print(f"mixed adjoint vs oracle n={n} m={m} block={block} wXL={wXL}", flush=True)
+    from cugwas_proto_amd.adjoint import snp_loss_grad
+    pr, w = _problem(n, m, wXL=wXL, seed=n + m + 1)
+    XR, XL, y, wd = _dev_inputs(pr, w)
+    loss, gy, gh = snp_loss_grad(y, pr.h2, XR, wd, pr.Phi, XL, 1.0, block, precision="mixed")
+    loss_o, gy_o, gh_o = orc.snp_loss_adjoint(pr.Phi, pr.h2, 1.0, pr.XL, pr.XR, pr.y, w)
+    assert abs(loss - loss_o) <= TOL * abs(loss_o)
+    assert _rel(gy.cpu().numpy(), gy_o) <= TOL
+    assert abs(gh - gh_o) <= TOL * abs(gh_o)
+    l64, gy64, gh64 = snp_loss_grad(y, pr.h2, XR, wd, pr.Phi, XL, 1.0, block)
+    assert _rel(gy.cpu().numpy(), gy64.cpu().numpy()) <= TOL and abs(gh - gh64) <= TOL * abs(gh64)
+
+
+def test_mixed_adjoint_needs_mixed_begin(gpu, pkg):
+    """adjoint_block_i8 after an FP64-mode begin is a state error."""
+    import cugwas_proto_amd.fgls as F
+    pr, w = _problem(128, 64, seed=4)
+    XR, XL, y, wd = _dev_inputs(pr, w)
+    L = orc.cholesky_lower(orc.build_M(pr.Phi, pr.h2, 1.0))
+    dc = pkg.DeviceContext(pr.n, 3, 1, max_block=64)
+    dc.set_trait(torch.from_numpy(np.ascontiguousarray(L.T)).cuda(), XL, y)
+    dc.adjoint_begin(torch.from_numpy(np.ascontiguousarray(pr.Phi.T)).cuda(), 1.0)
+    dc.set_mode(pkg.FGLS_MODE_MIXED_I8)
+    X8 = dc.to_i8(XR)
+    with pytest.raises(pkg.FglsError) as ei:
+        dc.adjoint_block_i8(X8, wd)
+    assert ei.value.code == pkg.FGLS_ERR_STATE and "mixed mode" in ei.value.msg
+    dc.close()
+
+
 def test_adjoint_matches_torch_autograd_dense(gpu, pkg):
     """Dense check at m = 2000 SNPs: torch.autograd through explicit M^-1."""
     print("adjoint vs dense autograd n=160 m=2000", flush=True)
