@@ -53,11 +53,13 @@ class SnpLoss(torch.autograd.Function):
     (SNP i in row i, ldx >= n; copied with zero padding unless ldx % 4 == 0),
     w (m,), Phi
     (n, n) symmetric, XL (wXL, n) covariates (row k = column k of X_L),
-    sigma2 (float), block (SNPs per device call).
+    sigma2 (float), block (SNPs per device call), precision ("fp64", or
+    "mixed": int8 genotypes for the forward and backward products).
     """
 
     @staticmethod
-    def forward(ctx, y, h2, XR, w, Phi, XL, sigma2: float = 1.0, block: int = 2048):
+    def forward(ctx, y, h2, XR, w, Phi, XL, sigma2: float = 1.0, block: int = 2048,
+                precision: str = "fp64"):
         n = int(y.shape[0])
         Phi_h = Phi.detach().cpu().numpy() if isinstance(Phi, torch.Tensor) else np.asarray(Phi)
         if Phi_h.shape != (n, n):
@@ -66,9 +68,13 @@ class SnpLoss(torch.autograd.Function):
         m = int(XR.shape[0])
         if XR.shape[1] < n or w.shape[0] != m:
             raise FglsError(FGLS_ERR_ARG, "XR must be (m, ldx >= n) and w (m,)")
+        if precision not in ("fp64", "mixed"):
+            raise FglsError(FGLS_ERR_ARG, "precision must be fp64 or mixed")
         h2v = float(h2)
         L = factor_M(Phi_h, h2v, float(sigma2))
         dc = DeviceContext(n, wXL, 1, max_block=min(block, m))
+        if precision == "mixed":
+            dc.set_mode(FGLS_MODE_MIXED_I8)
         Ld = torch.from_numpy(np.ascontiguousarray(L.T)).cuda()      # column-major (n, n)
         XLd = _as_f64_cuda(XL, "XL")
         yd = _as_f64_cuda(y, "y")
@@ -79,15 +85,19 @@ class SnpLoss(torch.autograd.Function):
         beta = torch.empty(m, dtype=torch.float64, device="cuda")
         B = torch.empty((dc.max_block, p), dtype=torch.float64, device="cuda")
         V = torch.empty((dc.max_block, p * p), dtype=torch.float64, device="cuda")
+        X8 = dc.to_i8(XRd) if precision == "mixed" else None
         for i0 in range(0, m, dc.max_block):
             cnt = min(dc.max_block, m - i0)
-            dc.solve_block(XRd[i0:i0 + cnt], cnt, B, V, snp_base=i0)
+            if X8 is not None:
+                dc.solve_block_i8(X8[i0:i0 + cnt], cnt, B, V, snp_base=i0)
+            else:
+                dc.solve_block(XRd[i0:i0 + cnt], cnt, B, V, snp_base=i0)
             beta[i0:i0 + cnt] = B[:cnt, p - 1]
         dc.check()
         loss = torch.dot(wd, beta * beta)
         ctx.dc = dc
         ctx.Phi = torch.from_numpy(np.ascontiguousarray(Phi_h.T)).cuda()
-        ctx.XR, ctx.w, ctx.sigma2, ctx.h2_like = XRd, wd, float(sigma2), h2
+        ctx.XR, ctx.X8, ctx.w, ctx.sigma2, ctx.h2_like = XRd, X8, wd, float(sigma2), h2
         ctx.y_device = y.device
         return loss.to(y.device)
 
@@ -98,7 +108,10 @@ class SnpLoss(torch.autograd.Function):
         dc.adjoint_begin(ctx.Phi, ctx.sigma2)
         for i0 in range(0, m, dc.max_block):
             cnt = min(dc.max_block, m - i0)
-            dc.adjoint_block(ctx.XR[i0:i0 + cnt], ctx.w[i0:i0 + cnt], cnt, snp_base=i0)
+            if ctx.X8 is not None:
+                dc.adjoint_block_i8(ctx.X8[i0:i0 + cnt], ctx.w[i0:i0 + cnt], cnt, snp_base=i0)
+            else:
+                dc.adjoint_block(ctx.XR[i0:i0 + cnt], ctx.w[i0:i0 + cnt], cnt, snp_base=i0)
         _, gy, gh = dc.adjoint_end()
         dc.check()
         go = grad_out.detach().to(device="cuda", dtype=torch.float64)
@@ -110,12 +123,12 @@ class SnpLoss(torch.autograd.Function):
             grad_h2 = None
         ctx.dc = None
         dc.close()
-        return grad_y, grad_h2, None, None, None, None, None, None
+        return grad_y, grad_h2, None, None, None, None, None, None, None
 
 
-def snp_loss(y, h2, XR, w, Phi, XL, sigma2: float = 1.0, block: int = 2048):
+def snp_loss(y, h2, XR, w, Phi, XL, sigma2: float = 1.0, block: int = 2048, precision: str = "fp64"):
     """Differentiable l(y, h2) = sum_i w_i b_i[p-1]^2 (see SnpLoss)."""
-    return SnpLoss.apply(y, h2, XR, w, Phi, XL, sigma2, block)
+    return SnpLoss.apply(y, h2, XR, w, Phi, XL, sigma2, block, precision)
 
 
 def snp_loss_grad(y, h2: float, XR, w, Phi, XL, sigma2: float = 1.0, block: int = 2048,

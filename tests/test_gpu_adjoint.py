@@ -131,6 +131,38 @@ def test_autograd_function_and_central_differences(gpu, pkg):
     assert abs(fd_y - gd) <= TOL_FD * abs(gd)
 
 
+def test_autograd_function_mixed_and_central_differences(gpu, pkg):
+    """SnpLoss with precision="mixed" (int8 forward and backward products):
+    its autograd gradients equal snp_loss_grad's mixed ones and match central
+    differences of the mixed forward loss (the int8 path is smooth in y and
+    h2: only L^-1 and H change with h2, the genotypes stay exact)."""
+    print("SnpLoss mixed autograd + FD n=200 m=600", flush=True)
+    from cugwas_proto_amd.adjoint import snp_loss, snp_loss_grad
+    pr, w = _problem(200, 600, seed=19, h2=0.4)
+    XR, XL, y, wd = _dev_inputs(pr, w)
+    yv = y.clone().requires_grad_(True)
+    hv = torch.tensor(pr.h2, dtype=torch.float64, requires_grad=True)
+    loss = snp_loss(yv, hv, XR, wd, pr.Phi, XL, 1.0, 256, "mixed")
+    gy_a, gh_a = torch.autograd.grad(loss, (yv, hv))
+    loss_g, gy, gh = snp_loss_grad(y, pr.h2, XR, wd, pr.Phi, XL, 1.0, 256, precision="mixed")
+    assert abs(float(loss.detach()) - loss_g) <= 1e-12 * abs(loss_g)
+    assert _rel(gy_a.cpu().numpy(), gy.cpu().numpy()) <= 1e-12
+    assert abs(float(gh_a) - gh) <= 1e-12 * abs(gh)
+    eps = 1e-6
+
+    def f(yy, hh):
+        with torch.no_grad():
+            return float(snp_loss(yy, torch.tensor(hh, dtype=torch.float64), XR, wd, pr.Phi, XL, 1.0, 256,
+                                  "mixed"))
+
+    fd_h = (f(y, pr.h2 + eps) - f(y, pr.h2 - eps)) / (2 * eps)
+    assert abs(fd_h - gh) <= TOL_FD * abs(gh)
+    d = torch.from_numpy(np.random.default_rng(2).standard_normal(pr.n)).cuda()
+    fd_y = (f(y + eps * d, pr.h2) - f(y - eps * d, pr.h2)) / (2 * eps)
+    gd = float(torch.dot(gy, d))
+    assert abs(fd_y - gd) <= TOL_FD * abs(gd)
+
+
 def test_adjoint_blocking_and_repeat(gpu, pkg):
     """Block size does not change the result beyond rounding, and a second
     begin/blocks/end pass on the same context reproduces the first exactly."""
