@@ -158,11 +158,11 @@ def cpu_baseline(kind, Phi, L, XL, y, X, n, wXL, h2, sample, threads):
     return {"value": rate, "unit": "SNPs/s", "cores": threads, "kind": kind, "sample": desc}
 
 
-def run_adjoint(torch, pkg, dev, m, steps):
+def run_adjoint(torch, pkg, dev, m, steps, n=5000, wXL=3):
     """Time the adjoint of l = sum_i w_i b_i[p-1]^2 (dl/dy, dl/dh2) over m SNPs
-    at n = 5000, p = 4, h2 = 0.6, w_i ~ U(0, 1), blocks of 2048 SNPs, next to
-    the forward solve of the same blocks."""
-    n, wXL, block = 5000, 3, 2048
+    (default n = 5000, p = 4), h2 = 0.6, w_i ~ U(0, 1), blocks of 2048 SNPs,
+    next to the forward solve of the same blocks; then the mixed-mode adjoint."""
+    block = 2048
     ldx = (n + 3) // 4 * 4
     Phi, L, XL, y, X, X8 = make_inputs(torch, n, m, wXL, 0.6, 3, ldx, dev)
     g = torch.Generator(device=dev)
@@ -377,6 +377,8 @@ def main():
     ap.add_argument("--eigen-paths", default="fp64,mixed")
     ap.add_argument("--adjoint-m", type=int, default=200_000,
                     help="SNPs of the adjoint measurement at n=5000, p=4, blocks of 2048 (0: skip)")
+    ap.add_argument("--adjoint-n", type=int, default=5000)
+    ap.add_argument("--adjoint-wxl", type=int, default=3)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (a known-size kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -603,8 +605,8 @@ def main():
     if args.adjoint_m > 0:
         del X, X8, D8, paths
         torch.cuda.empty_cache()
-        adjoint = run_adjoint(torch, pkg, dev, args.adjoint_m, args.steps)
-        log(f"adjoint (n=5000, m={args.adjoint_m}, block 2048): {adjoint['snps_per_s']:.0f} SNPs/s, "
+        adjoint = run_adjoint(torch, pkg, dev, args.adjoint_m, args.steps, args.adjoint_n, args.adjoint_wxl)
+        log(f"adjoint (n={args.adjoint_n}, m={args.adjoint_m}, block 2048): {adjoint['snps_per_s']:.0f} SNPs/s, "
             f"forward {adjoint['forward_snps_per_s']:.0f} SNPs/s; mixed {adjoint['mixed']['snps_per_s']:.0f} "
             f"SNPs/s (dl/dy vs FP64 {adjoint['mixed']['grad_y_rel_vs_fp64']:.1e})")
     out = {
