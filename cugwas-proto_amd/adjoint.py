@@ -15,7 +15,10 @@ gradients; this is an addition on top of its forward path.
 * backward: ``fgls_hip_adjoint_begin/block/end`` recompute L^-1 X_R block by
   block and return dl/dy and dl/dh2.
 
-Gradients flow to ``y`` and ``h2`` only.
+With ``precision="mixed"`` the genotypes go to the device as int8 and the
+block products of both passes run on the int8 MFMA
+(``fgls_hip_solve_block_i8``, ``fgls_hip_adjoint_block_i8``).  Gradients
+flow to ``y`` and ``h2`` only.
 """
 from __future__ import annotations
 
