@@ -600,15 +600,24 @@ def main():
     if args.eigen_traits > 0:
         ctx.close()
         torch.cuda.empty_cache()
-        eigen = run_eigen(torch, pkg, args, Phi, XL, y, X, X8, paths.get("fp64"), dev)
+        try:
+            eigen = run_eigen(torch, pkg, args, Phi, XL, y, X, X8, paths.get("fp64"), dev)
+        except pkg.FglsError as err:   # reported in the JSON line; the headline paths stand
+            eigen = {"error": str(err)}
+            log(f"eigen leg failed: {err}")
     adjoint = None
     if args.adjoint_m > 0:
         del X, X8, D8, paths
         torch.cuda.empty_cache()
-        adjoint = run_adjoint(torch, pkg, dev, args.adjoint_m, args.steps, args.adjoint_n, args.adjoint_wxl)
-        log(f"adjoint (n={args.adjoint_n}, m={args.adjoint_m}, block 2048): {adjoint['snps_per_s']:.0f} SNPs/s, "
-            f"forward {adjoint['forward_snps_per_s']:.0f} SNPs/s; mixed {adjoint['mixed']['snps_per_s']:.0f} "
-            f"SNPs/s (dl/dy vs FP64 {adjoint['mixed']['grad_y_rel_vs_fp64']:.1e})")
+        try:
+            adjoint = run_adjoint(torch, pkg, dev, args.adjoint_m, args.steps, args.adjoint_n, args.adjoint_wxl)
+            log(f"adjoint (n={args.adjoint_n}, m={args.adjoint_m}, block 2048): {adjoint['snps_per_s']:.0f} "
+                f"SNPs/s, forward {adjoint['forward_snps_per_s']:.0f} SNPs/s; mixed "
+                f"{adjoint['mixed']['snps_per_s']:.0f} SNPs/s (dl/dy vs FP64 "
+                f"{adjoint['mixed']['grad_y_rel_vs_fp64']:.1e})")
+        except pkg.FglsError as err:   # reported in the JSON line; the headline paths stand
+            adjoint = {"error": str(err)}
+            log(f"adjoint leg failed: {err}")
     out = {
         "metric": METRIC,
         "value": res["snps_per_s"],
