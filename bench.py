@@ -551,6 +551,29 @@ def main():
             check[mode] = dict(e, snps=int(idx.size))
             log(f"{mode} parity vs oracle on {idx.size} SNPs: {e}")
 
+    # PCIe: pinned host -> device copy of 1 GiB (the file pipeline's upload
+    # path), against Gen5 x16's 64 GB/s per direction
+    pcie = None
+    try:
+        hbuf = torch.empty(1 << 27, dtype=torch.float64, pin_memory=True)
+        dbuf = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+        dbuf.copy_(hbuf, non_blocking=True)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(4):
+            dbuf.copy_(hbuf, non_blocking=True)
+        e1.record()
+        torch.cuda.synchronize()
+        gbs = 4 * hbuf.numel() * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+        pcie = {"achieved_GBps": gbs, "peak_GBps": 64.0, "frac": gbs / 64.0,
+                "fp64_snps_per_s_bound": gbs * 1e9 / (8.0 * n), "int8_snps_per_s_bound": gbs * 1e9 / n,
+                "u2_snps_per_s_bound": gbs * 1e9 / ((n + 3) // 4)}
+        del hbuf, dbuf
+        log(f"PCIe H2D (pinned, 1 GiB): {gbs:.1f} GB/s")
+    except RuntimeError as err:
+        log(f"PCIe H2D measurement skipped: {err}")
+
     cpu = None
     if args.cpu_baseline != "none":
         threads = host_threads()
@@ -587,9 +610,17 @@ def main():
     res = paths[head]
     summary = {}
     for mode, r in paths.items():
+        # the per-SNP stage (k_finalize) reads (p+1) partials per row tile and
+        # writes b and V: its HBM rate against the 8 TB/s peak
+        tiles = (n + 127) // 128 * (4 if mode != "fp64" else 1)
+        fin_bytes = block * ((p + 1) * tiles * 8 + (p + p * p) * 8)
+        fin_gbs = fin_bytes / (r["finalize_ms"] * 1e-3) / 1e9 if r["finalize_ms"] > 0 else None
         summary[mode] = {"snps_per_s": r["snps_per_s"], "ms_per_step": r["elapsed"] / args.steps * 1e3,
                          "reduce_kernel_ms": r["reduce_ms"], "finalize_kernel_ms": r["finalize_ms"],
                          "tile_kernel_ms": r["tile_ms"],
+                         "finalize_hbm": {"bytes_per_launch": fin_bytes, "achieved_GBps": fin_gbs,
+                                          "peak_GBps": PEAK_HBM_GBS,
+                                          "frac": fin_gbs / PEAK_HBM_GBS if fin_gbs else None},
                          "mode_setup_s": r["setup_s"], "roofline": roofline(mode, r)}
     if "mixed" in paths and "fp64" in paths:
         e = orc_parity(paths["mixed"], paths["fp64"], p)
@@ -641,6 +672,7 @@ def main():
         "roofline": roofline(head, res),
         "cpu_baseline": cpu,
         "trait_setup_s": trait_setup,
+        "pcie_h2d": pcie,
         "paths": summary,
         "parity_vs_oracle": check,
         "eigen": eigen,
