@@ -280,3 +280,30 @@ def test_u2_unpack_kernel(pkg, gpu):
         got = X8.cpu().numpy()
         assert np.array_equal(got[:, :n].T, X.astype(np.int8))
         assert not got[:, n:].any()
+
+
+@pytest.mark.parametrize("variant", ["chol", "eigen"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pipeline_edge_sizes(pkg, gpu, tmp_path, variant, precision):
+    """m = 0 writes empty B.out / V.out; one SNP; a tiny n (7 < 16) with one
+    covariate; both variants and precisions against the oracle."""
+    run = pkg.fgls_chol_gpu if variant == "chol" else pkg.fgls_eigen
+    f = lambda d, s: os.path.join(d, s)
+    for n, wXL, m, t in ((40, 2, 0, 1), (40, 2, 1, 2), (7, 1, 9, 1)):
+        prob = make_problem(n=n, wXL=wXL, m=max(m, 1), seed=n + m, t=t)
+        d = str(tmp_path / f"{variant}_{precision}_{n}_{m}")
+        os.makedirs(d)
+        H, Sig = np.linspace(0.3, 0.6, t), np.ones(t)
+        write_problem_dir(prob, d, H=H, Sig=Sig)
+        if m == 0:
+            open(f(d, "XR.in"), "wb").close()
+        p = wXL + 1
+        run(n, p, m, t, wXL, 1, 256, 1, 2, f(d, "Phi.in"), f(d, "H.in"), f(d, "Sig.in"), f(d, "XL.in"),
+            f(d, "XR.in"), f(d, "Y.in"), f(d, "B.out"), f(d, "V.out"), options={"precision": precision})
+        B = np.fromfile(f(d, "B.out"), dtype="<f8")
+        V = np.fromfile(f(d, "V.out"), dtype="<f8")
+        assert B.size == t * m * p and V.size == t * m * p * p
+        if m:
+            Bo, Vo = orc.fgls_chol(prob.Phi, H, Sig, prob.XL, prob.XR[:, :m], prob.Y)
+            e = orc.parity_errors(B, V, Bo, Vo, p)
+            assert e["B_scaled"] < 1e-10 and e["V_scaled"] < 1e-10 and e["V_upper_zero"], (n, m, e)
