@@ -18,6 +18,7 @@ from .fgls import (  # noqa: F401
     last_error,
     lib,
     LIB_PATH,
+    pack_i16,
     pack_u2,
     tile_i8,
     u2_to_i8,

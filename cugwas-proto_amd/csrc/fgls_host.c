@@ -66,7 +66,8 @@ typedef struct {
   int io_threads;    /* parallel preads of X_R per block */
   int host_potrf;    /* potrf=host: M and its factor on the host (OpenBLAS) */
   int traits_per_pass;  /* traits sharing one read of X_R (0: as many as fit) */
-  int xr_i8;         /* xr_format: 0 f64, 1 i8 (n bytes per column), 2 u2 (ceil(n/4) bytes per column) */
+  int xr_i8;         /* xr_format: 0 f64, 1 i8 (n bytes per column), 2 u2 (ceil(n/4) bytes per
+                        column), 3 i16 (dosages x * dosage_scale, 2 n bytes per column) */
   double dosage_scale;  /* precision=mixed for dosages: x * scale = 128 hi + lo (0: genotypes) */
   int eig_host;      /* eig=host: eigendecomposition of Phi by host LAPACK dsyevr (default: device) */
 } options_t;
@@ -94,7 +95,8 @@ static void parse_options(const char *s, options_t *o) {
     if (!eq) continue;
     *eq = 0;
     if (!strcmp(tok, "xr_format")) {
-      o->xr_i8 = !strcmp(eq + 1, "i8") ? 1 : !strcmp(eq + 1, "u2") ? 2 : (!strcmp(eq + 1, "f64") ? 0 : -1);
+      o->xr_i8 = !strcmp(eq + 1, "i8") ? 1 : !strcmp(eq + 1, "u2") ? 2 : !strcmp(eq + 1, "i16") ? 3
+                                                                   : (!strcmp(eq + 1, "f64") ? 0 : -1);
       continue;
     }
     if (!strcmp(tok, "potrf")) {
@@ -332,11 +334,15 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   if (opt.eig_host < 0) return fgls_set_error(FGLS_ERR_ARG, "eig must be gpu or host");
   if (opt.mixed < 0) return fgls_set_error(FGLS_ERR_ARG, "precision must be fp64 or mixed");
   if (opt.host_potrf < 0) return fgls_set_error(FGLS_ERR_ARG, "potrf must be gpu or host");
-  if (opt.xr_i8 < 0) return fgls_set_error(FGLS_ERR_ARG, "xr_format must be f64, i8 or u2");
+  if (opt.xr_i8 < 0) return fgls_set_error(FGLS_ERR_ARG, "xr_format must be f64, i8, u2 or i16");
   if (opt.dosage_scale < 0.0) return fgls_set_error(FGLS_ERR_ARG, "dosage_scale must be positive");
-  if (opt.dosage_scale > 0.0 && (!opt.mixed || opt.xr_i8))
-    return fgls_set_error(FGLS_ERR_ARG, "dosage_scale needs precision=mixed and an FP64 X_R file");
-  const int dos = opt.dosage_scale > 0.0;
+  const int xr16 = opt.xr_i8 == 3;   /* 16-bit dosage file */
+  if (xr16 && !(opt.dosage_scale > 0.0))
+    return fgls_set_error(FGLS_ERR_ARG, "xr_format=i16 needs dosage_scale");
+  if (opt.dosage_scale > 0.0 && !xr16 && (!opt.mixed || opt.xr_i8))
+    return fgls_set_error(FGLS_ERR_ARG, "dosage_scale needs precision=mixed and an FP64 X_R file, or "
+                                        "xr_format=i16");
+  const int dos = opt.dosage_scale > 0.0 && opt.mixed;
   if (opt.mixed && wXR != 1)
     return fgls_set_error(FGLS_ERR_ARG, "precision=mixed needs wXR = 1 (got %d)", wXR);
   if (eig && wXR != 1)
@@ -377,8 +383,8 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   int8_t *dX8[2] = {NULL, NULL};
   uint8_t *dP2[2] = {NULL, NULL};   /* xr_format=u2: packed columns of the slot */
   int8_t *dX8T[2] = {NULL, NULL};   /* mixed: the slot's genotypes tiled once for every trait */
-  const size_t col_bytes = opt.xr_i8 == 2 ? (size_t)(n + 3) / 4 : opt.xr_i8 == 1 ? (size_t)n
-                                                                               : (size_t)n * sizeof(double);
+  const size_t col_bytes = xr16 ? (size_t)n * 2 : opt.xr_i8 == 2 ? (size_t)(n + 3) / 4
+                                               : opt.xr_i8 == 1 ? (size_t)n : (size_t)n * sizeof(double);
   unsigned int *dBad = NULL;
   const int ld8 = (n + 63) & ~63;
   double *dL = NULL, *dXL = NULL, *dy = NULL, *dX[2] = {NULL, NULL};
@@ -510,7 +516,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
       const size_t b8 = (size_t)block * wXR * ld8 * (dos ? 2 : 1);
       HCHK(hipMalloc((void **)&dX8[s], b8));
       HCHK(hipMemset(dX8[s], 0, b8));
-      if (opt.xr_i8 == 2) HCHK(hipMalloc((void **)&dP2[s], (size_t)block * wXR * col_bytes));
+      if (opt.xr_i8 >= 2) HCHK(hipMalloc((void **)&dP2[s], (size_t)block * wXR * col_bytes));
       if (opt.mixed) HCHK(hipMalloc((void **)&dX8T[s], fgls_hip_x8t_bytes(n, (int)block, dos ? 2 : 1)));
     }
   }
@@ -596,7 +602,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
         }
         /* device X_R slot s is free once the solves of chunk c-2 have run */
         HCHK(hipStreamWaitEvent(s_copy, ev_free[s], 0));
-        if (opt.xr_i8 == 2)
+        if (opt.xr_i8 >= 2)
           HCHK(hipMemcpyAsync(dP2[s], R.slot[s], (size_t)cnt * wXR * col_bytes, hipMemcpyHostToDevice, s_copy));
         else if (opt.xr_i8)
           HCHK(hipMemcpy2DAsync(dX8[s], (size_t)ld8, R.slot[s], (size_t)n, (size_t)n,
@@ -613,6 +619,13 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           HCHK(hipMemsetAsync(dBad + s, 0, sizeof(unsigned int), s_comp));
           RCHK(fgls_hip_u2_to_i8(dP2[s], (int64_t)col_bytes, n, cnt * wXR, dX8[s], ld8, dBad + s, s_comp));
           HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
+        } else if (xr16 && opt.mixed) {
+          /* 16-bit dosages -> the two int8 digit planes; out-of-range values fail the run */
+          HCHK(hipMemsetAsync(dBad + s, 0, sizeof(unsigned int), s_comp));
+          RCHK(fgls_hip_i16_to_dosage_i8((const int16_t *)dP2[s], n, n, cnt, dX8[s], ld8, dBad + s, s_comp));
+          HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
+        } else if (xr16) {
+          RCHK(fgls_hip_i16_to_f64((const int16_t *)dP2[s], n, n, cnt * wXR, opt.dosage_scale, dX[s], ldx, s_comp));
         }
         if (opt.mixed && !opt.xr_i8) {
           /* exact FP64 -> int8 on the device; a value that is not an integer
@@ -623,7 +636,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           else
             RCHK(fgls_hip_to_i8(dX[s], ldx, n, cnt, dX8[s], ld8, dBad + s, s_comp));
           HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
-        } else if (!opt.mixed && opt.xr_i8) {
+        } else if (!opt.mixed && (opt.xr_i8 == 1 || opt.xr_i8 == 2)) {
           RCHK(fgls_hip_i8_to_f64(dX8[s], ld8, n, cnt * wXR, dX[s], ldx, s_comp));
         }
         const int planes = dos ? 2 : 1;
@@ -672,6 +685,9 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           if (opt.xr_i8 == 2)
             rc = fgls_set_error(FGLS_ERR_ARG, "%s: %u 2-bit codes of SNPs %lld.. are 3 (not a genotype)",
                                 XR_path, hBad[ps], (long long)i0);
+          else if (xr16)
+            rc = fgls_set_error(FGLS_ERR_ARG, "%s: %u 16-bit dosage values of SNPs %lld.. are outside "
+                                "[-16256, 16383]", XR_path, hBad[ps], (long long)i0);
           else if (dos)
             rc = fgls_set_error(FGLS_ERR_ARG,
                                 "precision=mixed: %u dosage values of SNPs %lld.. are not multiples of "

@@ -578,6 +578,41 @@ __global__ __launch_bounds__(256) void k_u2_to_i8(int n, const uint8_t* __restri
   if (nbad) atomicAdd(bad, nbad);
 }
 
+// 16-bit dosage file values v = x * scale (column-major, ld ldv) -> the two
+// int8 digit planes of the mixed path (v = 128 hi + lo, lo at X8, hi at
+// X8 + plane_off; rows n..ldx8-1 zeroed); v outside [-16256, 16383] is
+// counted in *bad and written as 0.
+__global__ __launch_bounds__(256) void k_i16_to_dosage_i8(int n, const int16_t* __restrict__ V,
+                                                          int64_t ldv, int8_t* __restrict__ X8,
+                                                          int64_t ldx8, int64_t plane_off,
+                                                          unsigned int* __restrict__ bad) {
+  const int col = blockIdx.x;
+  unsigned int nb = 0;
+  for (int r = threadIdx.x; r < ldx8; r += 256) {
+    int lo = 0, hi = 0;
+    if (r < n) {
+      const int v = V[(size_t)col * ldv + r];
+      if (v < -16256 || v > 16383) {
+        ++nb;
+      } else {
+        hi = v >> 7;
+        lo = v & 127;
+      }
+    }
+    X8[(size_t)col * ldx8 + r] = (int8_t)lo;
+    X8[plane_off + (size_t)col * ldx8 + r] = (int8_t)hi;
+  }
+  if (nb) atomicAdd(bad, nb);
+}
+
+// 16-bit dosage file values -> FP64 x = v / scale (rows n..ldx-1 zeroed).
+__global__ __launch_bounds__(256) void k_i16_to_f64(int n, const int16_t* __restrict__ V, int64_t ldv,
+                                                    double scale, double* __restrict__ X, int64_t ldx) {
+  const int col = blockIdx.x;
+  for (int r = threadIdx.x; r < ldx; r += 256)
+    X[(size_t)col * ldx + r] = (r < n) ? (double)V[(size_t)col * ldv + r] / scale : 0.0;
+}
+
 // int8 genotypes -> FP64 (column-major, ld ldx; rows n..ldx-1 written as 0).
 __global__ __launch_bounds__(256) void k_i8_to_f64(int n, const int8_t* __restrict__ X8, int64_t ldx8,
                                                    double* __restrict__ X, int64_t ldx) {
@@ -1361,6 +1396,28 @@ int fgls_hip_u2_to_i8(const uint8_t* dP, int64_t ldp, int n, int ncols, int8_t* 
   if (ncols == 0) return FGLS_OK;
   hipLaunchKernelGGL(k_u2_to_i8, dim3(ncols), dim3(256), 0, (hipStream_t)stream, n, dP, ldp, dX8, ldx8,
                      dbad);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_i16_to_dosage_i8(const int16_t* dV, int64_t ldv, int n, int ncols, int8_t* dX8,
+                              int64_t ldx8, unsigned int* dbad, void* stream) {
+  if (n < 1 || ncols < 0 || !dV || !dX8 || !dbad || ldv < n || ldx8 < n)
+    return fail(FGLS_ERR_ARG, "bad arguments to fgls_hip_i16_to_dosage_i8");
+  if (ncols == 0) return FGLS_OK;
+  hipLaunchKernelGGL(k_i16_to_dosage_i8, dim3(ncols), dim3(256), 0, (hipStream_t)stream, n, dV, ldv, dX8,
+                     ldx8, (int64_t)ncols * ldx8, dbad);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_i16_to_f64(const int16_t* dV, int64_t ldv, int n, int ncols, double scale, double* dX,
+                        int64_t ldx, void* stream) {
+  if (n < 1 || ncols < 0 || !dV || !dX || ldv < n || ldx < n || !(scale > 0.0))
+    return fail(FGLS_ERR_ARG, "bad arguments to fgls_hip_i16_to_f64");
+  if (ncols == 0) return FGLS_OK;
+  hipLaunchKernelGGL(k_i16_to_f64, dim3(ncols), dim3(256), 0, (hipStream_t)stream, n, dV, ldv, scale, dX,
+                     ldx);
   HIP_TRY(hipGetLastError());
   return FGLS_OK;
 }

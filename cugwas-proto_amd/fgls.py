@@ -80,6 +80,8 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_solve_block_dosage": (i32, [vp, dp, i64, i32, ctypes.c_double, dp, dp, i64, vp]),
             "fgls_hip_i8_to_f64": (i32, [dp, i64, i32, i32, dp, i64, vp]),
             "fgls_hip_u2_to_i8": (i32, [dp, i64, i32, i32, dp, i64, dp, vp]),
+            "fgls_hip_i16_to_dosage_i8": (i32, [dp, i64, i32, i32, dp, i64, dp, vp]),
+            "fgls_hip_i16_to_f64": (i32, [dp, i64, i32, i32, ctypes.c_double, dp, i64, vp]),
             "fgls_hip_x8t_bytes": (ctypes.c_size_t, [i32, i32, i32]),
             "fgls_hip_tile_i8": (i32, [dp, i64, i32, i32, i32, dp, vp]),
             "fgls_hip_block_reduce_i8t": (i32, [vp, dp, i32, i32, ctypes.c_double, vp]),
@@ -136,6 +138,7 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_set_trait_phi", "fgls_hip_get_factor", "fgls_hip_to_dosage_i8",
                         "fgls_hip_block_reduce_dosage", "fgls_hip_solve_block_dosage",
                         "fgls_hip_i8_to_f64", "fgls_hip_u2_to_i8", "fgls_hip_x8t_bytes",
+                        "fgls_hip_i16_to_dosage_i8", "fgls_hip_i16_to_f64",
                         "fgls_hip_tile_i8", "fgls_hip_block_reduce_i8t", "fgls_hip_solve_block_i8t",
                         "fgls_hip_eig_solve_block_i8t",
                         "fgls_factor_M", "fgls_eigen", "fgls_eigen_gpu_ex", "fgls_eigen_decompose",
@@ -642,6 +645,21 @@ def tile_i8(X8, n: int, count: int | None = None, planes: int = 1, X8T=None, str
         raise FglsError(FGLS_ERR_ARG, f"X8T needs {nbytes} bytes")
     _check(lib().fgls_hip_tile_i8(_ptr(X8), X8.shape[1], n, count, planes, _ptr(X8T), _stream(stream)))
     return X8T
+
+
+def pack_i16(X, scale: float = 1000.0):
+    """Dosages X (n x m; column j = SNP j) with a fixed number of decimals ->
+    the 16-bit layout of xr_format=i16: an (m, n) int16 array of
+    v = rint(x * scale); raises unless |x * scale - v| <= 1e-6 and v fits in
+    [-16256, 16383]."""
+    import numpy as np
+    X = np.asarray(X, dtype=np.float64)
+    if X.ndim == 1:
+        X = X[:, None]
+    v = np.rint(X * scale)
+    if np.any(np.abs(X * scale - v) > 1e-6) or v.min(initial=0) < -16256 or v.max(initial=0) > 16383:
+        raise FglsError(FGLS_ERR_ARG, f"dosages must be multiples of 1/{scale:g} within [-16256, 16383]/scale")
+    return np.ascontiguousarray(v.astype(np.int16).T)
 
 
 def u2_to_i8(P, n: int, X8=None, stream=None):

@@ -54,7 +54,7 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *                  reference's FGLS_GPU_SERIAL, fgls_chol_gpu.c:44-46)
  *   stats=1        print stage timings and rates to stderr
  *   io_threads=<k> parallel preads of each X_R block (default 4)
- *   dosage_scale=<s>  with precision=mixed: X_R values are dosages with a
+ *   dosage_scale=<s>  with precision=mixed (or with xr_format=i16): X_R values are dosages with a
  *                  fixed number of decimals (s = 1000 for 3 decimals), run as
  *                  two int8 digits per value (twice the int8 work of
  *                  genotypes); a value that is not a multiple of 1/s is an
@@ -68,6 +68,10 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *                  bits 2(r%4)..2(r%4)+1 of byte r/4, codes 0/1/2 = the
  *                  genotype, code 3 is an FGLS_ERR_ARG): 32x fewer bytes than
  *                  FP64; unpacked to int8 on the device (fgls_hip_u2_to_i8)
+ *   xr_format=i16  with dosage_scale=<s>: XR_path holds int16 values
+ *                  v = x * s (2 bytes per value, the FP64 file's order),
+ *                  4x fewer bytes than FP64; the mixed path splits them into
+ *                  its two digit planes, the FP64 path uses x = v / s
  *   traits_per_pass=<k>  traits solved per read of each X_R block (default:
  *                  all t traits when their contexts fit in 60 % of the free
  *                  device memory; the reference re-reads X_R for every

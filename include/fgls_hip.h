@@ -182,6 +182,15 @@ int fgls_hip_solve_block_i8t(fgls_hip_ctx *ctx, const int8_t *dX8T, int count, i
  * reference (its X_R is FP64 only, io.c:35-55).  Asynchronous. */
 int fgls_hip_u2_to_i8(const uint8_t *dP, int64_t ldp, int n, int ncols,
                       int8_t *dX8, int64_t ldx8, unsigned int *dbad, void *stream);
+/* 16-bit dosage values v = x * scale (column-major int16, ld ldv elements)
+ * -> the two int8 digit planes of fgls_hip_solve_block_dosage (hi plane at
+ * dX8 + ncols * ldx8; v outside [-16256, 16383] counted into *dbad), or ->
+ * FP64 x = v / scale (ld ldx).  For compact dosage files; no counterpart in
+ * the reference (its X_R is FP64 only, io.c:35-55).  Asynchronous. */
+int fgls_hip_i16_to_dosage_i8(const int16_t *dV, int64_t ldv, int n, int ncols,
+                              int8_t *dX8, int64_t ldx8, unsigned int *dbad, void *stream);
+int fgls_hip_i16_to_f64(const int16_t *dV, int64_t ldv, int n, int ncols, double scale,
+                        double *dX, int64_t ldx, void *stream);
 /* int8 genotypes (ld ldx8) -> FP64 (ld ldx, rows n..ldx-1 zeroed), for
  * compact genotype input on the FP64 path.  No counterpart in the
  * reference (its X_R is FP64 only, io.c:35-55).  Asynchronous. */

@@ -141,3 +141,15 @@ def test_pack_u2_layout(pkg):
     for bad in (np.array([[3]]), np.array([[-1]]), np.array([[0.5]])):
         with pytest.raises(pkg.FglsError):
             pkg.pack_u2(bad)
+
+
+def test_pack_i16_layout(pkg):
+    """The 16-bit dosage layout of xr_format=i16: SNP-major int16 v = rint(1000 x);
+    values that are not multiples of 1/1000, or out of range, are rejected."""
+    X = np.array([[0.0, 1.999], [0.5, 0.001], [2.0, 1.234]])
+    P = pkg.pack_i16(X, 1000.0)
+    assert P.dtype == np.int16 and P.shape == (2, 3)
+    assert P.tolist() == [[0, 500, 2000], [1999, 1, 1234]]
+    for bad in (np.array([[0.0005]]), np.array([[17.0]])):
+        with pytest.raises(pkg.FglsError):
+            pkg.pack_i16(bad, 1000.0)

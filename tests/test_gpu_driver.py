@@ -307,3 +307,46 @@ def test_pipeline_edge_sizes(pkg, gpu, tmp_path, variant, precision):
             Bo, Vo = orc.fgls_chol(prob.Phi, H, Sig, prob.XL, prob.XR[:, :m], prob.Y)
             e = orc.parity_errors(B, V, Bo, Vo, p)
             assert e["B_scaled"] < 1e-10 and e["V_scaled"] < 1e-10 and e["V_upper_zero"], (n, m, e)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_i16_dosage_file(pkg, gpu, tmp_path, precision):
+    """xr_format=i16 (16-bit dosage file, v = 1000 x) gives exactly the files
+    of the FP64 dosage file (x = round(U(0, 2), 3)), for both variants; a value
+    out of range is a defined error."""
+    print(f"16-bit dosage file, precision={precision}", flush=True)
+    n, m, t, wXL = 333, 700, 2, 3
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=29, t=t, dosage="imputed")
+    d = str(tmp_path)
+    H, Sig = np.array([0.3, 0.55]), np.array([1.0, 1.2])
+    write_problem_dir(prob, d, H=H, Sig=Sig)
+    f = lambda s: os.path.join(d, s)
+    pkg.pack_i16(prob.XR, 1000.0).tofile(f("XR16.in"))
+    assert os.path.getsize(f("XR16.in")) == 2 * n * m
+    p = wXL + 1
+    base = {"precision": precision, "block": 256}
+    if precision == "mixed":
+        base["dosage_scale"] = 1000
+    for run in (pkg.fgls_chol_gpu, pkg.fgls_eigen):
+        out = {}
+        for fmt, xr in (("f64", "XR.in"), ("i16", "XR16.in")):
+            opts = dict(base, xr_format=fmt)
+            if fmt == "i16":
+                opts["dosage_scale"] = 1000
+            run(n, p, m, t, wXL, 1, 256, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"), f("XL.in"), f(xr),
+                f("Y.in"), f(f"B_{fmt}.out"), f(f"V_{fmt}.out"), options=opts)
+            out[fmt] = (read_out(d, f"B_{fmt}.out", t * m * p), read_out(d, f"V_{fmt}.out", t * m * p * p))
+        assert np.array_equal(out["i16"][0], out["f64"][0]), run.__name__
+        assert np.array_equal(out["i16"][1], out["f64"][1]), run.__name__
+    Bo, Vo = orc.fgls_chol(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y)
+    e = orc.parity_errors(out["i16"][0], out["i16"][1], Bo, Vo, p)
+    assert e["B_scaled"] < 2e-11 and e["V_scaled"] < 2e-11, e
+    if precision == "mixed":
+        bad = np.fromfile(f("XR16.in"), dtype=np.int16)
+        bad[5 * n + 7] = 20000
+        bad.tofile(f("XR16bad.in"))
+        with pytest.raises(pkg.FglsError) as ei:
+            pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, 256, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"), f("XL.in"),
+                              f("XR16bad.in"), f("Y.in"), f("B.out"), f("V.out"),
+                              options=dict(base, xr_format="i16", dosage_scale=1000))
+        assert ei.value.code == pkg.FGLS_ERR_ARG and "outside" in ei.value.msg
