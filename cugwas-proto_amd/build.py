@@ -68,7 +68,9 @@ def build(verbose: bool = False) -> dict:
         # the int8 kernel keeps its MFMA accumulators in VGPRs (see fgls_i8.hip)
         "fgls_i8": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
         "fgls_adjoint": [],
-        "fgls_potrf": [],
+        # the 64 x 64 leaf of the factorisation keeps its row and column in
+        # registers: its loops must unroll completely (see fgls_potrf.hip)
+        "fgls_potrf": ["-mllvm", "-pragma-unroll-threshold=500000"],
         "fgls_eigen": [],
     }
     hip_objs = []

@@ -41,7 +41,6 @@ struct fgls_hip_ctx {
   double* part;
   double* z;      // lazily allocated (generic per-SNP path, adjoint)
   fgls_adjoint_state* adj;  // lazily allocated (adjoint)
-  double* panel;  // lazily allocated (device Cholesky of M): (ldn + 128) x 64 row-major
   unsigned long long* err;
   int* status;
 };
@@ -122,6 +121,13 @@ constexpr int TRAIT_LD = 2 * PMAX + PMAX * PMAX + 8;
 // b and V of every SNP of a block from its transformed columns Z.
 void launch_snp_stage_any(const fgls_hip_ctx* c, const double* Z, int64_t ldz, int count,
                           double* dB, double* dV, int64_t base, hipStream_t st);
+
+// T = L^-1 (row-major, ctx->Linv) by recursive inversion on MFMA products;
+// with `factor`, first the Cholesky factor of the matrix in ctx->L (both
+// triangles), written over it (info of a failed pivot in ctx->status[1]).
+// Without it, ctx->L holds the factor in its column-major lower triangle;
+// its strict upper triangle is used as workspace.
+int chol_inv(fgls_hip_ctx* c, bool factor, hipStream_t st);
 
 // The per-trait precompute from the factor already in ctx->L (L^-1, W,
 // B_t, V_tl, res_sigma, slices); synchronizes `st`.  With potrf_info set,

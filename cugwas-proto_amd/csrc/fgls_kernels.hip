@@ -98,42 +98,33 @@ __global__ __launch_bounds__(64) void k_diag_inv(int n, const double* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Blocked left-looking TRSM, L Z = B, one workgroup per panel of NC (64 or
-// 32) columns.
+// Blocked left-looking TRSM, L Z = B in place (B column-major, ld ldb), one
+// workgroup per panel of 64 columns.
 // For every 64-row block I:   R = B_I - sum_{J<I} L_IJ Z_J   (FP64 MFMA)
 //                             Z_I = Dinv_I R                 (FP64 MFMA, R in LDS)
-// IDENT: B is the identity, panel P's rows above its first column are zero
-// (L^-1 is lower triangular), so the sweep starts at the block holding it.
-// The sweep over row blocks is sequential inside a panel: narrower panels
-// (NC = 32) give more workgroups and a shorter sweep per workgroup.
-// ROWMAJOR: Z(r, c) is stored at Z[r*ldz + c] (used for Linv), else at
-// Z[c*ldz + r]; B is always column-major and may alias Z (in place) when
-// !ROWMAJOR.
+// The sweep over row blocks is sequential inside a panel.
 // ---------------------------------------------------------------------------
-template <bool IDENT, bool ROWMAJOR, int NC = NB>
 __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const double* __restrict__ L,
                                                     int64_t ldl, const double* __restrict__ Dinv,
-                                                    const double* B, int64_t ldb, double* Z,
-                                                    int64_t ldz) {
-  constexpr int TJ = NC / 16;
+                                                    double* Z, int64_t ldz) {
+  constexpr int NC = NB, TJ = NC / 16;
   __shared__ double Rs[NB][NC + 2];
   const int P = blockIdx.x;
   const int c0 = P * NC;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int nblk = (n + NB - 1) / NB;
-  const int Istart = IDENT ? c0 / NB : 0;
   int colc[TJ];
 #pragma unroll
   for (int tj = 0; tj < TJ; ++tj) colc[tj] = min(c0 + 16 * tj + li, nrhs - 1);
 
-  for (int I = Istart; I < nblk; ++I) {
+  for (int I = 0; I < nblk; ++I) {
     v4d acc[TJ];
 #pragma unroll
     for (int tj = 0; tj < TJ; ++tj) acc[tj] = v4d{0.0, 0.0, 0.0, 0.0};
     const int rowA = I * NB + 16 * w + li;
     const bool rowok = rowA < n;
-    for (int k0 = Istart * NB; k0 < I * NB; k0 += 16) {
+    for (int k0 = 0; k0 < I * NB; k0 += 16) {
       double a[4];
 #pragma unroll
       for (int s = 0; s < 4; ++s)
@@ -141,13 +132,8 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const doubl
       double b[TJ][4];
 #pragma unroll
       for (int tj = 0; tj < TJ; ++tj) {
-        if (ROWMAJOR) {
 #pragma unroll
-          for (int s = 0; s < 4; ++s) b[tj][s] = Z[(size_t)(k0 + 4 * lg + s) * ldz + colc[tj]];
-        } else {
-#pragma unroll
-          for (int s = 0; s < 4; ++s) b[tj][s] = Z[(size_t)colc[tj] * ldz + k0 + 4 * lg + s];
-        }
+        for (int s = 0; s < 4; ++s) b[tj][s] = Z[(size_t)colc[tj] * ldz + k0 + 4 * lg + s];
       }
 #pragma unroll
       for (int s = 0; s < 4; ++s)
@@ -161,9 +147,7 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const doubl
       for (int v = 0; v < 4; ++v) {
         const int r = 16 * w + lg + 4 * v, c = 16 * tj + li;
         const int grow = I * NB + r, gcol = c0 + c;
-        double bv;
-        if (IDENT) bv = (grow == gcol) ? 1.0 : 0.0;
-        else bv = (grow < n && gcol < nrhs) ? B[(size_t)gcol * ldb + grow] : 0.0;
+        const double bv = (grow < n && gcol < nrhs) ? Z[(size_t)gcol * ldz + grow] : 0.0;
         Rs[r][c] = bv - acc[tj][v];
       }
     }
@@ -188,10 +172,7 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const doubl
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int grow = I * NB + 16 * w + lg + 4 * v, gcol = c0 + 16 * tj + li;
-        if (grow < n && gcol < nrhs) {
-          if (ROWMAJOR) Z[(size_t)grow * ldz + gcol] = zc[tj][v];
-          else Z[(size_t)gcol * ldz + grow] = zc[tj][v];
-        }
+        if (grow < n && gcol < nrhs) Z[(size_t)gcol * ldz + grow] = zc[tj][v];
       }
     }
     // Z_I is read back by other waves of this workgroup in the next blocks.
@@ -1115,10 +1096,10 @@ int finish_trait(fgls_hip_ctx* c, const double* dXL, int64_t ldxl, const double*
                              (size_t)n * sizeof(double), (size_t)wXL, hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipMemcpyAsync(c->Wtmp + (size_t)wXL * ldn, dy, (size_t)n * sizeof(double),
                          hipMemcpyDeviceToDevice, st));
+  // L^-1 (already formed with the device factor)
+  if (!potrf_info && (rc = chol_inv(c, false, st))) return rc;
+  // inverses of L's 64 x 64 diagonal blocks, for fgls_hip_trsm
   hipLaunchKernelGGL(k_diag_inv, dim3(c->nblk), dim3(64), 0, st, n, c->L, (int64_t)ldn, c->Dinv);
-  hipLaunchKernelGGL((k_trsm_panel<true, true, 32>), dim3((n + 31) / 32), dim3(256), 0, st, n, n,
-                     c->L, (int64_t)ldn, c->Dinv, (const double*)nullptr, (int64_t)0, c->Linv,
-                     (int64_t)ldn);
   // [X~_L | y~] = L^-1 [X_L | y] through the explicit inverse: wXL + 1 columns
   // of the triangular product spread over all row tiles (a TRSM would run
   // the n rows one after another in a single workgroup)
@@ -1217,7 +1198,7 @@ int fgls_hip_ctx_destroy(fgls_hip_ctx* c) {
   hipSetDevice(c->device);
   hipDeviceSynchronize();
   void* ptrs[] = {c->L, c->Linv, c->Dinv, c->W, c->Wtmp, c->trait, c->part, c->z, c->err, c->status,
-                  c->A8, c->invsc, c->bad, c->panel, c->X8t};
+                  c->A8, c->invsc, c->bad, c->X8t};
   for (void* p : ptrs)
     if (p) hipFree(p);
   fgls_adjoint_free(c->adj);
@@ -1288,8 +1269,8 @@ int fgls_hip_trsm(fgls_hip_ctx* c, double* dX, int64_t ldx, int ncols, void* str
   if (!dX || ldx < c->n) return fail(FGLS_ERR_ARG, "X needs ld >= n");
   if ((rc = set_device(c))) return rc;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL((k_trsm_panel<false, false>), dim3((ncols + NB - 1) / NB), dim3(256), 0, st,
-                     c->n, ncols, c->L, (int64_t)c->ldn, c->Dinv, dX, ldx, dX, ldx);
+  hipLaunchKernelGGL(k_trsm_panel, dim3((ncols + NB - 1) / NB), dim3(256), 0, st, c->n, ncols,
+                     c->L, (int64_t)c->ldn, c->Dinv, dX, ldx);
   HIP_TRY(hipGetLastError());
   return FGLS_OK;
 }

@@ -1,20 +1,45 @@
-// fgls_potrf.hip -- the per-trait Cholesky factor on the device:
-//   M = sigma2 (h2 Phi + (1 - h2) I) = L L'      (FP64, lower)
-// replacing the host M construction and dpotrf of fgls_chol_gpu.c:406-430.
+// fgls_potrf.hip -- the per-trait Cholesky factor and its inverse on the
+// device:
+//   M = sigma2 (h2 Phi + (1 - h2) I) = L L',   T = L^-1      (FP64, lower)
+// replacing the host M construction and dpotrf of fgls_chol_gpu.c:406-430 and
+// the triangular solves with L that the reference repeats for every block
+// (fgls_chol_gpu.c:493-499, 536-542), which this library replaces by
+// products with T.
 //
-// Two-level right-looking factorisation.  Outer panels of OB = 256 columns;
-// inside a panel, 64-column steps:
-//   k_chol_diag   factors the 64 x 64 diagonal block in LDS and forms the
-//                 inverse of its factor (one workgroup);
-//   k_chol_panel  L21 = A21 L11^-T for the rows below (64-row workgroups);
-//                 the rows are also written row-major to a panel buffer P
-//                 (ld 256) so that updates read 4 consecutive k per lane;
-//   k_chol_update A -= P P' on lower tiles: first the rest of the outer
-//                 panel (K = 64), then, once the outer panel is done, the
-//                 whole trailing matrix (K = 256: 4x fewer passes over it
-//                 than 64-column steps would make).
-// The first non-positive pivot is recorded as LAPACK's info (1-based order
-// of the failing leading minor); later panels still run on harmless values.
+// Recursive factorisation and inversion over the padded order N = ldn (a
+// multiple of 128), split at multiples of 64 (top-down, Gustavson's
+// recursion with the inverse carried along):
+//
+//   [S11  .  ]   [L11   0 ] [L11'  L21']        T11 = L11^-1
+//   [S21 S22 ] = [L21  L22] [ 0    L22'],       T22 = L22^-1
+//
+//   node(S):  node(S11)                  -> L11, T11
+//             L21  = S21 T11'                     (product, B upper)
+//             S22 -= L21 L21'                     (product, lower half)
+//             node(S22)                  -> L22, T22
+//             X    = L21 T11                      (product, B lower)
+//             T21  = -T22 X     (as T21' = -X' T22', product, B upper)
+//
+// 64 x 64 leaves factor and invert in one wave (k_leaf).  Every product runs
+// on k_tgemm, an FP64 MFMA kernel whose operands are both read along k
+// (A(r, k) at A[r ld + k], B(k, c) at B[c ld + k]) with a column-major
+// result.  To give every product such operands the two n x n buffers are
+// kept symmetric-stored while the recursion runs ("R-view" = the buffer read
+// row-major, element (i, j) at i ld + j):
+//   L buffer: the R-view lower triangle holds the Schur complements S, then
+//             L row-major; the R-view upper holds L column-major (the final
+//             factor; its strict upper triangle in column-major terms is L').
+//   T buffer: R-view lower = T row-major (the final L^-1), R-view upper =
+//             T column-major while the recursion runs (zeroed at the end).
+// The temporary X of a node lives in the T buffer's R-view upper-right block
+// of that node, which holds T21 column-major only once the node is done.
+// Rows and columns n..N-1 are identity in the leaves' arithmetic and zero in
+// the stored factor and inverse.  The first non-positive pivot is recorded as
+// LAPACK's info (1-based order of the failing leading minor); later leaves
+// still run on harmless values.
+//
+// Flops: n^3/3 for the factor and n^3/3 for the inverse; 3/4 of them are in
+// the two top levels' products, which fill the device.
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -27,10 +52,6 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int PB = 64;   // inner panel width
-constexpr int OB = 256;  // outer panel width (ld of P)
-constexpr int UT = 128;  // update tile (2 x 2 waves of 64 x 64)
-
 using fgls::fail;
 
 #define HIP_TRY(call)                                                                   \
@@ -42,6 +63,18 @@ using fgls::fail;
 
 __device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ v4d ld4(const double* p) {
+  return *reinterpret_cast<const v4d*>(p);
+}
+
+// Value of x in lane l (l uniform), as a wave-uniform scalar.
+__device__ __forceinline__ double rlane(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // M (ldn x ldn, column-major) = alpha Phi + beta I on the n x n block, zero
@@ -57,202 +90,330 @@ __global__ __launch_bounds__(256) void k_build_M(int n, int ldn, const double* _
   }
 }
 
-// Diagonal block at column j (nb = min(64, n - j) columns; the block is
-// completed with the identity to 64): unblocked Cholesky in LDS, then the
-// inverse of the factor by column-parallel forward substitution.  Writes the
-// factor back (strict upper part of the block zeroed) and the inverse,
-// column-major 64 x 64, to Dinv_j.
-__global__ __launch_bounds__(256) void k_chol_diag(int n, int j, double* __restrict__ L, int64_t ldl,
-                                                   double* __restrict__ Dinv_j, int* __restrict__ info) {
-  __shared__ double A[PB][PB + 1];
-  __shared__ double Di[PB][PB + 1];
-  const int tid = threadIdx.x;
-  const int nb = min(PB, n - j);
-  for (int idx = tid; idx < PB * PB; idx += 256) {
-    const int r = idx % PB, c = idx / PB;
-    double v;
-    if (r < nb && c < nb) v = (r >= c) ? L[(size_t)(j + c) * ldl + j + r] : 0.0;
-    else v = (r == c) ? 1.0 : 0.0;
-    A[r][c] = v;
+// ---------------------------------------------------------------------------
+// 64 x 64 diagonal block at o = o0 + 64 blockIdx.x, one wave.
+// FACTOR: lane i holds row i of S (from the L buffer's R-view lower
+//         triangle) in registers and factors it right-looking; at step k
+//         every lane publishes its entry of column k in LDS and reads the
+//         column back as wave-uniform broadcasts.
+// else:   lane i holds row i of the caller's factor (column-major lower
+//         triangle).
+// Then T = L^-1 by forward substitution with lane c owning column c of T (the
+// rows of L read from LDS as broadcasts).  L (FACTOR) and T are written to
+// both triangles of their buffers through LDS, so that every store
+// instruction writes one contiguous 512-byte row or column segment.
+// Rows >= n are identity rows; their stored values are zero.  All loops
+// unroll completely (the file is built with a raised
+// -pragma-unroll-threshold) so that a[] and t[] stay in registers.
+// ---------------------------------------------------------------------------
+template <bool FACTOR>
+__global__ __launch_bounds__(64) void k_leaf(int n, int o0, double* __restrict__ Lb,
+                                             double* __restrict__ Tb, int64_t ld,
+                                             int* __restrict__ info) {
+  __shared__ double Ls[64][66];
+  __shared__ double col[2][64];
+  __shared__ double rcp[64];
+  const int o = o0 + 64 * (int)blockIdx.x;
+  const int i = threadIdx.x;
+  const int gi = o + i;
+  const bool live = gi < n;
+  const int nl = min(64, n - o);   // live rows of the block
+  double a[64];
+  if (FACTOR) {
+    const double* row = Lb + (size_t)gi * ld + o;
+#pragma unroll
+    for (int j4 = 0; j4 < 16; ++j4) {
+      const v4d v = ld4(row + 4 * j4);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[4 * j4 + q] = v[q];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 64; ++j) a[j] = Lb[(size_t)(o + j) * ld + gi];
   }
-  __syncthreads();
-  for (int k = 0; k < PB; ++k) {
-    if (tid == 0) {
-      double d = A[k][k];
+#pragma unroll
+  for (int j = 0; j < 64; ++j) a[j] = live ? (j <= i ? a[j] : 0.0) : (j == i ? 1.0 : 0.0);
+
+  if (FACTOR) {
+    int bad = 0;
+#pragma unroll
+    for (int k = 0; k < 64; ++k) {
+      double* cb = col[k & 1];
+      cb[i] = a[k];   // S(i, k) after k updates
+      __syncthreads();
+      double d = cb[k];
       if (!(d > 0.0)) {
-        if (*info == 0) *info = j + k + 1;
+        if (bad == 0) bad = o + k + 1;
         d = 1.0;
       }
-      A[k][k] = sqrt(d);
+      const double p = sqrt(d), ip = 1.0 / p, ip2 = ip * ip;
+      // S(i, j) -= L(i, k) L(j, k) = S(i, k) S(j, k) / d for j > k; lanes
+      // i < j update entries above their diagonal, which are never read
+      const double f = a[k] * ip2;
+#pragma unroll
+      for (int j = 0; j < 64; ++j)
+        if (j > k) a[j] = fma(-f, cb[j], a[j]);
+      a[k] = (i > k) ? a[k] * ip : ((i == k) ? p : 0.0);
     }
-    __syncthreads();
-    const double dk = A[k][k];
-    for (int i = k + 1 + tid; i < PB; i += 256) A[i][k] /= dk;
-    __syncthreads();
-    const int m = PB - 1 - k;
-    for (int p = tid; p < m * m; p += 256) {
-      const int i = k + 1 + p / m, l = k + 1 + p % m;
-      if (l <= i) A[i][l] -= A[i][k] * A[l][k];
-    }
-    __syncthreads();
+    if (bad != 0 && i == 0 && *info == 0) *info = bad;
   }
-  // thread t < 64 owns column t of the inverse: A * d = e_t
-  if (tid < PB) {
-    const int t = tid;
-    for (int row = 0; row < PB; ++row) {
-      double d = 0.0;
-      if (row >= t) {
-        double s = (row == t) ? 1.0 : 0.0;
-        for (int q = t; q < row; ++q) s -= A[row][q] * Di[q][t];
-        d = s / A[row][row];
+
+#pragma unroll
+  for (int j = 0; j < 64; ++j) Ls[i][j] = (j <= i) ? a[j] : 0.0;
+  __syncthreads();
+  if (FACTOR) {
+    // R-view lower (L row-major): row o + r, lanes over its columns
+    for (int r = 0; r < nl; ++r)
+      if (i <= r) Lb[(size_t)(o + r) * ld + o + i] = Ls[r][i];
+    // R-view upper (L column-major): row o + j holds column j, lanes over rows
+    for (int j = 0; j < nl; ++j)
+      if (i >= j && i < nl) Lb[(size_t)(o + j) * ld + o + i] = Ls[i][j];
+  }
+  rcp[i] = 1.0 / Ls[i][i];
+  __syncthreads();
+  // lane c: column c of T, T(r, c) = (delta_rc - sum_{q<r} L(r, q) T(q, c)) / L(r, r);
+  // T(q, c) = 0 for q < c, so the sum is empty at r = c
+  const int c = i;
+  double t[64];
+#pragma unroll
+  for (int r = 0; r < 64; ++r) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q0 = 0; q0 < 64; q0 += 16) {
+      if (q0 < r) {
+#pragma unroll
+        for (int q = q0; q < q0 + 16; ++q)
+          if (q < r) s[q & 3] = fma(Ls[r][q], t[q], s[q & 3]);
+        // keep the broadcast loads of later chunks from piling up in registers
+        __builtin_amdgcn_sched_barrier(0);
       }
-      Di[row][t] = d;
     }
+    const double sum = (s[0] + s[1]) + (s[2] + s[3]);
+    t[r] = (r == c) ? rcp[r] : -sum * rcp[r];
   }
+  __syncthreads();   // every lane is done reading L
+#pragma unroll
+  for (int r = 0; r < 64; ++r) Ls[r][c] = (r >= c && o + r < n) ? t[r] : 0.0;
   __syncthreads();
-  for (int idx = tid; idx < PB * PB; idx += 256) {
-    const int r = idx % PB, c = idx / PB;
-    if (r < nb && c < nb) L[(size_t)(j + c) * ldl + j + r] = (r >= c) ? A[r][c] : 0.0;
-    Dinv_j[(size_t)c * PB + r] = Di[r][c];
+  // R-view lower (T row-major) and upper (T column-major), zeros for rows >= n
+  for (int r = 0; r < 64; ++r) {
+    if (i <= r) Tb[(size_t)(o + r) * ld + o + i] = Ls[r][i];
+    if (i >= r) Tb[(size_t)(o + r) * ld + o + i] = Ls[i][r];
   }
 }
 
-// L21 = A21 L11^-T for rows [i0, i0 + 64) below the diagonal block at j
-// (j + 64 < n, so the panel is full); also P[i * OB + poff + c] = L21(i, c).
-__global__ __launch_bounds__(256) void k_chol_panel(int n, int j, double* __restrict__ L, int64_t ldl,
-                                                    const double* __restrict__ Dinv_j,
-                                                    double* __restrict__ P, int poff) {
-  __shared__ double A[PB][PB + 1];
-  __shared__ double Di[PB][PB + 1];
-  const int tid = threadIdx.x;
-  const int i0 = j + PB + blockIdx.x * PB;
-  const int rows = min(PB, n - i0);
-  for (int idx = tid; idx < PB * PB; idx += 256) {
-    const int r = idx % PB, c = idx / PB;
-    A[r][c] = (r < rows) ? L[(size_t)(j + c) * ldl + i0 + r] : 0.0;
-    Di[r][c] = Dinv_j[(size_t)c * PB + r];
-  }
-  __syncthreads();
-  // thread: row r = tid % 64, columns [16 (tid / 64), +16)
-  const int r = tid % PB, c0 = 16 * (tid / PB);
-  double x[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) x[q] = 0.0;
-  for (int k = 0; k < PB; ++k) {
-    const double a = A[r][k];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) x[q] = fma(a, Di[c0 + q][k], x[q]);  // (L11^-T)(k, c) = L11^-1(c, k)
-  }
-  if (r < rows) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) L[(size_t)(j + c0 + q) * ldl + i0 + r] = x[q];
-    v4d* prow = reinterpret_cast<v4d*>(P + (size_t)(i0 + r) * OB + poff + c0);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) prow[q] = v4d{x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]};
-  }
-}
+// ---------------------------------------------------------------------------
+// C = alpha A B (+ C with ACC) on an M x N block, K terms, FP64 MFMA, over
+// 128 x 128 tiles (2 x 2 waves of 64 x 64).  A(r, k) = A[r ld + k],
+// B(k, c) = B[c ld + k], C(r, c) = C[c ld + r]; M, N, K multiples of 64.
+// TRI selects the terms and tiles that can be non-zero:
+//   TG_BLOWER  B lower triangular (k >= c): k from the wave's first column;
+//   TG_BUPPER  B upper triangular (k <= c): k up to the wave's last column;
+//   TG_CUPPER  only the blocks with r <= c (one triangle of a symmetric C).
+// The triangular B is a diagonal block of a symmetric-stored buffer, so its
+// other triangle is not zero: inside the wave's 64 x 64 diagonal block the
+// terms of that triangle are masked.
+// Grid: (row tiles, column tiles), heaviest column tiles first.  Operand
+// fetch and MFMA order as the triangular product of the block loop: one
+// 32-byte load per lane and operand per K = 16 step, one step ahead.
+// ---------------------------------------------------------------------------
+enum { TG_DENSE = 0, TG_BLOWER = 1, TG_BUPPER = 2, TG_CUPPER = 3 };
 
-// A(row, col) -= sum_{k<K} P(row, k) P(col, k) for col in [c0, c1), row in
-// [c0, n), row >= col; P row-major with ld OB (the columns of the current
-// panel).  Grid: (row tiles, column tiles) of 128 from c0; tiles entirely
-// above the diagonal exit.  MFMA operands as in the FP64 product: lane group
-// g supplies k = 4g + s, one 32-byte load per lane and operand; swapped
-// operands put 16 lanes on 16 consecutive rows of one column for the
-// read-modify-write.
-template <int K>
-__global__ __launch_bounds__(256) void k_chol_update(int n, int c0g, int c1g, double* __restrict__ L,
-                                                     int64_t ldl, const double* __restrict__ P) {
-  const int ti = blockIdx.x, tj = blockIdx.y;
-  if (tj > ti) return;
+template <int TRI, bool ACC>
+__global__ __launch_bounds__(256) void k_tgemm(int M, int N, int K, double alpha,
+                                               const double* __restrict__ A,
+                                               const double* __restrict__ B,
+                                               double* __restrict__ C, int64_t ld) {
+  const int CT = (N + 127) / 128;
+  const int rt = blockIdx.x;
+  const int ct = (TRI == TG_BUPPER) ? CT - 1 - (int)blockIdx.y : (int)blockIdx.y;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wr = w >> 1, wc = w & 1;
   const int li = lane & 15, lg = lane >> 4;
-  const int r0 = c0g + ti * UT + wr * 64;
-  const int c0 = c0g + tj * UT + wc * 64;
-  if (r0 >= n || c0 >= c1g || c0 > r0 + 63) return;   // nothing of the region here
+  const int r0 = rt * 128 + wr * 64;
+  const int c0 = ct * 128 + wc * 64;
+  if (r0 >= M || c0 >= N) return;
+  if (TRI == TG_CUPPER && r0 > c0) return;
+  // unmasked range [kb, ke); a triangular B adds the diagonal block [c0, c0 + 64)
+  const int kb = (TRI == TG_BLOWER) ? c0 + 64 : 0;
+  const int ke = (TRI == TG_BUPPER) ? c0 : K;
+
   const double* Ap[4];
   const double* Bp[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    Ap[q] = P + (size_t)(r0 + 16 * q + li) * OB + 4 * lg;
-    Bp[q] = P + (size_t)(c0 + 16 * q + li) * OB + 4 * lg;
-  }
+  for (int ti = 0; ti < 4; ++ti) Ap[ti] = A + (size_t)(r0 + 16 * ti + li) * ld + 4 * lg;
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) Bp[tj] = B + (size_t)(c0 + 16 * tj + li) * ld + 4 * lg;
+
   v4d acc[4][4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[a][c] = v4d{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 2
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    v4d av[4], bv[4];
+    for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  v4d a[4], b[4];
+  auto mfma_step = [&]() {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      av[q] = *reinterpret_cast<const v4d*>(Ap[q] + k0);
-      bv[q] = *reinterpret_cast<const v4d*>(Bp[q] + k0);
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = mfma_f64(b[tj][s], a[ti][s], acc[ti][tj]);
+  };
+  // the 4 K-steps of the diagonal block, B's other triangle masked
+  auto diag_steps = [&]() {
+#pragma unroll
+    for (int k0 = c0; k0 < c0 + 64; k0 += 16) {
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) a[ti] = ld4(Ap[ti] + k0);
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        b[tj] = ld4(Bp[tj] + k0);
+        const int c = c0 + 16 * tj + li;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int k = k0 + 4 * lg + s;
+          const bool keep = (TRI == TG_BUPPER) ? (k <= c) : (k >= c);
+          b[tj][s] = keep ? b[tj][s] : 0.0;
+        }
+      }
+      mfma_step();
     }
+  };
+
+  if (TRI == TG_BLOWER) diag_steps();
+  if (kb < ke) {
 #pragma unroll
-    for (int s2 = 0; s2 < 4; ++s2)
+    for (int ti = 0; ti < 4; ++ti) a[ti] = ld4(Ap[ti] + kb);
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+    for (int tj = 0; tj < 4; ++tj) b[tj] = ld4(Bp[tj] + kb);
+    for (int k0 = kb + 16; k0 < ke; k0 += 16) {
+      v4d an[4], bn[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[a][c] = mfma_f64(bv[c][s2], av[a][s2], acc[a][c]);
+      for (int ti = 0; ti < 4; ++ti) an[ti] = ld4(Ap[ti] + k0);
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) bn[tj] = ld4(Bp[tj] + k0);
+      mfma_step();
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) a[ti] = an[ti];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) b[tj] = bn[tj];
+    }
+    mfma_step();
   }
-  // element v of acc[a][c] is (row r0 + 16a + li, column c0 + 16c + lg + 4v)
+  if (TRI == TG_BUPPER) diag_steps();
+
+  // element v of acc[ti][tj] is (row r0+16ti+li, column c0+16tj+lg+4v): 16
+  // lanes write 16 consecutive rows of one column
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int tj = 0; tj < 4; ++tj) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int col = c0 + 16 * c + lg + 4 * v;
-      if (col >= c1g) continue;
+      double* Cc = C + (size_t)(c0 + 16 * tj + lg + 4 * v) * ld + r0 + li;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const int row = r0 + 16 * a + li;
-        if (row < n && row >= col) L[(size_t)col * ldl + row] -= acc[a][c][v];
+      for (int ti = 0; ti < 4; ++ti) {
+        const double z = alpha * acc[ti][tj][v];
+        Cc[16 * ti] = ACC ? Cc[16 * ti] + z : z;
       }
     }
   }
 }
 
-template <int K>
-void launch_update(int n, int c0, int c1, double* L, int64_t ldl, const double* P, hipStream_t st) {
-  if (c0 >= n || c1 <= c0) return;
-  const int tr = (n - c0 + UT - 1) / UT, tc = (c1 - c0 + UT - 1) / UT;
-  hipLaunchKernelGGL(k_chol_update<K>, dim3(tr, tc), dim3(256), 0, st, n, c0, c1, L, ldl, P);
+// dst(c, r) = src(r, c) for an R x C block (64-multiples): src(r, c) at
+// src[r ld + c], written to dst[c ld + r].  Grid (R / 64, C / 64).
+__global__ __launch_bounds__(256) void k_tcopy(const double* __restrict__ src,
+                                               double* __restrict__ dst, int64_t ld) {
+  __shared__ double t[64][65];
+  const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  for (int idx = threadIdx.x; idx < 4096; idx += 256) {
+    const int rr = idx >> 6, cc = idx & 63;
+    t[rr][cc] = src[(size_t)(r0 + rr) * ld + c0 + cc];
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < 4096; idx += 256) {
+    const int cc = idx >> 6, rr = idx & 63;
+    dst[(size_t)(c0 + cc) * ld + r0 + rr] = t[rr][cc];
+  }
+}
+
+// Zero the strict upper triangle of the row-major N x N matrix T.
+__global__ __launch_bounds__(256) void k_zero_upper(int N, double* __restrict__ T, int64_t ld) {
+  const int r = blockIdx.x;
+  for (int c = r + 1 + threadIdx.x; c < N; c += 256) T[(size_t)r * ld + c] = 0.0;
+}
+
+void tgemm(int tri, int M, int N, int K, double alpha, const double* A, const double* B, double* C,
+           int64_t ld, hipStream_t st) {
+  const dim3 grid((M + 127) / 128, (N + 127) / 128), blk(256);
+  if (tri == TG_BLOWER)
+    hipLaunchKernelGGL((k_tgemm<TG_BLOWER, false>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
+  else if (tri == TG_BUPPER)
+    hipLaunchKernelGGL((k_tgemm<TG_BUPPER, false>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
+  else   // the symmetric update, accumulated into C
+    hipLaunchKernelGGL((k_tgemm<TG_CUPPER, true>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
+}
+
+void tcopy(const double* src, double* dst, int R, int C, int64_t ld, hipStream_t st) {
+  hipLaunchKernelGGL(k_tcopy, dim3(R / 64, C / 64), dim3(256), 0, st, src, dst, ld);
+}
+
+// One node of the recursion on rows/columns [o, o + size) (see the top).
+void chol_inv_node(int n, int o, int size, bool factor, double* L, double* T, int64_t ld,
+                   int* info, hipStream_t st) {
+  if (size == 64) {
+    if (factor) hipLaunchKernelGGL(k_leaf<true>, dim3(1), dim3(64), 0, st, n, o, L, T, ld, info);
+    return;   // without FACTOR every leaf ran up front
+  }
+  const int s = (size <= 128) ? 64 : ((size / 2 + 127) / 128) * 128;
+  const int s2 = size - s;
+  chol_inv_node(n, o, s, factor, L, T, ld, info, st);
+  double* L21r = L + (size_t)(o + s) * ld + o;   // R-view lower-left: S21, then L21 row-major
+  double* L21c = L + (size_t)o * ld + o + s;     // R-view upper-right: L21 column-major
+  const double* T11 = T + (size_t)o * ld + o;
+  const double* T22 = T + (size_t)(o + s) * ld + o + s;
+  double* Xc = T + (size_t)o * ld + o + s;       // R-view upper-right: X, then T21 column-major
+  double* T21r = T + (size_t)(o + s) * ld + o;   // R-view lower-left: T21 row-major
+  if (factor) {
+    tgemm(TG_BUPPER, s2, s, s, 1.0, L21r, T11, L21c, ld, st);        // L21 = S21 T11'
+    tcopy(L21c, L21r, s, s2, ld, st);
+    tgemm(TG_CUPPER, s2, s2, s, -1.0, L21r, L21r, L + (size_t)(o + s) * ld + o + s, ld,
+          st);                                                               // S22 -= L21 L21'
+  } else {
+    tcopy(L21c, L21r, s, s2, ld, st);
+  }
+  chol_inv_node(n, o + s, s2, factor, L, T, ld, info, st);
+  tgemm(TG_BLOWER, s2, s, s, 1.0, L21r, T11, Xc, ld, st);            // X = L21 T11
+  tgemm(TG_BUPPER, s, s2, s2, -1.0, Xc, T22, T21r, ld, st);          // T21' = -X' T22'
+  tcopy(T21r, Xc, s2, s, ld, st);
 }
 
 int factor_on_device(fgls_hip_ctx* c, const double* dPhi, int64_t ldphi, double h2, double sigma2,
                      hipStream_t st) {
   const int n = c->n, ldn = c->ldn;
-  if (!c->panel) {
-    const size_t bytes = (size_t)(ldn + UT) * OB * sizeof(double);
-    if (hipMalloc((void**)&c->panel, bytes) != hipSuccess)
-      return fail(FGLS_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
-    HIP_TRY(hipMemset(c->panel, 0, bytes));
-  }
   HIP_TRY(hipMemsetAsync(c->status + 1, 0, sizeof(int), st));
   hipLaunchKernelGGL(k_build_M, dim3((ldn + 255) / 256, ldn), dim3(256), 0, st, n, ldn, dPhi, ldphi,
                      sigma2 * h2, sigma2 * (1.0 - h2), c->L);
-  for (int j = 0; j < n; j += OB) {
-    const int jend = min(j + OB, n);
-    for (int jj = j; jj < jend; jj += PB) {
-      double* Dj = c->Dinv + (size_t)(jj / PB) * PB * PB;
-      hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, st, n, jj, c->L, (int64_t)ldn, Dj,
-                         c->status + 1);
-      const int below = n - jj - PB;
-      if (below <= 0) break;
-      hipLaunchKernelGGL(k_chol_panel, dim3((below + PB - 1) / PB), dim3(256), 0, st, n, jj, c->L,
-                         (int64_t)ldn, (const double*)Dj, c->panel, jj - j);
-      // the rest of this outer panel, from this step's 64 columns
-      launch_update<PB>(n, jj + PB, jend, c->L, (int64_t)ldn, c->panel + (jj - j), st);
-    }
-    // the trailing matrix, from the whole outer panel
-    if (jend - j == OB) launch_update<OB>(n, jend, n, c->L, (int64_t)ldn, c->panel, st);
-  }
+  HIP_TRY(hipGetLastError());
+  return fgls::chol_inv(c, true, st);
+}
+
+}  // namespace
+
+namespace fgls {
+
+int chol_inv(fgls_hip_ctx* c, bool factor, hipStream_t st) {
+  const int n = c->n, N = c->ldn;
+  const int64_t ld = N;
+  if (!factor)
+    hipLaunchKernelGGL(k_leaf<false>, dim3(N / 64), dim3(64), 0, st, n, 0, c->L, c->Linv, ld,
+                       (int*)nullptr);
+  chol_inv_node(n, 0, N, factor, c->L, c->Linv, ld, c->status + 1, st);
+  hipLaunchKernelGGL(k_zero_upper, dim3(N), dim3(256), 0, st, N, c->Linv, ld);
   HIP_TRY(hipGetLastError());
   return FGLS_OK;
 }
 
-}  // namespace
+}  // namespace fgls
 
 extern "C" {
 

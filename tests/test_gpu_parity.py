@@ -180,15 +180,17 @@ def test_singular_snp_reports_index(pkg, gpu):
 
 def test_sigma_invariance_and_scaling(pkg, gpu):
     """b and V do not depend on sigma; scaling a SNP column by c scales its
-    coefficient by 1/c and its standard error by 1/|c| (closed form)."""
+    coefficient by 1/c and its standard error by 1/|c| (closed form).  The two
+    sigmas give different factors, so b and V agree to FP64 rounding of the
+    whole solve (element-wise relative, the driver's metric), not bitwise."""
     prob = make_problem(n=400, wXL=3, m=64, seed=13)
     _, ctx1 = setup(pkg, prob, sigma=1.0, max_block=64)
     _, ctx2 = setup(pkg, prob, sigma=3.7, max_block=64)
     X = pkg.colmajor(prob.XR)
     B1, V1 = ctx1.solve_block(X)
     B2, V2 = ctx2.solve_block(X)
-    assert orc.max_rel_err(B2.cpu().numpy(), B1.cpu().numpy()) < 1e-12
-    assert orc.max_rel_err(V2.cpu().numpy(), V1.cpu().numpy(), p=4, lower_only=True) < 1e-12
+    assert orc.max_rel_err(B2.cpu().numpy(), B1.cpu().numpy()) < 1e-11
+    assert orc.max_rel_err(V2.cpu().numpy(), V1.cpu().numpy(), p=4, lower_only=True) < 1e-11
     B3, V3 = ctx1.solve_block(pkg.colmajor(prob.XR * -2.5))
     assert orc.max_rel_err(B3.cpu().numpy()[:, 3] * -2.5, B1.cpu().numpy()[:, 3]) < 1e-12
     assert orc.max_rel_err(V3.cpu().numpy()[:, 15] * 2.5, V1.cpu().numpy()[:, 15]) < 1e-12

@@ -74,3 +74,52 @@ def test_device_factor_reports_first_bad_pivot(gpu, pkg, bad_col):
         assert f"dpotrf(M) failed (info: {info})" in ei.value.msg
         # the context stays usable with an SPD trait
         dc.set_trait_phi(_colmajor(np.eye(n)), 0.5, 1.0, _colmajor(XL), torch.ones(n, dtype=torch.float64).cuda())
+
+
+def _inverse_through_trmm(dc, n):
+    """L^-1 as the device product L^-1 I (fgls_hip_trmm), returned (n, n)."""
+    ldx = (n + 3) // 4 * 4
+    eye = torch.zeros((n, ldx), dtype=torch.float64)
+    eye[:, :n] = torch.eye(n, dtype=torch.float64)
+    Z = dc.trmm(eye.cuda(), n)
+    return Z[:, :n].cpu().numpy().T   # Z[c, r] = L^-1(r, c)
+
+
+@pytest.mark.parametrize("n", [5, 63, 64, 100, 257, 700, 1000, 2049])
+def test_device_inverse_of_factor(gpu, pkg, n):
+    """L^-1 formed with the device factor (recursive inversion): lower
+    triangular (zeros above the diagonal, exactly) and equal to the inverse of
+    LAPACK's factor; also L^-1 M L^-T = I."""
+    print(f"device factor + inverse n={n}", flush=True)
+    rng = np.random.default_rng(n + 11)
+    Phi = make_phi(rng, n)
+    h2, sigma2 = 0.6, 1.7
+    XL = np.column_stack([np.ones(n), rng.standard_normal((n, 2))])
+    with pkg.DeviceContext(n, 3, 1, max_block=64) as dc:
+        dc.set_trait_phi(_colmajor(Phi), h2, sigma2, _colmajor(XL),
+                         torch.from_numpy(rng.standard_normal(n)).cuda())
+        T = _inverse_through_trmm(dc, n)
+    M = orc.build_M(Phi, h2, sigma2)
+    Tref = np.linalg.inv(orc.cholesky_lower(M))
+    assert np.all(np.triu(T, 1) == 0.0)
+    assert np.linalg.norm(T - Tref) / np.linalg.norm(Tref) < 1e-13
+    assert np.abs(T @ M @ T.T - np.eye(n)).max() < 1e-12
+
+
+@pytest.mark.parametrize("n", [64, 129, 1000, 1500])
+def test_inverse_of_given_factor(gpu, pkg, n):
+    """fgls_hip_set_trait with the caller's factor: L^-1 by the same recursion
+    without the factorisation, whatever the factor's strict upper triangle
+    holds."""
+    print(f"inverse of a given factor n={n}", flush=True)
+    rng = np.random.default_rng(n + 3)
+    M = orc.build_M(make_phi(rng, n), 0.4, 1.0)
+    L = orc.cholesky_lower(M)
+    Lg = L + np.triu(rng.standard_normal((n, n)), 1)     # unspecified upper part
+    XL = np.column_stack([np.ones(n), rng.standard_normal((n, 2))])
+    with pkg.DeviceContext(n, 3, 1, max_block=64) as dc:
+        dc.set_trait(_colmajor(Lg), _colmajor(XL), torch.from_numpy(rng.standard_normal(n)).cuda())
+        T = _inverse_through_trmm(dc, n)
+    Tref = np.linalg.inv(L)
+    assert np.all(np.triu(T, 1) == 0.0)
+    assert np.linalg.norm(T - Tref) / np.linalg.norm(Tref) < 1e-13
