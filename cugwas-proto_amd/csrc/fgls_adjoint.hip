@@ -337,7 +337,7 @@ int fgls_hip_adjoint_begin(fgls_hip_ctx* c, const double* dPhi, int64_t ldphi, d
   HIP_TRY(hipGetLastError());
   if (c->mode == FGLS_MODE_MIXED_I8) {
     // H = G L^-1 row-major = (L^-T G)' : T = (L^-1)' as a row-major L^-T,
-    // then the dense product T G (G symmetric) stored column-major
+    // then the upper triangular product T G (G symmetric) stored column-major
     const size_t nn = (size_t)ldn * ldn;
     if (!a->H) {
       if (hipMalloc((void**)&a->H, nn * sizeof(double)) != hipSuccess ||
@@ -348,7 +348,7 @@ int fgls_hip_adjoint_begin(fgls_hip_ctx* c, const double* dPhi, int64_t ldphi, d
       if ((rc = fill(a->H, nn, 0.0, st))) return rc;
     }
     hipLaunchKernelGGL(k_transpose, dim3(ldn / 32, ldn / 32), dim3(256), 0, st, ldn, c->Linv, a->T);
-    fgls::launch_product_store(c, a->T, true, a->G, ldn, n, a->H, ldn, st);
+    fgls::launch_product_store_upper(c, a->T, a->G, ldn, n, a->H, ldn, st);
     fgls::launch_slices(ldn, a->H, a->H8, a->hinvsc, st);
     HIP_TRY(hipGetLastError());
     a->h8_ready = true;

@@ -83,6 +83,10 @@ __host__ __device__ inline int trait_res_off(int wXL) { return wXL + wXL * wXL; 
 // full row-major ldn x ldn matrix.
 void launch_product_store(const fgls_hip_ctx* c, const double* A, bool dense, const double* X,
                           int64_t ldx, int ncols, double* Z, int64_t ldz, hipStream_t st);
+// Z = A X for an upper triangular row-major ldn x ldn A (zeros below the
+// diagonal are skipped).
+void launch_product_store_upper(const fgls_hip_ctx* c, const double* A, const double* X,
+                                int64_t ldx, int ncols, double* Z, int64_t ldz, hipStream_t st);
 // Y = A X for a dense row-major A; per 128-row tile and column, the sums of
 // y * X[row, col] and y * ctx->W[:, k] go to ctx->part (p + 1 per column).
 void launch_product_dot(const fgls_hip_ctx* c, const double* A, const double* X, int64_t ldx,

@@ -185,7 +185,8 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const doubl
 // Triangular product Z = Linv * X over 128x128 tiles (2x2 waves of 64x64),
 // FP64 MFMA, operands fetched straight into registers (one K=16 step ahead).
 // Linv is row-major (ld ldli >= RT*128) and lower triangular: row tile rt
-// only needs k < (rt+1)*128 (DENSE: a full row-major matrix, all k < n).
+// only needs k < (rt+1)*128 (DENSE: a full row-major matrix, all k < n;
+// UPPER: upper triangular, k from the wave's first row).
 // X is column-major with ld ldx (% 4 == 0); rows k >= n are masked.
 // Epilogue:
 //   EPI_STORE : Z written column-major (ld ldz)
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const doubl
 // ---------------------------------------------------------------------------
 enum { EPI_STORE = 0, EPI_REDUCE = 1, EPI_DOT = 2, EPI_MULTI = 3 };
 
-template <int EPI, int NQ, bool DENSE = false>
+template <int EPI, int NQ, bool DENSE = false, bool UPPER = false>
 __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
                                               const double* __restrict__ Linv, int64_t ldli,
                                               const double* __restrict__ X, int64_t ldx,
@@ -208,15 +209,18 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
                                               double* __restrict__ part, int64_t ldp,
                                               int ntr = 1) {
   const int ct = blockIdx.x;
-  const int rt = RT - 1 - (int)blockIdx.y;
+  const int rt = UPPER ? (int)blockIdx.y : RT - 1 - (int)blockIdx.y;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wr = w >> 1, wc = w & 1;
   const int li = lane & 15, lg = lane >> 4;
   const int r0 = rt * TM + wr * 64;
   const int c0 = ct * TN + wc * 64;
   const int n16 = (n + 15) & ~15;
-  const int kend = (r0 < n16) ? (DENSE ? n16 : min(r0 + 64, n16)) : 0;   // multiple of 16
+  const int kend = (r0 < n16) ? ((DENSE || UPPER) ? n16 : min(r0 + 64, n16)) : 0;   // multiple of 16
   const int kfull = min(kend, n & ~15);                   // steps with no masking
+  // UPPER: an upper triangular A (zeros below the diagonal): k from the
+  // wave's first row
+  const int kb = (UPPER && r0 < kfull) ? r0 : 0;
 
   const double* Ap[4];
   const double* Bp[4];
@@ -240,10 +244,10 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
   v4d a[4], b[4];
   if (kfull > 0) {
 #pragma unroll
-    for (int ti = 0; ti < 4; ++ti) a[ti] = ld4(Ap[ti]);
+    for (int ti = 0; ti < 4; ++ti) a[ti] = ld4(Ap[ti] + kb);
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj) b[tj] = ld4(Bp[tj]);
-    for (int k0 = 16; k0 < kfull; k0 += 16) {
+    for (int tj = 0; tj < 4; ++tj) b[tj] = ld4(Bp[tj] + kb);
+    for (int k0 = kb + 16; k0 < kfull; k0 += 16) {
       v4d an[4], bn[4];
 #pragma unroll
       for (int ti = 0; ti < 4; ++ti) an[ti] = ld4(Ap[ti] + k0);
@@ -1008,6 +1012,14 @@ void launch_product_store(const fgls_hip_ctx* c, const double* A, bool dense, co
     hipLaunchKernelGGL((k_trmm<EPI_STORE, 2>), grid, dim3(256), 0, st, c->n, ncols, c->RT, A,
                        (int64_t)c->ldn, X, ldx, Z, ldz, c->W, (int64_t)c->ldn, (double*)nullptr,
                        (int64_t)0);
+}
+
+void launch_product_store_upper(const fgls_hip_ctx* c, const double* A, const double* X,
+                                int64_t ldx, int ncols, double* Z, int64_t ldz, hipStream_t st) {
+  dim3 grid((ncols + TN - 1) / TN, c->RT);
+  hipLaunchKernelGGL((k_trmm<EPI_STORE, 2, false, true>), grid, dim3(256), 0, st, c->n, ncols, c->RT,
+                     A, (int64_t)c->ldn, X, ldx, Z, ldz, c->W, (int64_t)c->ldn, (double*)nullptr,
+                     (int64_t)0);
 }
 
 void launch_product_dot(const fgls_hip_ctx* c, const double* A, const double* X, int64_t ldx,
