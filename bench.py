@@ -386,8 +386,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the SNP-range sharding across GPUs is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path "
+              "(SNP ranges would be split across GPUs, one process and one copy of L per GPU)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
