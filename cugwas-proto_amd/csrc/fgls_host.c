@@ -128,22 +128,121 @@ static void parse_options(const char *s, options_t *o) {
 
 /* Read exactly `count` doubles at element offset `start` (io.c:7-19, but a
  * short read is an error here instead of a printed warning). */
-static int read_exact(const char *path, double *buf, size_t count, size_t start) {
-  int fd = open(path, O_RDONLY);
-  if (fd < 0) return fgls_set_error(FGLS_ERR_IO, "cannot open %s: %s", path, strerror(errno));
-  size_t want = count * sizeof(double), got = 0;
-  off_t off = (off_t)(start * sizeof(double));
+/* pread of [off, off + want) into dst; returns the bytes read (short at EOF) */
+static size_t pread_full(int fd, char *dst, size_t want, off_t off) {
+  size_t got = 0;
   while (got < want) {
-    ssize_t r = pread(fd, (char *)buf + got, want - got, off + (off_t)got);
+    ssize_t r = pread(fd, dst + got, want - got, off + (off_t)got);
     if (r < 0 && errno == EINTR) continue;
     if (r <= 0) break;
     got += (size_t)r;
+  }
+  return got;
+}
+
+/* count doubles from element `start` of a raw FP64 file (io.c:7-19).  Reads
+ * of 64 MB and more are split into 64 MB pieces read by `threads` threads
+ * (the first touch of a fresh buffer and the page-cache copy run in
+ * parallel: Phi.in is n^2 doubles, 7.2 GB at n = 30,000). */
+static int read_exact_mt(const char *path, double *buf, size_t count, size_t start, int threads) {
+  int fd = open(path, O_RDONLY);
+  if (fd < 0) return fgls_set_error(FGLS_ERR_IO, "cannot open %s: %s", path, strerror(errno));
+  const size_t want = count * sizeof(double);
+  const off_t off = (off_t)(start * sizeof(double));
+  const size_t piece = (size_t)64 << 20;
+  const long npieces = (long)((want + piece - 1) / piece);
+  size_t got = 0;
+  if (npieces <= 1 || threads <= 1) {
+    got = pread_full(fd, (char *)buf, want, off);
+  } else {
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) reduction(+ : got)
+    for (long i = 0; i < npieces; ++i) {
+      const size_t o = (size_t)i * piece, w = (want - o < piece) ? want - o : piece;
+      got += pread_full(fd, (char *)buf + o, w, off + (off_t)o);
+    }
   }
   close(fd);
   if (got != want)
     return fgls_set_error(FGLS_ERR_IO, "%s: read %zu elements - %zu expected", path,
                           got / sizeof(double), count);
   return FGLS_OK;
+}
+
+static int read_exact(const char *path, double *buf, size_t count, size_t start) {
+  return read_exact_mt(path, buf, count, start, 1);
+}
+
+/* The file holds at least count doubles (the error read_exact would give). */
+static int check_size(const char *path, size_t count) {
+  int fd = open(path, O_RDONLY);
+  if (fd < 0) return fgls_set_error(FGLS_ERR_IO, "cannot open %s: %s", path, strerror(errno));
+  struct stat sb;
+  const int ok = fstat(fd, &sb) == 0;
+  close(fd);
+  if (!ok) return fgls_set_error(FGLS_ERR_IO, "cannot stat %s: %s", path, strerror(errno));
+  if ((size_t)sb.st_size < count * sizeof(double))
+    return fgls_set_error(FGLS_ERR_IO, "%s: read %zu elements - %zu expected", path,
+                          (size_t)sb.st_size / sizeof(double), count);
+  return FGLS_OK;
+}
+
+/* count doubles of a raw FP64 file into device memory at dst, through two
+ * 256 MB pinned staging buffers: `threads` threads read one buffer while the
+ * other is copied (Phi.in goes to the device this way when no host copy is
+ * needed; it is never held whole in host memory). */
+static int upload_file(const char *path, size_t count, double *dst, int threads) {
+  const size_t want = count * sizeof(double), chunk = (size_t)256 << 20, piece = (size_t)16 << 20;
+  const size_t bsz = want < chunk ? want : chunk;
+  char *stage[2] = {NULL, NULL};
+  hipEvent_t ev[2] = {NULL, NULL};
+  hipStream_t st = NULL;
+  int rc = FGLS_OK;
+  int fd = open(path, O_RDONLY);
+  if (fd < 0) return fgls_set_error(FGLS_ERR_IO, "cannot open %s: %s", path, strerror(errno));
+  if (hipStreamCreate(&st) != hipSuccess || hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) != hipSuccess) {
+    rc = fgls_set_error(FGLS_ERR_HIP, "stream or event creation failed");
+    goto out;
+  }
+  for (int s = 0; s < 2; ++s)
+    if (hipHostMalloc((void **)&stage[s], bsz, hipHostMallocDefault) != hipSuccess) {
+      rc = fgls_set_error(FGLS_ERR_NOMEM, "pinned allocation of %zu bytes failed", bsz);
+      goto out;
+    }
+  for (size_t o = 0, c = 0; o < want; o += chunk, ++c) {
+    const int s = (int)(c & 1);
+    const size_t w = (want - o < chunk) ? want - o : chunk;
+    if (c >= 2 && hipEventSynchronize(ev[s]) != hipSuccess) {
+      rc = fgls_set_error(FGLS_ERR_HIP, "upload of %s failed", path);
+      break;
+    }
+    const long np = (long)((w + piece - 1) / piece);
+    size_t got = 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) reduction(+ : got)
+    for (long i = 0; i < np; ++i) {
+      const size_t po = (size_t)i * piece, pw = (w - po < piece) ? w - po : piece;
+      got += pread_full(fd, stage[s] + po, pw, (off_t)(o + po));
+    }
+    if (got != w) {
+      rc = fgls_set_error(FGLS_ERR_IO, "%s: read %zu elements - %zu expected", path,
+                          (o + got) / sizeof(double), count);
+      break;
+    }
+    if (hipMemcpyAsync((char *)dst + o, stage[s], w, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipEventRecord(ev[s], st) != hipSuccess) {
+      rc = fgls_set_error(FGLS_ERR_HIP, "upload of %s failed", path);
+      break;
+    }
+  }
+out:
+  if (st) hipStreamSynchronize(st);
+  for (int s = 0; s < 2; ++s) {
+    if (stage[s]) hipHostFree(stage[s]);
+    if (ev[s]) hipEventDestroy(ev[s]);
+  }
+  if (st) hipStreamDestroy(st);
+  close(fd);
+  return rc;
 }
 
 static int write_at(int fd, const char *name, const void *buf, size_t bytes, off_t off) {
@@ -401,17 +500,23 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   double t_potrf = 0.0, t_trait = 0.0, t_loop = 0.0;
 
   const size_t nn = (size_t)n * n;
-  Phi = (double *)malloc(nn * sizeof(double));
-  M = (opt.host_potrf || eig) ? (double *)malloc(nn * sizeof(double)) : Phi;
+  /* Phi is held on the host only for the host factor (M per trait) or the
+   * host eigendecomposition (Z); otherwise it is streamed to the device */
+  const int host_phi = opt.host_potrf || (eig && opt.eig_host);
+  if (host_phi) {
+    Phi = (double *)malloc(nn * sizeof(double));
+    M = (double *)malloc(nn * sizeof(double));
+  }
   h = (double *)calloc((size_t)t, sizeof(double));
   sig = (double *)calloc((size_t)t, sizeof(double));
   XL = (double *)calloc((size_t)n * (wXL > 0 ? wXL : 1), sizeof(double));
   y = (double *)calloc((size_t)n, sizeof(double));
-  if (!Phi || !M || !h || !sig || !XL || !y) {
+  if ((host_phi && (!Phi || !M)) || !h || !sig || !XL || !y) {
     rc = fgls_set_error(FGLS_ERR_NOMEM, "host allocation of %zu bytes failed", 2 * nn * sizeof(double));
     goto done;
   }
-  RCHK(read_exact(Phi_path, Phi, nn, 0));
+  if (host_phi) RCHK(read_exact_mt(Phi_path, Phi, nn, 0, threads));
+  else RCHK(check_size(Phi_path, nn));
   RCHK(read_exact(h2_path, h, (size_t)t, 0));
   RCHK(read_exact(sigma2_path, sig, (size_t)t, 0));
   if (wXL > 0) RCHK(read_exact(XL_path, XL, (size_t)wXL * n, 0));
@@ -471,7 +576,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   /* device potrf: Phi is uploaded once and every trait's M is built and
    * factored on the device; host potrf: each trait's factor is uploaded */
   HCHK(hipMalloc((void **)&dL, nn * sizeof(double)));
-  if (!opt.host_potrf && !eig) HCHK(hipMemcpy(dL, Phi, nn * sizeof(double), hipMemcpyHostToDevice));
+  if (!opt.host_potrf && !eig) RCHK(upload_file(Phi_path, nn, dL, threads));
   HCHK(hipMalloc((void **)&dXL, (size_t)n * (wXL > 0 ? wXL : 1) * sizeof(double)));
   HCHK(hipMalloc((void **)&dy, (size_t)n * K * sizeof(double)));
   if (wXL > 0) HCHK(hipMemcpy(dXL, XL, (size_t)n * wXL * sizeof(double), hipMemcpyHostToDevice));
@@ -495,9 +600,8 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     } else {
       double *dPhi = NULL;
       HCHK(hipMalloc((void **)&dPhi, nn * sizeof(double)));
-      rc = hipMemcpy(dPhi, Phi, nn * sizeof(double), hipMemcpyHostToDevice) == hipSuccess
-               ? fgls_hip_eigen_decompose(n, dPhi, n, dL, n, dW, opt.device, s_comp)
-               : fgls_set_error(FGLS_ERR_HIP, "upload of Phi failed");
+      rc = upload_file(Phi_path, nn, dPhi, threads);
+      if (rc == FGLS_OK) rc = fgls_hip_eigen_decompose(n, dPhi, n, dL, n, dW, opt.device, s_comp);
       hipFree(dPhi);
       if (rc != FGLS_OK) goto done;
     }
