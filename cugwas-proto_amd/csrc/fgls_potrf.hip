@@ -319,6 +319,116 @@ __global__ __launch_bounds__(256) void k_tgemm(int M, int N, int K, double alpha
   }
 }
 
+// The same product for small blocks, split over K: one workgroup per 64 x 64
+// block of C, its 4 waves take consecutive quarters of the block's K range
+// (in 16-steps) and their partial sums are added through LDS in a fixed
+// order.  Four times the waves of k_tgemm on the levels of the recursion
+// whose products have few 64 x 64 blocks.  Grid: (row blocks, column blocks).
+template <int TRI, bool ACC>
+__global__ __launch_bounds__(256) void k_tgemm_sk(int M, int N, int K, double alpha,
+                                                  const double* __restrict__ A,
+                                                  const double* __restrict__ B,
+                                                  double* __restrict__ C, int64_t ld) {
+  __shared__ double red[64][64];   // one wave's partial block, lane-major
+  const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  if (r0 >= M || c0 >= N) return;
+  if (TRI == TG_CUPPER && r0 > c0) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int kb = (TRI == TG_BLOWER) ? c0 : 0;
+  const int ke = (TRI == TG_BUPPER) ? min(c0 + 64, K) : K;
+  const int q = (((ke - kb) / 16 + 3) / 4) * 16;      // this wave's share, whole 16-steps
+  const int kbw = min(kb + w * q, ke), kew = min(kbw + q, ke);
+
+  const double* Ap[4];
+  const double* Bp[4];
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti) Ap[ti] = A + (size_t)(r0 + 16 * ti + li) * ld + 4 * lg;
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) Bp[tj] = B + (size_t)(c0 + 16 * tj + li) * ld + 4 * lg;
+  v4d acc[4][4];
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+  v4d a[4], b[4];
+  // operands of step k0; inside the diagonal block B's other triangle is masked
+  auto load = [&](int k0, v4d (&av)[4], v4d (&bv)[4]) {
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) av[ti] = ld4(Ap[ti] + k0);
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) bv[tj] = ld4(Bp[tj] + k0);
+    if ((TRI == TG_BLOWER || TRI == TG_BUPPER) && k0 >= c0 && k0 < c0 + 64) {
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        const int c = c0 + 16 * tj + li;
+#pragma unroll
+        for (int s2 = 0; s2 < 4; ++s2) {
+          const int k = k0 + 4 * lg + s2;
+          const bool keep = (TRI == TG_BUPPER) ? (k <= c) : (k >= c);
+          bv[tj][s2] = keep ? bv[tj][s2] : 0.0;
+        }
+      }
+    }
+  };
+  auto mfma_step = [&]() {
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = mfma_f64(b[tj][s2], a[ti][s2], acc[ti][tj]);
+  };
+  if (kbw < kew) {
+    load(kbw, a, b);
+    for (int k0 = kbw + 16; k0 < kew; k0 += 16) {
+      v4d an[4], bn[4];
+      load(k0, an, bn);
+      mfma_step();
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) a[ti] = an[ti];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) b[tj] = bn[tj];
+    }
+    mfma_step();
+  }
+  // wave 0 adds the partial sums of waves 1, 2, 3 in turn
+  for (int src = 1; src < 4; ++src) {
+    if (w == src) {
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) red[16 * ti + 4 * tj + v][lane] = acc[ti][tj][v];
+    }
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[ti][tj][v] += red[16 * ti + 4 * tj + v][lane];
+    }
+    __syncthreads();
+  }
+  if (w != 0) return;
+  // element v of acc[ti][tj] is (row r0+16ti+li, column c0+16tj+lg+4v)
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double* Cc = C + (size_t)(c0 + 16 * tj + lg + 4 * v) * ld + r0 + li;
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) {
+        const double z = alpha * acc[ti][tj][v];
+        Cc[16 * ti] = ACC ? Cc[16 * ti] + z : z;
+      }
+    }
+  }
+}
+
 // dst(c, r) = src(r, c) for an R x C block (64-multiples): src(r, c) at
 // src[r ld + c], written to dst[c ld + r].  Grid (R / 64, C / 64).
 __global__ __launch_bounds__(256) void k_tcopy(const double* __restrict__ src,
@@ -344,7 +454,18 @@ __global__ __launch_bounds__(256) void k_zero_upper(int N, double* __restrict__ 
 
 void tgemm(int tri, int M, int N, int K, double alpha, const double* A, const double* B, double* C,
            int64_t ld, hipStream_t st) {
-  const dim3 grid((M + 127) / 128, (N + 127) / 128), blk(256);
+  const dim3 blk(256);
+  if ((M / 64) * (N / 64) <= 4096) {   // few 64 x 64 blocks: split K over the waves
+    const dim3 grid(M / 64, N / 64);
+    if (tri == TG_BLOWER)
+      hipLaunchKernelGGL((k_tgemm_sk<TG_BLOWER, false>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
+    else if (tri == TG_BUPPER)
+      hipLaunchKernelGGL((k_tgemm_sk<TG_BUPPER, false>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
+    else
+      hipLaunchKernelGGL((k_tgemm_sk<TG_CUPPER, true>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
+    return;
+  }
+  const dim3 grid((M + 127) / 128, (N + 127) / 128);
   if (tri == TG_BLOWER)
     hipLaunchKernelGGL((k_tgemm<TG_BLOWER, false>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
   else if (tri == TG_BUPPER)
