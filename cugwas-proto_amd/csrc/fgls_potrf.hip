@@ -146,20 +146,31 @@ __global__ __launch_bounds__(64) void k_leaf(int n, int o0, double* __restrict__
         if (bad == 0) bad = o + k + 1;
         d = 1.0;
       }
-      const double p = sqrt(d), ip = 1.0 / p, ip2 = ip * ip;
-      // S(i, j) -= L(i, k) L(j, k) = S(i, k) S(j, k) / d for j > k; lanes
-      // i < j update entries above their diagonal, which are never read
-      const double f = a[k] * ip2;
+      const double rd = 1.0 / d, p = sqrt(d), ip = p * rd;   // 1/d and sqrt(d) side by side
+      // S(i, j) -= L(i, k) L(j, k) = S(i, k) S(j, k) / d for j > k, the
+      // column read in chunks of 16 broadcasts (one LDS wait per chunk);
+      // lanes i < j update entries above their diagonal, which are never read
+      const double f = a[k] * rd;
 #pragma unroll
-      for (int j = 0; j < 64; ++j)
-        if (j > k) a[j] = fma(-f, cb[j], a[j]);
-      a[k] = (i > k) ? a[k] * ip : ((i == k) ? p : 0.0);
+      for (int j0 = 0; j0 < 64; j0 += 16) {
+        if (j0 + 15 > k) {
+          double cv[16];
+#pragma unroll
+          for (int jj = 0; jj < 16; ++jj) cv[jj] = (j0 + jj > k) ? cb[j0 + jj] : 0.0;
+#pragma unroll
+          for (int jj = 0; jj < 16; ++jj)
+            if (j0 + jj > k) a[j0 + jj] = fma(-f, cv[jj], a[j0 + jj]);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      // column k of L is final: to LDS, out of registers
+      Ls[i][k] = (i > k) ? a[k] * ip : ((i == k) ? p : 0.0);
     }
     if (bad != 0 && i == 0 && *info == 0) *info = bad;
-  }
-
+  } else {
 #pragma unroll
-  for (int j = 0; j < 64; ++j) Ls[i][j] = (j <= i) ? a[j] : 0.0;
+    for (int j = 0; j < 64; ++j) Ls[i][j] = (j <= i) ? a[j] : 0.0;
+  }
   __syncthreads();
   if (FACTOR) {
     // R-view lower (L row-major): row o + r, lanes over its columns
