@@ -292,13 +292,20 @@ __global__ __launch_bounds__(256) void k_tgemm(int M, int N, int K, double alpha
     }
   };
 
-  if (TRI == TG_BLOWER) diag_steps();
+  // The diagonal block first; a B-upper product then runs k downwards from
+  // it (7 % faster at n = 30,000 than upwards from k = 0), a B-lower one
+  // upwards; the dense update runs upwards (downwards or from a
+  // tile-dependent start measured no gain).
+  if (TRI == TG_BLOWER || TRI == TG_BUPPER) diag_steps();
   if (kb < ke) {
+    constexpr int DK = (TRI == TG_BUPPER) ? -16 : 16;
+    const int kfirst = (TRI == TG_BUPPER) ? ke - 16 : kb;
+    const int nst = (ke - kb) / 16;
 #pragma unroll
-    for (int ti = 0; ti < 4; ++ti) a[ti] = ld4(Ap[ti] + kb);
+    for (int ti = 0; ti < 4; ++ti) a[ti] = ld4(Ap[ti] + kfirst);
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj) b[tj] = ld4(Bp[tj] + kb);
-    for (int k0 = kb + 16; k0 < ke; k0 += 16) {
+    for (int tj = 0; tj < 4; ++tj) b[tj] = ld4(Bp[tj] + kfirst);
+    for (int st = 1, k0 = kfirst + DK; st < nst; ++st, k0 += DK) {
       v4d an[4], bn[4];
 #pragma unroll
       for (int ti = 0; ti < 4; ++ti) an[ti] = ld4(Ap[ti] + k0);
@@ -312,7 +319,6 @@ __global__ __launch_bounds__(256) void k_tgemm(int M, int N, int K, double alpha
     }
     mfma_step();
   }
-  if (TRI == TG_BUPPER) diag_steps();
 
   // element v of acc[ti][tj] is (row r0+16ti+li, column c0+16tj+lg+4v): 16
   // lanes write 16 consecutive rows of one column
