@@ -117,6 +117,24 @@ __global__ __launch_bounds__(256) void k_transpose(int ldn, const double* __rest
   for (int j = 0; j < 32; j += 8) out[(size_t)(bx + ty + j) * ldn + by + tx] = tile[tx][ty + j];
 }
 
+// The 64 x 64 blocks above the diagonal of a symmetric ldn x ldn column-major
+// matrix from those below it, through 32 x 33 LDS tiles (32-tiles whose
+// 64-blocks lie above the diagonal; the others exit).
+__global__ __launch_bounds__(256) void k_mirror_upper(int ldn, double* __restrict__ Z) {
+  __shared__ double tile[32][33];
+  const int ti = blockIdx.x, tj = blockIdx.y;   // 32-tile (rows, columns) to fill
+  if ((ti >> 1) >= (tj >> 1)) return;
+  const int r0 = ti * 32, c0 = tj * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // ty 0..7
+  // source: Z(c0 + tx, r0 + ty + j), below the diagonal
+#pragma unroll
+  for (int j = 0; j < 32; j += 8) tile[ty + j][tx] = Z[(size_t)(r0 + ty + j) * ldn + c0 + tx];
+  __syncthreads();
+  // Z(r0 + tx, c0 + ty + j) = Z(c0 + ty + j, r0 + tx)
+#pragma unroll
+  for (int j = 0; j < 32; j += 8) Z[(size_t)(c0 + ty + j) * ldn + r0 + tx] = tile[tx][ty + j];
+}
+
 // WGW = W' (G W), p x p column-major; one workgroup.
 __global__ __launch_bounds__(256) void k_wgw(int n, int p, const double* __restrict__ W,
                                              const double* __restrict__ GW, int64_t ldn,
@@ -327,7 +345,9 @@ int fgls_hip_adjoint_begin(fgls_hip_ctx* c, const double* dPhi, int64_t ldphi, d
                      dPhi, ldphi, a->G);
   fgls::launch_product_store(c, c->Linv, false, a->G, ldn, n, a->T, ldn, st);
   hipLaunchKernelGGL(k_transpose, dim3(ldn / 32, ldn / 32), dim3(256), 0, st, ldn, a->T, a->G);
-  fgls::launch_product_store(c, c->Linv, false, a->G, ldn, n, a->T, ldn, st);
+  // G is symmetric: its blocks on and below the diagonal, then the mirror
+  fgls::launch_product_store_lower_half(c, a->G, ldn, a->T, ldn, st);
+  hipLaunchKernelGGL(k_mirror_upper, dim3(ldn / 32, ldn / 32), dim3(256), 0, st, ldn, a->T);
   // columns n..ldn-1 of the product were not written: T holds G with the
   // padding of the buffer it was cleared with; copy it to G
   HIP_TRY(hipMemcpyAsync(a->G, a->T, (size_t)ldn * ldn * sizeof(double), hipMemcpyDeviceToDevice, st));

@@ -83,6 +83,10 @@ __host__ __device__ inline int trait_res_off(int wXL) { return wXL + wXL * wXL; 
 // full row-major ldn x ldn matrix.
 void launch_product_store(const fgls_hip_ctx* c, const double* A, bool dense, const double* X,
                           int64_t ldx, int ncols, double* Z, int64_t ldz, hipStream_t st);
+// Z = L^-1 X (n columns) for a symmetric result: only the 64 x 64 blocks on
+// or below the diagonal of Z are written.
+void launch_product_store_lower_half(const fgls_hip_ctx* c, const double* X, int64_t ldx,
+                                     double* Z, int64_t ldz, hipStream_t st);
 // Z = A X for an upper triangular row-major ldn x ldn A (zeros below the
 // diagonal are skipped).
 void launch_product_store_upper(const fgls_hip_ctx* c, const double* A, const double* X,

@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int nrhs, const doubl
 // ---------------------------------------------------------------------------
 enum { EPI_STORE = 0, EPI_REDUCE = 1, EPI_DOT = 2, EPI_MULTI = 3 };
 
-template <int EPI, int NQ, bool DENSE = false, bool UPPER = false>
+template <int EPI, int NQ, bool DENSE = false, bool UPPER = false, bool LOWER_OUT = false>
 __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
                                               const double* __restrict__ Linv, int64_t ldli,
                                               const double* __restrict__ X, int64_t ldx,
@@ -215,6 +215,10 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
   const int li = lane & 15, lg = lane >> 4;
   const int r0 = rt * TM + wr * 64;
   const int c0 = ct * TN + wc * 64;
+  // LOWER_OUT (EPI_STORE, a symmetric result): only the 64 x 64 blocks on or
+  // below the diagonal are formed; the caller mirrors the others
+  static_assert(!LOWER_OUT || EPI == EPI_STORE, "the lower-half store has no reductions");
+  if (LOWER_OUT && c0 > r0) return;
   const int n16 = (n + 15) & ~15;
   const int kend = (r0 < n16) ? ((DENSE || UPPER) ? n16 : min(r0 + 64, n16)) : 0;   // multiple of 16
   const int kfull = min(kend, n & ~15);                   // steps with no masking
@@ -1012,6 +1016,14 @@ void launch_product_store(const fgls_hip_ctx* c, const double* A, bool dense, co
     hipLaunchKernelGGL((k_trmm<EPI_STORE, 2>), grid, dim3(256), 0, st, c->n, ncols, c->RT, A,
                        (int64_t)c->ldn, X, ldx, Z, ldz, c->W, (int64_t)c->ldn, (double*)nullptr,
                        (int64_t)0);
+}
+
+void launch_product_store_lower_half(const fgls_hip_ctx* c, const double* X, int64_t ldx,
+                                     double* Z, int64_t ldz, hipStream_t st) {
+  dim3 grid((c->n + TN - 1) / TN, c->RT);
+  hipLaunchKernelGGL((k_trmm<EPI_STORE, 2, false, false, true>), grid, dim3(256), 0, st, c->n, c->n,
+                     c->RT, c->Linv, (int64_t)c->ldn, X, ldx, Z, ldz, c->W, (int64_t)c->ldn,
+                     (double*)nullptr, (int64_t)0);
 }
 
 void launch_product_store_upper(const fgls_hip_ctx* c, const double* A, const double* X,
