@@ -23,7 +23,8 @@
 // 64 x 64 leaves factor and invert in one wave (k_leaf).  Every product runs
 // on k_tgemm, an FP64 MFMA kernel whose operands are both read along k
 // (A(r, k) at A[r ld + k], B(k, c) at B[c ld + k]) with a column-major
-// result.  To give every product such operands the two n x n buffers are
+// result, or, when it has at most 4,096 64 x 64 blocks of output, on
+// k_tgemm_sk, which splits K over the four waves of a block.  To give every product such operands the two n x n buffers are
 // kept symmetric-stored while the recursion runs ("R-view" = the buffer read
 // row-major, element (i, j) at i ld + j):
 //   L buffer: the R-view lower triangle holds the Schur complements S, then
@@ -39,7 +40,7 @@
 // still run on harmless values.
 //
 // Flops: n^3/3 for the factor and n^3/3 for the inverse; 3/4 of them are in
-// the two top levels' products, which fill the device.
+// the top level's four products, which fill the device.
 
 #include <hip/hip_runtime.h>
 #include <math.h>
