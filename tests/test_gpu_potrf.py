@@ -85,11 +85,13 @@ def _inverse_through_trmm(dc, n):
     return Z[:, :n].cpu().numpy().T   # Z[c, r] = L^-1(r, c)
 
 
-@pytest.mark.parametrize("n", [5, 63, 64, 100, 257, 700, 1000, 2049])
+@pytest.mark.parametrize("n", [5, 63, 64, 100, 257, 700, 1000, 2049, 8400])
 def test_device_inverse_of_factor(gpu, pkg, n):
     """L^-1 formed with the device factor (recursive inversion): lower
     triangular (zeros above the diagonal, exactly) and equal to the inverse of
-    LAPACK's factor; also L^-1 M L^-T = I."""
+    LAPACK's factor; also L^-1 M L^-T = I.  n = 8400 has top-level products
+    with more than 4,096 64 x 64 output blocks (the tiled product; smaller
+    ones split K)."""
     print(f"device factor + inverse n={n}", flush=True)
     rng = np.random.default_rng(n + 11)
     Phi = make_phi(rng, n)
@@ -106,7 +108,7 @@ def test_device_inverse_of_factor(gpu, pkg, n):
     assert np.abs(T @ M @ T.T - np.eye(n)).max() < 1e-12
 
 
-@pytest.mark.parametrize("n", [64, 129, 1000, 1500])
+@pytest.mark.parametrize("n", [64, 129, 1000, 1500, 8400])
 def test_inverse_of_given_factor(gpu, pkg, n):
     """fgls_hip_set_trait with the caller's factor: L^-1 by the same recursion
     without the factorisation, whatever the factor's strict upper triangle
