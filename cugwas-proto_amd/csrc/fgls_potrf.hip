@@ -70,14 +70,6 @@ __device__ __forceinline__ v4d ld4(const double* p) {
   return *reinterpret_cast<const v4d*>(p);
 }
 
-// Value of x in lane l (l uniform), as a wave-uniform scalar.
-__device__ __forceinline__ double rlane(double x, int l) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)b, l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
 // M (ldn x ldn, column-major) = alpha Phi + beta I on the n x n block, zero
 // elsewhere (fgls_chol_gpu.c:406-414: alpha = sigma2 h2, beta = sigma2 (1-h2)).
 __global__ __launch_bounds__(256) void k_build_M(int n, int ldn, const double* __restrict__ Phi,
@@ -229,7 +221,7 @@ __global__ __launch_bounds__(64) void k_leaf(int n, int o0, double* __restrict__
 // fetch and MFMA order as the triangular product of the block loop: one
 // 32-byte load per lane and operand per K = 16 step, one step ahead.
 // ---------------------------------------------------------------------------
-enum { TG_DENSE = 0, TG_BLOWER = 1, TG_BUPPER = 2, TG_CUPPER = 3 };
+enum { TG_BLOWER = 1, TG_BUPPER = 2, TG_CUPPER = 3 };
 
 template <int TRI, bool ACC>
 __global__ __launch_bounds__(256) void k_tgemm(int M, int N, int K, double alpha,
