@@ -302,12 +302,17 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
           z[tj][ti][v] = t * isc;
         }
       }
+    // group k's weights of the tile's 32 rows go through LDS (one load per
+    // thread and group) instead of NQ dependent global loads per row and lane
+    __shared__ double wsh[NQ][I8_TM];
     for (int k = 0; k < ntr; ++k) {
       const double* Wk = W + (size_t)k * NQ * ldw;
+      __syncthreads();   // every wave is done with group k - 1
+      for (int idx = tid; idx < NQ * I8_TM; idx += 256)
+        wsh[idx / I8_TM][idx % I8_TM] = Wk[(size_t)(idx / I8_TM) * ldw + r0 + idx % I8_TM];
+      __syncthreads();
 #pragma unroll
       for (int tj = 0; tj < NF; ++tj) {
-        int oz;   // opaque zero: W is re-read per fragment instead of held live
-        asm volatile("v_mov_b32 %0, 0" : "=v"(oz));
         double q[NQ];
 #pragma unroll
         for (int qi = 0; qi < NQ; ++qi) q[qi] = 0.0;
@@ -315,11 +320,11 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
         for (int ti = 0; ti < 2; ++ti) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int row = r0 + 16 * ti + 4 * lg + v + oz;
+            const int rl = 16 * ti + 4 * lg + v;   // row r0 + rl
             const double zz = z[tj][ti][v];
-            q[0] = fma(zz * Wk[row], zz, q[0]);
+            q[0] = fma(zz * wsh[0][rl], zz, q[0]);
 #pragma unroll
-            for (int j = 1; j < NQ; ++j) q[j] = fma(zz, Wk[(size_t)j * ldw + row], q[j]);
+            for (int j = 1; j < NQ; ++j) q[j] = fma(zz, wsh[j][rl], q[j]);
           }
         }
 #pragma unroll
@@ -386,7 +391,15 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   } else if constexpr (NQ <= 5) {   // p <= 4: sums interleaved with the conversion (fastest here)
     // z(row r0+16ti+4lg+v, col c0+16tj+li) = inv_scale[row] * sum_s R^(-s) acc[s];
     // one 16-column fragment at a time, so that the accumulators of finished
-    // fragments are free and the FP64 sums stay within the register budget
+    // fragments are free and the FP64 sums stay within the register budget.
+    // The tile's 32 row scales and weight rows go through LDS once (one load
+    // per thread) instead of 40 dependent global loads per fragment and lane.
+    __shared__ double wsh[NQ][I8_TM];
+    for (int idx = tid; idx < NQ * I8_TM; idx += 256) {
+      const int qi = idx / I8_TM, r = idx % I8_TM;
+      wsh[qi][r] = (qi == 0) ? inv_scale[r0 + r] : W[(size_t)(qi - 1) * ldw + r0 + r];
+    }
+    __syncthreads();
   #pragma unroll
     for (int tj = 0; tj < 4; ++tj) {
       double q[NQ];
@@ -396,15 +409,15 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
       for (int ti = 0; ti < 2; ++ti) {
   #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int row = r0 + 16 * ti + 4 * lg + v;
+          const int rl = 16 * ti + 4 * lg + v;   // row r0 + rl
           double t = (double)acc[S - 1][ti][tj][v];
   #pragma unroll
           for (int s2 = S - 2; s2 >= 0; --s2)
             t = fma(t, 1.0 / FGLS_SLICE_RADIX, (double)acc[s2][ti][tj][v]);
-          const double z = t * inv_scale[row];
+          const double z = t * wsh[0][rl];
           q[0] = fma(z, z, q[0]);
   #pragma unroll
-          for (int k = 0; k < NQ - 1; ++k) q[1 + k] = fma(z, W[(size_t)k * ldw + row], q[1 + k]);
+          for (int k = 0; k < NQ - 1; ++k) q[1 + k] = fma(z, wsh[1 + k][rl], q[1 + k]);
         }
       }
   #pragma unroll
