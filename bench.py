@@ -50,12 +50,12 @@ NSLICE = 6
 # Bytes per launch of the two reduce kernels at n = 10,000 and 65,536 SNPs per
 # launch, from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of
 # this benchmark (KiB per dispatch, averaged over the launches of a 131,072-SNP
-# pass).  Scale factors from a plain 1 GiB device copy measured in the same
-# runs: it reads as FETCH_SIZE 524,313 KiB (x 2.0) and writes as WRITE_SIZE
-# 1,048,576 KiB (x 1.0).  FETCH_SIZE counts L2 misses sent to the fabric
+# pass).  Scale factors from a plain 1 GiB device copy measured by the same
+# commands: it reads as FETCH_SIZE 524,313 KiB (x 2.0) and writes as
+# WRITE_SIZE 1,048,576 KiB (x 1.0).  FETCH_SIZE counts L2 misses sent to the fabric
 # (some are served by the MALL), so this is an upper bound on HBM reads.
 TRAFFIC_N, TRAFFIC_BLOCK = 10000, 65536
-TRAFFIC_KIB = {"fp64": (106697640.9, 202240.0), "mixed": (22413608.3, 808960.0)}
+TRAFFIC_KIB = {"fp64": (106724017.8, 202240.0), "mixed": (22650707.5, 808960.0)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
