@@ -71,15 +71,33 @@ __device__ __forceinline__ v4d ld4(const double* p) {
 }
 
 // M (ldn x ldn, column-major) = alpha Phi + beta I on the n x n block, zero
-// elsewhere (fgls_chol_gpu.c:406-414: alpha = sigma2 h2, beta = sigma2 (1-h2)).
+// elsewhere (fgls_chol_gpu.c:406-414: alpha = sigma2 h2, beta = sigma2 (1-h2)),
+// from the lower triangle of Phi only, written to both triangles of M (the
+// recursion reads its row-major lower triangle).  One workgroup per 64 x 64
+// block (I, J) with I >= J: the block is read once (coalesced down the
+// columns), written in place and, transposed through LDS, at (J, I).
 __global__ __launch_bounds__(256) void k_build_M(int n, int ldn, const double* __restrict__ Phi,
                                                  int64_t ldphi, double alpha, double beta,
                                                  double* __restrict__ M) {
-  const int c = blockIdx.y;
-  for (int r = blockIdx.x * 256 + threadIdx.x; r < ldn; r += gridDim.x * 256) {
+  __shared__ double t[64][65];
+  const int I = blockIdx.x, J = blockIdx.y;
+  if (J > I) return;
+  const int r0 = I * 64, c0 = J * 64;
+  const bool diag = I == J;
+  for (int idx = threadIdx.x; idx < 4096; idx += 256) {
+    const int cc = idx >> 6, rr = idx & 63;
+    const int r = r0 + rr, c = c0 + cc;
     double v = 0.0;
-    if (r < n && c < n) v = alpha * Phi[(size_t)c * ldphi + r] + (r == c ? beta : 0.0);
-    M[(size_t)c * ldn + r] = v;
+    if (r < n && c < n && (!diag || rr >= cc))
+      v = alpha * Phi[(size_t)c * ldphi + r] + (r == c ? beta : 0.0);
+    t[rr][cc] = v;
+    if (!diag || rr >= cc) M[(size_t)c * ldn + r] = v;
+  }
+  __syncthreads();
+  // M(c, r) = M(r, c): block (J, I), the strict upper half of a diagonal block
+  for (int idx = threadIdx.x; idx < 4096; idx += 256) {
+    const int rr = idx >> 6, cc = idx & 63;
+    if (!diag || rr > cc) M[(size_t)(r0 + rr) * ldn + c0 + cc] = t[rr][cc];
   }
 }
 
@@ -522,7 +540,7 @@ int factor_on_device(fgls_hip_ctx* c, const double* dPhi, int64_t ldphi, double 
                      hipStream_t st) {
   const int n = c->n, ldn = c->ldn;
   HIP_TRY(hipMemsetAsync(c->status + 1, 0, sizeof(int), st));
-  hipLaunchKernelGGL(k_build_M, dim3((ldn + 255) / 256, ldn), dim3(256), 0, st, n, ldn, dPhi, ldphi,
+  hipLaunchKernelGGL(k_build_M, dim3(ldn / 64, ldn / 64), dim3(256), 0, st, n, ldn, dPhi, ldphi,
                      sigma2 * h2, sigma2 * (1.0 - h2), c->L);
   HIP_TRY(hipGetLastError());
   return fgls::chol_inv(c, true, st);

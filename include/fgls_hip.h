@@ -66,8 +66,9 @@ int fgls_hip_ctx_ldn(const fgls_hip_ctx *ctx);
  * dL: n x n lower Cholesky factor of M (upper triangle ignored), ld ldl.
  * dXL: n x wXL covariates (ld ldxl), dy: the trait vector (n).
  * Computes on the device, in FP64:
- *   - the inverses of the 64x64 diagonal blocks of L (LDS-staged solves),
- *   - L^-1 (blocked TRSM against the identity; MFMA off-diagonal updates),
+ *   - L^-1 (recursive inversion: 64x64 diagonal blocks inverted in one
+ *     wave each, the off-diagonal blocks as FP64 MFMA products),
+ *   - the inverses of the 64x64 diagonal blocks of L (for fgls_hip_trsm),
  *   - X~_L = L^-1 X_L and y~ = L^-1 y,
  *   - B_t = X~_L' y~, V_tl = lower(X~_L' X~_L) and the null-model residual
  *     variance res_sigma = |L^-1 (y - X_L b_L)|^2 / (n - wXL).
@@ -235,9 +236,10 @@ int fgls_hip_check(fgls_hip_ctx *ctx, void *stream, int64_t *bad_snp,
 
 /*
  * Per-trait precompute from Phi instead of a host factor: builds
- * M = sigma2 (h2 Phi + (1 - h2) I) and its lower Cholesky factor on the
- * device (FP64, blocked, 64-column panels) and then does what
- * fgls_hip_set_trait does.  Replaces the host M construction and dpotrf of
+ * M = sigma2 (h2 Phi + (1 - h2) I) and, on the device in FP64, its lower
+ * Cholesky factor together with L^-1 (one recursion: 64 x 64 leaves, MFMA
+ * products for the off-diagonal blocks) and then does what
+ * fgls_hip_set_trait does after its inversion.  Replaces the host M construction and dpotrf of
  * fgls_chol_gpu.c:406-430 and the L upload of fgls_chol_gpu.c:434-445.
  * dPhi: device, column-major, ld >= n (only the lower triangle is read).
  * A non-SPD M gives FGLS_ERR_NOTSPD with "dpotrf(M) failed (info: i)",

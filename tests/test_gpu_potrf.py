@@ -125,3 +125,22 @@ def test_inverse_of_given_factor(gpu, pkg, n):
     Tref = np.linalg.inv(L)
     assert np.all(np.triu(T, 1) == 0.0)
     assert np.linalg.norm(T - Tref) / np.linalg.norm(Tref) < 1e-13
+
+
+@pytest.mark.parametrize("n", [200, 1100])
+def test_device_factor_reads_lower_triangle_of_phi(gpu, pkg, n):
+    """set_trait_phi reads only the lower triangle of Phi (NaN above it)."""
+    rng = np.random.default_rng(n + 5)
+    Phi = make_phi(rng, n)
+    Phl = Phi.copy()
+    Phl[np.triu_indices(n, 1)] = np.nan
+    XL = np.column_stack([np.ones(n), rng.standard_normal((n, 2))])
+    y = rng.standard_normal(n)
+    with pkg.DeviceContext(n, 3, 1, max_block=64) as dc:
+        dc.set_trait_phi(_colmajor(Phl), 0.45, 1.2, _colmajor(XL), torch.from_numpy(y).cuda())
+        L = np.tril(dc.factor().cpu().numpy().T)
+        T = _inverse_through_trmm(dc, n)
+    Lref = orc.cholesky_lower(orc.build_M(Phi, 0.45, 1.2))
+    assert np.all(np.isfinite(L)) and np.all(np.isfinite(T))
+    assert np.linalg.norm(L - Lref) / np.linalg.norm(Lref) < 1e-13
+    assert np.linalg.norm(T - np.linalg.inv(Lref)) / np.linalg.norm(np.linalg.inv(Lref)) < 1e-13
