@@ -1441,8 +1441,6 @@ int fgls_hip_i8_to_f64(const int8_t* dX8, int64_t ldx8, int n, int ncols, double
 
 namespace {
 
-// The int8 product of a block: genotypes (planes = 1) or dosage digits
-// (planes = 2: lo at dX8, hi at dX8 + count * ldx8, x = (128 hi + lo) / scale).
 // The int8 product of a block of tiled genotypes (planes = 1) or dosage
 // digits (planes = 2, x = (128 hi + lo) / scale).
 int reduce_i8t(fgls_hip_ctx* c, const int8_t* dX8T, int count, int planes, double scale,
