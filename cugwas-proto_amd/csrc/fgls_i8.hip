@@ -62,7 +62,8 @@ constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); }
 // 13.2 -> 12.7 ms per 65,536 SNPs at n = 10^4 with bitwise equal results.
 // The buffer resource of a workgroup covers its two 16-row slice groups (one
 // contiguous range of 2 KS S KB, far below 2^31 for every n the int8 path
-// accepts); a load outside it would read zeros, not fault.
+// accepts); the per-lane voffset picks the wave's block inside it and the
+// uniform soffset the step, both within that range at every step.
 //
 // Step k (reading half k & 1): wait for this wave's loads of step k, issue
 // its genotype loads of step k + 1, barrier (every wave's slices of step k
