@@ -65,14 +65,17 @@ constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); }
 // accepts); the per-lane voffset picks the wave's block inside it and the
 // uniform soffset the step, both within that range at every step.
 //
-// Step k (reading half k & 1): wait for this wave's loads of step k, issue
-// its genotype loads of step k + 1, barrier (every wave's slices of step k
-// are in LDS and every wave has finished reading the other half in step
-// k - 1), issue the slices of step k + 1, then 12 LDS fragment reads and 48
-// MFMAs, each read one fragment ahead of the four MFMAs that use it.  The
-// loop runs two steps per trip so that the LDS halves and the genotype
-// registers alternate without copies (a copy of a register that a load is
-// filling would wait for that load).
+// Step k (reading half k & 1): wait for this wave's slices of step k,
+// barrier (every wave's slices of step k are in LDS and every wave has
+// finished reading the other half in step k - 1), issue the slices and then
+// the genotype loads of step k + 1, wait for this wave's genotype fragments
+// of step k (issued after its slices in step k - 1, so the barrier never
+// waits for genotype loads), then 12 LDS fragment reads and 48 MFMAs, each
+// read one fragment ahead of the four MFMAs that use it (vmcnt retires in
+// issue order: the 4 genotype loads follow the slices, and the S/2 + 4 loads
+// of step k + 1 follow them).  The loop runs two steps per trip so that the
+// LDS halves and the genotype registers alternate without copies (a copy of
+// a register that a load is filling would wait for that load).
 // Prefetch addresses are clamped to the last step, so every load is in
 // bounds.  The barrier carries no memory fence: a workgroup-scope release
 // fence would also wait for the LDS loads just issued; the vmcnt wait before
@@ -109,15 +112,21 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i
                                         SliceStage<S>& wr, fgls_v4i asrc, const int (&avo)[S / 2],
                                         const int8_t* Xw, const int (&boff)[4], int k, int last,
                                         int w, int lane) {
-  __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+  // this wave's slices of step k are in LDS (its genotype loads of step k,
+  // issued after them, may still be in flight)
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(4));
   const int k1 = min(k + 1, last);
-#pragma unroll
-  for (int tj = 0; tj < 4; ++tj)
-    bn[tj] = *reinterpret_cast<const fgls_v4i*>(Xw + 1024 * k1 + boff[tj]);
   __builtin_amdgcn_sched_barrier(0);
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_sched_barrier(0);
   slice_dma<S>(asrc, avo, k1 * S * 1024, wr, w);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj)
+    bn[tj] = *reinterpret_cast<const fgls_v4i*>(Xw + 1024 * k1 + boff[tj]);
+  __builtin_amdgcn_sched_barrier(0);
+  // the genotype fragments of step k (older than the S/2 + 4 loads just issued)
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(S / 2 + 4));
   __builtin_amdgcn_sched_barrier(0);
   // A fragments one ahead: fragment f = 2 s + ti
   fgls_v4i af[2];
@@ -219,9 +228,11 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   if (nsteps > 0) {
     const int last = nsteps - 1;
     fgls_v4i b0[4], b1[4];
+    slice_dma<S>(asrc, avo, 0, st0, w);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int tj = 0; tj < 4; ++tj) b0[tj] = *reinterpret_cast<const fgls_v4i*>(Xw + boff[tj]);
-    slice_dma<S>(asrc, avo, 0, st0, w);
+    __builtin_amdgcn_sched_barrier(0);
     int k = 0;
     for (; k + 2 <= nsteps; k += 2) {
       step_i8<S>(acc, b0, b1, st0, st1, asrc, avo, Xw, boff, k, last, w, lane);
