@@ -55,7 +55,7 @@ NSLICE = 6
 # WRITE_SIZE 1,048,576 KiB (x 1.0).  FETCH_SIZE counts L2 misses sent to the fabric
 # (some are served by the MALL), so this is an upper bound on HBM reads.
 TRAFFIC_N, TRAFFIC_BLOCK = 10000, 65536
-TRAFFIC_KIB = {"fp64": (106697733.0, 202240.0), "mixed": (21516934.6, 808960.0)}
+TRAFFIC_KIB = {"fp64": (106698200.6, 202240.0), "mixed": (23309193.3, 808960.0)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
