@@ -87,6 +87,7 @@ def make_inputs(torch, n, m, wXL, h2, seed, ldx, dev):
     Phi = 0.5 * (Phi + Phi.T)
     d = Phi.diagonal().sqrt()
     Phi = Phi / d[:, None] / d[None, :]
+    Phi = torch.tril(Phi) + torch.tril(Phi, -1).T     # exactly symmetric (the scaling rounds per side)
     Phi.fill_diagonal_(1.0)
     M = h2 * Phi
     M.diagonal().add_(1.0 - h2)
@@ -156,6 +157,122 @@ def cpu_baseline(kind, Phi, L, XL, y, X, n, wXL, h2, sample, threads):
         rate = sample / (time.perf_counter() - t0)
         desc = f"FP64 oracle (numpy/scipy dtrsm + per-SNP stage), n={n}, p={wXL + 1}, {sample} SNPs"
     return {"value": rate, "unit": "SNPs/s", "cores": threads, "kind": kind, "sample": desc}
+
+
+def read_rate(path, nbytes, threads, direct=False):
+    """GB/s of reading the first nbytes of path into one pinned buffer with
+    `threads` threads of 64 MB preads, as the file pipeline's reader does
+    (direct=True: O_DIRECT, i.e. from the disk, not the page cache).  None
+    when the file system refuses O_DIRECT."""
+    import concurrent.futures
+    import torch
+    piece = 64 << 20
+    nbytes = nbytes // piece * piece
+    if nbytes == 0:
+        return None
+    buf = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True).numpy()
+    flags = os.O_RDONLY | (os.O_DIRECT if direct else 0)
+    try:
+        fd = os.open(path, flags)
+    except OSError:
+        return None
+    mv = memoryview(buf)
+
+    def rd(i):
+        o = i * piece
+        got = 0
+        while got < piece:
+            r = os.preadv(fd, [mv[o + got:o + piece]], o + got)
+            if r <= 0:
+                break
+            got += r
+        return got
+
+    try:
+        t0 = time.perf_counter()
+        with concurrent.futures.ThreadPoolExecutor(threads) as pool:
+            got = sum(pool.map(rd, range(nbytes // piece)))
+        dt = time.perf_counter() - t0
+    except OSError:
+        return None
+    finally:
+        os.close(fd)
+    return got / dt / 1e9 if got == nbytes else None
+
+
+def run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, ref, dev):
+    """The drop-in's streamed path: fgls_chol_gpu_ex (files in, files out) at
+    the workload's n and p on the first --pipeline-m SNPs of the same
+    genotypes, written on this machine as the reference's FP64 XR.in, an int8
+    file and a 2-bit file (the page cache holds them: the reads are warm).
+    Per run: block-loop SNPs/s, the reader's X_R rate (bytes / time inside
+    pread) and the effective rate (bytes / block-loop time), the upload rate
+    (bytes / copy-stream time of the H2D copies) against PCIe Gen5 x16's
+    64 GB/s, and the whole call.  The FP64 file is swept over io_threads."""
+    import shutil
+    import numpy as np
+    n, wXL = args.n, args.wXL
+    p = wXL + 1
+    m = min(args.pipeline_m, X.shape[0])
+    d = tempfile.mkdtemp(prefix="fgls_pipe_", dir=args.pipeline_dir)
+    f = lambda s: os.path.join(d, s)
+    out = {"m": m, "n": n, "p": p, "block": min(args.block, m), "dir_fs": None, "runs": []}
+    try:
+        t0 = time.perf_counter()
+        np.ascontiguousarray(Phi.cpu().numpy(), dtype="<f8").tofile(f("Phi.in"))
+        np.asarray([args.h2], "<f8").tofile(f("H.in"))
+        np.asarray([1.0], "<f8").tofile(f("Sig.in"))
+        XL.T.contiguous().cpu().numpy().astype("<f8").tofile(f("XL.in"))
+        y.cpu().numpy().astype("<f8").tofile(f("Y.in"))
+        C = 16384
+        with open(f("XR.in"), "wb") as a, open(f("XR8.in"), "wb") as b, open(f("XR2.in"), "wb") as c:
+            for i0 in range(0, m, C):
+                i1 = min(i0 + C, m)
+                a.write(X[i0:i1, :n].cpu().numpy().tobytes())
+                x8 = X8[i0:i1, :n].cpu().numpy()
+                b.write(x8.tobytes())
+                c.write(pkg.pack_u2(x8.T).tobytes())
+        log(f"pipeline: files of {m} SNPs written in {time.perf_counter() - t0:.1f} s "
+            f"(XR.in {os.path.getsize(f('XR.in')) / 1e9:.1f} GB)")
+        xr_bytes = os.path.getsize(f("XR.in"))
+        rates = {"page_cache_GBps": {}, "o_direct_GBps": None}
+        for th in (4, 16):
+            rates["page_cache_GBps"][th] = read_rate(f("XR.in"), min(xr_bytes, 8 << 30), th)
+        rates["o_direct_GBps"] = read_rate(f("XR.in"), min(xr_bytes, 4 << 30), 16, direct=True)
+        out["standalone_read"] = rates
+        log(f"pipeline: standalone reads of XR.in {rates}")
+        runs = [("f64", "fp64", 4), ("f64", "fp64", 8), ("f64", "fp64", 16), ("f64", "fp64", 0),
+                ("f64", "mixed", 0), ("i8", "mixed", 0), ("u2", "mixed", 0), ("u2", "fp64", 0)]
+        Bref = ref["B"][:m].cpu().numpy().ravel() if ref is not None else None
+        for fmt, prec, io in runs:
+            xr = {"f64": "XR.in", "i8": "XR8.in", "u2": "XR2.in"}[fmt]
+            opts = {"precision": prec, "xr_format": fmt, "block": out["block"]}
+            if io:
+                opts["io_threads"] = io
+            t0 = time.perf_counter()
+            pkg.fgls_chol_gpu(n, p, m, 1, wXL, 1, 4096, 1, host_threads(), f("Phi.in"), f("H.in"), f("Sig.in"),
+                              f("XL.in"), f(xr), f("Y.in"), f("B.out"), f("V.out"), options=opts)
+            wall = time.perf_counter() - t0
+            st = pkg.last_run_stats()
+            r = {"xr_format": fmt, "precision": prec, "io_threads": st["io_threads"],
+                 "block_loop_snps_per_s": m / st["loop_s"], "block_loop_s": st["loop_s"],
+                 "whole_call_s": st["total_s"], "python_wall_s": wall, "trait_setup_s": st["trait_s"],
+                 "xr_GB": st["xr_bytes"] / 1e9,
+                 "xr_read_GBps": st["xr_bytes"] / st["read_busy_s"] / 1e9 if st["read_busy_s"] > 0 else None,
+                 "xr_effective_GBps": st["xr_bytes"] / st["loop_s"] / 1e9,
+                 "h2d_GBps": st["h2d_bytes"] / st["h2d_busy_s"] / 1e9 if st["h2d_busy_s"] > 0 else None,
+                 "h2d_peak_GBps": 64.0, "cache": "warm (files written on this machine just before)"}
+            if prec == "fp64" and Bref is not None:
+                r["B_bitwise_equal_to_device_resident_fp64"] = bool(
+                    np.array_equal(np.fromfile(f("B.out"), dtype="<f8"), Bref))
+            out["runs"].append(r)
+            log(f"pipeline {fmt}/{prec} io_threads={r['io_threads']}: {r['block_loop_snps_per_s']:.0f} SNPs/s "
+                f"(loop {st['loop_s']:.2f} s, call {st['total_s']:.2f} s), X_R read "
+                f"{r['xr_read_GBps'] or 0:.1f} GB/s in pread, {r['xr_effective_GBps']:.1f} GB/s effective, "
+                f"H2D {r['h2d_GBps'] or 0:.1f} GB/s")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
 
 
 def run_adjoint(torch, pkg, dev, m, steps, n=5000, wXL=3):
@@ -366,7 +483,7 @@ def main():
     ap.add_argument("--h2", type=float, default=0.5)
     ap.add_argument("--block", type=int, default=65536)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--check-snps", type=int, default=512)
+    ap.add_argument("--check-snps", type=int, default=4096)
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "none"], default="auto")
     ap.add_argument("--cpu-snps", type=int, default=16384)
     ap.add_argument("--paths", default="fp64,mixed,dosage",
@@ -379,6 +496,10 @@ def main():
                     help="SNPs of the adjoint measurement at n=5000, p=4, blocks of 2048 (0: skip)")
     ap.add_argument("--adjoint-n", type=int, default=5000)
     ap.add_argument("--adjoint-wxl", type=int, default=3)
+    ap.add_argument("--pipeline-m", type=int, default=262144,
+                    help="SNPs of the file-pipeline leg (fgls_chol_gpu_ex on FP64, int8 and 2-bit X_R "
+                         "files written here; 0: skip)")
+    ap.add_argument("--pipeline-dir", default=None, help="directory for the pipeline leg's files (default: TMPDIR)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (a known-size kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -527,30 +648,47 @@ def main():
         log(f"{mode}: {paths[mode]['snps_per_s']:.0f} SNPs/s, reduce kernel {paths[mode]['reduce_ms']:.2f} ms "
             f"per {block} SNPs")
 
-    # parity spot check of each path against the FP64 oracle on a sample of SNPs
+    # parity of each path against the FP64 oracle on a sample of SNPs: the
+    # first 64, --check-snps random ones, the last one and the 64 where the
+    # mixed and FP64 paths differ most element-wise (relative error of b);
+    # parity_errors gives the scaled bars, error_stats the distribution of
+    # the element-wise errors (driver.c:159-177's metric)
     check = {}
     if args.check_snps > 0:
         from oracle import fgls_oracle as orc
-        idx = np.unique(np.concatenate([np.arange(min(args.check_snps // 2, m)),
-                                        np.random.default_rng(0).integers(0, m, args.check_snps // 2)]))
+        pick = [np.arange(min(64, m)), np.random.default_rng(0).integers(0, m, args.check_snps), [m - 1]]
+        if "mixed" in paths and "fp64" in paths:
+            gap = (paths["mixed"]["B"] - paths["fp64"]["B"]).abs() / paths["fp64"]["B"].abs()
+            gap = torch.nan_to_num(gap, nan=0.0, posinf=1e300).amax(dim=1)
+            pick.append(torch.topk(gap, min(64, m)).indices.cpu().numpy())
+            del gap
+        idx = np.unique(np.concatenate(pick)).astype(np.int64)
+        t0 = time.perf_counter()
         Lc = orc.cholesky_lower(orc.build_M(Phi.cpu().numpy(), args.h2, 1.0))   # LAPACK's factor
         tr = orc.trait_precompute(Lc, XL.cpu().numpy(), y.cpu().numpy())
-        Xs = X[torch.as_tensor(idx, device=dev), :n].cpu().numpy().T.copy()
+        sel = torch.as_tensor(idx, device=dev)
+        Xs = X[sel, :n].cpu().numpy().T.copy()
         b_ref, V_ref = orc.solve_snps(Xs, tr, 1)
         Vr = V_ref.transpose(0, 2, 1).reshape(idx.size, -1)
-        sel = torch.as_tensor(idx, device=dev)
+        log(f"oracle on {idx.size} SNPs: {time.perf_counter() - t0:.1f} s")
         for mode, res in paths.items():
+            bref, vref = b_ref, Vr
             if mode == "dosage":
-                Xd = np.stack([(D8[i // block][i % block, :n].cpu().numpy().astype(np.int64) +
-                                128 * D8[i // block][min(block, m - (i // block) * block) + i % block, :n]
-                                .cpu().numpy().astype(np.int64)) / 1000.0 for i in idx], axis=1)
-                bd, Vd = orc.solve_snps(Xd, tr, 1)
-                e = orc.parity_errors(res["B"][sel].cpu().numpy(), res["V"][sel].cpu().numpy(), bd,
-                                      Vd.transpose(0, 2, 1).reshape(idx.size, -1), p)
-            else:
-                e = orc.parity_errors(res["B"][sel].cpu().numpy(), res["V"][sel].cpu().numpy(), b_ref, Vr, p)
-            check[mode] = dict(e, snps=int(idx.size))
-            log(f"{mode} parity vs oracle on {idx.size} SNPs: {e}")
+                # the dosages of the sample, 1000 x = lo + 128 hi, from the digit planes
+                Xd = np.empty((n, idx.size))
+                for bi, i0 in enumerate(range(0, m, block)):
+                    cnt = min(block, m - i0)
+                    j = np.nonzero((idx >= i0) & (idx < i0 + cnt))[0]
+                    if j.size:
+                        r = torch.as_tensor(idx[j] - i0, device=dev)
+                        v = D8[bi][r, :n].to(torch.int32) + 128 * D8[bi][r + cnt, :n].to(torch.int32)
+                        Xd[:, j] = (v.to(torch.float64) / 1000.0).cpu().numpy().T
+                bref, Vd = orc.solve_snps(Xd, tr, 1)
+                vref = Vd.transpose(0, 2, 1).reshape(idx.size, -1)
+            Bs, Vs = res["B"][sel].cpu().numpy(), res["V"][sel].cpu().numpy()
+            e = orc.parity_errors(Bs, Vs, bref, vref, p)
+            check[mode] = dict(e, snps=int(idx.size), elementwise=orc.error_stats(Bs, Vs, bref, vref, p))
+            log(f"{mode} parity vs oracle on {idx.size} SNPs: {e}; element-wise {check[mode]['elementwise']}")
 
     # PCIe: pinned host -> device copy of 1 GiB (the file pipeline's upload
     # path), against Gen5 x16's 64 GB/s per direction
@@ -626,8 +764,19 @@ def main():
     if "mixed" in paths and "fp64" in paths:
         e = orc_parity(paths["mixed"], paths["fp64"], p)
         summary["mixed_vs_fp64_all_snps"] = e
+        from oracle import fgls_oracle as orc
+        summary["mixed_vs_fp64_all_snps_elementwise"] = orc.error_stats(
+            paths["mixed"]["B"].cpu().numpy(), paths["mixed"]["V"].cpu().numpy(),
+            paths["fp64"]["B"].cpu().numpy(), paths["fp64"]["V"].cpu().numpy(), p)
         summary["speedup_mixed_over_fp64"] = paths["mixed"]["snps_per_s"] / paths["fp64"]["snps_per_s"]
         log(f"mixed vs fp64 over all {m} SNPs: {e}; speedup {summary['speedup_mixed_over_fp64']:.2f}x")
+    pipeline = None
+    if args.pipeline_m > 0:
+        try:
+            pipeline = run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, paths.get("fp64"), dev)
+        except (pkg.FglsError, OSError) as err:   # reported in the JSON line; the headline paths stand
+            pipeline = {"error": str(err)}
+            log(f"pipeline leg failed: {err}")
     eigen = None
     if args.eigen_traits > 0:
         ctx.close()
@@ -678,6 +827,7 @@ def main():
         "parity_vs_oracle": check,
         "eigen": eigen,
         "adjoint": adjoint,
+        "pipeline": pipeline,
     }
     print(json.dumps(out), flush=True)
 

@@ -16,11 +16,13 @@ from .fgls import (  # noqa: F401
     fgls_chol_gpu,
     fgls_eigen,
     last_error,
+    last_run_stats,
     lib,
     LIB_PATH,
     pack_i16,
     pack_u2,
     tile_i8,
+    to_i8,
     u2_to_i8,
     FGLS_OK,
     FGLS_ERR_ARG,

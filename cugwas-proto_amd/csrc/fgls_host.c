@@ -72,6 +72,15 @@ typedef struct {
   int eig_host;      /* eig=host: eigendecomposition of Phi by host LAPACK dsyevr (default: device) */
 } options_t;
 
+/* Timings of the calling thread's last completed fgls_*_gpu_ex call. */
+static __thread fgls_run_stats g_last_stats;
+
+int fgls_last_run_stats(fgls_run_stats *out) {
+  if (!out) return fgls_set_error(FGLS_ERR_ARG, "NULL output pointer");
+  *out = g_last_stats;
+  return FGLS_OK;
+}
+
 static double now_s(void) {
   struct timespec ts;
   clock_gettime(CLOCK_MONOTONIC, &ts);
@@ -497,7 +506,9 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   int fdB = -1, fdV = -1;
   pthread_t rthread;
   int reader_started = 0;
-  double t_potrf = 0.0, t_trait = 0.0, t_loop = 0.0;
+  double t_potrf = 0.0, t_trait = 0.0, t_loop = 0.0, h2d_ms = 0.0;
+  const double t_call0 = now_s();
+  hipEvent_t ev_c0[2] = {NULL, NULL}, ev_c1[2] = {NULL, NULL};   /* timed uploads */
 
   const size_t nn = (size_t)n * n;
   /* Phi is held on the host only for the host factor (M per trait) or the
@@ -570,6 +581,8 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   HCHK(hipStreamCreateWithFlags(&s_comp, hipStreamNonBlocking));
   for (int s = 0; s < 2; ++s) {
     HCHK(hipEventCreateWithFlags(&ev_h2d[s], hipEventDisableTiming));
+    HCHK(hipEventCreate(&ev_c0[s]));
+    HCHK(hipEventCreate(&ev_c1[s]));
     HCHK(hipEventCreateWithFlags(&ev_out[s], hipEventDisableTiming));
     HCHK(hipEventCreateWithFlags(&ev_free[s], hipEventDisableTiming));
   }
@@ -706,6 +719,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
         }
         /* device X_R slot s is free once the solves of chunk c-2 have run */
         HCHK(hipStreamWaitEvent(s_copy, ev_free[s], 0));
+        HCHK(hipEventRecord(ev_c0[s], s_copy));
         if (opt.xr_i8 >= 2)
           HCHK(hipMemcpyAsync(dP2[s], R.slot[s], (size_t)cnt * wXR * col_bytes, hipMemcpyHostToDevice, s_copy));
         else if (opt.xr_i8)
@@ -715,6 +729,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           HCHK(hipMemcpy2DAsync(dX[s], (size_t)ldx * sizeof(double), R.slot[s],
                                 (size_t)n * sizeof(double), (size_t)n * sizeof(double),
                                 (size_t)cnt * wXR, hipMemcpyHostToDevice, s_copy));
+        HCHK(hipEventRecord(ev_c1[s], s_copy));
         HCHK(hipEventRecord(ev_h2d[s], s_copy));
         HCHK(hipStreamWaitEvent(s_comp, ev_h2d[s], 0));
         if (opt.xr_i8 == 2) {
@@ -742,6 +757,11 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
         } else if (!opt.mixed && (opt.xr_i8 == 1 || opt.xr_i8 == 2)) {
           RCHK(fgls_hip_i8_to_f64(dX8[s], ld8, n, cnt * wXR, dX[s], ldx, s_comp));
+        } else if (opt.mixed && opt.xr_i8 == 1) {
+          /* an int8 file goes to the product as is: -128 is out of its range */
+          HCHK(hipMemsetAsync(dBad + s, 0, sizeof(unsigned int), s_comp));
+          RCHK(fgls_hip_check_i8(dX8[s], ld8, n, cnt * wXR, dBad + s, s_comp));
+          HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
         }
         const int planes = dos ? 2 : 1;
         if (opt.mixed) RCHK(fgls_hip_tile_i8(dX8[s], ld8, n, cnt, planes, dX8T[s], s_comp));
@@ -778,6 +798,11 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
         if (opt.serial) HCHK(hipDeviceSynchronize());
         /* the pinned read slot can be refilled once its upload has landed */
         HCHK(hipEventSynchronize(ev_h2d[s]));
+        {
+          float ms = 0.0f;
+          HCHK(hipEventElapsedTime(&ms, ev_c0[s], ev_c1[s]));
+          h2d_ms += ms;
+        }
         sem_post(&R.free_[s]);
       }
       if (c > 0) {  /* write back chunk c-1 */
@@ -823,6 +848,24 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     t_loop += now_s() - tl0;
   }
   HCHK(hipStreamSynchronize(s_comp));
+  {
+    fgls_run_stats st;
+    memset(&st, 0, sizeof(st));
+    st.total_s = now_s() - t_call0;
+    st.factor_s = eig ? t_eig : t_potrf;
+    st.trait_s = t_trait;
+    st.loop_s = t_loop;
+    st.read_busy_s = R.busy_s;
+    st.h2d_busy_s = h2d_ms * 1e-3;
+    st.xr_bytes = (double)m * R.t * wXR * (double)col_bytes;
+    st.h2d_bytes = st.xr_bytes;
+    st.trait_snps = (int64_t)m * t;
+    st.block = block;
+    st.passes = R.t;
+    st.traits_per_pass = K;
+    st.io_threads = R.io_threads;
+    g_last_stats = st;
+  }
   if (opt.stats) {
     const double snps = (double)m * t;
     fprintf(stderr,
@@ -850,6 +893,8 @@ done:
   if (s_copy) hipStreamSynchronize(s_copy);
   for (int s = 0; s < 2; ++s) {
     if (ev_h2d[s]) hipEventDestroy(ev_h2d[s]);
+    if (ev_c0[s]) hipEventDestroy(ev_c0[s]);
+    if (ev_c1[s]) hipEventDestroy(ev_c1[s]);
     if (ev_out[s]) hipEventDestroy(ev_out[s]);
     if (ev_free[s]) hipEventDestroy(ev_free[s]);
     if (dX[s]) hipFree(dX[s]);

@@ -30,6 +30,7 @@
 #include <stdio.h>
 #include <stdarg.h>
 #include <string.h>
+#include <float.h>
 #include <math.h>
 
 #include "fgls_device.h"
@@ -514,8 +515,11 @@ __global__ __launch_bounds__(256) void k_to_i8(int n, const double* __restrict__
 
 // FP64 dosages -> two int8 digit planes: v = rint(x * scale) = 128 hi + lo,
 // lo in [0, 127] at X8, hi in [-127, 127] at X8 + plane_off (rows n..ldx8-1
-// zeroed).  A value with |x * scale - v| > 1e-6 or v outside
-// [-16256, 16383] is counted in *bad and written as 0.
+// zeroed).  A value with |x * scale - v| > 8 eps max(1, |x * scale|) (eps =
+// 2^-52: x holds the nearest double to v / scale, and x * scale adds one
+// more rounding, so an exact multiple of 1/scale is off by at most
+// 2 eps |x * scale|) or v outside [-16256, 16383] is counted in *bad and
+// written as 0.
 __global__ __launch_bounds__(256) void k_to_dosage_i8(int n, const double* __restrict__ X, int64_t ldx,
                                                       double scale, int8_t* __restrict__ X8,
                                                       int64_t ldx8, int64_t plane_off,
@@ -527,7 +531,7 @@ __global__ __launch_bounds__(256) void k_to_dosage_i8(int n, const double* __res
     if (r < n) {
       const double y = X[(size_t)col * ldx + r] * scale;
       const double v = rint(y);
-      if (!(fabs(y - v) <= 1e-6 && v >= -16256.0 && v <= 16383.0)) {
+      if (!(fabs(y - v) <= 8.0 * DBL_EPSILON * fmax(1.0, fabs(y)) && v >= -16256.0 && v <= 16383.0)) {
         ++nb;
       } else {
         const int iv = (int)v;
@@ -600,6 +604,17 @@ __global__ __launch_bounds__(256) void k_i16_to_f64(int n, const int16_t* __rest
   const int col = blockIdx.x;
   for (int r = threadIdx.x; r < ldx; r += 256)
     X[(size_t)col * ldx + r] = (r < n) ? (double)V[(size_t)col * ldv + r] / scale : 0.0;
+}
+
+// int8 genotypes as given (column-major, ld ldx8): values of -128 in rows
+// < n are counted in *bad (the int8 product's exactness bound assumes
+// |x| <= 127).  One workgroup per column, one byte per thread and step.
+__global__ __launch_bounds__(256) void k_check_i8(int n, const int8_t* __restrict__ X8, int64_t ldx8,
+                                                  unsigned int* __restrict__ bad) {
+  const int col = blockIdx.x;
+  unsigned int nb = 0;
+  for (int r = threadIdx.x; r < n; r += 256) nb += (X8[(size_t)col * ldx8 + r] == -128) ? 1u : 0u;
+  if (nb) atomicAdd(bad, nb);
 }
 
 // int8 genotypes -> FP64 (column-major, ld ldx; rows n..ldx-1 written as 0).
@@ -1423,6 +1438,16 @@ int fgls_hip_i16_to_f64(const int16_t* dV, int64_t ldv, int n, int ncols, double
   if (ncols == 0) return FGLS_OK;
   hipLaunchKernelGGL(k_i16_to_f64, dim3(ncols), dim3(256), 0, (hipStream_t)stream, n, dV, ldv, scale, dX,
                      ldx);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_check_i8(const int8_t* dX8, int64_t ldx8, int n, int ncols, unsigned int* dbad,
+                      void* stream) {
+  if (n < 1 || ncols < 0 || ldx8 < n || !dX8 || !dbad)
+    return fail(FGLS_ERR_ARG, "bad arguments to fgls_hip_check_i8");
+  if (ncols == 0) return FGLS_OK;
+  hipLaunchKernelGGL(k_check_i8, dim3(ncols), dim3(256), 0, (hipStream_t)stream, n, dX8, ldx8, dbad);
   HIP_TRY(hipGetLastError());
   return FGLS_OK;
 }

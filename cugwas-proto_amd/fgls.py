@@ -41,6 +41,15 @@ FGLS_MODE_MIXED_I8 = 1
 _lib = None
 
 
+class RunStats(ctypes.Structure):
+    """fgls_run_stats (include/fgls_chol_gpu.h)."""
+    _fields_ = [("total_s", ctypes.c_double), ("factor_s", ctypes.c_double), ("trait_s", ctypes.c_double),
+                ("loop_s", ctypes.c_double), ("read_busy_s", ctypes.c_double),
+                ("h2d_busy_s", ctypes.c_double), ("xr_bytes", ctypes.c_double),
+                ("h2d_bytes", ctypes.c_double), ("trait_snps", ctypes.c_int64), ("block", ctypes.c_int64),
+                ("passes", ctypes.c_int), ("traits_per_pass", ctypes.c_int), ("io_threads", ctypes.c_int)]
+
+
 class FglsError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[{code}] {msg}")
@@ -79,6 +88,7 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_block_reduce_dosage": (i32, [vp, dp, i64, i32, ctypes.c_double, vp]),
             "fgls_hip_solve_block_dosage": (i32, [vp, dp, i64, i32, ctypes.c_double, dp, dp, i64, vp]),
             "fgls_hip_i8_to_f64": (i32, [dp, i64, i32, i32, dp, i64, vp]),
+            "fgls_hip_check_i8": (i32, [dp, i64, i32, i32, dp, vp]),
             "fgls_hip_u2_to_i8": (i32, [dp, i64, i32, i32, dp, i64, dp, vp]),
             "fgls_hip_i16_to_dosage_i8": (i32, [dp, i64, i32, i32, dp, i64, dp, vp]),
             "fgls_hip_i16_to_f64": (i32, [dp, i64, i32, i32, ctypes.c_double, dp, i64, vp]),
@@ -96,6 +106,7 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_adjoint_end": (i32, [vp, dp, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_double), vp]),
             "fgls_factor_M": (i32, [i32, dp, i64, ctypes.c_double, ctypes.c_double, dp, i64, i32]),
+            "fgls_last_run_stats": (i32, [ctypes.POINTER(RunStats)]),
             "fgls_chol_gpu": (i32, [i32] * 9 + [ctypes.c_char_p] * 8),
             "fgls_chol_gpu_ex": (i32, [i32] * 9 + [ctypes.c_char_p] * 8 + [ctypes.c_void_p]),
             "fgls_eigen": (i32, [i32] * 9 + [ctypes.c_char_p] * 8),
@@ -137,11 +148,13 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_adjoint_block_i8",
                         "fgls_hip_set_trait_phi", "fgls_hip_get_factor", "fgls_hip_to_dosage_i8",
                         "fgls_hip_block_reduce_dosage", "fgls_hip_solve_block_dosage",
-                        "fgls_hip_i8_to_f64", "fgls_hip_u2_to_i8", "fgls_hip_x8t_bytes",
+                        "fgls_hip_i8_to_f64", "fgls_hip_check_i8", "fgls_hip_u2_to_i8",
+                        "fgls_hip_x8t_bytes",
                         "fgls_hip_i16_to_dosage_i8", "fgls_hip_i16_to_f64",
                         "fgls_hip_tile_i8", "fgls_hip_block_reduce_i8t", "fgls_hip_solve_block_i8t",
                         "fgls_hip_eig_solve_block_i8t",
-                        "fgls_factor_M", "fgls_eigen", "fgls_eigen_gpu_ex", "fgls_eigen_decompose",
+                        "fgls_factor_M", "fgls_last_run_stats", "fgls_eigen", "fgls_eigen_gpu_ex",
+                        "fgls_eigen_decompose",
                         "fgls_hip_eigen_decompose",
                         "fgls_hip_eig_create", "fgls_hip_eig_destroy", "fgls_hip_eig_ldn",
                         "fgls_hip_eig_set_mode", "fgls_hip_eig_set_basis", "fgls_hip_eig_set_traits",
@@ -305,18 +318,7 @@ class DeviceContext:
     def to_i8(self, X, ncols: int | None = None, X8=None, stream=None):
         """FP64 block (ncols, ldx) -> int8 block (ncols, ld8); raises if a value
         is not an integer in [-127, 127]."""
-        import torch
-        _need_cuda_f64(X, "X")
-        ncols = X.shape[0] if ncols is None else ncols
-        if X8 is None:
-            X8 = torch.empty((ncols, self.ld8), dtype=torch.int8, device=X.device)
-        bad = torch.zeros(1, dtype=torch.int32, device=X.device)
-        _check(lib().fgls_hip_to_i8(_ptr(X), X.shape[1], self.n, ncols, _ptr(X8), X8.shape[1],
-                                    _ptr(bad), _stream(stream)))
-        nb = int(bad.item())
-        if nb:
-            raise FglsError(FGLS_ERR_ARG, f"{nb} genotype values are not integers in [-127, 127]")
-        return X8
+        return to_i8(X, self.n, ncols, X8, stream)
 
     def to_dosage_i8(self, X, scale: float = 1000.0, ncols: int | None = None, X8=None, stream=None):
         """FP64 dosages (ncols, ldx) -> two int8 digit planes, a (2 * ncols, ld8)
@@ -650,16 +652,38 @@ def tile_i8(X8, n: int, count: int | None = None, planes: int = 1, X8T=None, str
 def pack_i16(X, scale: float = 1000.0):
     """Dosages X (n x m; column j = SNP j) with a fixed number of decimals ->
     the 16-bit layout of xr_format=i16: an (m, n) int16 array of
-    v = rint(x * scale); raises unless |x * scale - v| <= 1e-6 and v fits in
+    v = rint(x * scale); raises unless x * scale is within 8 eps max(1, |x * scale|)
+    of v (eps = 2^-52, the device conversion's test) and v fits in
     [-16256, 16383]."""
     import numpy as np
     X = np.asarray(X, dtype=np.float64)
     if X.ndim == 1:
         X = X[:, None]
-    v = np.rint(X * scale)
-    if np.any(np.abs(X * scale - v) > 1e-6) or v.min(initial=0) < -16256 or v.max(initial=0) > 16383:
+    y = X * scale
+    v = np.rint(y)
+    tol = 8.0 * np.finfo(np.float64).eps * np.maximum(1.0, np.abs(y))
+    if np.any(np.abs(y - v) > tol) or v.min(initial=0) < -16256 or v.max(initial=0) > 16383:
         raise FglsError(FGLS_ERR_ARG, f"dosages must be multiples of 1/{scale:g} within [-16256, 16383]/scale")
     return np.ascontiguousarray(v.astype(np.int16).T)
+
+
+def to_i8(X, n: int, ncols: int | None = None, X8=None, stream=None):
+    """FP64 genotypes X ((ncols, ldx) CUDA tensor, rows 0..n-1 of each column
+    used) -> int8 (ncols, ceil(n/64)*64) column-major block, the input of the
+    int8 paths (fgls_hip_to_i8); raises if a value is not an integer in
+    [-127, 127]."""
+    import torch
+    _need_cuda_f64(X, "X")
+    ncols = X.shape[0] if ncols is None else ncols
+    if X8 is None:
+        X8 = torch.empty((ncols, (n + 63) // 64 * 64), dtype=torch.int8, device=X.device)
+    bad = torch.zeros(1, dtype=torch.int32, device=X.device)
+    _check(lib().fgls_hip_to_i8(_ptr(X), X.shape[1], n, ncols, _ptr(X8), X8.shape[1], _ptr(bad),
+                                _stream(stream)))
+    nb = int(bad.item())
+    if nb:
+        raise FglsError(FGLS_ERR_ARG, f"{nb} genotype values are not integers in [-127, 127]")
+    return X8
 
 
 def u2_to_i8(P, n: int, X8=None, stream=None):
@@ -723,6 +747,14 @@ def fgls_chol_gpu(n, p, m, t, wXL, wXR, x_b, y_b, num_threads,
     rc = L.fgls_chol_gpu_ex(*args, *paths, opts)
     _check(rc)
     return rc
+
+
+def last_run_stats() -> dict:
+    """Timings of this thread's last successful fgls_chol_gpu / fgls_eigen
+    call (fgls_last_run_stats), as a dict of the fgls_run_stats fields."""
+    st = RunStats()
+    _check(lib().fgls_last_run_stats(ctypes.byref(st)))
+    return {name: getattr(st, name) for name, _ in RunStats._fields_}
 
 
 def fgls_eigen(n, p, m, t, wXL, wXR, x_b, y_b, num_threads,

@@ -20,7 +20,11 @@
  *     reported as "[Error]: <message>" on stderr followed by
  *     exit(EXIT_FAILURE) (the reference's error_msg, common.c:11-17) instead
  *     of crashing; no LXR.out file is created (fgls_chol_gpu.c:338);
- *   - no per-block progress lines on stdout (fgls_chol_gpu.c:501-504).
+ *   - no per-block progress lines on stdout (fgls_chol_gpu.c:501-504);
+ *   - p = wXL + wXR <= 16 (the per-SNP p x p stage is unrolled per p);
+ *     larger p is an error ("[Error]: ..." + exit), which the reference
+ *     does not have.  precision=mixed (fgls_chol_gpu_ex options) further
+ *     needs wXR = 1 and n <= 133,000.
  * y_b is ignored with the reference's warning (fgls_chol_gpu.c:185-186);
  * num_threads sets the host BLAS threads for the per-trait Cholesky of M.
  */
@@ -91,6 +95,31 @@ int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR,
                      const char *Phi_path, const char *h2_path, const char *sigma2_path,
                      const char *XL_path, const char *XR_path, const char *Y_path,
                      const char *B_path, const char *V_path, const char *options);
+
+/*
+ * Timings of the calling thread's last successful fgls_chol_gpu_ex /
+ * fgls_eigen_gpu_ex call (the reference reports wall time only, driver.c:52-100;
+ * its VampirTrace regions, fgls_chol_gpu.c:57-71, are the counterpart).
+ * Seconds from a monotonic clock unless stated.
+ */
+typedef struct {
+  double total_s;       /* the whole call: file reads, setups, block loops, writes */
+  double factor_s;      /* host dpotrf of M (potrf=host) or the eigendecomposition */
+  double trait_s;       /* per-trait setups (device factor, L^-1, precompute) */
+  double loop_s;        /* the block loops (read, upload, solve, write back) */
+  double read_busy_s;   /* reader thread time inside pread of X_R */
+  double h2d_busy_s;    /* copy-stream time of the X_R uploads (HIP events) */
+  double xr_bytes;      /* bytes of X_R read (every pass) */
+  double h2d_bytes;     /* bytes of X_R uploaded */
+  int64_t trait_snps;   /* m * t */
+  int64_t block;        /* SNPs per device launch */
+  int passes;           /* reads of X_R (t / traits_per_pass, rounded up) */
+  int traits_per_pass;
+  int io_threads;
+} fgls_run_stats;
+
+/* FGLS_OK and *out filled (all zero before the first successful call). */
+int fgls_last_run_stats(fgls_run_stats *out);
 
 /*
  * Host factor of one trait's covariance: L (column-major, ld ldl) receives

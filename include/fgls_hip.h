@@ -143,8 +143,10 @@ int fgls_hip_to_i8(const double *dX, int64_t ldx, int n, int ncols,
  * (ld ldx8): lo at dX8, hi at dX8 + count * ldx8.  The product costs twice
  * the int8 work of genotypes; b and V as fgls_hip_solve_block_i8.
  * to_dosage_i8 builds the planes from FP64 values (ncols columns, planes
- * ncols * ldx8 apart), counting into *dbad the values that are not within
- * 1e-6 of an integer after scaling or out of range.  Asynchronous.
+ * ncols * ldx8 apart), counting into *dbad the values out of range and those
+ * whose y = x * scale is farther than 8 eps max(1, |y|) (eps = 2^-52) from
+ * an integer: exact multiples of 1/scale pass, anything else is an error,
+ * never rounded.  Asynchronous.
  */
 int fgls_hip_to_dosage_i8(const double *dX, int64_t ldx, int n, int ncols, double scale,
                           int8_t *dX8, int64_t ldx8, unsigned int *dbad, void *stream);
@@ -192,6 +194,13 @@ int fgls_hip_i16_to_dosage_i8(const int16_t *dV, int64_t ldv, int n, int ncols,
                               int8_t *dX8, int64_t ldx8, unsigned int *dbad, void *stream);
 int fgls_hip_i16_to_f64(const int16_t *dV, int64_t ldv, int n, int ncols, double scale,
                         double *dX, int64_t ldx, void *stream);
+/* Count into *dbad (device counter, caller-cleared) the values of -128 in
+ * rows 0..n-1 of ncols int8 genotype columns (ld ldx8): the int8 product
+ * takes |x| <= 127 only.  For int8 genotypes that did not come through
+ * fgls_hip_to_i8 (e.g. an int8 X_R file).  No counterpart in the
+ * reference.  Asynchronous. */
+int fgls_hip_check_i8(const int8_t *dX8, int64_t ldx8, int n, int ncols,
+                      unsigned int *dbad, void *stream);
 /* int8 genotypes (ld ldx8) -> FP64 (ld ldx, rows n..ldx-1 zeroed), for
  * compact genotype input on the FP64 path.  No counterpart in the
  * reference (its X_R is FP64 only, io.c:35-55).  Asynchronous. */

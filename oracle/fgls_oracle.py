@@ -15,7 +15,8 @@ variant (src/fgls_chol_gpu.c):
                         fgls_chol.c:386-439  (per-SNP stage)
 * ``fgls_chol``         fgls_chol.c:223-491  (trait loop, output layout
                                               fgls_chol.c:458-474)
-* ``read_inputs`` / ``write_outputs``  io.c:7-33 raw little-endian doubles.
+* ``read_doubles`` / ``read_inputs``  io.c:7-19 raw little-endian doubles (the
+                        output files are written by the tests with numpy, io.c:21-33).
 * ``eigen_decompose``, ``eigen_trait``, ``eigen_solve_snps``, ``fgls_eigen``
                         the eigenbasis solver src/fgls_eigen.c: dsyevr of Phi
                         (:780-806), Winv = sqrt(1/(sigma h W + sigma (1-h)))
@@ -350,3 +351,41 @@ def parity_errors(B, V, Bref, Vref, p: int) -> dict:
             "B_norm": float(np.max(bn)) if bn.size else 0.0,
             "V_norm": float(np.max(vn)) if vn.size else 0.0,
             "V_upper_zero": bool(np.all(V[:, r[r != c], c[r != c]] == 0.0)) if p > 1 else True}
+
+
+def error_stats(B, V, Bref, Vref, p: int) -> dict:
+    """Distribution of the element-wise relative errors (driver.c:159-177's
+    |a - b| / |b|, entry by entry) of the coefficients b and of the variances
+    var(b) = se^2 (the squared diagonal of V), both in the on-disk layout.
+
+    Returns, for "b" and "var": the maximum, the median and the 99.9th
+    percentile over all entries, and for "b" also the maximum over the
+    entries with |b_ref| >= 1e-3 se (coefficients not negligible against
+    their standard error; nearer zero the relative error of any two FP64
+    computations grows without bound).  A zero reference entry counts as
+    error 0 when the other entry is zero too, else as infinity."""
+    B = np.asarray(B, dtype=np.float64).reshape(-1, p)
+    Bref = np.asarray(Bref, dtype=np.float64).reshape(-1, p)
+    V = np.asarray(V, dtype=np.float64).reshape(-1, p, p)
+    Vref = np.asarray(Vref, dtype=np.float64).reshape(-1, p, p)
+    se = np.einsum("icc->ic", Vref)
+    var_ref, var_new = se ** 2, np.einsum("icc->ic", V) ** 2
+
+    def rel(a, b):
+        d = np.abs(a - b)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = d / np.abs(b)
+        r[(b == 0) & (d == 0)] = 0.0
+        return r.ravel()
+
+    def dist(r):
+        if r.size == 0:
+            return {"max": 0.0, "median": 0.0, "p999": 0.0}
+        return {"max": float(r.max()), "median": float(np.median(r)), "p999": float(np.quantile(r, 0.999))}
+
+    eb = rel(B, Bref)
+    big = (np.abs(Bref) >= 1e-3 * se).ravel()
+    out = {"b": dist(eb), "var": dist(rel(var_new, var_ref)), "entries": int(eb.size)}
+    out["b"]["max_if_b_ge_1e-3_se"] = float(eb[big].max()) if big.any() else 0.0
+    out["b"]["share_b_ge_1e-3_se"] = float(big.mean()) if big.size else 0.0
+    return out

@@ -14,15 +14,22 @@ import torch
 
 def dense_loss(y, h, Phi, XL, XR, w, sigma=1.0):
     """y: (n,) tensor, h: 0-dim tensor; Phi n x n, XL n x wXL, XR n x m,
-    w (m,) as float64 CPU tensors.  Returns the 0-dim loss tensor."""
+    w (m,) as float64 CPU tensors.  Returns the 0-dim loss tensor.
+
+    S_i = X_i' M^-1 X_i and r_i = X_i' M^-1 y are assembled from the two
+    dense products M^-1 X_L and M^-1 X_R (blocks shared by every SNP, and one
+    column per SNP), then solved per SNP."""
     n = Phi.shape[0]
     M = sigma * (h * Phi + (1.0 - h) * torch.eye(n, dtype=torch.float64))
     Minv = torch.linalg.inv(M)
-    m = XR.shape[1]
-    X = torch.cat([XL.unsqueeze(0).expand(m, -1, -1), XR.T.unsqueeze(2)], dim=2)   # m x n x p
-    MX = Minv @ X                                                                  # m x n x p
-    S = X.transpose(1, 2) @ MX                                                     # m x p x p
-    r = (MX.transpose(1, 2) @ y).unsqueeze(2)                                      # m x p x 1
+    m, wXL = XR.shape[1], XL.shape[1]
+    ML, MR = Minv @ XL, Minv @ XR                                  # n x wXL, n x m
+    A = (XL.T @ ML).expand(m, wXL, wXL)                            # X_L' M^-1 X_L
+    c = XR.T @ ML                                                  # m x wXL: x_i' M^-1 X_L
+    q = (XR * MR).sum(0)                                           # m: x_i' M^-1 x_i
+    S = torch.cat([torch.cat([A, c.unsqueeze(2)], 2),
+                   torch.cat([c.unsqueeze(1), q.view(m, 1, 1)], 2)], 1)   # m x p x p
+    r = torch.cat([(ML.T @ y).expand(m, wXL), (MR.T @ y).unsqueeze(1)], 1).unsqueeze(2)
     b = torch.linalg.solve(S, r)[..., 0]
     beta = b[:, -1]
     return (w * beta * beta).sum()

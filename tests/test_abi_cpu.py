@@ -150,6 +150,9 @@ def test_pack_i16_layout(pkg):
     P = pkg.pack_i16(X, 1000.0)
     assert P.dtype == np.int16 and P.shape == (2, 3)
     assert P.tolist() == [[0, 500, 2000], [1999, 1, 1234]]
-    for bad in (np.array([[0.0005]]), np.array([[17.0]])):
+    for bad in (np.array([[0.0005]]), np.array([[17.0]]), np.array([[1.234 + 1e-11]])):
         with pytest.raises(pkg.FglsError):
             pkg.pack_i16(bad, 1000.0)
+    # every 3-decimal value in [0, 2] passes (x * 1000 is within 2 eps of v)
+    allv = np.arange(2001) / 1000.0
+    assert np.array_equal(pkg.pack_i16(allv, 1000.0)[0], np.arange(2001))

@@ -202,6 +202,26 @@ def test_int8_genotype_file(pkg, gpu, tmp_path, precision):
                           f("XL.in"), f("XR8s.in"), f("Y.in"), f("B_s.out"), f("V_s.out"),
                           options={"precision": precision, "xr_format": "i8"})
     assert ei.value.code == pkg.FGLS_ERR_IO and "not read completely" in ei.value.msg
+    # -128 is outside the int8 product's range: a defined error on the mixed
+    # path; the FP64 path converts it exactly and compares with the oracle
+    X = prob.XR.astype(np.int8)
+    X[12, 1033] = -128
+    X.ravel(order="F").tofile(f("XR8neg.in"))
+    opts = {"precision": precision, "xr_format": "i8", "block": 512}
+    if precision == "mixed":
+        with pytest.raises(pkg.FglsError) as ei:
+            pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, 256, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                              f("XL.in"), f("XR8neg.in"), f("Y.in"), f("B_n.out"), f("V_n.out"), options=opts)
+        assert ei.value.code == pkg.FGLS_ERR_ARG and "1 genotype values of SNPs 1024" in ei.value.msg
+    else:
+        pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, 256, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f("XR8neg.in"), f("Y.in"), f("B_n.out"), f("V_n.out"), options=opts)
+        XR = prob.XR.copy()
+        XR[12, 1033] = -128.0
+        Bo, Vo = orc.fgls_chol(prob.Phi, H, Sig, prob.XL, XR, prob.Y, wXR=1)
+        e = orc.parity_errors(read_out(d, "B_n.out", t * m * p), read_out(d, "V_n.out", t * m * p * p),
+                              Bo, Vo, p)
+        assert e["B_scaled"] < 1e-10 and e["V_scaled"] < 1e-10, e
 
 
 def test_fgls_chol_gpu_mixed_dosages(pkg, gpu, tmp_path):

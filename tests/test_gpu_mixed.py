@@ -120,6 +120,13 @@ def test_mixed_dosages_match_oracle(pkg, gpu, n, m, wXL, block):
     bad[0, 5] += 0.0004
     with pytest.raises(pkg.FglsError):
         ctx.to_dosage_i8(bad, 1000.0)
+    # ... also when the offset is tiny (1e-8 / scale): nothing is rounded
+    bad = XR[:4].clone()
+    bad[1, 7] += 1e-11
+    with pytest.raises(pkg.FglsError) as ei:
+        ctx.to_dosage_i8(bad, 1000.0)
+    assert "1 dosage values" in ei.value.msg
+    ctx.to_dosage_i8(XR[:4].clone(), 1000.0)      # the exact values pass
     ctx.close()
 
 

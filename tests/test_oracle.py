@@ -287,3 +287,21 @@ def test_adjoint_oracle_catches_planted_mistakes():
     assert np.max(np.abs(gy - gy_r)) <= tol_y and abs(gh - gh_r) <= 1e-9 * abs(gh_r)
     assert np.max(np.abs(0.5 * gy - gy_r)) > 1e6 * tol_y
     assert abs(-gh - gh_r) > 1e6 * 1e-9 * abs(gh_r)
+
+
+def test_error_stats_by_hand():
+    """error_stats on four hand-checked entries (p = 1): relative errors of b
+    are 0, 0.2/2.2, 0.9 and 0 (0/0); only the first two have |b| >= 1e-3 se;
+    var = se^2 gives 0, 3, 0, 0."""
+    B, Bref = np.array([1.0, 2.0, 1e-6, 0.0]), np.array([1.0, 2.2, 1e-5, 0.0])
+    V, Vref = np.array([1.0, 2.0, 1.0, 3.0]), np.array([1.0, 1.0, 1.0, 3.0])
+    s = orc.error_stats(B, V, Bref, Vref, 1)
+    assert s["entries"] == 4
+    assert s["b"]["max"] == pytest.approx(0.9) and s["b"]["median"] == pytest.approx(0.1 / 2.2)
+    assert s["b"]["max_if_b_ge_1e-3_se"] == pytest.approx(0.2 / 2.2)
+    assert s["b"]["share_b_ge_1e-3_se"] == 0.5
+    assert s["var"]["max"] == pytest.approx(3.0) and s["var"]["median"] == 0.0
+    # p > 1: only the diagonal of V enters var, in the on-disk order c*p + r
+    V2 = np.array([[4.0, 0.5, 0.0, 9.0]])
+    s2 = orc.error_stats(np.ones((1, 2)), V2 * [[1.0, 1.0, 1.0, 2.0]], np.ones((1, 2)), V2, 2)
+    assert s2["var"]["max"] == pytest.approx(3.0) and s2["b"]["max"] == 0.0
