@@ -31,6 +31,7 @@
 #include <stdarg.h>
 #include <string.h>
 #include <float.h>
+#include <mutex>
 #include <math.h>
 
 #include "fgls_device.h"
@@ -960,22 +961,37 @@ int set_device(const fgls_hip_ctx* c) {
   return FGLS_OK;
 }
 
+// b and V from RT row tiles of partial sums in `part` (leading dimension
+// ldp): the context's own buffer, or the no-alloc entry point's scratch.
+template <int P>
+void launch_finalize_from(const fgls_hip_ctx* c, int RT, const double* part, int64_t ldp, int count,
+                          double* dB, double* dV, int64_t base, hipStream_t st) {
+  dim3 grid((count + 127) / 128);
+  hipLaunchKernelGGL(k_finalize<P>, grid, dim3(128), 0, st, count, RT, part, ldp, 1, c->trait,
+                     (int64_t)0, dB, (int64_t)0, dV, (int64_t)0, c->err, base);
+}
+
 template <int P>
 void launch_finalize(const fgls_hip_ctx* c, int count, double* dB, double* dV, int64_t base,
                      hipStream_t st) {
-  dim3 grid((count + 127) / 128);
-  hipLaunchKernelGGL(k_finalize<P>, grid, dim3(128), 0, st, count, c->last_rt, c->part,
-                     (int64_t)c->max_block, 1, c->trait, (int64_t)0, dB, (int64_t)0, dV, (int64_t)0,
-                     c->err, base);
+  launch_finalize_from<P>(c, c->last_rt, c->part, (int64_t)c->max_block, count, dB, dV, base, st);
+}
+
+// The FP64 triangular product with the fused per-row-tile reductions into
+// `part` (leading dimension ldp).
+template <int P>
+void launch_trmm_reduce_into(const fgls_hip_ctx* c, const double* X, int64_t ldx, int count, double* part,
+                             int64_t ldp, hipStream_t st) {
+  dim3 grid((count + TN - 1) / TN, c->RT);
+  hipLaunchKernelGGL((k_trmm<EPI_REDUCE, P + 1>), grid, dim3(256), 0, st, c->n, count, c->RT,
+                     c->Linv, (int64_t)c->ldn, X, ldx, (double*)nullptr, (int64_t)0, c->W,
+                     (int64_t)c->ldn, part, ldp);
 }
 
 template <int P>
 void launch_trmm_reduce(const fgls_hip_ctx* c, const double* X, int64_t ldx, int count,
                         hipStream_t st) {
-  dim3 grid((count + TN - 1) / TN, c->RT);
-  hipLaunchKernelGGL((k_trmm<EPI_REDUCE, P + 1>), grid, dim3(256), 0, st, c->n, count, c->RT,
-                     c->Linv, (int64_t)c->ldn, X, ldx, (double*)nullptr, (int64_t)0, c->W,
-                     (int64_t)c->ldn, c->part, (int64_t)c->max_block);
+  launch_trmm_reduce_into<P>(c, X, ldx, count, c->part, (int64_t)c->max_block, st);
 }
 
 constexpr int NSLICE = FGLS_NSLICE;   // int8 digit slices of L^-1 in the mixed mode
@@ -1640,6 +1656,93 @@ int fgls_hip_check(fgls_hip_ctx* c, void* stream, int64_t* bad_snp, int* info) {
   HIP_TRY(hipMemsetAsync(c->err, 0xff, sizeof(unsigned long long), st));
   HIP_TRY(hipStreamSynchronize(st));
   return fail(FGLS_ERR_NOTSPD, "dpotrf(V) failed (info: %d) - i: %lld", inf, (long long)snp);
+}
+
+// ---------------------------------------------------------------------------
+// fgls_hip_solve_block_noalloc: fgls_hip_solve_block with its scratch in a
+// file-scope device array instead of context buffers, for callers that cannot
+// allocate.  The scratch holds the per-row-tile partial sums of the fused
+// product (wXR = 1) or the transformed columns L^-1 X_R (wXR > 1); every
+// entry a call reads was written earlier in the same call, so nothing is
+// cleared.  Each device holds its own copy for the life of the process;
+// calls on one device share it, so calls that overlap in time on one device
+// (other streams, other threads) race.
+
+}  // extern "C"
+
+namespace {
+
+constexpr size_t kScratchBytes = FGLS_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices];   // device address of `scratch` per device
+
+// The device address of this device's copy of `scratch` (the symbol's name
+// is a host-side stand-in); looked up once per device with that device
+// current.
+int scratch_ptr(int device, double** out) {
+  if (device < 0 || device >= kMaxScratchDevices)
+    return fail(FGLS_ERR_ARG, "device %d beyond the %d devices the scratch table holds", device,
+                kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FGLS_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = (double*)p;
+  }
+  *out = g_scratch_ptr[device];
+  return FGLS_OK;
+}
+
+// SNPs per call that fit the scratch: (p + 1) partial sums per 128-row tile
+// and SNP for wXR = 1, the wXR transformed columns (ldn rows) otherwise.
+int noalloc_max_count(const fgls_hip_ctx* c) {
+  const size_t per = c->wXR == 1 ? (size_t)c->RT * (c->p + 1) * sizeof(double)
+                                 : (size_t)c->ldn * c->wXR * sizeof(double);
+  const size_t cap = kScratchBytes / per;
+  return (int)(cap < (size_t)INT32_MAX ? cap : (size_t)INT32_MAX);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgls_hip_solve_block_noalloc(fgls_hip_ctx* c, const double* dXR, int64_t ldxr, int count, double* dB,
+                                 double* dV, int64_t snp_base, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->have_trait) return fail(FGLS_ERR_STATE, "no trait set");
+  const int cap = noalloc_max_count(c);
+  if (count < 0 || count > cap)
+    return fail(FGLS_ERR_ARG,
+                "count=%d outside [0, %d]: the no-alloc solve holds its scratch in %zu bytes "
+                "(at n=%d, p=%d, wXR=%d at most %d SNPs per call)",
+                count, cap, kScratchBytes, c->n, c->p, c->wXR, cap);
+  if (count == 0) return FGLS_OK;
+  if (!dXR || !dB || !dV) return fail(FGLS_ERR_ARG, "NULL pointer");
+  if (ldxr < c->n || (ldxr & 3) || !aligned32(dXR))
+    return fail(FGLS_ERR_ARG, "X_R needs ld >= n, ld %% 4 == 0 and 32-byte alignment");
+  if ((rc = set_device(c))) return rc;
+  double* ws = nullptr;
+  if ((rc = scratch_ptr(c->device, &ws))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (c->wXR == 1) {
+    const int64_t ldp = count;
+    FGLS_P_SWITCH(c->p, launch_trmm_reduce_into, c, dXR, ldxr, count, ws, ldp, st);
+    FGLS_P_SWITCH(c->p, launch_finalize_from, c, c->RT, ws, ldp, count, dB, dV, snp_base, st);
+  } else {
+    dim3 grid((count * c->wXR + TN - 1) / TN, c->RT);
+    hipLaunchKernelGGL((k_trmm<EPI_STORE, 2>), grid, dim3(256), 0, st, c->n, count * c->wXR, c->RT, c->Linv,
+                       (int64_t)c->ldn, dXR, ldxr, ws, (int64_t)c->ldn, c->W, (int64_t)c->ldn,
+                       (double*)nullptr, (int64_t)0);
+    FGLS_P_SWITCH(c->p, launch_snp_stage, c, ws, (int64_t)c->ldn, count, dB, dV, snp_base, st);
+  }
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
 }
 
 }  // extern "C"

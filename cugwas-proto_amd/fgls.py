@@ -74,6 +74,7 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_set_trait": (i32, [vp, dp, i64, dp, i64, dp, vp]),
             "fgls_hip_get_trait": (i32, [vp, dp, dp, dp]),
             "fgls_hip_solve_block": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
+            "fgls_hip_solve_block_noalloc": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
             "fgls_hip_block_reduce": (i32, [vp, dp, i64, i32, vp]),
             "fgls_hip_block_finalize": (i32, [vp, i32, dp, dp, i64, vp]),
             "fgls_hip_ctx_set_mode": (i32, [vp, i32]),
@@ -140,7 +141,8 @@ def lib() -> ctypes.CDLL:
 def exported_symbols() -> list[str]:
     return [s for s in ("fgls_hip_last_error", "fgls_hip_device_count", "fgls_hip_ctx_create",
                         "fgls_hip_ctx_destroy", "fgls_hip_ctx_ldn", "fgls_hip_set_trait",
-                        "fgls_hip_get_trait", "fgls_hip_solve_block", "fgls_hip_block_reduce",
+                        "fgls_hip_get_trait", "fgls_hip_solve_block", "fgls_hip_solve_block_noalloc",
+                        "fgls_hip_block_reduce",
                         "fgls_hip_block_finalize", "fgls_hip_ctx_set_mode", "fgls_hip_block_reduce_i8",
                         "fgls_hip_solve_block_i8", "fgls_hip_to_i8", "fgls_hip_trsm",
                         "fgls_hip_trmm", "fgls_hip_snp_stage", "fgls_hip_check",
@@ -293,6 +295,18 @@ class DeviceContext:
             V = torch.empty((count, self.p * self.p), dtype=torch.float64, device=XR.device)
         _check(lib().fgls_hip_solve_block(self._h, _ptr(XR), XR.shape[1], count, _ptr(B), _ptr(V),
                                           snp_base, _stream(stream)))
+        return B, V
+
+    def solve_block_noalloc(self, XR, B, V, count: int | None = None, snp_base: int = 0, stream=None):
+        """fgls_hip_solve_block_noalloc: solve_block with its scratch in the
+        library's static device array (no allocation; B and V are given)."""
+        _need_cuda_f64(XR, "XR")
+        _need_cuda_f64(B, "B")
+        _need_cuda_f64(V, "V")
+        if count is None:
+            count = XR.shape[0] // self.wXR
+        _check(lib().fgls_hip_solve_block_noalloc(self._h, _ptr(XR), XR.shape[1], count, _ptr(B), _ptr(V),
+                                                  snp_base, _stream(stream)))
         return B, V
 
     def block_reduce(self, XR, count: int | None = None, stream=None):

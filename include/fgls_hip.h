@@ -235,6 +235,30 @@ int fgls_hip_snp_stage(fgls_hip_ctx *ctx, const double *dZ, int64_t ldz,
                        void *stream);
 
 /*
+ * fgls_hip_solve_block for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, results
+ * and failure records (fgls_hip_check) as fgls_hip_solve_block, FP64
+ * arithmetic in either mode; it calls no hipMalloc, hipFree or allocating
+ * library routine.  Its scratch -- the per-row-tile partial sums of the
+ * fused product (wXR = 1) or the transformed columns L^-1 X_R (wXR > 1) --
+ * is one file-scope device array of FGLS_NOALLOC_SCRATCH_BYTES that each
+ * device holds for the life of the process, so count is limited to
+ *   wXR = 1:  count * ceil(n/128) * (p + 1) * 8 <= FGLS_NOALLOC_SCRATCH_BYTES
+ *             (21,236 SNPs at n = 10,000, p = 4)
+ *   wXR > 1:  count * wXR * ceil(n/128)*128 * 8 <= FGLS_NOALLOC_SCRATCH_BYTES
+ * and a larger count is FGLS_ERR_ARG.  Every call on a device shares that
+ * device's array: calls that overlap in time on one device (other streams
+ * or threads) race.  The context's own partial sums are untouched, so a
+ * pending fgls_hip_block_reduce / fgls_hip_block_finalize pair stays valid.
+ * Asynchronous on `stream`.
+ */
+#define FGLS_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int fgls_hip_solve_block_noalloc(fgls_hip_ctx *ctx, const double *dXR, int64_t ldxr,
+                                 int count, double *dB, double *dV, int64_t snp_base,
+                                 void *stream);
+
+/*
  * Synchronise the context's work on `stream` and report a recorded per-SNP
  * factorisation failure: returns FGLS_ERR_NOTSPD with *bad_snp (0-based,
  * global index) and *info (LAPACK-style pivot index, 1-based), after which
