@@ -170,7 +170,9 @@ def read_rate(path, nbytes, threads, direct=False):
     nbytes = nbytes // piece * piece
     if nbytes == 0:
         return None
-    buf = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True).numpy()
+    raw = torch.empty(nbytes + 4096, dtype=torch.uint8, pin_memory=True)
+    skip = (-raw.data_ptr()) % 4096                 # O_DIRECT wants page-aligned buffers
+    buf = raw[skip:skip + nbytes].numpy()
     flags = os.O_RDONLY | (os.O_DIRECT if direct else 0)
     try:
         fd = os.open(path, flags)
@@ -216,7 +218,7 @@ def run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, ref, dev):
     m = min(args.pipeline_m, X.shape[0])
     d = tempfile.mkdtemp(prefix="fgls_pipe_", dir=args.pipeline_dir)
     f = lambda s: os.path.join(d, s)
-    out = {"m": m, "n": n, "p": p, "block": min(args.block, m), "dir_fs": None, "runs": []}
+    out = {"m": m, "n": n, "p": p, "runs": []}
     try:
         t0 = time.perf_counter()
         np.ascontiguousarray(Phi.cpu().numpy(), dtype="<f8").tofile(f("Phi.in"))
@@ -225,28 +227,40 @@ def run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, ref, dev):
         XL.T.contiguous().cpu().numpy().astype("<f8").tofile(f("XL.in"))
         y.cpu().numpy().astype("<f8").tofile(f("Y.in"))
         C = 16384
+        nb = (n + 3) // 4
         with open(f("XR.in"), "wb") as a, open(f("XR8.in"), "wb") as b, open(f("XR2.in"), "wb") as c:
             for i0 in range(0, m, C):
                 i1 = min(i0 + C, m)
-                a.write(X[i0:i1, :n].cpu().numpy().tobytes())
-                x8 = X8[i0:i1, :n].cpu().numpy()
-                b.write(x8.tobytes())
-                c.write(pkg.pack_u2(x8.T).tobytes())
+                a.write(X[i0:i1, :n].contiguous().cpu().numpy().tobytes())
+                b.write(X8[i0:i1, :n].contiguous().cpu().numpy().tobytes())
+                # the 2-bit layout of pack_u2 (individual r in bits 2(r%4).. of
+                # byte r/4), formed on the device
+                q = torch.zeros((i1 - i0, 4 * nb), dtype=torch.uint8, device=dev)
+                q[:, :n] = X8[i0:i1, :n].to(torch.uint8)
+                q = q.view(i1 - i0, nb, 4)
+                u2 = q[:, :, 0] | (q[:, :, 1] << 2) | (q[:, :, 2] << 4) | (q[:, :, 3] << 6)
+                c.write(u2.cpu().numpy().tobytes())
+                del q, u2
         log(f"pipeline: files of {m} SNPs written in {time.perf_counter() - t0:.1f} s "
             f"(XR.in {os.path.getsize(f('XR.in')) / 1e9:.1f} GB)")
         xr_bytes = os.path.getsize(f("XR.in"))
         rates = {"page_cache_GBps": {}, "o_direct_GBps": None}
-        for th in (4, 16):
+        read_rate(f("XR.in"), min(xr_bytes, 8 << 30), 16)                 # warm-up
+        for th in (1, 4, 8, 16):
             rates["page_cache_GBps"][th] = read_rate(f("XR.in"), min(xr_bytes, 8 << 30), th)
         rates["o_direct_GBps"] = read_rate(f("XR.in"), min(xr_bytes, 4 << 30), 16, direct=True)
         out["standalone_read"] = rates
         log(f"pipeline: standalone reads of XR.in {rates}")
-        runs = [("f64", "fp64", 4), ("f64", "fp64", 8), ("f64", "fp64", 16), ("f64", "fp64", 0),
-                ("f64", "mixed", 0), ("i8", "mixed", 0), ("u2", "mixed", 0), ("u2", "fp64", 0)]
+        # (X_R file, precision, io_threads (0: default), SNPs per block)
+        # the first run is the drop-in's default (block = a multiple of x_b near
+        # 16,384, io_threads = the host cores up to 16) on the reference's FP64 XR.in
+        runs = [("f64", "fp64", 0, 16384), ("f64", "mixed", 0, 16384), ("f64", "fp64", 4, 16384),
+                ("f64", "fp64", 8, 16384), ("f64", "fp64", 0, 65536), ("i8", "fp64", 0, 16384),
+                ("i8", "mixed", 0, 16384), ("u2", "mixed", 0, 16384), ("u2", "fp64", 0, 16384)]
         Bref = ref["B"][:m].cpu().numpy().ravel() if ref is not None else None
-        for fmt, prec, io in runs:
+        for fmt, prec, io, blk in runs:
             xr = {"f64": "XR.in", "i8": "XR8.in", "u2": "XR2.in"}[fmt]
-            opts = {"precision": prec, "xr_format": fmt, "block": out["block"]}
+            opts = {"precision": prec, "xr_format": fmt, "block": min(blk, m)}
             if io:
                 opts["io_threads"] = io
             t0 = time.perf_counter()
@@ -254,7 +268,7 @@ def run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, ref, dev):
                               f("XL.in"), f(xr), f("Y.in"), f("B.out"), f("V.out"), options=opts)
             wall = time.perf_counter() - t0
             st = pkg.last_run_stats()
-            r = {"xr_format": fmt, "precision": prec, "io_threads": st["io_threads"],
+            r = {"xr_format": fmt, "precision": prec, "io_threads": st["io_threads"], "block": st["block"],
                  "block_loop_snps_per_s": m / st["loop_s"], "block_loop_s": st["loop_s"],
                  "whole_call_s": st["total_s"], "python_wall_s": wall, "trait_setup_s": st["trait_s"],
                  "xr_GB": st["xr_bytes"] / 1e9,
@@ -266,12 +280,17 @@ def run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, ref, dev):
                 r["B_bitwise_equal_to_device_resident_fp64"] = bool(
                     np.array_equal(np.fromfile(f("B.out"), dtype="<f8"), Bref))
             out["runs"].append(r)
-            log(f"pipeline {fmt}/{prec} io_threads={r['io_threads']}: {r['block_loop_snps_per_s']:.0f} SNPs/s "
+            log(f"pipeline {fmt}/{prec} io_threads={r['io_threads']} block={r['block']}: "
+                f"{r['block_loop_snps_per_s']:.0f} SNPs/s "
                 f"(loop {st['loop_s']:.2f} s, call {st['total_s']:.2f} s), X_R read "
                 f"{r['xr_read_GBps'] or 0:.1f} GB/s in pread, {r['xr_effective_GBps']:.1f} GB/s effective, "
                 f"H2D {r['h2d_GBps'] or 0:.1f} GB/s")
     finally:
         shutil.rmtree(d, ignore_errors=True)
+    if out["runs"]:
+        r0 = out["runs"][0]
+        out["default_fp64_file"] = {k: r0[k] for k in ("block_loop_snps_per_s", "xr_read_GBps",
+                                                        "xr_effective_GBps", "h2d_GBps", "whole_call_s")}
     return out
 
 

@@ -309,23 +309,17 @@ static void *reader_main(void *arg) {
     const size_t bytes = (size_t)cnt * R->wXR * col_bytes;
     const off_t off = (off_t)i0 * R->wXR * (off_t)col_bytes;
     double t0 = now_s();
-    /* io_threads contiguous pieces in parallel (page-cache reads are
-     * memory-copy bound per thread) */
+    /* 32 MB preads handed out to io_threads threads in turn (page-cache
+     * reads are memory-copy bound per thread; small pieces keep every
+     * thread busy to the end of the block) */
     size_t got = 0;
     const int T = R->io_threads > 0 ? R->io_threads : 1;
-#pragma omp parallel num_threads(T) reduction(+ : got)
-    {
-      const int tid = omp_get_thread_num();
-      const size_t lo = bytes / (size_t)T * (size_t)tid;
-      const size_t hi = tid == T - 1 ? bytes : bytes / (size_t)T * (size_t)(tid + 1);
-      size_t g = 0;
-      while (lo + g < hi) {
-        ssize_t r = pread(R->fd, (char *)R->slot[s] + lo + g, hi - lo - g, off + (off_t)(lo + g));
-        if (r < 0 && errno == EINTR) continue;
-        if (r <= 0) break;
-        g += (size_t)r;
-      }
-      got += g;
+    const size_t piece = (size_t)32 << 20;
+    const long np = (long)((bytes + piece - 1) / piece);
+#pragma omp parallel for num_threads(T) schedule(dynamic, 1) reduction(+ : got)
+    for (long i = 0; i < np; ++i) {
+      const size_t o = (size_t)i * piece, w = (bytes - o < piece) ? bytes - o : piece;
+      got += pread_full(R->fd, (char *)R->slot[s] + o, w, off + (off_t)o);
     }
     R->busy_s += now_s() - t0;
     if (got != bytes) {
@@ -663,7 +657,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     R.m = m;
     R.block = block;
     R.nchunks = nchunks;
-    R.io_threads = opt.io_threads > 0 ? opt.io_threads : 4;
+    R.io_threads = opt.io_threads > 0 ? opt.io_threads : (host_cores() < 16 ? host_cores() : 16);
     for (int s = 0; s < 2; ++s) {
       sem_init(&R.free_[s], 0, 1);
       sem_init(&R.full[s], 0, 0);

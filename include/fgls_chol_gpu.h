@@ -57,7 +57,8 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *   serial=1       synchronise after every stage (debug aid, like the
  *                  reference's FGLS_GPU_SERIAL, fgls_chol_gpu.c:44-46)
  *   stats=1        print stage timings and rates to stderr
- *   io_threads=<k> parallel preads of each X_R block (default 4)
+ *   io_threads=<k> threads reading each X_R block in 32 MB preads (default:
+ *                  the host cores available, at most 16)
  *   dosage_scale=<s>  with precision=mixed (or with xr_format=i16): X_R values are dosages with a
  *                  fixed number of decimals (s = 1000 for 3 decimals), run as
  *                  two int8 digits per value (twice the int8 work of
