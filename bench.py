@@ -55,7 +55,7 @@ NSLICE = 6
 # WRITE_SIZE 1,048,576 KiB (x 1.0).  FETCH_SIZE counts L2 misses sent to the fabric
 # (some are served by the MALL), so this is an upper bound on HBM reads.
 TRAFFIC_N, TRAFFIC_BLOCK = 10000, 65536
-TRAFFIC_KIB = {"fp64": (106698200.6, 202240.0), "mixed": (23309193.3, 808960.0)}
+TRAFFIC_KIB = {"fp64": (106698200.6, 202240.0), "mixed8": (23309193.3, 808960.0)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
@@ -405,18 +405,18 @@ def run_eigen(torch, pkg, args, Phi, XL, y, X, X8, chol_fp64, dev):
         starts = list(range(0, m, block))
         evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in starts]
 
-        XT = (torch.empty(pkg.lib().fgls_hip_x8t_bytes(n, block, 1), dtype=torch.int8, device=dev)
-              if mode == "mixed" else None)
+        XT = torch.empty(pkg.x2t_bytes(n, block), dtype=torch.uint8, device=dev) if mode == "mixed" else None
+        bad = torch.zeros(1, dtype=torch.int32, device=dev)
 
         def step(record):
             for bi, i0 in enumerate(starts):
                 cnt = min(block, m - i0)
-                if mode == "mixed":
-                    pkg.tile_i8(X8[i0:i0 + cnt], n, cnt, X8T=XT)
+                if mode == "mixed":                 # hard calls as 2-bit tiles
+                    pkg.tile_u2(X8[i0:i0 + cnt], n, cnt, X2T=XT, bad=bad)
                 if record:
                     evs[bi][0].record()
                 if mode == "mixed":
-                    e.solve_block_i8t(XT, cnt, 1, 1.0, Bo[:, i0:i0 + cnt], Vo[:, i0:i0 + cnt], snp_base=i0)
+                    e.solve_block_u2t(XT, cnt, Bo[:, i0:i0 + cnt], Vo[:, i0:i0 + cnt], snp_base=i0)
                 else:
                     e.solve_block(X[i0:i0 + cnt], cnt, Bo[:, i0:i0 + cnt], Vo[:, i0:i0 + cnt], snp_base=i0)
                 if record:
@@ -505,9 +505,10 @@ def main():
     ap.add_argument("--check-snps", type=int, default=4096)
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "none"], default="auto")
     ap.add_argument("--cpu-snps", type=int, default=16384)
-    ap.add_argument("--paths", default="fp64,mixed,dosage",
-                    help="comma-separated: fp64, mixed (int8 genotypes), dosage (3-decimal dosages "
-                         "as two int8 digit planes)")
+    ap.add_argument("--paths", default="fp64,mixed,mixed8,dosage",
+                    help="comma-separated: fp64, mixed (hard-call genotypes as 2-bit tiles), mixed8 "
+                         "(the same genotypes as 8-bit tiles), dosage (3-decimal dosages as two int8 "
+                         "digit planes)")
     ap.add_argument("--eigen-traits", type=int, default=4,
                     help="traits of the eigenbasis variant's run over the same genotypes (0: skip)")
     ap.add_argument("--eigen-paths", default="fp64,mixed")
@@ -599,7 +600,18 @@ def main():
         timed apart."""
         nonlocal X8T
         tile = None
-        if mode in ("dosage", "mixed"):
+        if mode == "mixed":
+            # hard-call genotypes as 2-bit tiles (fgls_hip_tile_u2): the int8
+            # product loads a quarter of the genotype bytes
+            t0 = time.perf_counter()
+            ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)
+            torch.cuda.synchronize()
+            setup_s = time.perf_counter() - t0
+            X2T = torch.empty(pkg.x2t_bytes(n, block), dtype=torch.uint8, device=dev)
+            bad = torch.zeros(1, dtype=torch.int32, device=dev)
+            tile = lambda i0, cnt: pkg.tile_u2(X8[i0:i0 + cnt], n, cnt, X2T=X2T, bad=bad)
+            reduce = lambda i0, cnt: ctx.block_reduce_u2t(X2T, cnt)
+        elif mode in ("dosage", "mixed8"):
             t0 = time.perf_counter()
             ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)      # slices L^-1 of the current trait
             torch.cuda.synchronize()
@@ -657,6 +669,8 @@ def main():
             log(f"{mode} step {k + 1}/{args.steps} done")
         elapsed = time.perf_counter() - t0
         ctx.check()
+        if mode == "mixed" and int(bad.item()):
+            raise SystemExit(f"{int(bad.item())} genotypes outside 0..3")
         return {"B": Bo, "V": Vo, "elapsed": elapsed, "snps_per_s": m * args.steps / elapsed,
                 "reduce_ms": red_ms / max(nfull, 1), "finalize_ms": fin_ms / max(nfull, 1),
                 "tile_ms": tile_ms / max(nfull, 1), "setup_s": setup_s}
@@ -758,8 +772,10 @@ def main():
         ach = ops / (res["reduce_ms"] * 1e-3) / 1e12
         return {"bound": "mfma", "achieved": ach, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_INT8_TOPS, "traffic": traffic(mode),
-                "kernel": ("k_trmm_i8 dosage planes (fgls_hip_block_reduce_i8t, planes 2)" if planes == 2
-                           else "k_trmm_i8 (fgls_hip_block_reduce_i8t)"), "achieved_is": "int8 TOP/s",
+                "kernel": {"dosage": "k_trmm_i8 dosage planes (fgls_hip_block_reduce_i8t, planes 2)",
+                           "mixed": "k_trmm_i8 on 2-bit genotype tiles (fgls_hip_block_reduce_u2t)",
+                           "mixed8": "k_trmm_i8 on 8-bit genotype tiles (fgls_hip_block_reduce_i8t)"}[mode],
+                "achieved_is": "int8 TOP/s",
                 "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block,
                 "int8_ops_per_snp": planes * NSLICE * float(n) * n,
                 "fp64_equivalent_tflops": flop / (res["reduce_ms"] * 1e-3) / 1e12}
@@ -788,6 +804,9 @@ def main():
             paths["mixed"]["B"].cpu().numpy(), paths["mixed"]["V"].cpu().numpy(),
             paths["fp64"]["B"].cpu().numpy(), paths["fp64"]["V"].cpu().numpy(), p)
         summary["speedup_mixed_over_fp64"] = paths["mixed"]["snps_per_s"] / paths["fp64"]["snps_per_s"]
+    if "mixed" in paths and "mixed8" in paths:
+        summary["mixed_2bit_tiles_bitwise_equal_8bit_tiles"] = bool(
+            torch.equal(paths["mixed"]["B"], paths["mixed8"]["B"]) and torch.equal(paths["mixed"]["V"], paths["mixed8"]["V"]))
         log(f"mixed vs fp64 over all {m} SNPs: {e}; speedup {summary['speedup_mixed_over_fp64']:.2f}x")
     pipeline = None
     if args.pipeline_m > 0:

@@ -22,8 +22,11 @@ from .fgls import (  # noqa: F401
     pack_i16,
     pack_u2,
     tile_i8,
+    tile_u2,
+    tile_u2_packed,
     to_i8,
     u2_to_i8,
+    x2t_bytes,
     FGLS_OK,
     FGLS_ERR_ARG,
     FGLS_ERR_HIP,

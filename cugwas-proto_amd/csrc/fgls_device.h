@@ -66,6 +66,30 @@ __host__ __device__ constexpr size_t fgls_x8t_bytes(int n, int count, int planes
 __host__ __device__ constexpr int64_t fgls_x8t_plane(int n, int count) {
   return (int64_t)fgls_x8t_bytes(n, count, 1);
 }
+// Bytes of 2-bit tiled genotypes (values 0..3, hard calls) of `count`
+// columns: one 256-byte block per (16 SNP columns, 64-individual step), in
+// the B-operand lane order of the int8 product; lane l = 16 g + i of a block
+// holds, in one dword, the 16 genotypes k = 64 ks + 16 g + j (j < 16) of
+// column 16 cg + i, genotype j = 4 q + b in bits 8 b + 2 q .. 8 b + 2 q + 1,
+// so that (x >> 2 q) & 0x03030303 is dword q of the lane's int8 operand.
+__host__ __device__ constexpr size_t fgls_x2t_bytes(int n, int count) {
+  return (size_t)((count + 15) / 16) * 16 * (size_t)((n + 63) / 64) * 16;
+}
+// Column-major int8 genotypes (ld ldx8 >= ceil(n/64)*64, ldx8 % 16 == 0,
+// 16-byte aligned) -> 2-bit tiles; values of rows < n outside 0..3 are
+// counted in *bad and written as 0.
+void fgls_launch_tile_u2(int n, int count, const int8_t* X8, int64_t ldx8, uint8_t* X2T, unsigned int* bad,
+                         hipStream_t st);
+// 2-bit genotype file columns (ceil(n/4) bytes at P + c ldp, individual r in
+// bits 2 (r % 4) of byte r / 4) -> 2-bit tiles; code 3 in rows < n is
+// counted in *bad and written as 0.
+void fgls_launch_tile_u2_packed(int n, int count, const uint8_t* P, int64_t ldp, uint8_t* X2T,
+                                unsigned int* bad, hipStream_t st);
+// The int8 product on 2-bit tiles (fgls_launch_trmm_i8 with X8T in the 2-bit
+// layout; genotypes only).
+void fgls_launch_trmm_u2(int p, int n, int count, int RT32, int KS, const int8_t* A8,
+                         const double* inv_scale, const uint8_t* X2T, const double* W, int64_t ldw,
+                         double* part, int64_t ldp, int eig_ntr, hipStream_t st);
 // Column-major genotypes (ld ldx8 >= ceil(n/64)*64, ldx8 % 16 == 0, 16-byte
 // aligned; `planes` planes of count columns, count ldx8 bytes apart) -> the
 // tiled layout of the int8 product (fgls_i8.hip).

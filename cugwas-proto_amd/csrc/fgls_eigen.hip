@@ -168,6 +168,17 @@ int reduce_i8t(fgls_hip_eig* e, const int8_t* dX8T, int count, int planes, doubl
   return FGLS_OK;
 }
 
+// The int8 product of 2-bit tiled genotypes (fgls_x2t_bytes).
+int reduce_u2t(fgls_hip_eig* e, const uint8_t* dX2T, int count, hipStream_t st) {
+  if (e->mode != FGLS_MODE_MIXED_I8) return fail(FGLS_ERR_STATE, "eigen context is not in the mixed mode");
+  if (!dX2T || ((uintptr_t)dX2T & 3)) return fail(FGLS_ERR_ARG, "2-bit tiles need 4-byte alignment");
+  fgls_launch_trmm_u2(e->p, e->n, count, e->RT8, e->ldn / 64, e->A8, e->invsc, dX2T, e->Wg, (int64_t)e->ldn,
+                      e->part, (int64_t)e->max_block, e->ntr, st);
+  HIP_TRY(hipGetLastError());
+  e->last_rt = e->RT8;
+  return FGLS_OK;
+}
+
 // Column-major genotypes: tiled into the context's buffer, then the product.
 int reduce_i8(fgls_hip_eig* e, const int8_t* dX8, int64_t ldx8, int count, int planes, double scale,
               hipStream_t st) {
@@ -505,6 +516,16 @@ int fgls_hip_eig_solve_block_i8t(fgls_hip_eig* e, const int8_t* dX8T, int count,
   if ((rc = check_outputs(e, count, dB, ldb, dV, ldv))) return rc;
   hipStream_t st = (hipStream_t)stream;
   if ((rc = reduce_i8t(e, dX8T, count, planes, scale, st))) return rc;
+  return solve_common(e, count, dB, ldb, dV, ldv, snp_base, st);
+}
+
+int fgls_hip_eig_solve_block_u2t(fgls_hip_eig* e, const uint8_t* dX2T, int count, double* dB, int64_t ldb,
+                                 double* dV, int64_t ldv, int64_t snp_base, void* stream) {
+  int rc = pre_solve(e, count);
+  if (rc || count == 0) return rc;
+  if ((rc = check_outputs(e, count, dB, ldb, dV, ldv))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if ((rc = reduce_u2t(e, dX2T, count, st))) return rc;
   return solve_common(e, count, dB, ldb, dV, ldv, snp_base, st);
 }
 

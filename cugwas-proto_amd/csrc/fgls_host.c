@@ -485,6 +485,8 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   int8_t *dX8[2] = {NULL, NULL};
   uint8_t *dP2[2] = {NULL, NULL};   /* xr_format=u2: packed columns of the slot */
   int8_t *dX8T[2] = {NULL, NULL};   /* mixed: the slot's genotypes tiled once for every trait */
+  uint8_t *dX2T[2] = {NULL, NULL};  /* mixed with a 2-bit file: the slot's 2-bit tiles */
+  const int u2t = opt.mixed && opt.xr_i8 == 2;   /* 2-bit file straight to 2-bit tiles */
   const size_t col_bytes = xr16 ? (size_t)n * 2 : opt.xr_i8 == 2 ? (size_t)(n + 3) / 4
                                                : opt.xr_i8 == 1 ? (size_t)n : (size_t)n * sizeof(double);
   unsigned int *dBad = NULL;
@@ -628,7 +630,8 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
       HCHK(hipMalloc((void **)&dX8[s], b8));
       HCHK(hipMemset(dX8[s], 0, b8));
       if (opt.xr_i8 >= 2) HCHK(hipMalloc((void **)&dP2[s], (size_t)block * wXR * col_bytes));
-      if (opt.mixed) HCHK(hipMalloc((void **)&dX8T[s], fgls_hip_x8t_bytes(n, (int)block, dos ? 2 : 1)));
+      if (u2t) HCHK(hipMalloc((void **)&dX2T[s], fgls_hip_x2t_bytes(n, (int)block)));
+      else if (opt.mixed) HCHK(hipMalloc((void **)&dX8T[s], fgls_hip_x8t_bytes(n, (int)block, dos ? 2 : 1)));
     }
   }
   if (nchunks > 0) {
@@ -726,7 +729,13 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
         HCHK(hipEventRecord(ev_c1[s], s_copy));
         HCHK(hipEventRecord(ev_h2d[s], s_copy));
         HCHK(hipStreamWaitEvent(s_comp, ev_h2d[s], 0));
-        if (opt.xr_i8 == 2) {
+        if (u2t) {
+          /* 2-bit codes -> the product's 2-bit tiles; code 3 is counted and
+           * fails the run at write-back */
+          HCHK(hipMemsetAsync(dBad + s, 0, sizeof(unsigned int), s_comp));
+          RCHK(fgls_hip_tile_u2_packed(dP2[s], (int64_t)col_bytes, n, cnt, dX2T[s], dBad + s, s_comp));
+          HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
+        } else if (opt.xr_i8 == 2) {
           /* 2-bit codes -> int8 on the device; code 3 is counted and fails
            * the run at write-back */
           HCHK(hipMemsetAsync(dBad + s, 0, sizeof(unsigned int), s_comp));
@@ -758,10 +767,12 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
         }
         const int planes = dos ? 2 : 1;
-        if (opt.mixed) RCHK(fgls_hip_tile_i8(dX8[s], ld8, n, cnt, planes, dX8T[s], s_comp));
+        if (opt.mixed && !u2t) RCHK(fgls_hip_tile_i8(dX8[s], ld8, n, cnt, planes, dX8T[s], s_comp));
         if (eig) {
           const int64_t ldb = block * p, ldv = block * pp;
-          if (opt.mixed)
+          if (u2t)
+            RCHK(fgls_hip_eig_solve_block_u2t(ectx, dX2T[s], cnt, dBs[s], ldb, dVs[s], ldv, i0, s_comp));
+          else if (opt.mixed)
             RCHK(fgls_hip_eig_solve_block_i8t(ectx, dX8T[s], cnt, planes, dos ? opt.dosage_scale : 1.0, dBs[s],
                                               ldb, dVs[s], ldv, i0, s_comp));
           else
@@ -777,7 +788,9 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
             HCHK(hipMemcpyAsync(hV[q], dV[q], (size_t)cnt * pp * sizeof(double), hipMemcpyDeviceToHost, s_comp));
             continue;
           }
-          if (opt.mixed)
+          if (u2t)
+            RCHK(fgls_hip_solve_block_u2t(ctx[k], dX2T[s], cnt, dB[q], dV[q], base, s_comp));
+          else if (opt.mixed)
             RCHK(fgls_hip_solve_block_i8t(ctx[k], dX8T[s], cnt, planes, dos ? opt.dosage_scale : 1.0, dB[q],
                                           dV[q], base, s_comp));
           else
@@ -896,6 +909,7 @@ done:
     if (dX8[s]) hipFree(dX8[s]);
     if (dP2[s]) hipFree(dP2[s]);
     if (dX8T[s]) hipFree(dX8T[s]);
+    if (dX2T[s]) hipFree(dX2T[s]);
   }
   for (int s = 0; s < 2; ++s) {
     if (dBs[s]) hipFree(dBs[s]);

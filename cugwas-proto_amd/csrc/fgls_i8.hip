@@ -104,11 +104,28 @@ __device__ __forceinline__ void slice_dma(fgls_v4i rs, const int (&vo)[S / 2], i
     fgls_buffer_load_lds(rs, FGLS_LDS(&dst.blk[w + 4 * j][0]), 16, vo[j], off, 0, 0);
 }
 
+// Genotype (B) fragments: 16 int8 values per lane (fgls_v4i) from 1 KB
+// blocks, or, from 2-bit tiles (U2, hard calls 0..3), one dword per lane from
+// 256-byte blocks, unpacked in registers: dword q of the int8 operand is
+// (x >> 2 q) & 0x03030303 (the layout of fgls_x2t_bytes).
+template <bool U2> struct BFrag { using T = fgls_v4i; static constexpr int BLOCK = 1024, LANE = 16; };
+template <> struct BFrag<true> { using T = unsigned int; static constexpr int BLOCK = 256, LANE = 4; };
+
+template <bool U2>
+__device__ __forceinline__ typename BFrag<U2>::T load_b(const int8_t* p) {
+  return *reinterpret_cast<const typename BFrag<U2>::T*>(p);
+}
+__device__ __forceinline__ fgls_v4i unpack_b(fgls_v4i x) { return x; }
+__device__ __forceinline__ fgls_v4i unpack_b(unsigned int x) {
+  constexpr unsigned int m = 0x03030303u;
+  return fgls_v4i{(int)(x & m), (int)((x >> 2) & m), (int)((x >> 4) & m), (int)((x >> 6) & m)};
+}
+
 // One 64-k step: rd = LDS half of step k, wr = half that receives step k + 1,
 // b = genotype fragments of step k, bn receives those of step k + 1.
-template <int S>
-__device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i (&b)[4],
-                                        fgls_v4i (&bn)[4], const SliceStage<S>& rd,
+template <int S, bool U2>
+__device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const typename BFrag<U2>::T (&b)[4],
+                                        typename BFrag<U2>::T (&bn)[4], const SliceStage<S>& rd,
                                         SliceStage<S>& wr, fgls_v4i asrc, const int (&avo)[S / 2],
                                         const int8_t* Xw, const int (&boff)[4], int k, int last,
                                         int w, int lane) {
@@ -122,12 +139,14 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i
   slice_dma<S>(asrc, avo, k1 * S * 1024, wr, w);
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int tj = 0; tj < 4; ++tj)
-    bn[tj] = *reinterpret_cast<const fgls_v4i*>(Xw + 1024 * k1 + boff[tj]);
+  for (int tj = 0; tj < 4; ++tj) bn[tj] = load_b<U2>(Xw + BFrag<U2>::BLOCK * k1 + boff[tj]);
   __builtin_amdgcn_sched_barrier(0);
   // the genotype fragments of step k (older than the S/2 + 4 loads just issued)
   __builtin_amdgcn_s_waitcnt(vmcnt_imm(S / 2 + 4));
   __builtin_amdgcn_sched_barrier(0);
+  fgls_v4i bu[4];
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) bu[tj] = unpack_b(b[tj]);
   // A fragments one ahead: fragment f = 2 s + ti
   fgls_v4i af[2];
   af[0] = rd.blk[0][lane];
@@ -137,7 +156,7 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i
     const int s2 = f >> 1, ti = f & 1;
     if (f + 1 < 2 * S) af[(f + 1) & 1] = rd.blk[((f + 1) & 1) * S + ((f + 1) >> 1)][lane];
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(af[f & 1], b[tj], acc[s2][ti][tj]);
+    for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(af[f & 1], bu[tj], acc[s2][ti][tj]);
   }
   // issue order: one LDS read (the next fragment), then the four MFMAs of
   // the current one
@@ -168,7 +187,7 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const fgls_v4i
 // into part as I8_RED does z'z and z'W[:, k].
 enum { I8_RED = 0, I8_STORE = 1, I8_DOT = 2 };
 
-template <int NQ, int S, bool DOS = false, bool EIG = false, int MODE = I8_RED>
+template <int NQ, int S, bool DOS = false, bool EIG = false, int MODE = I8_RED, bool U2 = false>
 __global__ __launch_bounds__(256, 2) void k_trmm_i8(
     int n, int count, int RT32, int CT, int KS, const int8_t* __restrict__ A8,
     const double* __restrict__ inv_scale, const int8_t* __restrict__ X8T, int64_t ld8,
@@ -176,6 +195,9 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
     int64_t plane_off = 0, double inv_xscale = 1.0, int ntr = 1, double* __restrict__ Zio = nullptr,
     int64_t ldz = 0) {
   static_assert(S % 2 == 0, "the 2 x S slice blocks are spread over 4 waves");
+  static_assert(!(U2 && DOS), "2-bit tiles hold genotypes, not dosage digit planes");
+  using BT = typename BFrag<U2>::T;
+  constexpr int BB = BFrag<U2>::BLOCK, BL = BFrag<U2>::LANE;
   constexpr int WN = DOS ? 32 : 64;   // SNP columns per wave
   __shared__ SliceStage<S> st0, st1;
   int rt, ct;
@@ -208,12 +230,12 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   const int KSX = (int)(ld8 / 64);
   const int CG = (count + 15) / 16;
   const int cgb = min(c0 / 16, CG - 1);
-  const int8_t* Xw = X8T + (size_t)cgb * KSX * 1024;
+  const int8_t* Xw = X8T + (size_t)cgb * KSX * BB;
   int boff[4];
 #pragma unroll
   for (int tj = 0; tj < 4; ++tj) {
     const int cj = DOS ? (tj & 1) : tj;   // DOS: 0, 1 lo digits; 2, 3 hi digits
-    boff[tj] = (min(c0 / 16 + cj, CG - 1) - cgb) * KSX * 1024 + 16 * lane +
+    boff[tj] = (min(c0 / 16 + cj, CG - 1) - cgb) * KSX * BB + BL * lane +
                ((DOS && tj >= 2) ? (int)plane_off : 0);
   }
 
@@ -227,18 +249,18 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
 
   if (nsteps > 0) {
     const int last = nsteps - 1;
-    fgls_v4i b0[4], b1[4];
+    BT b0[4], b1[4];
     slice_dma<S>(asrc, avo, 0, st0, w);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj) b0[tj] = *reinterpret_cast<const fgls_v4i*>(Xw + boff[tj]);
+    for (int tj = 0; tj < 4; ++tj) b0[tj] = load_b<U2>(Xw + boff[tj]);
     __builtin_amdgcn_sched_barrier(0);
     int k = 0;
     for (; k + 2 <= nsteps; k += 2) {
-      step_i8<S>(acc, b0, b1, st0, st1, asrc, avo, Xw, boff, k, last, w, lane);
-      step_i8<S>(acc, b1, b0, st1, st0, asrc, avo, Xw, boff, k + 1, last, w, lane);
+      step_i8<S, U2>(acc, b0, b1, st0, st1, asrc, avo, Xw, boff, k, last, w, lane);
+      step_i8<S, U2>(acc, b1, b0, st1, st0, asrc, avo, Xw, boff, k + 1, last, w, lane);
     }
-    if (k < nsteps) step_i8<S>(acc, b0, b1, st0, st1, asrc, avo, Xw, boff, k, last, w, lane);
+    if (k < nsteps) step_i8<S, U2>(acc, b0, b1, st0, st1, asrc, avo, Xw, boff, k, last, w, lane);
     // no LDS load may still be in flight when the workgroup ends
     __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
   }
@@ -536,6 +558,76 @@ __global__ __launch_bounds__(256) void k_tile_i8(int count, int KSX, const int8_
   }
 }
 
+// Column-major int8 genotypes -> 2-bit tiles (fgls_x2t_bytes): one workgroup
+// per 16-column group, thread t = 64 ks + 16 g + i packs the 16 genotypes
+// k = 64 ks + 16 g + j of column 16 cg + i into one dword (genotype j = 4 q + b
+// at bit 8 b + 2 q).  Values of rows < n outside 0..3 are counted in *bad
+// and written as 0; rows >= n are written as 0; columns past count repeat
+// the last column.
+__global__ __launch_bounds__(256) void k_tile_u2(int n, int count, int KSX, const int8_t* __restrict__ X8,
+                                                 int64_t ldx8, unsigned int* __restrict__ X2T,
+                                                 unsigned int* __restrict__ bad) {
+  const int cg = blockIdx.x;
+  unsigned int* dst = X2T + (size_t)cg * KSX * 64;
+  unsigned int nb = 0;
+  for (int t = threadIdx.x; t < KSX * 64; t += 256) {
+    const int ks = t >> 6, g = (t >> 4) & 3, i = t & 15;
+    const int col = min(cg * 16 + i, count - 1);
+    const int r0 = 64 * ks + 16 * g;
+    const fgls_v4i v = *reinterpret_cast<const fgls_v4i*>(X8 + (size_t)col * ldx8 + r0);
+    unsigned int x = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int gv = (int)(signed char)((v[q] >> (8 * b)) & 0xff);
+        unsigned int c = 0;
+        if (r0 + 4 * q + b < n) {
+          if (gv < 0 || gv > 3) ++nb;
+          else c = (unsigned int)gv;
+        }
+        x |= c << (8 * b + 2 * q);
+      }
+    dst[t] = x;
+  }
+  if (nb) atomicAdd(bad, nb);
+}
+
+// 2-bit genotype file columns (ceil(n/4) bytes per column at P + c ldp; the
+// file's genotype r in bits 2 (r % 4) of byte r / 4) -> 2-bit tiles: the 16
+// genotypes of a lane are 4 file bytes (genotype 4 m + q' at file bit
+// 8 m + 2 q'), moved to bit 8 q' + 2 m.  Code 3 in rows < n is counted in
+// *bad and written as 0; rows >= n and bytes past the column are 0.
+__global__ __launch_bounds__(256) void k_tile_u2_packed(int n, int count, int KSX, const uint8_t* __restrict__ P,
+                                                        int64_t ldp, unsigned int* __restrict__ X2T,
+                                                        unsigned int* __restrict__ bad) {
+  const int cg = blockIdx.x;
+  const int nbytes = (n + 3) / 4;
+  unsigned int* dst = X2T + (size_t)cg * KSX * 64;
+  unsigned int nb = 0;
+  for (int t = threadIdx.x; t < KSX * 64; t += 256) {
+    const int ks = t >> 6, g = (t >> 4) & 3, i = t & 15;
+    const int col = min(cg * 16 + i, count - 1);
+    const int r0 = 64 * ks + 16 * g;
+    const uint8_t* src = P + (size_t)col * ldp;
+    unsigned int x = 0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int bi = r0 / 4 + m;
+      const unsigned int byte = bi < nbytes ? src[bi] : 0u;
+#pragma unroll
+      for (int q2 = 0; q2 < 4; ++q2) {
+        unsigned int c = (byte >> (2 * q2)) & 3u;
+        if (r0 + 4 * m + q2 >= n) c = 0;
+        else if (c == 3u) { ++nb; c = 0; }
+        x |= c << (8 * q2 + 2 * m);
+      }
+    }
+    dst[t] = x;
+  }
+  if (nb) atomicAdd(bad, nb);
+}
+
 template <int P, bool EIG>
 void launch(int n, int count, int RT32, int KS, const int8_t* A8, const double* isc,
             const int8_t* X8, int64_t ldx8, const double* W, int64_t ldw, double* part, int64_t ldp,
@@ -572,6 +664,22 @@ void launch_any(int n, int count, int RT32, int KS, const int8_t* A8, const doub
 namespace {
 
 template <int P>
+void launch_u2(int n, int count, int RT32, int KS, const int8_t* A8, const double* isc, const uint8_t* X2T,
+               const double* W, int64_t ldw, double* part, int64_t ldp, int eig_ntr, hipStream_t st) {
+  const int CT = (count + I8_TN - 1) / I8_TN;
+  const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
+  const int64_t ld8 = ((int64_t)n + 63) / 64 * 64;
+  if (eig_ntr > 0)
+    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, true, I8_RED, true>), dim3(G), dim3(256), 0, st,
+                       n, count, RT32, CT, KS, A8, isc, (const int8_t*)X2T, ld8, W, ldw, part, ldp, (int64_t)0,
+                       1.0, eig_ntr);
+  else
+    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, false, I8_RED, true>), dim3(G), dim3(256), 0, st,
+                       n, count, RT32, CT, KS, A8, isc, (const int8_t*)X2T, ld8, W, ldw, part, ldp, (int64_t)0,
+                       1.0, 1);
+}
+
+template <int P>
 void launch_dot(int n, int count, int RT32, int KS, const int8_t* A8, const double* isc,
                 const int8_t* X8T, int64_t ld8, const double* W, int64_t ldw, const double* Zin,
                 int64_t ldz, double* part, int64_t ldp, hipStream_t st) {
@@ -599,6 +707,26 @@ void fgls_launch_i8_dot(int p, int n, int count, int RT32, int KS, const int8_t*
                         hipStream_t st) {
   FGLS_I8_P_SWITCH(p, launch_dot, n, count, RT32, KS, A8, inv_scale, X8T, ld8, W, ldw, Zin, ldz, part,
                    ldp, st);
+}
+
+void fgls_launch_trmm_u2(int p, int n, int count, int RT32, int KS, const int8_t* A8,
+                         const double* inv_scale, const uint8_t* X2T, const double* W, int64_t ldw,
+                         double* part, int64_t ldp, int eig_ntr, hipStream_t st) {
+  FGLS_I8_P_SWITCH(p, launch_u2, n, count, RT32, KS, A8, inv_scale, X2T, W, ldw, part, ldp, eig_ntr, st);
+}
+
+void fgls_launch_tile_u2(int n, int count, const int8_t* X8, int64_t ldx8, uint8_t* X2T, unsigned int* bad,
+                         hipStream_t st) {
+  const int KSX = (n + 63) / 64;
+  hipLaunchKernelGGL(k_tile_u2, dim3((count + 15) / 16), dim3(256), 0, st, n, count, KSX, X8, ldx8,
+                     reinterpret_cast<unsigned int*>(X2T), bad);
+}
+
+void fgls_launch_tile_u2_packed(int n, int count, const uint8_t* P, int64_t ldp, uint8_t* X2T,
+                                unsigned int* bad, hipStream_t st) {
+  const int KSX = (n + 63) / 64;
+  hipLaunchKernelGGL(k_tile_u2_packed, dim3((count + 15) / 16), dim3(256), 0, st, n, count, KSX, P, ldp,
+                     reinterpret_cast<unsigned int*>(X2T), bad);
 }
 
 void fgls_launch_tile_i8(int n, int count, int planes, const int8_t* X8, int64_t ldx8, int8_t* X8T,

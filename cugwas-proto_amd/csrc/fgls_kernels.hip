@@ -1517,6 +1517,25 @@ int check_x8(int n, const int8_t* dX8, int64_t ldx8) {
   return FGLS_OK;
 }
 
+// The int8 product of a block of 2-bit tiled genotypes (fgls_x2t_bytes).
+int reduce_u2t(fgls_hip_ctx* c, const uint8_t* dX2T, int count, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->have_trait) return fail(FGLS_ERR_STATE, "no trait set");
+  if (c->mode != FGLS_MODE_MIXED_I8) return fail(FGLS_ERR_STATE, "context is not in the mixed mode");
+  if (count < 0 || count > c->max_block)
+    return fail(FGLS_ERR_ARG, "count=%d outside [0, %d]", count, c->max_block);
+  if (count == 0) return FGLS_OK;
+  if (!dX2T || ((uintptr_t)dX2T & 3)) return fail(FGLS_ERR_ARG, "2-bit tiles need 4-byte alignment");
+  if ((rc = set_device(c))) return rc;
+  fgls_launch_trmm_u2(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, dX2T, c->W, (int64_t)c->ldn,
+                      c->part, (int64_t)c->max_block, 0, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  c->reduced = count;
+  c->last_rt = c->RT8;
+  return FGLS_OK;
+}
+
 // Column-major genotypes: tiled into the context's buffer, then the product.
 int reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count, int planes, double scale,
               void* stream) {
@@ -1558,6 +1577,47 @@ int fgls_hip_tile_i8(const int8_t* dX8, int64_t ldx8, int n, int count, int plan
   fgls_launch_tile_i8(n, count, planes, dX8, ldx8, dX8T, (hipStream_t)stream);
   HIP_TRY(hipGetLastError());
   return FGLS_OK;
+}
+
+size_t fgls_hip_x2t_bytes(int n, int count) {
+  if (n < 1 || count < 0) return 0;
+  return fgls_x2t_bytes(n, count);
+}
+
+int fgls_hip_tile_u2(const int8_t* dX8, int64_t ldx8, int n, int count, uint8_t* dX2T, unsigned int* dbad,
+                     void* stream) {
+  if (n < 1 || count < 0 || !dX2T || !dbad || ((uintptr_t)dX2T & 3))
+    return fail(FGLS_ERR_ARG, "bad arguments to fgls_hip_tile_u2");
+  int rc = check_x8(n, dX8, ldx8);
+  if (rc) return rc;
+  if (count == 0) return FGLS_OK;
+  fgls_launch_tile_u2(n, count, dX8, ldx8, dX2T, dbad, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_tile_u2_packed(const uint8_t* dP, int64_t ldp, int n, int count, uint8_t* dX2T,
+                            unsigned int* dbad, void* stream) {
+  if (n < 1 || count < 0 || !dP || !dX2T || !dbad || ((uintptr_t)dX2T & 3) || ldp < (n + 3) / 4)
+    return fail(FGLS_ERR_ARG, "bad arguments to fgls_hip_tile_u2_packed");
+  if (count == 0) return FGLS_OK;
+  fgls_launch_tile_u2_packed(n, count, dP, ldp, dX2T, dbad, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
+}
+
+int fgls_hip_block_reduce_u2t(fgls_hip_ctx* c, const uint8_t* dX2T, int count, void* stream) {
+  return reduce_u2t(c, dX2T, count, stream);
+}
+
+int fgls_hip_solve_block_u2t(fgls_hip_ctx* c, const uint8_t* dX2T, int count, double* dB, double* dV,
+                             int64_t snp_base, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (count > 0 && (!dB || !dV)) return fail(FGLS_ERR_ARG, "NULL pointer");
+  rc = reduce_u2t(c, dX2T, count, stream);
+  if (rc) return rc;
+  return fgls_hip_block_finalize(c, count, dB, dV, snp_base, stream);
 }
 
 int fgls_hip_block_reduce_i8t(fgls_hip_ctx* c, const int8_t* dX8T, int count, int planes,

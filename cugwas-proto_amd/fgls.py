@@ -99,6 +99,12 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_solve_block_i8t": (i32, [vp, dp, i32, i32, ctypes.c_double, dp, dp, i64, vp]),
             "fgls_hip_eig_solve_block_i8t": (i32, [vp, dp, i32, i32, ctypes.c_double, dp, i64, dp, i64,
                                                    i64, vp]),
+            "fgls_hip_x2t_bytes": (ctypes.c_size_t, [i32, i32]),
+            "fgls_hip_tile_u2": (i32, [dp, i64, i32, i32, dp, dp, vp]),
+            "fgls_hip_tile_u2_packed": (i32, [dp, i64, i32, i32, dp, dp, vp]),
+            "fgls_hip_block_reduce_u2t": (i32, [vp, dp, i32, vp]),
+            "fgls_hip_solve_block_u2t": (i32, [vp, dp, i32, dp, dp, i64, vp]),
+            "fgls_hip_eig_solve_block_u2t": (i32, [vp, dp, i32, dp, i64, dp, i64, i64, vp]),
             "fgls_hip_set_trait_phi": (i32, [vp, dp, i64, ctypes.c_double, ctypes.c_double, dp, i64, dp, vp]),
             "fgls_hip_get_factor": (i32, [vp, dp, i64, vp]),
             "fgls_hip_adjoint_begin": (i32, [vp, dp, i64, ctypes.c_double, vp]),
@@ -154,7 +160,9 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_x8t_bytes",
                         "fgls_hip_i16_to_dosage_i8", "fgls_hip_i16_to_f64",
                         "fgls_hip_tile_i8", "fgls_hip_block_reduce_i8t", "fgls_hip_solve_block_i8t",
-                        "fgls_hip_eig_solve_block_i8t",
+                        "fgls_hip_eig_solve_block_i8t", "fgls_hip_x2t_bytes", "fgls_hip_tile_u2",
+                        "fgls_hip_tile_u2_packed", "fgls_hip_block_reduce_u2t", "fgls_hip_solve_block_u2t",
+                        "fgls_hip_eig_solve_block_u2t",
                         "fgls_factor_M", "fgls_last_run_stats", "fgls_eigen", "fgls_eigen_gpu_ex",
                         "fgls_eigen_decompose",
                         "fgls_hip_eigen_decompose",
@@ -388,6 +396,21 @@ class DeviceContext:
                                               _ptr(V), snp_base, _stream(stream)))
         return B, V
 
+    def block_reduce_u2t(self, X2T, count: int, stream=None):
+        """The int8 product of 2-bit tiled genotypes (tile_u2 / tile_u2_packed)."""
+        _check(lib().fgls_hip_block_reduce_u2t(self._h, _ptr(X2T), count, _stream(stream)))
+        return count
+
+    def solve_block_u2t(self, X2T, count: int, B=None, V=None, snp_base: int = 0, stream=None):
+        import torch
+        if B is None:
+            B = torch.empty((count, self.p), dtype=torch.float64, device=X2T.device)
+        if V is None:
+            V = torch.empty((count, self.p * self.p), dtype=torch.float64, device=X2T.device)
+        _check(lib().fgls_hip_solve_block_u2t(self._h, _ptr(X2T), count, _ptr(B), _ptr(V), snp_base,
+                                              _stream(stream)))
+        return B, V
+
     def block_reduce_i8(self, X8, count: int | None = None, stream=None):
         count = X8.shape[0] if count is None else count
         _check(lib().fgls_hip_block_reduce_i8(self._h, _ptr(X8), X8.shape[1], count, _stream(stream)))
@@ -603,6 +626,13 @@ class EigenContext:
                                                   _stream(stream)))
         return B, V
 
+    def solve_block_u2t(self, X2T, count: int, B=None, V=None, snp_base: int = 0, stream=None):
+        """2-bit tiled genotypes (tile_u2 / tile_u2_packed)."""
+        B, V = self._outputs(count, B, V, X2T.device)
+        _check(lib().fgls_hip_eig_solve_block_u2t(self._h, _ptr(X2T), count, _ptr(B), B.stride(0), _ptr(V),
+                                                  V.stride(0), snp_base, _stream(stream)))
+        return B, V
+
     def check(self, stream=None):
         tr = ctypes.c_int(-1)
         snp = ctypes.c_int64(-1)
@@ -661,6 +691,48 @@ def tile_i8(X8, n: int, count: int | None = None, planes: int = 1, X8T=None, str
         raise FglsError(FGLS_ERR_ARG, f"X8T needs {nbytes} bytes")
     _check(lib().fgls_hip_tile_i8(_ptr(X8), X8.shape[1], n, count, planes, _ptr(X8T), _stream(stream)))
     return X8T
+
+
+def x2t_bytes(n: int, count: int) -> int:
+    """Bytes of the 2-bit tiled genotypes of count SNPs (fgls_hip_x2t_bytes)."""
+    return int(lib().fgls_hip_x2t_bytes(n, count))
+
+
+def tile_u2(X8, n: int, count: int | None = None, X2T=None, bad=None, stream=None):
+    """Column-major int8 hard-call genotypes X8 ((count, ld8) tensor, values
+    0..3) -> the 2-bit tiles of the int8 product (a flat uint8 tensor of
+    x2t_bytes(n, count) bytes).  Raises if a value is outside 0..3, unless a
+    device counter `bad` (int32, caller-cleared) is given, which then
+    receives the count without a synchronisation."""
+    import torch
+    count = X8.shape[0] if count is None else count
+    nbytes = x2t_bytes(n, count)
+    if X2T is None:
+        X2T = torch.empty(nbytes, dtype=torch.uint8, device=X8.device)
+    if X2T.numel() < nbytes:
+        raise FglsError(FGLS_ERR_ARG, f"X2T needs {nbytes} bytes")
+    own = bad is None
+    if own:
+        bad = torch.zeros(1, dtype=torch.int32, device=X8.device)
+    _check(lib().fgls_hip_tile_u2(_ptr(X8), X8.shape[1], n, count, _ptr(X2T), _ptr(bad), _stream(stream)))
+    if own and int(bad.item()):
+        raise FglsError(FGLS_ERR_ARG, f"{int(bad.item())} genotype values are outside 0..3")
+    return X2T
+
+
+def tile_u2_packed(P, n: int, count: int | None = None, X2T=None, stream=None):
+    """2-bit genotype file columns P ((count, ldp) uint8 tensor, pack_u2's
+    layout) -> the 2-bit tiles of the int8 product; raises if a code is 3."""
+    import torch
+    count = P.shape[0] if count is None else count
+    nbytes = x2t_bytes(n, count)
+    if X2T is None:
+        X2T = torch.empty(nbytes, dtype=torch.uint8, device=P.device)
+    bad = torch.zeros(1, dtype=torch.int32, device=P.device)
+    _check(lib().fgls_hip_tile_u2_packed(_ptr(P), P.shape[1], n, count, _ptr(X2T), _ptr(bad), _stream(stream)))
+    if int(bad.item()):
+        raise FglsError(FGLS_ERR_ARG, f"{int(bad.item())} 2-bit codes are 3 (not a genotype)")
+    return X2T
 
 
 def pack_i16(X, scale: float = 1000.0):

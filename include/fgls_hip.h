@@ -177,6 +177,30 @@ int fgls_hip_solve_block_i8t(fgls_hip_ctx *ctx, const int8_t *dX8T, int count, i
                              double scale, double *dB, double *dV, int64_t snp_base,
                              void *stream);
 
+/*
+ * 2-bit tiled genotypes (hard calls 0, 1, 2): the int8 product with its
+ * genotype operand loaded 4 values per byte and unpacked in registers, a
+ * quarter of the bytes of fgls_hip_tile_i8's layout per MFMA (bitwise the
+ * same b and V).  Layout: one 256-byte block per (16 SNP columns, 64
+ * individuals), fgls_hip_x2t_bytes(n, count) bytes, 4-byte aligned; values
+ * 0..3 only.  tile_u2 builds it from column-major int8 genotypes (as for
+ * fgls_hip_solve_block_i8), tile_u2_packed from the 2-bit file layout of
+ * fgls_hip_u2_to_i8 (ceil(n/4) bytes per column at dP + c*ldp); a value
+ * outside 0..3 (int8 input) or a code 3 (packed input) in rows < n is
+ * counted into *dbad (device counter, caller-cleared) and written as 0.
+ * reduce/solve take the tiles in the mixed mode (wXR = 1) as the *_i8t entry
+ * points do; eig_solve_block_u2t is the eigen variant's.  No counterpart in
+ * the reference (its X_R is FP64, io.c:35-55).  Asynchronous.
+ */
+size_t fgls_hip_x2t_bytes(int n, int count);
+int fgls_hip_tile_u2(const int8_t *dX8, int64_t ldx8, int n, int count, uint8_t *dX2T,
+                     unsigned int *dbad, void *stream);
+int fgls_hip_tile_u2_packed(const uint8_t *dP, int64_t ldp, int n, int count, uint8_t *dX2T,
+                            unsigned int *dbad, void *stream);
+int fgls_hip_block_reduce_u2t(fgls_hip_ctx *ctx, const uint8_t *dX2T, int count, void *stream);
+int fgls_hip_solve_block_u2t(fgls_hip_ctx *ctx, const uint8_t *dX2T, int count, double *dB,
+                             double *dV, int64_t snp_base, void *stream);
+
 /* 2-bit genotypes -> int8 (column-major, ld ldx8 >= n, ldx8 % 4 == 0, rows
  * n..ldx8-1 zeroed): column c is ceil(n/4) bytes at dP + c*ldp, individual r
  * in bits 2(r%4)..2(r%4)+1 of byte r/4 (the first individual in the low
@@ -391,6 +415,10 @@ int fgls_hip_eig_solve_block_dosage(fgls_hip_eig *e, const int8_t *dX8,
 int fgls_hip_eig_solve_block_i8t(fgls_hip_eig *e, const int8_t *dX8T, int count,
                                  int planes, double scale, double *dB, int64_t ldb,
                                  double *dV, int64_t ldv, int64_t snp_base, void *stream);
+/* The same from 2-bit tiles (fgls_hip_tile_u2, fgls_hip_tile_u2_packed). */
+int fgls_hip_eig_solve_block_u2t(fgls_hip_eig *e, const uint8_t *dX2T, int count, double *dB,
+                                 int64_t ldb, double *dV, int64_t ldv, int64_t snp_base,
+                                 void *stream);
 int fgls_hip_eig_check(fgls_hip_eig *e, void *stream, int *trait,
                        int64_t *bad_snp, int *info);
 

@@ -162,3 +162,67 @@ def test_tiled_genotype_entry_points(pkg, gpu):
     e.check()
     e.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("n,m", [(650, 203), (1001, 77), (64, 16), (7, 5), (4096, 300)])
+def test_u2_tiles_equal_i8_tiles(pkg, gpu, n, m):
+    """2-bit tiles (tile_u2 from int8 columns, tile_u2_packed from the 2-bit
+    file layout) give bitwise the results of the 8-bit tiles in both
+    variants: the product unpacks them to the same int8 operands.  Padding
+    rows of the int8 input (rows >= n) may hold anything."""
+    import torch
+    print(f"u2 tiles n={n} m={m}", flush=True)
+    prob = make_problem(n=n, wXL=min(3, n - 1), m=m, seed=n + 17)
+    _, ctx = setup(pkg, prob, max_block=max(m, 64))
+    ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)
+    X8 = ctx.to_i8(pkg.colmajor(prob.XR, ld=(n + 3) // 4 * 4))
+    B1, V1 = ctx.solve_block_i8t(pkg.tile_i8(X8, n), m)
+    X8g = X8.clone()
+    X8g[:, n:] = 100                                         # garbage padding rows
+    T2 = pkg.tile_u2(X8g, n)
+    assert T2.numel() == pkg.x2t_bytes(n, m) == ((m + 15) // 16) * 16 * ((n + 63) // 64) * 16
+    B2, V2 = ctx.solve_block_u2t(T2, m)
+    P = torch.from_numpy(pkg.pack_u2(prob.XR)).cuda()
+    B3, V3 = ctx.solve_block_u2t(pkg.tile_u2_packed(P, n), m)
+    ctx.check()
+    assert torch.equal(B1, B2) and torch.equal(V1, V2)
+    assert torch.equal(B1, B3) and torch.equal(V1, V3)
+    assert torch.equal(T2, pkg.tile_u2_packed(P, n))
+    # the same through a block_reduce / block_finalize pair
+    ctx.block_reduce_u2t(T2, m)
+    B4 = torch.empty_like(B1)
+    V4 = torch.empty_like(V1)
+    ctx.block_finalize(m, B4, V4)
+    assert torch.equal(B1, B4) and torch.equal(V1, V4)
+    # eigen variant, two traits
+    Z, W = pkg.eigen_decompose(prob.Phi, threads=4)
+    with pkg.EigenContext(n, prob.wXL, max_traits=2, max_block=max(m, 64)) as e:
+        e.set_mode(pkg.FGLS_MODE_MIXED_I8)
+        e.set_basis(torch.from_numpy(np.ascontiguousarray(Z.T)).cuda(), torch.from_numpy(W).cuda(),
+                    pkg.colmajor(prob.XL))
+        e.set_traits([0.4, 0.7], [1.0, 1.0], pkg.colmajor(np.column_stack([prob.y, prob.y[::-1]])))
+        E1 = e.solve_block_i8t(pkg.tile_i8(X8, n), m)
+        E2 = e.solve_block_u2t(T2, m)
+        e.check()
+        assert torch.equal(E1[0], E2[0]) and torch.equal(E1[1], E2[1])
+    ctx.close()
+
+
+def test_u2_tiles_reject_values_outside_0_3(pkg, gpu):
+    import torch
+    n, m = 300, 40
+    X8 = torch.randint(0, 3, (m, 320), dtype=torch.int8, device="cuda")
+    X8[:, n:] = -5                                           # padding: ignored
+    pkg.tile_u2(X8, n)
+    X8[3, 17] = 3                                            # 3 fits in two bits: allowed
+    pkg.tile_u2(X8, n)
+    X8[5, 299] = 4
+    X8[6, 0] = -1
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.tile_u2(X8, n)
+    assert "2 genotype values" in ei.value.msg
+    P = torch.from_numpy(pkg.pack_u2(np.zeros((n, m)))).cuda()
+    P[7, 74] |= 0xC0                                         # individual 299: code 3
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.tile_u2_packed(P, n)
+    assert "1 2-bit codes" in ei.value.msg
