@@ -20,7 +20,7 @@
 //             X    = L21 T11                      (product, B lower)
 //             T21  = -T22 X     (as T21' = -X' T22', product, B upper)
 //
-// 64 x 64 leaves factor and invert in one wave (k_leaf).  Every product runs
+// 64 x 64 leaves factor and invert in one workgroup (k_leaf).  Every product runs
 // on k_tgemm, an FP64 MFMA kernel whose operands are both read along k
 // (A(r, k) at A[r ld + k], B(k, c) at B[c ld + k]) with a column-major
 // result, or, when it has at most 4,096 64 x 64 blocks of output, on
@@ -101,126 +101,181 @@ __global__ __launch_bounds__(256) void k_build_M(int n, int ldn, const double* _
   }
 }
 
+// Value of x in lane l (l wave-uniform), as a scalar broadcast.
+__device__ __forceinline__ double lane_bcast(double x, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+  return __hiloint2double(hi, lo);
+}
+
 // ---------------------------------------------------------------------------
-// 64 x 64 diagonal block at o = o0 + 64 blockIdx.x, one wave.
-// FACTOR: lane i holds row i of S (from the L buffer's R-view lower
-//         triangle) in registers and factors it right-looking; at step k
-//         every lane publishes its entry of column k in LDS and reads the
-//         column back as wave-uniform broadcasts.
-// else:   lane i holds row i of the caller's factor (column-major lower
-//         triangle).
-// Then T = L^-1 by forward substitution with lane c owning column c of T (the
-// rows of L read from LDS as broadcasts).  L (FACTOR) and T are written to
-// both triangles of their buffers through LDS, so that every store
-// instruction writes one contiguous 512-byte row or column segment.
-// Rows >= n are identity rows; their stored values are zero.  All loops
-// unroll completely (the file is built with a raised
-// -pragma-unroll-threshold) so that a[] and t[] stay in registers.
+// 64 x 64 diagonal block at o = o0 + 64 blockIdx.x, one workgroup of 4 waves,
+// blocked by 16 with the block in LDS (A: S, then L; Tm: T = L^-1, lower
+// triangles):
+// FACTOR: S from the L buffer's R-view lower triangle; for each 16-column
+//         panel jb: wave 0 factors the 16 x 16 diagonal block in registers
+//         (lane r holds row r; the column and the pivot are scalar
+//         broadcasts, v_readlane) and inverts it (lane c: column c of T_jj);
+//         then all waves form the panel below, L_ij = S_ij T_jj', and
+//         update the trailing lower triangle, S_ik -= L_ij L_kj'.
+// else:   L from the caller's factor (column-major lower triangle); each
+//         wave inverts one 16 x 16 diagonal block.
+// Then the off-diagonal blocks of T by block rows, T_IJ = -T_II sum_{J<=K<I}
+// L_IK T_KJ.  L (FACTOR) and T are written to both triangles of their
+// buffers, every store instruction covering contiguous row segments.  Rows
+// >= n are identity rows; their stored values are zero.  The first
+// non-positive pivot is recorded in info (1-based), and the pivot is replaced
+// by 1 so that the later arithmetic stays finite.
 // ---------------------------------------------------------------------------
 template <bool FACTOR>
-__global__ __launch_bounds__(64) void k_leaf(int n, int o0, double* __restrict__ Lb,
-                                             double* __restrict__ Tb, int64_t ld,
-                                             int* __restrict__ info) {
-  __shared__ double Ls[64][66];
-  __shared__ double col[2][64];
-  __shared__ double rcp[64];
+__global__ __launch_bounds__(256) void k_leaf(int n, int o0, double* __restrict__ Lb,
+                                              double* __restrict__ Tb, int64_t ld,
+                                              int* __restrict__ info) {
+  __shared__ double A[64][65];
+  __shared__ double Tm[64][65];
+  __shared__ double X[3][16][17];
   const int o = o0 + 64 * (int)blockIdx.x;
-  const int i = threadIdx.x;
-  const int gi = o + i;
-  const bool live = gi < n;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nl = min(64, n - o);   // live rows of the block
-  double a[64];
-  if (FACTOR) {
-    const double* row = Lb + (size_t)gi * ld + o;
-#pragma unroll
-    for (int j4 = 0; j4 < 16; ++j4) {
-      const v4d v = ld4(row + 4 * j4);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) a[4 * j4 + q] = v[q];
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < 64; ++j) a[j] = Lb[(size_t)(o + j) * ld + gi];
+  for (int idx = tid; idx < 4096; idx += 256) {
+    // FACTOR: row-major source, c fastest; else column-major, r fastest
+    const int r = FACTOR ? idx >> 6 : idx & 63, c = FACTOR ? idx & 63 : idx >> 6;
+    double v = (r == c) ? 1.0 : 0.0;
+    if (r < nl && c <= r)
+      v = FACTOR ? Lb[(size_t)(o + r) * ld + o + c] : Lb[(size_t)(o + c) * ld + o + r];
+    A[r][c] = v;
+    Tm[r][c] = 0.0;   // the strict upper triangle of T stays zero: the block
+                      // products below run over whole 16-blocks
   }
-#pragma unroll
-  for (int j = 0; j < 64; ++j) a[j] = live ? (j <= i ? a[j] : 0.0) : (j == i ? 1.0 : 0.0);
+  __syncthreads();
 
+  // T_jj = L_jj^-1 of the diagonal block at c0 by one wave: lane c (< 16)
+  // forms column c by forward substitution; a[] is row `lane` of L_jj
+  auto diag_inverse = [&](int c0, const double (&a)[16]) {
+    double irr[16];   // 1 / L(r, r): independent of the substitution chain
+#pragma unroll
+    for (int r = 0; r < 16; ++r) irr[r] = 1.0 / lane_bcast(a[r], r);
+    double t[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      double s4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < r; ++q) s4[q & 3] = fma(lane_bcast(a[q], r), t[q], s4[q & 3]);
+      const double sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+      t[r] = (r == lane) ? irr[r] : ((r > lane) ? -sum * irr[r] : 0.0);
+    }
+    if (lane < 16) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        if (r >= lane) Tm[c0 + r][c0 + lane] = t[r];
+    }
+  };
+
+  int bad = 0;
   if (FACTOR) {
-    int bad = 0;
+    for (int jb = 0; jb < 4; ++jb) {
+      const int c0 = 16 * jb;
+      if (w == 0) {
+        double a[16];
+        const int rr = min(lane, 15);
 #pragma unroll
-    for (int k = 0; k < 64; ++k) {
-      double* cb = col[k & 1];
-      cb[i] = a[k];   // S(i, k) after k updates
-      __syncthreads();
-      double d = cb[k];
-      if (!(d > 0.0)) {
-        if (bad == 0) bad = o + k + 1;
-        d = 1.0;
+        for (int j = 0; j < 16; ++j) a[j] = A[c0 + rr][c0 + j];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          double d = lane_bcast(a[k], k);   // S(k, k) after k updates
+          if (!(d > 0.0)) {
+            if (bad == 0) bad = o + c0 + k + 1;
+            d = 1.0;
+          }
+          const double p = sqrt(d), ip = 1.0 / p;
+          const double lk = (lane == k) ? p : a[k] * ip;   // L(lane, k)
+          a[k] = lk;
+#pragma unroll
+          for (int j = k + 1; j < 16; ++j) a[j] = fma(-lk, lane_bcast(lk, j), a[j]);
+        }
+        if (lane < 16) {
+#pragma unroll
+          for (int j = 0; j < 16; ++j)
+            if (j <= lane) A[c0 + lane][c0 + j] = a[j];
+        }
+        diag_inverse(c0, a);
       }
-      const double rd = 1.0 / d, p = sqrt(d), ip = p * rd;   // 1/d and sqrt(d) side by side
-      // S(i, j) -= L(i, k) L(j, k) = S(i, k) S(j, k) / d for j > k, the
-      // column read in chunks of 16 broadcasts (one LDS wait per chunk);
-      // lanes i < j update entries above their diagonal, which are never read
-      const double f = a[k] * rd;
+      __syncthreads();
+      // panel rows below: L(r, c0 + c) = sum_{q <= c} S(r, c0 + q) T_jj(c, q)
+      const int pr = 48 - c0;
+      double pv[3];
 #pragma unroll
-      for (int j0 = 0; j0 < 64; j0 += 16) {
-        if (j0 + 15 > k) {
-          double cv[16];
+      for (int e = 0; e < 3; ++e) {
+        const int idx = tid + 256 * e;
+        pv[e] = 0.0;
+        if (idx < pr * 16) {
+          const int r = c0 + 16 + (idx >> 4), c = idx & 15;
+          double s4[4] = {0.0, 0.0, 0.0, 0.0};   // T_jj(c, q) = 0 for q > c
 #pragma unroll
-          for (int jj = 0; jj < 16; ++jj) cv[jj] = (j0 + jj > k) ? cb[j0 + jj] : 0.0;
-#pragma unroll
-          for (int jj = 0; jj < 16; ++jj)
-            if (j0 + jj > k) a[j0 + jj] = fma(-f, cv[jj], a[j0 + jj]);
-          __builtin_amdgcn_sched_barrier(0);
+          for (int q = 0; q < 16; ++q) s4[q & 3] = fma(A[r][c0 + q], Tm[c0 + c][c0 + q], s4[q & 3]);
+          pv[e] = (s4[0] + s4[1]) + (s4[2] + s4[3]);
         }
       }
-      // column k of L is final: to LDS, out of registers
-      Ls[i][k] = (i > k) ? a[k] * ip : ((i == k) ? p : 0.0);
-    }
-    if (bad != 0 && i == 0 && *info == 0) *info = bad;
-  } else {
+      __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 64; ++j) Ls[i][j] = (j <= i) ? a[j] : 0.0;
-  }
-  __syncthreads();
-  if (FACTOR) {
-    // R-view lower (L row-major): row o + r, lanes over its columns
-    for (int r = 0; r < nl; ++r)
-      if (i <= r) Lb[(size_t)(o + r) * ld + o + i] = Ls[r][i];
-    // R-view upper (L column-major): row o + j holds column j, lanes over rows
-    for (int j = 0; j < nl; ++j)
-      if (i >= j && i < nl) Lb[(size_t)(o + j) * ld + o + i] = Ls[i][j];
-  }
-  rcp[i] = 1.0 / Ls[i][i];
-  __syncthreads();
-  // lane c: column c of T, T(r, c) = (delta_rc - sum_{q<r} L(r, q) T(q, c)) / L(r, r);
-  // T(q, c) = 0 for q < c, so the sum is empty at r = c
-  const int c = i;
-  double t[64];
-#pragma unroll
-  for (int r = 0; r < 64; ++r) {
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int q0 = 0; q0 < 64; q0 += 16) {
-      if (q0 < r) {
-#pragma unroll
-        for (int q = q0; q < q0 + 16; ++q)
-          if (q < r) s[q & 3] = fma(Ls[r][q], t[q], s[q & 3]);
-        // keep the broadcast loads of later chunks from piling up in registers
-        __builtin_amdgcn_sched_barrier(0);
+      for (int e = 0; e < 3; ++e) {
+        const int idx = tid + 256 * e;
+        if (idx < pr * 16) A[c0 + 16 + (idx >> 4)][c0 + (idx & 15)] = pv[e];
       }
-    }
-    const double sum = (s[0] + s[1]) + (s[2] + s[3]);
-    t[r] = (r == c) ? rcp[r] : -sum * rcp[r];
-  }
-  __syncthreads();   // every lane is done reading L
+      __syncthreads();
+      // trailing lower triangle: S(r, c) -= sum_q L(r, c0 + q) L(c, c0 + q)
+      for (int idx = tid; idx < pr * pr; idx += 256) {
+        const int r = c0 + 16 + idx / pr, c = c0 + 16 + idx % pr;
+        if (c <= r) {
+          double s4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int r = 0; r < 64; ++r) Ls[r][c] = (r >= c && o + r < n) ? t[r] : 0.0;
-  __syncthreads();
-  // R-view lower (T row-major) and upper (T column-major), zeros for rows >= n
-  for (int r = 0; r < 64; ++r) {
-    if (i <= r) Tb[(size_t)(o + r) * ld + o + i] = Ls[r][i];
-    if (i >= r) Tb[(size_t)(o + r) * ld + o + i] = Ls[i][r];
+          for (int q = 0; q < 16; ++q) s4[q & 3] = fma(A[r][c0 + q], A[c][c0 + q], s4[q & 3]);
+          A[r][c] -= (s4[0] + s4[1]) + (s4[2] + s4[3]);
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0 && bad != 0 && *info == 0) *info = bad;
+  } else {
+    double a[16];
+    const int c0 = 16 * w, rr = min(lane, 15);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) a[j] = A[c0 + rr][c0 + j];
+    diag_inverse(c0, a);
+    __syncthreads();
+  }
+
+  // off-diagonal blocks of T, block row I at a time
+  for (int I = 1; I < 4; ++I) {
+    for (int idx = tid; idx < I * 256; idx += 256) {
+      const int J = idx >> 8, r = (idx >> 4) & 15, c = idx & 15;
+      double s4[4] = {0.0, 0.0, 0.0, 0.0};   // T(kk, 16 J + c) = 0 for kk < 16 J + c
+      for (int k0 = 16 * J; k0 < 16 * I; k0 += 16) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) s4[q & 3] = fma(A[16 * I + r][k0 + q], Tm[k0 + q][16 * J + c], s4[q & 3]);
+      }
+      X[J][r][c] = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+    }
+    __syncthreads();
+    for (int idx = tid; idx < I * 256; idx += 256) {
+      const int J = idx >> 8, r = (idx >> 4) & 15, c = idx & 15;
+      double s4[4] = {0.0, 0.0, 0.0, 0.0};   // T_II(r, q) = 0 for q > r
+#pragma unroll
+      for (int q = 0; q < 16; ++q) s4[q & 3] = fma(Tm[16 * I + r][16 * I + q], X[J][q][c], s4[q & 3]);
+      Tm[16 * I + r][16 * J + c] = -((s4[0] + s4[1]) + (s4[2] + s4[3]));
+    }
+    __syncthreads();
+  }
+
+  // buffer row o + a, column o + b: b <= a holds the row-major entry (a, b),
+  // b > a the column-major entry (b, a); L only for live rows, T everywhere
+  // (zero for rows >= n)
+  for (int idx = tid; idx < 4096; idx += 256) {
+    const int ra = idx >> 6, cb = idx & 63;
+    const int r = max(ra, cb), c = min(ra, cb);   // the lower-triangle entry
+    if (FACTOR && r < nl) Lb[(size_t)(o + ra) * ld + o + cb] = A[r][c];
+    Tb[(size_t)(o + ra) * ld + o + cb] = (r < nl) ? Tm[r][c] : 0.0;
   }
 }
 
@@ -510,7 +565,7 @@ void tcopy(const double* src, double* dst, int R, int C, int64_t ld, hipStream_t
 void chol_inv_node(int n, int o, int size, bool factor, double* L, double* T, int64_t ld,
                    int* info, hipStream_t st) {
   if (size == 64) {
-    if (factor) hipLaunchKernelGGL(k_leaf<true>, dim3(1), dim3(64), 0, st, n, o, L, T, ld, info);
+    if (factor) hipLaunchKernelGGL(k_leaf<true>, dim3(1), dim3(256), 0, st, n, o, L, T, ld, info);
     return;   // without FACTOR every leaf ran up front
   }
   const int s = (size <= 128) ? 64 : ((size / 2 + 127) / 128) * 128;
@@ -554,7 +609,7 @@ int chol_inv(fgls_hip_ctx* c, bool factor, hipStream_t st) {
   const int n = c->n, N = c->ldn;
   const int64_t ld = N;
   if (!factor)
-    hipLaunchKernelGGL(k_leaf<false>, dim3(N / 64), dim3(64), 0, st, n, 0, c->L, c->Linv, ld,
+    hipLaunchKernelGGL(k_leaf<false>, dim3(N / 64), dim3(256), 0, st, n, 0, c->L, c->Linv, ld,
                        (int*)nullptr);
   chol_inv_node(n, 0, N, factor, c->L, c->Linv, ld, c->status + 1, st);
   hipLaunchKernelGGL(k_zero_upper, dim3(N), dim3(256), 0, st, N, c->Linv, ld);

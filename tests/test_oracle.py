@@ -187,7 +187,10 @@ def test_eigen_oracle_matches_reference_build(tmp_path):
     Sig = np.array([1.0, 2.0, 0.5])
     d = str(tmp_path)
     write_problem_dir(prob, d, H=H, Sig=Sig)
-    ref_runner.fgls_eigen_dir(d, n, wXL + 1, m, t, wXL, x_b, threads=2)
+    # one compute thread: with two, the reference's threaded loop and its
+    # asynchronous B/V writes (fgls_eigen.c:178-197, :574-600) wrote a wrong V
+    # about once in six runs here (B right, V_norm 0.56)
+    ref_runner.fgls_eigen_dir(d, n, wXL + 1, m, t, wXL, x_b, threads=1)
     B = np.fromfile(os.path.join(d, "B.out"))
     V = np.fromfile(os.path.join(d, "V.out"))
     Bo, Vo = orc.fgls_eigen(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y, wXR=1, x_b=x_b)
