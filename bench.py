@@ -47,15 +47,20 @@ PEAK_INT8_TOPS = 5000.0
 PEAK_HBM_GBS = 8000.0
 NSLICE = 6
 
-# Bytes per launch of the two reduce kernels at n = 10,000 and 65,536 SNPs per
+# Bytes per launch of the reduce kernels at n = 10,000 and 65,536 SNPs per
 # launch, from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of
 # this benchmark (KiB per dispatch, averaged over the launches of a 131,072-SNP
-# pass).  Scale factors from a plain 1 GiB device copy measured by the same
-# commands: it reads as FETCH_SIZE 524,313 KiB (x 2.0) and writes as
-# WRITE_SIZE 1,048,576 KiB (x 1.0).  FETCH_SIZE counts L2 misses sent to the fabric
+# pass): profile constants, not measured by the run that prints them.  Scale
+# factors from a plain 1 GiB device copy measured by the same commands: it
+# reads as FETCH_SIZE 524,315 KiB (x 2.0) and writes as WRITE_SIZE
+# 1,048,576 KiB (x 1.0).  FETCH_SIZE counts L2 misses sent to the fabric
 # (some are served by the MALL), so this is an upper bound on HBM reads.
 TRAFFIC_N, TRAFFIC_BLOCK = 10000, 65536
-TRAFFIC_KIB = {"fp64": (106698200.6, 202240.0), "mixed8": (23309193.3, 808960.0)}
+TRAFFIC_KIB = {"fp64": (106711329.6, 202240.0), "mixed": (12695667.6, 808960.0),
+               "mixed8": (20186549.5, 808960.0)}
+TRAFFIC_SOURCE = ("profile constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel at "
+                  "n=10000, 65,536 SNPs per launch, scaled by a 1 GiB copy (x2.0 / x1.0); "
+                  "not measured by this run")
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
@@ -765,6 +770,7 @@ def main():
             ach = flop / (res["reduce_ms"] * 1e-3) / 1e12
             return {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                     "frac": ach / PEAK_FP64_TFLOPS, "traffic": traffic(mode),
+                    "traffic_is": TRAFFIC_SOURCE if traffic(mode) is not None else None,
                     "kernel": "k_trmm<EPI_REDUCE> (fgls_hip_block_reduce)",
                     "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block, "flop_per_snp": float(n) * n}
         planes = 2 if mode == "dosage" else 1
@@ -772,6 +778,7 @@ def main():
         ach = ops / (res["reduce_ms"] * 1e-3) / 1e12
         return {"bound": "mfma", "achieved": ach, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_INT8_TOPS, "traffic": traffic(mode),
+                "traffic_is": TRAFFIC_SOURCE if traffic(mode) is not None else None,
                 "kernel": {"dosage": "k_trmm_i8 dosage planes (fgls_hip_block_reduce_i8t, planes 2)",
                            "mixed": "k_trmm_i8 on 2-bit genotype tiles (fgls_hip_block_reduce_u2t)",
                            "mixed8": "k_trmm_i8 on 8-bit genotype tiles (fgls_hip_block_reduce_i8t)"}[mode],
