@@ -207,11 +207,16 @@ def read_rate(path, nbytes, threads, direct=False):
     return got / dt / 1e9 if got == nbytes else None
 
 
-def run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, ref, dev):
+def run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, dev_paths, dev):
     """The drop-in's streamed path: fgls_chol_gpu_ex (files in, files out) at
     the workload's n and p on the first --pipeline-m SNPs of the same
     genotypes, written on this machine as the reference's FP64 XR.in, an int8
-    file and a 2-bit file (the page cache holds them: the reads are warm).
+    file and a 2-bit file (--pipeline-formats picks which; the page cache
+    holds them: the reads are warm).  With --pipeline-traits t > 1 the files
+    hold t traits: trait 0 is the benchmark's (h2, y), the others have h2 ~
+    U(0.1, 0.9) and their own y; all traits go through one pass over X_R, and
+    the per-trait setups (trait_setup_s, all traits) stand next to the block
+    loop.
     Per run: block-loop SNPs/s, the reader's X_R rate (bytes / time inside
     pread) and the effective rate (bytes / block-loop time), the upload rate
     (bytes / copy-stream time of the H2D copies) against PCIe Gen5 x16's
@@ -221,78 +226,100 @@ def run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, ref, dev):
     n, wXL = args.n, args.wXL
     p = wXL + 1
     m = min(args.pipeline_m, X.shape[0])
+    t = args.pipeline_traits
+    fmts = args.pipeline_formats.split(",")
     d = tempfile.mkdtemp(prefix="fgls_pipe_", dir=args.pipeline_dir)
     f = lambda s: os.path.join(d, s)
-    out = {"m": m, "n": n, "p": p, "runs": []}
+    out = {"m": m, "n": n, "p": p, "traits": t, "runs": []}
     try:
         t0 = time.perf_counter()
         np.ascontiguousarray(Phi.cpu().numpy(), dtype="<f8").tofile(f("Phi.in"))
-        np.asarray([args.h2], "<f8").tofile(f("H.in"))
-        np.asarray([1.0], "<f8").tofile(f("Sig.in"))
+        rng = np.random.default_rng(11)
+        H = np.concatenate([[args.h2], rng.uniform(0.1, 0.9, t - 1)])
+        out["h2"] = H.tolist()
+        H.astype("<f8").tofile(f("H.in"))
+        np.ones(t, "<f8").tofile(f("Sig.in"))
         XL.T.contiguous().cpu().numpy().astype("<f8").tofile(f("XL.in"))
-        y.cpu().numpy().astype("<f8").tofile(f("Y.in"))
+        Y = np.concatenate([y.cpu().numpy().astype("<f8"), rng.standard_normal(n * (t - 1))])
+        Y.astype("<f8").tofile(f("Y.in"))   # trait k at [k n, (k + 1) n)
         C = 16384
         nb = (n + 3) // 4
-        with open(f("XR.in"), "wb") as a, open(f("XR8.in"), "wb") as b, open(f("XR2.in"), "wb") as c:
+        files = {"f64": "XR.in", "i8": "XR8.in", "u2": "XR2.in"}
+        handles = {fmt: open(f(files[fmt]), "wb") for fmt in fmts}
+        try:
             for i0 in range(0, m, C):
                 i1 = min(i0 + C, m)
-                a.write(X[i0:i1, :n].contiguous().cpu().numpy().tobytes())
-                b.write(X8[i0:i1, :n].contiguous().cpu().numpy().tobytes())
-                # the 2-bit layout of pack_u2 (individual r in bits 2(r%4).. of
-                # byte r/4), formed on the device
-                q = torch.zeros((i1 - i0, 4 * nb), dtype=torch.uint8, device=dev)
-                q[:, :n] = X8[i0:i1, :n].to(torch.uint8)
-                q = q.view(i1 - i0, nb, 4)
-                u2 = q[:, :, 0] | (q[:, :, 1] << 2) | (q[:, :, 2] << 4) | (q[:, :, 3] << 6)
-                c.write(u2.cpu().numpy().tobytes())
-                del q, u2
-        log(f"pipeline: files of {m} SNPs written in {time.perf_counter() - t0:.1f} s "
-            f"(XR.in {os.path.getsize(f('XR.in')) / 1e9:.1f} GB)")
-        xr_bytes = os.path.getsize(f("XR.in"))
-        rates = {"page_cache_GBps": {}, "o_direct_GBps": None}
-        read_rate(f("XR.in"), min(xr_bytes, 8 << 30), 16)                 # warm-up
-        for th in (1, 4, 8, 16):
-            rates["page_cache_GBps"][th] = read_rate(f("XR.in"), min(xr_bytes, 8 << 30), th)
-        rates["o_direct_GBps"] = read_rate(f("XR.in"), min(xr_bytes, 4 << 30), 16, direct=True)
-        out["standalone_read"] = rates
-        log(f"pipeline: standalone reads of XR.in {rates}")
+                if "f64" in handles:
+                    handles["f64"].write(X[i0:i1, :n].contiguous().cpu().numpy().tobytes())
+                if "i8" in handles:
+                    handles["i8"].write(X8[i0:i1, :n].contiguous().cpu().numpy().tobytes())
+                if "u2" in handles:
+                    # the 2-bit layout of pack_u2 (individual r in bits 2(r%4).. of
+                    # byte r/4), formed on the device
+                    q = torch.zeros((i1 - i0, 4 * nb), dtype=torch.uint8, device=dev)
+                    q[:, :n] = X8[i0:i1, :n].to(torch.uint8)
+                    q = q.view(i1 - i0, nb, 4)
+                    u2 = q[:, :, 0] | (q[:, :, 1] << 2) | (q[:, :, 2] << 4) | (q[:, :, 3] << 6)
+                    handles["u2"].write(u2.cpu().numpy().tobytes())
+                    del q, u2
+        finally:
+            for h in handles.values():
+                h.close()
+        sizes = {fmt: os.path.getsize(f(files[fmt])) / 1e9 for fmt in fmts}
+        log(f"pipeline: files of {m} SNPs x {t} traits written in {time.perf_counter() - t0:.1f} s "
+            f"(GB: {sizes})")
+        if "f64" in fmts:
+            xr_bytes = os.path.getsize(f("XR.in"))
+            rates = {"page_cache_GBps": {}, "o_direct_GBps": None}
+            read_rate(f("XR.in"), min(xr_bytes, 8 << 30), 16)                 # warm-up
+            for th in (1, 4, 8, 16):
+                rates["page_cache_GBps"][th] = read_rate(f("XR.in"), min(xr_bytes, 8 << 30), th)
+            rates["o_direct_GBps"] = read_rate(f("XR.in"), min(xr_bytes, 4 << 30), 16, direct=True)
+            out["standalone_read"] = rates
+            log(f"pipeline: standalone reads of XR.in {rates}")
         # (X_R file, precision, io_threads (0: default), SNPs per block)
         # the first run is the drop-in's default (block = a multiple of x_b near
         # 16,384, io_threads = the host cores up to 16) on the reference's FP64 XR.in
         runs = [("f64", "fp64", 0, 16384), ("f64", "mixed", 0, 16384), ("f64", "fp64", 4, 16384),
                 ("f64", "fp64", 8, 16384), ("f64", "fp64", 0, 65536), ("i8", "fp64", 0, 16384),
                 ("i8", "mixed", 0, 16384), ("u2", "mixed", 0, 16384), ("u2", "fp64", 0, 16384)]
-        Bref = ref["B"][:m].cpu().numpy().ravel() if ref is not None else None
+        runs = [r for r in runs if r[0] in fmts]
+        # trait 0 of B.out against the device-resident path of the same precision
+        Bref = {prec: dev_paths[key]["B"][:m].cpu().numpy().ravel()
+                for prec, key in (("fp64", "fp64"), ("mixed", "mixed")) if key in dev_paths}
         for fmt, prec, io, blk in runs:
-            xr = {"f64": "XR.in", "i8": "XR8.in", "u2": "XR2.in"}[fmt]
+            xr = files[fmt]
             opts = {"precision": prec, "xr_format": fmt, "block": min(blk, m)}
             if io:
                 opts["io_threads"] = io
             t0 = time.perf_counter()
-            pkg.fgls_chol_gpu(n, p, m, 1, wXL, 1, 4096, 1, host_threads(), f("Phi.in"), f("H.in"), f("Sig.in"),
+            pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, 4096, 1, host_threads(), f("Phi.in"), f("H.in"), f("Sig.in"),
                               f("XL.in"), f(xr), f("Y.in"), f("B.out"), f("V.out"), options=opts)
             wall = time.perf_counter() - t0
             st = pkg.last_run_stats()
             r = {"xr_format": fmt, "precision": prec, "io_threads": st["io_threads"], "block": st["block"],
-                 "block_loop_snps_per_s": m / st["loop_s"], "block_loop_s": st["loop_s"],
+                 "traits_per_pass": st["traits_per_pass"], "passes": st["passes"],
+                 "block_loop_snps_per_s": m / st["loop_s"], "block_loop_trait_snps_per_s": m * t / st["loop_s"],
+                 "block_loop_s": st["loop_s"],
                  "whole_call_s": st["total_s"], "python_wall_s": wall, "trait_setup_s": st["trait_s"],
                  "xr_GB": st["xr_bytes"] / 1e9,
                  "xr_read_GBps": st["xr_bytes"] / st["read_busy_s"] / 1e9 if st["read_busy_s"] > 0 else None,
                  "xr_effective_GBps": st["xr_bytes"] / st["loop_s"] / 1e9,
                  "h2d_GBps": st["h2d_bytes"] / st["h2d_busy_s"] / 1e9 if st["h2d_busy_s"] > 0 else None,
                  "h2d_peak_GBps": 64.0, "cache": "warm (files written on this machine just before)"}
-            if prec == "fp64" and Bref is not None:
-                r["B_bitwise_equal_to_device_resident_fp64"] = bool(
-                    np.array_equal(np.fromfile(f("B.out"), dtype="<f8"), Bref))
+            if prec in Bref:
+                # B.out holds trait k at [k m p, (k + 1) m p)
+                b0 = np.fromfile(f("B.out"), dtype="<f8", count=m * p)
+                r[f"B_trait0_bitwise_equal_to_device_resident_{prec}"] = bool(np.array_equal(b0, Bref[prec]))
             out["runs"].append(r)
-            log(f"pipeline {fmt}/{prec} io_threads={r['io_threads']} block={r['block']}: "
+            log(f"pipeline {fmt}/{prec} io_threads={r['io_threads']} block={r['block']} traits={t}: "
                 f"{r['block_loop_snps_per_s']:.0f} SNPs/s "
-                f"(loop {st['loop_s']:.2f} s, call {st['total_s']:.2f} s), X_R read "
+                f"(loop {st['loop_s']:.2f} s, setups {st['trait_s']:.2f} s, call {st['total_s']:.2f} s), X_R read "
                 f"{r['xr_read_GBps'] or 0:.1f} GB/s in pread, {r['xr_effective_GBps']:.1f} GB/s effective, "
                 f"H2D {r['h2d_GBps'] or 0:.1f} GB/s")
     finally:
         shutil.rmtree(d, ignore_errors=True)
-    if out["runs"]:
+    if out["runs"] and out["runs"][0]["xr_format"] == "f64":
         r0 = out["runs"][0]
         out["default_fp64_file"] = {k: r0[k] for k in ("block_loop_snps_per_s", "xr_read_GBps",
                                                         "xr_effective_GBps", "h2d_GBps", "whole_call_s")}
@@ -525,6 +552,10 @@ def main():
                     help="SNPs of the file-pipeline leg (fgls_chol_gpu_ex on FP64, int8 and 2-bit X_R "
                          "files written here; 0: skip)")
     ap.add_argument("--pipeline-dir", default=None, help="directory for the pipeline leg's files (default: TMPDIR)")
+    ap.add_argument("--pipeline-formats", default="f64,i8,u2",
+                    help="X_R files the pipeline leg writes and runs: f64 (the reference's XR.in), i8, u2")
+    ap.add_argument("--pipeline-traits", type=int, default=1,
+                    help="traits in the pipeline leg's files (all in one pass over X_R)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (a known-size kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -818,7 +849,7 @@ def main():
     pipeline = None
     if args.pipeline_m > 0:
         try:
-            pipeline = run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, paths.get("fp64"), dev)
+            pipeline = run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, paths, dev)
         except (pkg.FglsError, OSError) as err:   # reported in the JSON line; the headline paths stand
             pipeline = {"error": str(err)}
             log(f"pipeline leg failed: {err}")
