@@ -4,6 +4,11 @@ Outputs (git-ignored, shipped with the tree):
   lib/libfgls_amd.so   HIP kernels + C-ABI (include/fgls_hip.h) + the C host
                        solver fgls_chol_gpu (include/fgls_chol_gpu.h)
   lib/driver.x         the reference's CLI contract (driver.c) over that solver
+  lib/driver_san.x     the same driver with the host C code (driver.c,
+                       fgls_host.c) built with AddressSanitizer and
+                       UndefinedBehaviorSanitizer (gcc, runtimes linked
+                       statically); the HIP objects are the regular ones, so
+                       only host code is instrumented
 """
 from __future__ import annotations
 
@@ -109,7 +114,29 @@ def build(verbose: bool = False) -> dict:
              "-L", LIBDIR, "-lfgls_amd", "-Wl,-rpath,$ORIGIN",
              "-Wl,-rpath-link," + blasdir], verbose)
         out["driver"] = exe
+        out["driver_san"] = build_sanitized_driver(objdir, hip_objs, blasdir, blasname, verbose)
     return out
+
+
+# host-only sanitizers: every instrumented object is compiled and linked by
+# gcc; the device objects come from the regular hipcc build above
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized_driver(objdir, hip_objs, blasdir, blasname, verbose):
+    objs = []
+    for name in ("fgls_host", "driver"):
+        obj = os.path.join(objdir, name + "_san.o")
+        run(["gcc", "-O1", "-std=gnu11", "-fPIC", "-Wall", "-Wno-unused-parameter", *SAN_FLAGS, "-fopenmp",
+             "-I", INCLUDE, "-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+             "-c", os.path.join(CSRC, name + ".c"), "-o", obj], verbose)
+        objs.append(obj)
+    exe = os.path.join(LIBDIR, "driver_san.x")
+    run(["g++", *SAN_FLAGS, "-static-libasan", "-static-libubsan", "-o", exe, *objs, *hip_objs,
+         "-L", "/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib",
+         "-L", blasdir, f"-l:{blasname}", f"-Wl,-rpath,{blasdir}", "-fopenmp", "-lpthread", "-lrt", "-ldl"],
+        verbose)
+    return exe
 
 
 if __name__ == "__main__":

@@ -6,17 +6,21 @@
  * single-GPU streaming pipeline:
  *
  *   reader thread:  pread(XR.in block c)  -> pinned host slot c%2
+ *                   (32 MB pieces over io_threads OpenMP threads)
  *   copy stream:    hipMemcpy2DAsync slot -> device X_R buffer c%2
- *   compute stream: fgls_hip_solve_block (TRMM + per-SNP stage) -> device B/V c%2
+ *   compute stream: conversions / tiles, then fgls_hip_solve_block* (product
+ *                   through L^-1 + per-SNP stage) per trait -> device B/V c%2
  *                   hipMemcpyAsync B/V -> pinned host B/V c%2
  *   main thread:    pwrite B/V of block c-1 at the reference's offsets
  *
  * Ordering is by hipEvents; each buffer pair is double buffered, so disk
  * reads, PCIe uploads, device compute and output writes of neighbouring
- * blocks overlap.  Per trait: M is formed and factorised on the host
- * (LAPACK dpotrf, as fgls_chol_gpu.c:406-430), L is uploaded once and the
- * device precompute runs (fgls_hip_set_trait).  X_R is streamed once per
- * trait (the CPU twin's trait loop, fgls_chol.c:223 and :336-337).
+ * blocks overlap.  Per trait: by default Phi is uploaded once and each
+ * trait's M is built, factorised and inverted on the device
+ * (fgls_hip_set_trait_phi); potrf=host forms and factorises M on the host
+ * (LAPACK dpotrf, as fgls_chol_gpu.c:406-430) and uploads L.  Several traits
+ * share one read of X_R (traits_per_pass; the CPU twin re-reads it per
+ * trait, fgls_chol.c:223 and :336-337).
  *
  * fgls_eigen() (include/fgls_eigen.h, the reference's src/fgls_eigen.c:37-236)
  * runs the same pipeline in the eigenbasis of Phi: one host dsyevr of Phi
@@ -25,6 +29,7 @@
  * (fgls_hip_eig_solve_block).
  */
 #define _GNU_SOURCE
+#include <dlfcn.h>
 #include <errno.h>
 #include <fcntl.h>
 #include <math.h>
@@ -70,7 +75,46 @@ typedef struct {
                         column), 3 i16 (dosages x * dosage_scale, 2 n bytes per column) */
   double dosage_scale;  /* precision=mixed for dosages: x * scale = 128 hi + lo (0: genotypes) */
   int eig_host;      /* eig=host: eigendecomposition of Phi by host LAPACK dsyevr (default: device) */
+  int no_roctx;      /* roctx=0: no roctx ranges around the pipeline stages */
 } options_t;
+
+/* roctx ranges around the pipeline's stages, the counterpart of the
+ * reference's VampirTrace regions (fgls_chol_gpu.c:57-59, :69-71): a
+ * profiler (rocprofv3 --marker-trace) shows the trait setups, the block loop
+ * and, per block, the read, the enqueue and the write-back.  The roctx
+ * library is resolved at the first use; without it the ranges are no-ops.
+ * Ranges nest per thread. */
+typedef int (*roctx_push_t)(const char *);
+typedef int (*roctx_pop_t)(void);
+static roctx_push_t g_roctx_push;
+static roctx_pop_t g_roctx_pop;
+static pthread_once_t g_roctx_once = PTHREAD_ONCE_INIT;
+
+static void roctx_load(void) {
+  static const char *const libs[] = {"librocprofiler-sdk-roctx.so.1", "libroctx64.so.4"};
+  for (size_t i = 0; i < sizeof(libs) / sizeof(libs[0]); ++i) {
+    void *h = dlopen(libs[i], RTLD_NOW | RTLD_LOCAL);
+    if (!h) continue;
+    roctx_push_t push = (roctx_push_t)dlsym(h, "roctxRangePushA");
+    roctx_pop_t pop = (roctx_pop_t)dlsym(h, "roctxRangePop");
+    if (push && pop) {
+      g_roctx_push = push;
+      g_roctx_pop = pop;
+      return;
+    }
+    dlclose(h);
+  }
+}
+
+static void range_push(int on, const char *name) {
+  if (!on) return;
+  pthread_once(&g_roctx_once, roctx_load);
+  if (g_roctx_push) g_roctx_push(name);
+}
+
+static void range_pop(int on) {
+  if (on && g_roctx_pop) g_roctx_pop();
+}
 
 /* Timings of the calling thread's last completed fgls_*_gpu_ex call. */
 static __thread fgls_run_stats g_last_stats;
@@ -132,6 +176,7 @@ static void parse_options(const char *s, options_t *o) {
     else if (!strcmp(tok, "stats")) o->stats = (int)v;
     else if (!strcmp(tok, "io_threads")) o->io_threads = (int)v;
     else if (!strcmp(tok, "traits_per_pass")) o->traits_per_pass = (int)v;
+    else if (!strcmp(tok, "roctx")) o->no_roctx = (v == 0);
   }
 }
 
@@ -293,6 +338,7 @@ typedef struct {
   char err[2][256];
   volatile int stop;
   int io_threads;           /* parallel preads per slot */
+  int trace;                /* roctx ranges */
   double busy_s;            /* time spent in pread */
 } reader_t;
 
@@ -309,6 +355,7 @@ static void *reader_main(void *arg) {
     const size_t bytes = (size_t)cnt * R->wXR * col_bytes;
     const off_t off = (off_t)i0 * R->wXR * (off_t)col_bytes;
     double t0 = now_s();
+    range_push(R->trace, "fgls: read X_R block");
     /* 32 MB preads handed out to io_threads threads in turn (page-cache
      * reads are memory-copy bound per thread; small pieces keep every
      * thread busy to the end of the block) */
@@ -321,6 +368,7 @@ static void *reader_main(void *arg) {
       const size_t o = (size_t)i * piece, w = (bytes - o < piece) ? bytes - o : piece;
       got += pread_full(R->fd, (char *)R->slot[s] + o, w, off + (off_t)o);
     }
+    range_pop(R->trace);
     R->busy_s += now_s() - t0;
     if (got != bytes) {
       R->status[s] = FGLS_ERR_IO;
@@ -431,7 +479,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
                     const char *Y_path, const char *B_path, const char *V_path,
                     const char *options) {
   int rc = FGLS_OK;
-  options_t opt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0.0, 0};
+  options_t opt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0.0, 0, 0};
   parse_options(options ? options : getenv("FGLS_OPTIONS"), &opt);
   if (opt.eig_host < 0) return fgls_set_error(FGLS_ERR_ARG, "eig must be gpu or host");
   if (opt.mixed < 0) return fgls_set_error(FGLS_ERR_ARG, "precision must be fp64 or mixed");
@@ -504,6 +552,22 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   int reader_started = 0;
   double t_potrf = 0.0, t_trait = 0.0, t_loop = 0.0, h2d_ms = 0.0;
   const double t_call0 = now_s();
+  /* roctx ranges of this thread; an error exit closes the open ones */
+  const int trace = !opt.no_roctx;
+  int rdepth = 0;
+#define RANGE_PUSH(name)       \
+  do {                         \
+    range_push(trace, (name)); \
+    ++rdepth;                  \
+  } while (0)
+#define RANGE_POP()        \
+  do {                     \
+    if (rdepth > 0) {      \
+      range_pop(trace);    \
+      --rdepth;            \
+    }                      \
+  } while (0)
+  RANGE_PUSH("fgls: inputs and device setup");
   hipEvent_t ev_c0[2] = {NULL, NULL}, ev_c1[2] = {NULL, NULL};   /* timed uploads */
 
   const size_t nn = (size_t)n * n;
@@ -661,6 +725,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     R.block = block;
     R.nchunks = nchunks;
     R.io_threads = opt.io_threads > 0 ? opt.io_threads : (host_cores() < 16 ? host_cores() : 16);
+    R.trace = trace;
     for (int s = 0; s < 2; ++s) {
       sem_init(&R.free_[s], 0, 1);
       sem_init(&R.full[s], 0, 0);
@@ -672,10 +737,12 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     reader_started = 1;
   }
   scipy_openblas_set_num_threads(threads);
+  RANGE_POP();
 
   int64_t it = 0;  /* global chunk counter across passes (reader order) */
   for (int j0 = 0; j0 < t; j0 += K) {
     const int kn = (t - j0) < K ? (t - j0) : K;   /* traits of this pass */
+    RANGE_PUSH("fgls: trait setup");
     if (eig) {
       /* Z' Y, weights, B_t, V_tl and res_sigma of the pass's traits */
       double t0 = now_s();
@@ -702,11 +769,14 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
       }
       t_trait += now_s() - t0;
     }
+    RANGE_POP();
 
     double tl0 = now_s();
+    RANGE_PUSH("fgls: block loop");
     for (int64_t c = 0; c <= nchunks; ++c, ++it) {
       const int s = (int)(it & 1);
       if (c < nchunks) {
+        RANGE_PUSH("fgls: upload and enqueue block");
         const int64_t i0 = c * block;
         const int cnt = (int)((m - i0) < block ? (m - i0) : block);
         sem_wait(&R.full[s]);
@@ -811,8 +881,10 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           h2d_ms += ms;
         }
         sem_post(&R.free_[s]);
+        RANGE_POP();
       }
       if (c > 0) {  /* write back chunk c-1 */
+        RANGE_PUSH("fgls: write back block");
         const int ps = (int)((it - 1) & 1);
         const int64_t i0 = (c - 1) * block;
         const int cnt = (int)((m - i0) < block ? (m - i0) : block);
@@ -849,10 +921,12 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           RCHK(write_at(fdV, V_path, hV[q], (size_t)cnt * pp * sizeof(double),
                         (off_t)(((int64_t)j * m + i0) * pp) * (off_t)sizeof(double)));
         }
+        RANGE_POP();
       }
     }
     --it;  /* the c == nchunks pass consumed no slot */
     t_loop += now_s() - tl0;
+    RANGE_POP();
   }
   HCHK(hipStreamSynchronize(s_comp));
   {
@@ -885,6 +959,9 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   }
 
 done:
+  while (rdepth > 0) RANGE_POP();
+#undef RANGE_PUSH
+#undef RANGE_POP
   if (reader_started) {
     R.stop = 1;
     for (int s = 0; s < 2; ++s) sem_post(&R.free_[s]);

@@ -57,6 +57,9 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *   serial=1       synchronise after every stage (debug aid, like the
  *                  reference's FGLS_GPU_SERIAL, fgls_chol_gpu.c:44-46)
  *   stats=1        print stage timings and rates to stderr
+ *   roctx=0        no roctx ranges around the pipeline stages (by default
+ *                  they are emitted when the ROCm roctx library loads; the
+ *                  reference's VampirTrace regions, fgls_chol_gpu.c:57-59)
  *   io_threads=<k> threads reading each X_R block in 32 MB preads (default:
  *                  the host cores available, at most 16)
  *   dosage_scale=<s>  with precision=mixed (or with xr_format=i16): X_R values are dosages with a
@@ -72,7 +75,10 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *                  (m*wXR columns in the FP64 file's order; individual r in
  *                  bits 2(r%4)..2(r%4)+1 of byte r/4, codes 0/1/2 = the
  *                  genotype, code 3 is an FGLS_ERR_ARG): 32x fewer bytes than
- *                  FP64; unpacked to int8 on the device (fgls_hip_u2_to_i8)
+ *                  FP64; with precision=mixed tiled on the device straight
+ *                  into the product's 2-bit operand tiles
+ *                  (fgls_hip_tile_u2_packed), else unpacked to int8
+ *                  (fgls_hip_u2_to_i8)
  *   xr_format=i16  with dosage_scale=<s>: XR_path holds int16 values
  *                  v = x * s (2 bytes per value, the FP64 file's order),
  *                  4x fewer bytes than FP64; the mixed path splits them into
