@@ -588,8 +588,12 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: {nel * 8} bytes read and written")
         del src, dst
+    # a dosage-only run draws no genotypes (a 2,000,000-SNP n = 20,000 FP64
+    # genotype matrix would not fit in HBM next to its dosage planes)
+    path_list = args.paths.split(",")
+    geno = any(pth in ("fp64", "mixed", "mixed8") for pth in path_list)
     t0 = time.perf_counter()
-    Phi, L, XL, y, X, X8 = make_inputs(torch, n, m, wXL, args.h2, args.seed, ldx, dev)
+    Phi, L, XL, y, X, X8 = make_inputs(torch, n, m if geno else 0, wXL, args.h2, args.seed, ldx, dev)
     torch.cuda.synchronize()
     log(f"inputs ready: n={n} m={m} X_R {X.numel() * 8 / 1e9:.1f} GB FP64 + {X8.numel() / 1e9:.1f} GB int8 "
         f"in HBM ({time.perf_counter() - t0:.1f}s)")
@@ -608,7 +612,7 @@ def main():
     starts = list(range(0, m, block))
 
     D8 = None
-    if "dosage" in args.paths.split(","):
+    if "dosage" in path_list:
         # dosages with 3 decimals, round(U(0, 2), 3), as per-block pairs of
         # int8 digit planes (lo, hi) of 1000 x; the FP64 values of a sample
         # are rebuilt for the parity check
@@ -736,9 +740,11 @@ def main():
         Lc = orc.cholesky_lower(orc.build_M(Phi.cpu().numpy(), args.h2, 1.0))   # LAPACK's factor
         tr = orc.trait_precompute(Lc, XL.cpu().numpy(), y.cpu().numpy())
         sel = torch.as_tensor(idx, device=dev)
-        Xs = X[sel, :n].cpu().numpy().T.copy()
-        b_ref, V_ref = orc.solve_snps(Xs, tr, 1)
-        Vr = V_ref.transpose(0, 2, 1).reshape(idx.size, -1)
+        b_ref = Vr = None
+        if geno:
+            Xs = X[sel, :n].cpu().numpy().T.copy()
+            b_ref, V_ref = orc.solve_snps(Xs, tr, 1)
+            Vr = V_ref.transpose(0, 2, 1).reshape(idx.size, -1)
         log(f"oracle on {idx.size} SNPs: {time.perf_counter() - t0:.1f} s")
         for mode, res in paths.items():
             bref, vref = b_ref, Vr
@@ -783,7 +789,9 @@ def main():
         log(f"PCIe H2D measurement skipped: {err}")
 
     cpu = None
-    if args.cpu_baseline != "none":
+    if args.cpu_baseline != "none" and not geno:
+        log("CPU baseline skipped: it runs on the genotype workload, which a dosage-only run does not draw")
+    elif args.cpu_baseline != "none":
         threads = host_threads()
         log(f"CPU baseline ({args.cpu_baseline}, {threads} threads, {args.cpu_snps} SNPs)")
         cpu = cpu_baseline(args.cpu_baseline, Phi, L, XL, y, X, n, wXL, args.h2, args.cpu_snps, threads)
@@ -818,7 +826,7 @@ def main():
                 "int8_ops_per_snp": planes * NSLICE * float(n) * n,
                 "fp64_equivalent_tflops": flop / (res["reduce_ms"] * 1e-3) / 1e12}
 
-    head = "mixed" if "mixed" in paths else "fp64"
+    head = next(h for h in ("mixed", "fp64", "mixed8", "dosage") if h in paths)
     res = paths[head]
     summary = {}
     for mode, r in paths.items():
@@ -842,19 +850,19 @@ def main():
             paths["mixed"]["B"].cpu().numpy(), paths["mixed"]["V"].cpu().numpy(),
             paths["fp64"]["B"].cpu().numpy(), paths["fp64"]["V"].cpu().numpy(), p)
         summary["speedup_mixed_over_fp64"] = paths["mixed"]["snps_per_s"] / paths["fp64"]["snps_per_s"]
+        log(f"mixed vs fp64 over all {m} SNPs: {e}; speedup {summary['speedup_mixed_over_fp64']:.2f}x")
     if "mixed" in paths and "mixed8" in paths:
         summary["mixed_2bit_tiles_bitwise_equal_8bit_tiles"] = bool(
             torch.equal(paths["mixed"]["B"], paths["mixed8"]["B"]) and torch.equal(paths["mixed"]["V"], paths["mixed8"]["V"]))
-        log(f"mixed vs fp64 over all {m} SNPs: {e}; speedup {summary['speedup_mixed_over_fp64']:.2f}x")
     pipeline = None
-    if args.pipeline_m > 0:
+    if args.pipeline_m > 0 and geno:
         try:
             pipeline = run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, paths, dev)
         except (pkg.FglsError, OSError) as err:   # reported in the JSON line; the headline paths stand
             pipeline = {"error": str(err)}
             log(f"pipeline leg failed: {err}")
     eigen = None
-    if args.eigen_traits > 0:
+    if args.eigen_traits > 0 and geno:
         ctx.close()
         torch.cuda.empty_cache()
         try:
@@ -886,7 +894,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "i8" if head == "mixed" else "f64",
+        "dtype": "f64" if head == "fp64" else "i8",
         "data": "synthetic (configuration generators, seeded); X_R resident in HBM",
         "config": {"workload": f"Cholesky FGLS, n={n}, p={p}, m={m}, t=1, h2={args.h2}; value = {head} path",
                    "n": n, "p": p, "m": m, "block": block, "path": head,
