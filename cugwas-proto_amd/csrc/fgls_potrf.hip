@@ -127,24 +127,35 @@ __device__ __forceinline__ double lane_bcast(double x, int l) {
 // non-positive pivot is recorded in info (1-based), and the pivot is replaced
 // by 1 so that the later arithmetic stays finite.
 // ---------------------------------------------------------------------------
-template <bool FACTOR>
-__global__ __launch_bounds__(256) void k_leaf(int n, int o0, double* __restrict__ Lb,
-                                              double* __restrict__ Tb, int64_t ld,
-                                              int* __restrict__ info) {
-  __shared__ double A[64][65];
-  __shared__ double Tm[64][65];
-  __shared__ double X[3][16][17];
-  const int o = o0 + 64 * (int)blockIdx.x;
+// The LDS of one 64 x 64 leaf: A holds S, then L; Tm holds T = L^-1 (lower
+// triangles; T's strict upper triangle is zero); X the block sums of T.
+struct LeafLds {
+  double A[64][65];
+  double Tm[64][65];
+  double X[3][16][17];
+};
+
+// Leaf body for the block at o, run by all 256 threads of the workgroup.
+// FROM_LDS: S is already in A (rows >= n as identity rows), written by the
+// caller before a barrier; otherwise it is read from the buffer.
+template <bool FACTOR, bool FROM_LDS>
+__device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, double* __restrict__ Tb, int64_t ld,
+                       int* __restrict__ info, LeafLds& sh) {
+  auto& A = sh.A;
+  auto& Tm = sh.Tm;
+  auto& X = sh.X;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nl = min(64, n - o);   // live rows of the block
+  const int nl = min(64, n - o);   // live rows of the block (may be <= 0)
   for (int idx = tid; idx < 4096; idx += 256) {
     // FACTOR: row-major source, c fastest; else column-major, r fastest
     const int r = FACTOR ? idx >> 6 : idx & 63, c = FACTOR ? idx & 63 : idx >> 6;
-    double v = (r == c) ? 1.0 : 0.0;
-    if (r < nl && c <= r)
-      v = FACTOR ? Lb[(size_t)(o + r) * ld + o + c] : Lb[(size_t)(o + c) * ld + o + r];
-    A[r][c] = v;
+    if (!FROM_LDS) {
+      double v = (r == c) ? 1.0 : 0.0;
+      if (r < nl && c <= r)
+        v = FACTOR ? Lb[(size_t)(o + r) * ld + o + c] : Lb[(size_t)(o + c) * ld + o + r];
+      A[r][c] = v;
+    }
     Tm[r][c] = 0.0;   // the strict upper triangle of T stays zero: the block
                       // products below run over whole 16-blocks
   }
@@ -277,6 +288,135 @@ __global__ __launch_bounds__(256) void k_leaf(int n, int o0, double* __restrict_
     if (FACTOR && r < nl) Lb[(size_t)(o + ra) * ld + o + cb] = A[r][c];
     Tb[(size_t)(o + ra) * ld + o + cb] = (r < nl) ? Tm[r][c] : 0.0;
   }
+}
+
+template <bool FACTOR>
+__global__ __launch_bounds__(256) void k_leaf(int n, int o0, double* __restrict__ Lb,
+                                              double* __restrict__ Tb, int64_t ld,
+                                              int* __restrict__ info) {
+  __shared__ LeafLds sh;
+  leaf64<FACTOR, false>(n, o0 + 64 * (int)blockIdx.x, Lb, Tb, ld, info, sh);
+}
+
+// ---------------------------------------------------------------------------
+// A whole 128-row node of the factorising recursion (rows [o, o + 128)) in
+// one workgroup, in place of 2 leaves, 4 products and 2 transposed copies:
+//   leaf(S11) -> L11, T11;  L21 = S21 T11';  S22 - L21 L21' and X = L21 T11
+//   (into LDS);  leaf(S22) -> L22, T22;  T21 = -T22 X.
+// Everything the node produces for itself stays in LDS (L21, X, the updated
+// S22, T11 and T22); only S11 and S21, written by earlier launches, are read
+// from the buffers.  The buffers end as after the separate launches: L and
+// T in both triangles of the two diagonal blocks, L21 row-major (R-view
+// lower-left) and column-major (R-view upper-right), T21 likewise.  The
+// 64 x 64 x 64 products run on FP64 MFMA with both operands in LDS (a
+// quadrant per wave), T's rows >= n zeroed first so that they match the
+// stored factor.
+// ---------------------------------------------------------------------------
+struct NodeLds {
+  LeafLds leaf;
+  double P[64][65];   // S21, then L21
+  double Q[64][65];   // X = L21 T11
+};
+
+// C(r, c) = sum_k U(r, k) V(c, k) (V_T: V(k, c) instead), k = 0..63, on FP64
+// MFMA: wave w takes the 32 x 32 quadrant at rows 32 (w >> 1), columns
+// 32 (w & 1); element v of acc[ti][tj] is (row r0 + 16 ti + li, column
+// c0 + 16 tj + lg + 4 v), the operand order of k_tgemm.
+template <bool V_T>
+__device__ __forceinline__ void lds_product(const double (&U)[64][65], const double (&V)[64][65], int w,
+                                            int lane, v4d (&acc)[2][2]) {
+  const int li = lane & 15, lg = lane >> 4;
+  const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1);
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+  for (int k0 = 0; k0 < 64; k0 += 16) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = k0 + 4 * lg + s;
+      double a[2], b[2];
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti) a[ti] = U[r0 + 16 * ti + li][k];
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj) b[tj] = V_T ? V[k][c0 + 16 * tj + li] : V[c0 + 16 * tj + li][k];
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = mfma_f64(b[tj], a[ti], acc[ti][tj]);
+    }
+  }
+}
+
+// f(r, c, value) for every element of the wave's quadrant
+template <typename F>
+__device__ __forceinline__ void for_quadrant(const v4d (&acc)[2][2], int w, int lane, F f) {
+  const int li = lane & 15, lg = lane >> 4;
+  const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1);
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) f(r0 + 16 * ti + li, c0 + 16 * tj + lg + 4 * v, acc[ti][tj][v]);
+}
+
+__global__ __launch_bounds__(256) void k_node128(int n, int o, double* __restrict__ Lb,
+                                                 double* __restrict__ Tb, int64_t ld,
+                                                 int* __restrict__ info) {
+  __shared__ NodeLds sh;
+  auto& Tm = sh.leaf.Tm;
+  auto& A = sh.leaf.A;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int o2 = o + 64;
+  const int nl1 = min(64, n - o), nl2 = min(64, n - o2);
+
+  leaf64<true, false>(n, o, Lb, Tb, ld, info, sh.leaf);
+  __syncthreads();   // the leaf's last reads of Tm are done
+  // T11 as stored: rows >= n zero; S21 (R-view lower-left, row-major)
+  for (int idx = tid; idx < 4096; idx += 256) {
+    const int r = idx >> 6, c = idx & 63;
+    if (r >= nl1) Tm[r][c] = 0.0;
+    sh.P[r][c] = Lb[(size_t)(o2 + r) * ld + o + c];
+  }
+  __syncthreads();
+  // L21(r, c) = sum_{k <= c} S21(r, k) T11(c, k)
+  v4d acc[2][2];
+  lds_product<false>(sh.P, Tm, w, lane, acc);
+  __syncthreads();   // every wave is done reading S21
+  for_quadrant(acc, w, lane, [&](int r, int c, double v) {
+    sh.P[r][c] = v;
+    Lb[(size_t)(o2 + r) * ld + o + c] = v;   // row-major (R-view lower-left)
+    Lb[(size_t)(o + c) * ld + o2 + r] = v;   // column-major (R-view upper-right)
+  });
+  __syncthreads();
+  // X(r, c) = sum_{k >= c} L21(r, k) T11(k, c) into Q
+  lds_product<true>(sh.P, Tm, w, lane, acc);
+  for_quadrant(acc, w, lane, [&](int r, int c, double v) { sh.Q[r][c] = v; });
+  // S22(r, c) - sum_k L21(r, k) L21(c, k): the second leaf's input in A (its
+  // lower triangle; rows >= n identity rows, as the leaf's own load)
+  lds_product<false>(sh.P, sh.P, w, lane, acc);
+  for_quadrant(acc, w, lane, [&](int r, int c, double v) {
+    double a = (r == c) ? 1.0 : 0.0;
+    if (r < nl2 && c <= r) a = Lb[(size_t)(o2 + r) * ld + o2 + c] - v;
+    A[r][c] = a;
+  });
+  __syncthreads();   // A complete; the first leaf's T11 in Tm is no longer needed
+  leaf64<true, true>(n, o2, Lb, Tb, ld, info, sh.leaf);
+  __syncthreads();
+  for (int idx = tid; idx < 4096; idx += 256) {
+    const int r = idx >> 6, c = idx & 63;
+    if (r >= nl2) Tm[r][c] = 0.0;   // T22 as stored
+  }
+  __syncthreads();
+  // T21(c, r) = -sum_{k <= c} T22(c, k) X(k, r)
+  lds_product<true>(Tm, sh.Q, w, lane, acc);
+  for_quadrant(acc, w, lane, [&](int c, int r, double v) {
+    Tb[(size_t)(o2 + c) * ld + o + r] = -v;   // row-major (R-view lower-left)
+    Tb[(size_t)(o + r) * ld + o2 + c] = -v;   // column-major (R-view upper-right)
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -567,6 +707,10 @@ void chol_inv_node(int n, int o, int size, bool factor, double* L, double* T, in
   if (size == 64) {
     if (factor) hipLaunchKernelGGL(k_leaf<true>, dim3(1), dim3(256), 0, st, n, o, L, T, ld, info);
     return;   // without FACTOR every leaf ran up front
+  }
+  if (size == 128 && factor) {
+    hipLaunchKernelGGL(k_node128, dim3(1), dim3(256), 0, st, n, o, L, T, ld, info);
+    return;
   }
   const int s = (size <= 128) ? 64 : ((size / 2 + 127) / 128) * 128;
   const int s2 = size - s;
