@@ -13,7 +13,7 @@
  *                   hipMemcpyAsync B/V -> pinned host B/V c%2
  *   main thread:    pwrite B/V of block c-1 at the reference's offsets
  *
- * Ordering is by hipEvents; each buffer pair is double buffered, so disk
+ * Ordering is by hipEvents; every buffer is double buffered (NSLOT), so disk
  * reads, PCIe uploads, device compute and output writes of neighbouring
  * blocks overlap.  Per trait: by default Phi is uploaded once and each
  * trait's M is built, factorised and inverted on the device
@@ -326,16 +326,21 @@ static int write_at(int fd, const char *name, const void *buf, size_t bytes, off
   } while (0)
 
 /* ---------------------------------------------------------------- reader */
+/* X_R blocks in flight: pinned read slots, device X_R buffers and B/V
+ * buffers, used in turn.  Three measured 5-10 % slower than two on the
+ * reference's FP64 file (the reader running further ahead competes with
+ * the uploads for host memory bandwidth) and the same on compact files. */
+#define NSLOT 2
 typedef struct {
   int fd;
   const char *path;
   int n, wXR, t;
   size_t col_bytes;         /* bytes per X_R column: 8 n (FP64), n (int8) or ceil(n/4) (2-bit) */
   int64_t m, block, nchunks;
-  void *slot[2];            /* pinned */
-  sem_t free_[2], full[2];
-  int status[2];            /* FGLS_OK or FGLS_ERR_IO per filled slot */
-  char err[2][256];
+  void *slot[NSLOT];            /* pinned */
+  sem_t free_[NSLOT], full[NSLOT];
+  int status[NSLOT];            /* FGLS_OK or FGLS_ERR_IO per filled slot */
+  char err[NSLOT][256];
   volatile int stop;
   int io_threads;           /* parallel preads per slot */
   int trace;                /* roctx ranges */
@@ -346,7 +351,7 @@ static void *reader_main(void *arg) {
   reader_t *R = (reader_t *)arg;
   const size_t col_bytes = R->col_bytes;
   for (int64_t it = 0; it < (int64_t)R->t * R->nchunks; ++it) {
-    const int s = (int)(it & 1);
+    const int s = (int)(it % NSLOT);
     sem_wait(&R->free_[s]);
     if (R->stop) break;
     const int64_t c = it % R->nchunks;
@@ -524,24 +529,24 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   /* per slot s and trait k of a pass: index s * K + k, views into one
    * buffer per slot (dBs[s] + k * block * p, ...) */
   double **hB = NULL, **hV = NULL, **dB = NULL, **dV = NULL;
-  double *hBs[2] = {NULL, NULL}, *hVs[2] = {NULL, NULL}, *dBs[2] = {NULL, NULL}, *dVs[2] = {NULL, NULL};
+  double *hBs[NSLOT] = {NULL}, *hVs[NSLOT] = {NULL}, *dBs[NSLOT] = {NULL}, *dVs[NSLOT] = {NULL};
   fgls_hip_eig *ectx = NULL;       /* eigen variant: one context for the K traits of a pass */
   double *Zh = NULL, *Wh = NULL, *dW = NULL, *yK = NULL;
   double t_eig = 0.0;
   unsigned long long *hErr = NULL;  /* pinned copies of the failure records */
   unsigned int *hBad = NULL;        /* pinned: genotypes of the slot that are not int8-exact */
-  int8_t *dX8[2] = {NULL, NULL};
-  uint8_t *dP2[2] = {NULL, NULL};   /* xr_format=u2: packed columns of the slot */
-  int8_t *dX8T[2] = {NULL, NULL};   /* mixed: the slot's genotypes tiled once for every trait */
-  uint8_t *dX2T[2] = {NULL, NULL};  /* mixed with a 2-bit file: the slot's 2-bit tiles */
+  int8_t *dX8[NSLOT] = {NULL};
+  uint8_t *dP2[NSLOT] = {NULL};   /* xr_format=u2: packed columns of the slot */
+  int8_t *dX8T[NSLOT] = {NULL};   /* mixed: the slot's genotypes tiled once for every trait */
+  uint8_t *dX2T[NSLOT] = {NULL};  /* mixed with a 2-bit file: the slot's 2-bit tiles */
   const int u2t = opt.mixed && opt.xr_i8 == 2;   /* 2-bit file straight to 2-bit tiles */
   const size_t col_bytes = xr16 ? (size_t)n * 2 : opt.xr_i8 == 2 ? (size_t)(n + 3) / 4
                                                : opt.xr_i8 == 1 ? (size_t)n : (size_t)n * sizeof(double);
   unsigned int *dBad = NULL;
   const int ld8 = (n + 63) & ~63;
-  double *dL = NULL, *dXL = NULL, *dy = NULL, *dX[2] = {NULL, NULL};
+  double *dL = NULL, *dXL = NULL, *dy = NULL, *dX[NSLOT] = {NULL};
   hipStream_t s_copy = NULL, s_comp = NULL;
-  hipEvent_t ev_h2d[2] = {NULL, NULL}, ev_out[2] = {NULL, NULL}, ev_free[2] = {NULL, NULL};
+  hipEvent_t ev_h2d[NSLOT] = {NULL}, ev_out[NSLOT] = {NULL}, ev_free[NSLOT] = {NULL};
   fgls_hip_ctx **ctx = NULL;
   int K = 1;                        /* traits per pass over X_R */
   reader_t R;
@@ -568,7 +573,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     }                      \
   } while (0)
   RANGE_PUSH("fgls: inputs and device setup");
-  hipEvent_t ev_c0[2] = {NULL, NULL}, ev_c1[2] = {NULL, NULL};   /* timed uploads */
+  hipEvent_t ev_c0[NSLOT] = {NULL}, ev_c1[NSLOT] = {NULL};   /* timed uploads */
 
   const size_t nn = (size_t)n * n;
   /* Phi is held on the host only for the host factor (M per trait) or the
@@ -612,7 +617,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     HCHK(hipMemGetInfo(&free_b, &total_b));
     const size_t ldn = ((size_t)n + 127) / 128 * 128;
     const size_t per = eig ? eig_trait_bytes(n, p, block, opt.mixed) : ctx_bytes(n, p, block, opt.mixed);
-    size_t shared = (size_t)block * wXR * ldx * sizeof(double) * 2 + nn * sizeof(double);
+    size_t shared = (size_t)block * wXR * ldx * sizeof(double) * NSLOT + nn * sizeof(double);
     if (eig) shared += ldn * ldn * (sizeof(double) + (opt.mixed ? 6 : 0)) + (64u << 20);
     long long fit = free_b > shared ? (long long)((free_b - shared) * 0.6 / per) : 1;
     if (fit < 1) fit = 1;
@@ -620,10 +625,10 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     if (K > t) K = t;
   }
   ctx = (fgls_hip_ctx **)calloc((size_t)K, sizeof(*ctx));
-  hB = (double **)calloc((size_t)2 * K, sizeof(double *));
-  hV = (double **)calloc((size_t)2 * K, sizeof(double *));
-  dB = (double **)calloc((size_t)2 * K, sizeof(double *));
-  dV = (double **)calloc((size_t)2 * K, sizeof(double *));
+  hB = (double **)calloc((size_t)NSLOT * K, sizeof(double *));
+  hV = (double **)calloc((size_t)NSLOT * K, sizeof(double *));
+  dB = (double **)calloc((size_t)NSLOT * K, sizeof(double *));
+  dV = (double **)calloc((size_t)NSLOT * K, sizeof(double *));
   if (!ctx || !hB || !hV || !dB || !dV) {
     rc = fgls_set_error(FGLS_ERR_NOMEM, "host allocation failed");
     goto done;
@@ -639,7 +644,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   }
   HCHK(hipStreamCreateWithFlags(&s_copy, hipStreamNonBlocking));
   HCHK(hipStreamCreateWithFlags(&s_comp, hipStreamNonBlocking));
-  for (int s = 0; s < 2; ++s) {
+  for (int s = 0; s < NSLOT; ++s) {
     HCHK(hipEventCreateWithFlags(&ev_h2d[s], hipEventDisableTiming));
     HCHK(hipEventCreate(&ev_c0[s]));
     HCHK(hipEventCreate(&ev_c1[s]));
@@ -681,13 +686,13 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     t_eig = now_s() - te;
     RCHK(fgls_hip_eig_set_basis(ectx, dL, n, dW, dXL, n, s_comp));
   }
-  HCHK(hipHostMalloc((void **)&hErr, (size_t)2 * K * sizeof(unsigned long long), hipHostMallocDefault));
-  for (int i = 0; i < 2 * K; ++i) hErr[i] = ~0ull;
-  HCHK(hipHostMalloc((void **)&hBad, 2 * sizeof(unsigned int), hipHostMallocDefault));
-  hBad[0] = hBad[1] = 0;
+  HCHK(hipHostMalloc((void **)&hErr, (size_t)NSLOT * K * sizeof(unsigned long long), hipHostMallocDefault));
+  for (int i = 0; i < NSLOT * K; ++i) hErr[i] = ~0ull;
+  HCHK(hipHostMalloc((void **)&hBad, NSLOT * sizeof(unsigned int), hipHostMallocDefault));
+  for (int s = 0; s < NSLOT; ++s) hBad[s] = 0;
   if ((opt.mixed || opt.xr_i8) && nchunks > 0) {
-    HCHK(hipMalloc((void **)&dBad, 2 * sizeof(unsigned int)));
-    for (int s = 0; s < 2; ++s) {
+    HCHK(hipMalloc((void **)&dBad, NSLOT * sizeof(unsigned int)));
+    for (int s = 0; s < NSLOT; ++s) {
       /* rows n..ld8-1 stay zero: uploads and conversions write rows < n;
        * dosages take two digit planes */
       const size_t b8 = (size_t)block * wXR * ld8 * (dos ? 2 : 1);
@@ -700,7 +705,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   }
   if (nchunks > 0) {
     const size_t xbytes = (size_t)block * wXR * ldx * sizeof(double);
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < NSLOT; ++s) {
       HCHK(hipMalloc((void **)&dX[s], xbytes));
       HCHK(hipMemset(dX[s], 0, xbytes));
       HCHK(hipHostMalloc((void **)&R.slot[s], (size_t)block * wXR * col_bytes, hipHostMallocDefault));
@@ -726,7 +731,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     R.nchunks = nchunks;
     R.io_threads = opt.io_threads > 0 ? opt.io_threads : (host_cores() < 16 ? host_cores() : 16);
     R.trace = trace;
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < NSLOT; ++s) {
       sem_init(&R.free_[s], 0, 1);
       sem_init(&R.full[s], 0, 0);
     }
@@ -774,7 +779,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     double tl0 = now_s();
     RANGE_PUSH("fgls: block loop");
     for (int64_t c = 0; c <= nchunks; ++c, ++it) {
-      const int s = (int)(it & 1);
+      const int s = (int)(it % NSLOT);
       if (c < nchunks) {
         RANGE_PUSH("fgls: upload and enqueue block");
         const int64_t i0 = c * block;
@@ -784,7 +789,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           rc = fgls_set_error(FGLS_ERR_IO, "%s", R.err[s]);
           goto done;
         }
-        /* device X_R slot s is free once the solves of chunk c-2 have run */
+        /* device X_R slot s is free once the solves of chunk c - NSLOT have run */
         HCHK(hipStreamWaitEvent(s_copy, ev_free[s], 0));
         HCHK(hipEventRecord(ev_c0[s], s_copy));
         if (opt.xr_i8 >= 2)
@@ -885,7 +890,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
       }
       if (c > 0) {  /* write back chunk c-1 */
         RANGE_PUSH("fgls: write back block");
-        const int ps = (int)((it - 1) & 1);
+        const int ps = (int)((it - 1) % NSLOT);
         const int64_t i0 = (c - 1) * block;
         const int cnt = (int)((m - i0) < block ? (m - i0) : block);
         HCHK(hipEventSynchronize(ev_out[ps]));
@@ -964,18 +969,18 @@ done:
 #undef RANGE_POP
   if (reader_started) {
     R.stop = 1;
-    for (int s = 0; s < 2; ++s) sem_post(&R.free_[s]);
+    for (int s = 0; s < NSLOT; ++s) sem_post(&R.free_[s]);
     pthread_join(rthread, NULL);
   }
   if (nchunks > 0 && R.slot[0] != NULL) {
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < NSLOT; ++s) {
       sem_destroy(&R.free_[s]);
       sem_destroy(&R.full[s]);
     }
   }
   if (s_comp) hipStreamSynchronize(s_comp);
   if (s_copy) hipStreamSynchronize(s_copy);
-  for (int s = 0; s < 2; ++s) {
+  for (int s = 0; s < NSLOT; ++s) {
     if (ev_h2d[s]) hipEventDestroy(ev_h2d[s]);
     if (ev_c0[s]) hipEventDestroy(ev_c0[s]);
     if (ev_c1[s]) hipEventDestroy(ev_c1[s]);
@@ -988,7 +993,7 @@ done:
     if (dX8T[s]) hipFree(dX8T[s]);
     if (dX2T[s]) hipFree(dX2T[s]);
   }
-  for (int s = 0; s < 2; ++s) {
+  for (int s = 0; s < NSLOT; ++s) {
     if (dBs[s]) hipFree(dBs[s]);
     if (dVs[s]) hipFree(dVs[s]);
     if (hBs[s]) hipHostFree(hBs[s]);
