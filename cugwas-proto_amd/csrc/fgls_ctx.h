@@ -129,6 +129,12 @@ constexpr int TRAIT_LD = 2 * PMAX + PMAX * PMAX + 8;
 // b and V of every SNP of a block from its transformed columns Z.
 void launch_snp_stage_any(const fgls_hip_ctx* c, const double* Z, int64_t ldz, int count,
                           double* dB, double* dV, int64_t base, hipStream_t st);
+// k_snp_stage with row weights dwt of the Z'Z sums (the eigen variant with
+// wXR > 1): p = wXL + wXR, W the weighted [Z'X_L, Z'y] of one trait.
+void launch_snp_stage_weighted_any(int p, int n, int count, int wXL, int wXR, const double* Z, int64_t ldz,
+                                   const double* W, int64_t ldw, const double* dwt, const double* trait,
+                                   double* dB, double* dV, unsigned long long* err, int64_t base,
+                                   hipStream_t st);
 
 // T = L^-1 (row-major, ctx->Linv) by recursive inversion on MFMA products;
 // with `factor`, first the Cholesky factor of the matrix in ctx->L (both

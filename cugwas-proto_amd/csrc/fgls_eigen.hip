@@ -59,6 +59,8 @@ struct fgls_hip_eig {
   int8_t* A8;        // slices of Z' (mixed mode)
   int8_t* X8t;       // tiled genotypes of the current block (mixed mode)
   size_t X8t_cap;
+  double* xp;        // Z' X_R of the current block (wXR > 1), ldn rows
+  size_t xp_cap;
   double* invsc;
   unsigned long long* err;   // per trait: (first failing SNP << 8) | info, all ones if none
   int* status;       // [0, max_traits): V_tl Cholesky info per trait; [max_traits]: bad weights
@@ -345,7 +347,7 @@ int fgls_hip_eig_destroy(fgls_hip_eig* e) {
   hipSetDevice(e->device);
   hipDeviceSynchronize();
   void* ptrs[] = {e->Zt, e->evals, e->XLin, e->XLp, e->Yin, e->Yp, e->Wg, e->Wsc, e->hyp,
-                  e->trait, e->part, e->A8, e->invsc, e->err, e->status, e->X8t};
+                  e->trait, e->part, e->A8, e->invsc, e->err, e->status, e->X8t, e->xp};
   for (void* q : ptrs)
     if (q) hipFree(q);
   delete e;
@@ -484,6 +486,47 @@ int fgls_hip_eig_solve_block(fgls_hip_eig* e, const double* dXR, int64_t ldxr, i
   HIP_TRY(hipGetLastError());
   e->last_rt = e->RT;
   return solve_common(e, count, dB, ldb, dV, ldv, snp_base, st);
+}
+
+int fgls_hip_eig_solve_block_wide(fgls_hip_eig* e, const double* dXR, int64_t ldxr, int count, int wXR,
+                                  double* dB, int64_t ldb, double* dV, int64_t ldv, int64_t snp_base,
+                                  void* stream) {
+  int rc = pre_solve(e, count);
+  if (rc || count == 0) return rc;
+  if (e->mode != FGLS_MODE_FP64)
+    return fail(FGLS_ERR_STATE, "the eigen variant with several columns per SNP runs in FP64");
+  if (wXR < 1 || e->wXL + wXR > fgls::PMAX)
+    return fail(FGLS_ERR_ARG, "wXR=%d: need wXR >= 1 and p = wXL + wXR <= %d", wXR, fgls::PMAX);
+  if (!dXR || !dB || !dV) return fail(FGLS_ERR_ARG, "NULL pointer");
+  const int64_t p = e->wXL + wXR;
+  if (e->ntr > 1 && (ldb < (int64_t)count * p || ldv < (int64_t)count * p * p))
+    return fail(FGLS_ERR_ARG, "per-trait output strides ldb=%lld ldv=%lld below count*p, count*p*p",
+                (long long)ldb, (long long)ldv);
+  if (ldxr < e->n || (ldxr & 3) || !aligned32(dXR))
+    return fail(FGLS_ERR_ARG, "X_R needs ld >= n, ld %% 4 == 0 and 32-byte alignment");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t ldn = (size_t)e->ldn;
+  const size_t need = ldn * (size_t)count * wXR * sizeof(double);
+  if (e->xp_cap < need) {
+    HIP_TRY(hipStreamSynchronize(st));
+    if (e->xp) HIP_TRY(hipFree(e->xp));
+    e->xp = nullptr;
+    e->xp_cap = 0;
+    if (hipMalloc((void**)&e->xp, need) != hipSuccess)
+      return fail(FGLS_ERR_NOMEM, "hipMalloc of %zu bytes failed", need);
+    e->xp_cap = need;
+  }
+  // x' = Z' x for every column of the block (fgls_eigen.c:494-512), then per
+  // trait the per-SNP stage on the d_k-weighted sums (fgls_eigen.c:513-538)
+  fgls::launch_dense_store(e->n, e->ldn, e->Zt, dXR, ldxr, count * wXR, e->xp, (int64_t)ldn, st);
+  for (int k = 0; k < e->ntr; ++k) {
+    const double* wg = e->Wg + (size_t)k * (e->p + 1) * ldn;   // [d_k, d_k Z'X_L, d_k Z'y_k]
+    fgls::launch_snp_stage_weighted_any((int)p, e->n, count, e->wXL, wXR, e->xp, (int64_t)ldn, wg + ldn,
+                                        (int64_t)ldn, wg, e->trait + (size_t)k * TRAIT_LD,
+                                        dB + (size_t)k * ldb, dV + (size_t)k * ldv, e->err + k, snp_base, st);
+  }
+  HIP_TRY(hipGetLastError());
+  return FGLS_OK;
 }
 
 int fgls_hip_eig_solve_block_i8(fgls_hip_eig* e, const int8_t* dX8, int64_t ldx8, int count,

@@ -500,8 +500,6 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   const int dos = opt.dosage_scale > 0.0 && opt.mixed;
   if (opt.mixed && wXR != 1)
     return fgls_set_error(FGLS_ERR_ARG, "precision=mixed needs wXR = 1 (got %d)", wXR);
-  if (eig && wXR != 1)
-    return fgls_set_error(FGLS_ERR_ARG, "the eigen variant needs wXR = 1 (got %d)", wXR);
 
   if (n < 1 || m < 0 || t < 1 || wXL < 0 || wXR < 1 || p != wXL + wXR || x_b < 1)
     return fgls_set_error(FGLS_ERR_ARG,
@@ -619,6 +617,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     const size_t per = eig ? eig_trait_bytes(n, p, block, opt.mixed) : ctx_bytes(n, p, block, opt.mixed);
     size_t shared = (size_t)block * wXR * ldx * sizeof(double) * NSLOT + nn * sizeof(double);
     if (eig) shared += ldn * ldn * (sizeof(double) + (opt.mixed ? 6 : 0)) + (64u << 20);
+    if (eig && wXR > 1) shared += ldn * (size_t)block * wXR * sizeof(double);   /* Z' X_R of a block */
     long long fit = free_b > shared ? (long long)((free_b - shared) * 0.6 / per) : 1;
     if (fit < 1) fit = 1;
     K = opt.traits_per_pass > 0 ? opt.traits_per_pass : (int)(fit < t ? fit : t);
@@ -850,6 +849,9 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           else if (opt.mixed)
             RCHK(fgls_hip_eig_solve_block_i8t(ectx, dX8T[s], cnt, planes, dos ? opt.dosage_scale : 1.0, dBs[s],
                                               ldb, dVs[s], ldv, i0, s_comp));
+          else if (wXR > 1)
+            RCHK(fgls_hip_eig_solve_block_wide(ectx, dX[s], ldx, cnt, wXR, dBs[s], ldb, dVs[s], ldv, i0,
+                                               s_comp));
           else
             RCHK(fgls_hip_eig_solve_block(ectx, dX[s], ldx, cnt, dBs[s], ldb, dVs[s], ldv, i0, s_comp));
           HCHK(hipMemcpyAsync(&hErr[s * K], fgls_eig_err_ptr(ectx), (size_t)kn * sizeof(unsigned long long),

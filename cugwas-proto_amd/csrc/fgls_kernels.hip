@@ -782,13 +782,17 @@ __global__ __launch_bounds__(128) void k_finalize(int count, int RT, const doubl
 
 // Per-SNP stage from transformed columns Z (any wXR): one wave per SNP,
 // wavefront reductions over the n rows, then the p x p tail on lane 0.
+// dwt (the eigen variant; nullptr for the Cholesky path): row weights of
+// the Z'Z sums, sum_r dwt_r z_a(r) z_b(r); W then holds the weighted
+// columns dwt [Z'X_L, Z'y], so that the z'W sums need no weight.
 template <int P>
 __global__ __launch_bounds__(256) void k_snp_stage(int n, int count, int wXL, int wXR,
                                                    const double* __restrict__ Z, int64_t ldz,
                                                    const double* __restrict__ W, int64_t ldw,
                                                    const double* __restrict__ trait,
                                                    double* __restrict__ Bout, double* __restrict__ Vout,
-                                                   unsigned long long* __restrict__ err, int64_t snp_base) {
+                                                   unsigned long long* __restrict__ err, int64_t snp_base,
+                                                   const double* __restrict__ dwt) {
   __shared__ double Ssh[4][P * P + P];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int snp = blockIdx.x * 4 + w;
@@ -801,9 +805,10 @@ __global__ __launch_bounds__(256) void k_snp_stage(int n, int count, int wXL, in
     for (int i = 0; i < P; ++i) t[i] = u[i] = 0.0;
     for (int r = lane; r < n; r += 64) {
       const double x = za[r];
+      const double xt = dwt ? x * dwt[r] : x;
 #pragma unroll
       for (int bb = 0; bb < P; ++bb)
-        if (bb <= a) t[bb] = fma(x, Z[(size_t)(snp * wXR + bb) * ldz + r], t[bb]);
+        if (bb <= a) t[bb] = fma(xt, Z[(size_t)(snp * wXR + bb) * ldz + r], t[bb]);
 #pragma unroll
       for (int k = 0; k < P; ++k)
         if (k <= wXL) u[k] = fma(x, W[(size_t)k * ldw + r], u[k]);
@@ -1008,7 +1013,16 @@ void launch_snp_stage(const fgls_hip_ctx* c, const double* Z, int64_t ldz, int c
                       double* dV, int64_t base, hipStream_t st) {
   dim3 grid((count + 3) / 4);
   hipLaunchKernelGGL(k_snp_stage<P>, grid, dim3(256), 0, st, c->n, count, c->wXL, c->wXR, Z, ldz,
-                     c->W, (int64_t)c->ldn, c->trait, dB, dV, c->err, base);
+                     c->W, (int64_t)c->ldn, c->trait, dB, dV, c->err, base, (const double*)nullptr);
+}
+
+template <int P>
+void launch_snp_stage_weighted(int n, int count, int wXL, int wXR, const double* Z, int64_t ldz,
+                               const double* W, int64_t ldw, const double* dwt, const double* trait,
+                               double* dB, double* dV, unsigned long long* err, int64_t base,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_snp_stage<P>, dim3((count + 3) / 4), dim3(256), 0, st, n, count, wXL, wXR, Z, ldz, W,
+                     ldw, trait, dB, dV, err, base, dwt);
 }
 
 bool aligned32(const void* p) { return ((uintptr_t)p & 31u) == 0; }
@@ -1137,6 +1151,14 @@ void launch_slices(int ldn, const double* A, int8_t* A8, double* invsc, hipStrea
 void launch_snp_stage_any(const fgls_hip_ctx* c, const double* Z, int64_t ldz, int count,
                           double* dB, double* dV, int64_t base, hipStream_t st) {
   FGLS_P_SWITCH(c->p, launch_snp_stage, c, Z, ldz, count, dB, dV, base, st);
+}
+
+void launch_snp_stage_weighted_any(int p, int n, int count, int wXL, int wXR, const double* Z, int64_t ldz,
+                                   const double* W, int64_t ldw, const double* dwt, const double* trait,
+                                   double* dB, double* dV, unsigned long long* err, int64_t base,
+                                   hipStream_t st) {
+  FGLS_P_SWITCH(p, launch_snp_stage_weighted, n, count, wXL, wXR, Z, ldz, W, ldw, dwt, trait, dB, dV, err,
+                base, st);
 }
 
 int finish_trait(fgls_hip_ctx* c, const double* dXL, int64_t ldxl, const double* dy, hipStream_t st,

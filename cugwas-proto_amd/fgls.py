@@ -128,6 +128,7 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_eig_set_traits": (i32, [vp, i32, dp, dp, dp, i64, vp]),
             "fgls_hip_eig_get_trait": (i32, [vp, i32, dp, dp, dp]),
             "fgls_hip_eig_solve_block": (i32, [vp, dp, i64, i32, dp, i64, dp, i64, i64, vp]),
+            "fgls_hip_eig_solve_block_wide": (i32, [vp, dp, i64, i32, i32, dp, i64, dp, i64, i64, vp]),
             "fgls_hip_eig_solve_block_i8": (i32, [vp, dp, i64, i32, dp, i64, dp, i64, i64, vp]),
             "fgls_hip_eig_solve_block_dosage": (i32, [vp, dp, i64, i32, ctypes.c_double, dp, i64, dp, i64,
                                                       i64, vp]),
@@ -168,7 +169,7 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_eigen_decompose",
                         "fgls_hip_eig_create", "fgls_hip_eig_destroy", "fgls_hip_eig_ldn",
                         "fgls_hip_eig_set_mode", "fgls_hip_eig_set_basis", "fgls_hip_eig_set_traits",
-                        "fgls_hip_eig_get_trait", "fgls_hip_eig_solve_block",
+                        "fgls_hip_eig_get_trait", "fgls_hip_eig_solve_block", "fgls_hip_eig_solve_block_wide",
                         "fgls_hip_eig_solve_block_i8", "fgls_hip_eig_solve_block_dosage",
                         "fgls_hip_eig_check")
             if hasattr(lib(), s)]
@@ -597,6 +598,24 @@ class EigenContext:
         B, V = self._outputs(count, B, V, XR.device)
         _check(lib().fgls_hip_eig_solve_block(self._h, _ptr(XR), XR.shape[1], count, _ptr(B),
                                               B.stride(0), _ptr(V), V.stride(0), snp_base, _stream(stream)))
+        return B, V
+
+    def solve_block_wide(self, XR, wXR: int, count: int | None = None, B=None, V=None, snp_base: int = 0,
+                         stream=None):
+        """Several columns per SNP (FP64): XR (count*wXR, ldx), SNP i's columns
+        at rows i*wXR .. i*wXR + wXR - 1.  Returns B (ntr, count, p) and
+        V (ntr, count, p*p) with p = wXL + wXR."""
+        import torch
+        _need_cuda_f64(XR, "XR")
+        count = XR.shape[0] // wXR if count is None else count
+        p = self.wXL + wXR
+        if B is None:
+            B = torch.empty((self.ntr, count, p), dtype=torch.float64, device=XR.device)
+        if V is None:
+            V = torch.empty((self.ntr, count, p * p), dtype=torch.float64, device=XR.device)
+        _check(lib().fgls_hip_eig_solve_block_wide(self._h, _ptr(XR), XR.shape[1], count, wXR, _ptr(B),
+                                                   B.stride(0), _ptr(V), V.stride(0), snp_base,
+                                                   _stream(stream)))
         return B, V
 
     def solve_block_i8(self, X8, count: int | None = None, B=None, V=None, snp_base: int = 0, stream=None):

@@ -16,10 +16,12 @@
  *   - no temporary files: the reference writes Z'X_L, Z'X_R and Z'Y to
  *     "<path>.tmp" files (common.c:98-100, fgls_eigen.c:693-784); here Z' x_R
  *     is formed per block on the device and never stored;
- *   - wXR must be 1 (every configuration of the hot path has one SNP column;
- *     the reference's fgls_eigen takes any wXR) and p = wXL + 1 <= 16:
- *     anything else is an error ("[Error]: ..." + exit), which the
- *     reference does not have;
+ *   - p = wXL + wXR <= 16, and precision=mixed needs wXR = 1: anything else
+ *     is an error ("[Error]: ..." + exit), which the reference does not
+ *     have.  With wXR > 1 (FP64) Z' X_R of a block is stored on the device
+ *     and each trait runs the per-SNP stage on its weighted sums
+ *     (fgls_hip_eig_solve_block_wide); with wXR = 1 the sums are fused into
+ *     the product;
  *   - failures are reported as "[Error]: <message>" + exit(EXIT_FAILURE)
  *     (common.c:11-17), including M_k not positive definite
  *     (h2 W_l + (1 - h2) <= 0 for some eigenvalue W_l of Phi).

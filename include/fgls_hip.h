@@ -404,6 +404,16 @@ int fgls_hip_eig_get_trait(fgls_hip_eig *e, int k, double *hBt, double *hVtl,
 int fgls_hip_eig_solve_block(fgls_hip_eig *e, const double *dXR, int64_t ldxr,
                              int count, double *dB, int64_t ldb, double *dV,
                              int64_t ldv, int64_t snp_base, void *stream);
+/* Several columns per SNP (wXR >= 1, FP64 mode): SNP i's columns at
+ * dXR + (i*wXR + a)*ldxr, a < wXR; p = wXL + wXR <= 16 values of b and
+ * p*p of V per SNP and trait (ldb >= count*p, ldv >= count*p*p for several
+ * traits).  One dense product Z' X_R per block into a context buffer, then
+ * per trait the per-SNP stage with the trait's weights d_k (the reference's
+ * fgls_eigen handles any wXR: src/fgls_eigen.c:494-538).  Asynchronous. */
+int fgls_hip_eig_solve_block_wide(fgls_hip_eig *e, const double *dXR, int64_t ldxr,
+                                  int count, int wXR, double *dB, int64_t ldb,
+                                  double *dV, int64_t ldv, int64_t snp_base,
+                                  void *stream);
 int fgls_hip_eig_solve_block_i8(fgls_hip_eig *e, const int8_t *dX8, int64_t ldx8,
                                 int count, double *dB, int64_t ldb, double *dV,
                                 int64_t ldv, int64_t snp_base, void *stream);

@@ -249,3 +249,64 @@ def test_eigen_singular_snp_reports_trait_and_index(pkg, gpu, tmp_path, mixed):
                        f("XL.in"), f("XR.in"), f("Y.in"), f("B.out"), f("V.out"),
                        options={"precision": "mixed" if mixed else "fp64"})
     assert ei.value.code == pkg.FGLS_ERR_NOTSPD and "i: 37" in ei.value.msg
+
+
+@pytest.mark.parametrize("n,m,wXL,wXR,t", [(700, 301, 3, 2, 2), (257, 130, 1, 3, 3), (400, 97, 5, 11, 1)])
+def test_eigen_wide_matches_oracle(pkg, gpu, n, m, wXL, wXR, t):
+    """Several columns per SNP (fgls_hip_eig_solve_block_wide): against the
+    oracle's fgls_eigen restatement (pinned at wXR = 2 by the reference's own
+    fgls_eigen) and the Cholesky form of each trait; p = 16 in the last case."""
+    print(f"eigen wide n={n} m={m} wXL={wXL} wXR={wXR} t={t}", flush=True)
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=7 * n + m, h2=0.4, t=t, wXR=wXR)
+    H = np.linspace(0.25, 0.75, t)
+    Sig = np.linspace(0.8, 1.6, t)
+    e = eig_setup(pkg, prob, H, Sig, max_block=max(m, 64))
+    XR = pkg.colmajor(prob.XR, ld=(n + 3) // 4 * 4)
+    B, V = e.solve_block_wide(XR, wXR, snp_base=3)
+    e.check()
+    p = wXL + wXR
+    assert B.shape == (t, m, p) and V.shape == (t, m, p * p)
+    Bo, Vo = orc.fgls_eigen(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y[:, :t], wXR=wXR)
+    err = orc.parity_errors(B.cpu().numpy().ravel(), V.cpu().numpy().ravel(), Bo, Vo, p)
+    assert err["V_upper_zero"] and err["B_scaled"] < TOL_FP64 and err["V_scaled"] < TOL_FP64, err
+    for j in range(t):
+        L = orc.cholesky_lower(orc.build_M(prob.Phi, float(H[j]), float(Sig[j])))
+        bc, Vc = orc.solve_snps(prob.XR, orc.trait_precompute(L, prob.XL, prob.Y[:, j]), wXR)
+        errc = orc.parity_errors(B[j].cpu().numpy(), V[j].cpu().numpy(), bc, v_flat(Vc), p)
+        assert errc["B_scaled"] < TOL_FP64 and errc["V_scaled"] < TOL_FP64, (j, errc)
+    e.close()
+
+
+def test_eigen_wide_files_and_errors(pkg, gpu, tmp_path):
+    """fgls_eigen with wXR = 2 through files (reader, one pass of 3 traits,
+    several blocks) against the oracle; the mixed mode and p > 16 are errors."""
+    n, wXL, wXR, m, t = 500, 3, 2, 401, 3
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=21, t=t, wXR=wXR)
+    d = str(tmp_path)
+    H = np.array([0.3, 0.5, 0.7])
+    Sig = np.array([1.0, 1.2, 0.8])
+    write_dir(prob, d, H, Sig)
+    p = wXL + wXR
+    f = lambda s: os.path.join(d, s)
+    pkg.fgls_eigen(n, p, m, t, wXL, wXR, 128, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"), f("XL.in"),
+                   f("XR.in"), f("Y.in"), f("B.out"), f("V.out"), options={"block": 128})
+    B = np.fromfile(f("B.out"), dtype="<f8")
+    V = np.fromfile(f("V.out"), dtype="<f8")
+    assert B.size == t * m * p and V.size == t * m * p * p
+    Bo, Vo = orc.fgls_eigen(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y, wXR, 128)
+    err = orc.parity_errors(B, V, Bo, Vo, p)
+    assert err["V_upper_zero"] and err["B_scaled"] < TOL_FP64 and err["V_scaled"] < TOL_FP64, err
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.fgls_eigen(n, p, m, t, wXL, wXR, 128, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"), f("XL.in"),
+                       f("XR.in"), f("Y.in"), f("B.out"), f("V.out"), options={"precision": "mixed"})
+    assert ei.value.code == pkg.FGLS_ERR_ARG
+    e = eig_setup(pkg, prob, H, Sig, max_block=64)
+    XR = pkg.colmajor(prob.XR[:, :64 * wXR], ld=(n + 3) // 4 * 4)
+    with pytest.raises(pkg.FglsError) as ei:
+        e.solve_block_wide(XR, 14)          # p = 3 + 14 > 16
+    assert ei.value.code == pkg.FGLS_ERR_ARG
+    e.set_mode(pkg.FGLS_MODE_MIXED_I8)
+    with pytest.raises(pkg.FglsError) as ei:
+        e.solve_block_wide(XR, wXR)
+    assert ei.value.code == pkg.FGLS_ERR_STATE
+    e.close()
