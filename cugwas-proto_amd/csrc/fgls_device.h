@@ -113,6 +113,9 @@ void fgls_launch_trmm_i8(int p, int n, int count, int RT32, int KS, const int8_t
 // y' Zin[:, col], part[(rt (p+1) + 1 + k) ldp + col] = y' W[:, k] (k < p).
 void fgls_launch_i8_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
                           const int8_t* X8T, int64_t ld8, double* Z, int64_t ldz, hipStream_t st);
+// The same store from 2-bit genotype tiles (fgls_x2t_bytes layout).
+void fgls_launch_u2_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
+                          const uint8_t* X2T, double* Z, int64_t ldz, hipStream_t st);
 void fgls_launch_i8_dot(int p, int n, int count, int RT32, int KS, const int8_t* A8,
                         const double* inv_scale, const int8_t* X8T, int64_t ld8, const double* W,
                         int64_t ldw, const double* Zin, int64_t ldz, double* part, int64_t ldp,

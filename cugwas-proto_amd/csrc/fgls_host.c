@@ -467,13 +467,14 @@ static size_t eig_trait_bytes(int n, int p, int64_t block, int mixed) {
 }
 
 /* Device bytes of one trait's context beyond the shared buffers. */
-static size_t ctx_bytes(int n, int p, int64_t block, int mixed) {
+static size_t ctx_bytes(int n, int p, int wXR, int64_t block, int mixed) {
   const size_t ldn = ((size_t)n + 127) / 128 * 128;
   const size_t rt = mixed ? ldn / 32 : ldn / 128;
   size_t b = 2 * ldn * ldn * sizeof(double);                    /* L, L^-1 */
   b += rt * (size_t)(p + 1) * (size_t)block * sizeof(double);    /* partial sums */
   b += (size_t)block * ((size_t)p + (size_t)p * p) * sizeof(double) * 2;   /* B, V slots */
-  if (mixed) b += ldn * ldn * 6;                                 /* int8 slices */
+  if (mixed) b += ldn * ldn * 6 + ldn * (size_t)block * wXR;    /* int8 slices, tiles */
+  if (wXR > 1) b += ldn * (size_t)block * wXR * sizeof(double);  /* L^-1 X_R of a block */
   return b + (64u << 20);
 }
 
@@ -498,8 +499,9 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     return fgls_set_error(FGLS_ERR_ARG, "dosage_scale needs precision=mixed and an FP64 X_R file, or "
                                         "xr_format=i16");
   const int dos = opt.dosage_scale > 0.0 && opt.mixed;
-  if (opt.mixed && wXR != 1)
-    return fgls_set_error(FGLS_ERR_ARG, "precision=mixed needs wXR = 1 (got %d)", wXR);
+  if (opt.mixed && wXR != 1 && (dos || eig))
+    return fgls_set_error(FGLS_ERR_ARG, "precision=mixed with wXR > 1 takes genotypes on the Cholesky path "
+                                        "(dosages and the eigen variant need wXR = 1; got %d)", wXR);
 
   if (n < 1 || m < 0 || t < 1 || wXL < 0 || wXR < 1 || p != wXL + wXR || x_b < 1)
     return fgls_set_error(FGLS_ERR_ARG,
@@ -614,7 +616,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
     size_t free_b = 0, total_b = 0;
     HCHK(hipMemGetInfo(&free_b, &total_b));
     const size_t ldn = ((size_t)n + 127) / 128 * 128;
-    const size_t per = eig ? eig_trait_bytes(n, p, block, opt.mixed) : ctx_bytes(n, p, block, opt.mixed);
+    const size_t per = eig ? eig_trait_bytes(n, p, block, opt.mixed) : ctx_bytes(n, p, wXR, block, opt.mixed);
     size_t shared = (size_t)block * wXR * ldx * sizeof(double) * NSLOT + nn * sizeof(double);
     if (eig) shared += ldn * ldn * (sizeof(double) + (opt.mixed ? 6 : 0)) + (64u << 20);
     if (eig && wXR > 1) shared += ldn * (size_t)block * wXR * sizeof(double);   /* Z' X_R of a block */
@@ -698,8 +700,8 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
       HCHK(hipMalloc((void **)&dX8[s], b8));
       HCHK(hipMemset(dX8[s], 0, b8));
       if (opt.xr_i8 >= 2) HCHK(hipMalloc((void **)&dP2[s], (size_t)block * wXR * col_bytes));
-      if (u2t) HCHK(hipMalloc((void **)&dX2T[s], fgls_hip_x2t_bytes(n, (int)block)));
-      else if (opt.mixed) HCHK(hipMalloc((void **)&dX8T[s], fgls_hip_x8t_bytes(n, (int)block, dos ? 2 : 1)));
+      if (u2t) HCHK(hipMalloc((void **)&dX2T[s], fgls_hip_x2t_bytes(n, (int)block * wXR)));
+      else if (opt.mixed) HCHK(hipMalloc((void **)&dX8T[s], fgls_hip_x8t_bytes(n, (int)block * wXR, dos ? 2 : 1)));
     }
   }
   if (nchunks > 0) {
@@ -807,7 +809,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           /* 2-bit codes -> the product's 2-bit tiles; code 3 is counted and
            * fails the run at write-back */
           HCHK(hipMemsetAsync(dBad + s, 0, sizeof(unsigned int), s_comp));
-          RCHK(fgls_hip_tile_u2_packed(dP2[s], (int64_t)col_bytes, n, cnt, dX2T[s], dBad + s, s_comp));
+          RCHK(fgls_hip_tile_u2_packed(dP2[s], (int64_t)col_bytes, n, cnt * wXR, dX2T[s], dBad + s, s_comp));
           HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
         } else if (opt.xr_i8 == 2) {
           /* 2-bit codes -> int8 on the device; code 3 is counted and fails
@@ -830,7 +832,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           if (dos)
             RCHK(fgls_hip_to_dosage_i8(dX[s], ldx, n, cnt, opt.dosage_scale, dX8[s], ld8, dBad + s, s_comp));
           else
-            RCHK(fgls_hip_to_i8(dX[s], ldx, n, cnt, dX8[s], ld8, dBad + s, s_comp));
+            RCHK(fgls_hip_to_i8(dX[s], ldx, n, cnt * wXR, dX8[s], ld8, dBad + s, s_comp));
           HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
         } else if (!opt.mixed && (opt.xr_i8 == 1 || opt.xr_i8 == 2)) {
           RCHK(fgls_hip_i8_to_f64(dX8[s], ld8, n, cnt * wXR, dX[s], ldx, s_comp));
@@ -841,7 +843,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
           HCHK(hipMemcpyAsync(&hBad[s], dBad + s, sizeof(unsigned int), hipMemcpyDeviceToHost, s_comp));
         }
         const int planes = dos ? 2 : 1;
-        if (opt.mixed && !u2t) RCHK(fgls_hip_tile_i8(dX8[s], ld8, n, cnt, planes, dX8T[s], s_comp));
+        if (opt.mixed && !u2t) RCHK(fgls_hip_tile_i8(dX8[s], ld8, n, cnt * wXR, planes, dX8T[s], s_comp));
         if (eig) {
           const int64_t ldb = block * p, ldv = block * pp;
           if (u2t)

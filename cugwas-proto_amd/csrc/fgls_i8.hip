@@ -701,6 +701,16 @@ void fgls_launch_i8_store(int n, int count, int RT32, int KS, const int8_t* A8, 
                      (double*)nullptr, (int64_t)0, (int64_t)0, 1.0, 1, Z, ldz);
 }
 
+void fgls_launch_u2_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
+                          const uint8_t* X2T, double* Z, int64_t ldz, hipStream_t st) {
+  const int CT = (count + I8_TN - 1) / I8_TN;
+  const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
+  const int64_t ld8 = ((int64_t)n + 63) / 64 * 64;
+  hipLaunchKernelGGL((k_trmm_i8<2, FGLS_NSLICE, false, false, I8_STORE, true>), dim3(G), dim3(256), 0, st, n,
+                     count, RT32, CT, KS, A8, inv_scale, (const int8_t*)X2T, ld8, (const double*)nullptr,
+                     (int64_t)0, (double*)nullptr, (int64_t)0, (int64_t)0, 1.0, 1, Z, ldz);
+}
+
 void fgls_launch_i8_dot(int p, int n, int count, int RT32, int KS, const int8_t* A8,
                         const double* inv_scale, const int8_t* X8T, int64_t ld8, const double* W,
                         int64_t ldw, const double* Zin, int64_t ldz, double* part, int64_t ldp,

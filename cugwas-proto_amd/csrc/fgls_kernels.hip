@@ -966,6 +966,18 @@ int set_device(const fgls_hip_ctx* c) {
   return FGLS_OK;
 }
 
+// The transformed columns L^-1 X_R of a block for wXR > 1 (ldn rows,
+// max_block * wXR columns), allocated at the first such reduce.
+int ensure_z(fgls_hip_ctx* c) {
+  if (c->z) return FGLS_OK;
+  const size_t bytes = (size_t)c->ldn * (size_t)c->max_block * c->wXR * sizeof(double);
+  if (hipMalloc((void**)&c->z, bytes) != hipSuccess) {
+    c->z = nullptr;
+    return fail(FGLS_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+  }
+  return FGLS_OK;
+}
+
 // b and V from RT row tiles of partial sums in `part` (leading dimension
 // ldp): the context's own buffer, or the no-alloc entry point's scratch.
 template <int P>
@@ -1383,11 +1395,7 @@ int fgls_hip_block_reduce(fgls_hip_ctx* c, const double* dXR, int64_t ldxr, int 
   if (c->wXR == 1) {
     FGLS_P_SWITCH(c->p, launch_trmm_reduce, c, dXR, ldxr, count, st);
   } else {
-    if (!c->z) {
-      const size_t bytes = (size_t)c->ldn * (size_t)c->max_block * c->wXR * sizeof(double);
-      if (hipMalloc((void**)&c->z, bytes) != hipSuccess)
-        return fail(FGLS_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
-    }
+    if ((rc = ensure_z(c))) return rc;
     rc = fgls_hip_trmm(c, dXR, ldxr, count * c->wXR, c->z, c->ldn, stream);
     if (rc) return rc;
   }
@@ -1401,8 +1409,6 @@ int fgls_hip_ctx_set_mode(fgls_hip_ctx* c, int mode) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (mode != FGLS_MODE_FP64 && mode != FGLS_MODE_MIXED_I8) return fail(FGLS_ERR_ARG, "unknown mode %d", mode);
-  if (mode == FGLS_MODE_MIXED_I8 && c->wXR != 1)
-    return fail(FGLS_ERR_ARG, "the mixed mode needs wXR = 1 (got %d)", c->wXR);
   if (mode == FGLS_MODE_MIXED_I8 && c->n > FGLS_I8_MAX_N)
     return fail(FGLS_ERR_ARG, "the mixed mode is exact for n <= %d (got %d)", FGLS_I8_MAX_N, c->n);
   if ((rc = set_device(c))) return rc;
@@ -1414,8 +1420,9 @@ int fgls_hip_ctx_set_mode(fgls_hip_ctx* c, int mode) {
       if (hipMalloc((void**)&c->invsc, (size_t)c->ldn * sizeof(double)) != hipSuccess)
         return fail(FGLS_ERR_NOMEM, "hipMalloc of the row scales failed");
     }
-    if ((rc = fgls::ensure_buffer(&c->X8t, &c->X8t_cap, fgls_x8t_bytes(c->n, c->max_block, 1))))
+    if ((rc = fgls::ensure_buffer(&c->X8t, &c->X8t_cap, fgls_x8t_bytes(c->n, c->max_block * c->wXR, 1))))
       return rc;
+    if (c->wXR > 1 && (rc = ensure_z(c))) return rc;
     if (c->part_rt < c->RT8) {
       double* np = nullptr;
       const size_t bytes = (size_t)c->RT8 * (c->p + 1) * (size_t)c->max_block * sizeof(double);
@@ -1505,7 +1512,9 @@ int fgls_hip_i8_to_f64(const int8_t* dX8, int64_t ldx8, int n, int ncols, double
 namespace {
 
 // The int8 product of a block of tiled genotypes (planes = 1) or dosage
-// digits (planes = 2, x = (128 hi + lo) / scale).
+// digits (planes = 2, x = (128 hi + lo) / scale).  wXR = 1: the fused
+// reductions into `part`; wXR > 1: the count * wXR tiled columns (SNP i's at
+// i*wXR + a) transformed into z for the per-SNP stage.
 int reduce_i8t(fgls_hip_ctx* c, const int8_t* dX8T, int count, int planes, double scale,
                void* stream) {
   int rc = check_ctx(c);
@@ -1515,6 +1524,8 @@ int reduce_i8t(fgls_hip_ctx* c, const int8_t* dX8T, int count, int planes, doubl
   if (count < 0 || count > c->max_block)
     return fail(FGLS_ERR_ARG, "count=%d outside [0, %d]", count, c->max_block);
   if (planes != 1 && planes != 2) return fail(FGLS_ERR_ARG, "planes must be 1 or 2");
+  if (planes == 2 && c->wXR != 1)
+    return fail(FGLS_ERR_ARG, "dosage digits need wXR = 1 (got %d)", c->wXR);
   if (count == 0) return FGLS_OK;
   if (!dX8T || ((uintptr_t)dX8T & 15)) return fail(FGLS_ERR_ARG, "tiled genotypes need 16-byte alignment");
   // 32-bit offsets to the hi plane of the tiled dosage digits
@@ -1522,10 +1533,17 @@ int reduce_i8t(fgls_hip_ctx* c, const int8_t* dX8T, int count, int planes, doubl
     return fail(FGLS_ERR_ARG, "int8 X_R block of %d columns too large", count);
   if (planes == 2 && !(scale > 0.0)) return fail(FGLS_ERR_ARG, "dosage scale must be positive");
   if ((rc = set_device(c))) return rc;
-  fgls_launch_trmm_i8(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, dX8T,
-                      round_up(c->n, 64), c->W, (int64_t)c->ldn, c->part, (int64_t)c->max_block,
-                      planes == 2 ? fgls_x8t_plane(c->n, count) : 0, planes == 2 ? 1.0 / scale : 1.0,
-                      0, (hipStream_t)stream);
+  const hipStream_t st = (hipStream_t)stream;
+  if (c->wXR > 1) {
+    if ((rc = ensure_z(c))) return rc;
+    fgls_launch_i8_store(c->n, count * c->wXR, c->RT8, c->ldn / 64, c->A8, c->invsc, dX8T, round_up(c->n, 64),
+                         c->z, (int64_t)c->ldn, st);
+  } else {
+    fgls_launch_trmm_i8(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, dX8T,
+                        round_up(c->n, 64), c->W, (int64_t)c->ldn, c->part, (int64_t)c->max_block,
+                        planes == 2 ? fgls_x8t_plane(c->n, count) : 0, planes == 2 ? 1.0 / scale : 1.0,
+                        0, st);
+  }
   HIP_TRY(hipGetLastError());
   c->reduced = count;
   c->last_rt = c->RT8;
@@ -1539,7 +1557,8 @@ int check_x8(int n, const int8_t* dX8, int64_t ldx8) {
   return FGLS_OK;
 }
 
-// The int8 product of a block of 2-bit tiled genotypes (fgls_x2t_bytes).
+// The int8 product of a block of 2-bit tiled genotypes (fgls_x2t_bytes of
+// count * wXR columns), as reduce_i8t.
 int reduce_u2t(fgls_hip_ctx* c, const uint8_t* dX2T, int count, void* stream) {
   int rc = check_ctx(c);
   if (rc) return rc;
@@ -1550,8 +1569,15 @@ int reduce_u2t(fgls_hip_ctx* c, const uint8_t* dX2T, int count, void* stream) {
   if (count == 0) return FGLS_OK;
   if (!dX2T || ((uintptr_t)dX2T & 3)) return fail(FGLS_ERR_ARG, "2-bit tiles need 4-byte alignment");
   if ((rc = set_device(c))) return rc;
-  fgls_launch_trmm_u2(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, dX2T, c->W, (int64_t)c->ldn,
-                      c->part, (int64_t)c->max_block, 0, (hipStream_t)stream);
+  const hipStream_t st = (hipStream_t)stream;
+  if (c->wXR > 1) {
+    if ((rc = ensure_z(c))) return rc;
+    fgls_launch_u2_store(c->n, count * c->wXR, c->RT8, c->ldn / 64, c->A8, c->invsc, dX2T, c->z,
+                         (int64_t)c->ldn, st);
+  } else {
+    fgls_launch_trmm_u2(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, dX2T, c->W, (int64_t)c->ldn,
+                        c->part, (int64_t)c->max_block, 0, st);
+  }
   HIP_TRY(hipGetLastError());
   c->reduced = count;
   c->last_rt = c->RT8;
@@ -1565,12 +1591,13 @@ int reduce_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, int count, int p
   if (rc) return rc;
   if (count < 0 || count > c->max_block)
     return fail(FGLS_ERR_ARG, "count=%d outside [0, %d]", count, c->max_block);
-  if (count == 0 || !c->have_trait || c->mode != FGLS_MODE_MIXED_I8)
+  if (count == 0 || !c->have_trait || c->mode != FGLS_MODE_MIXED_I8 || (planes == 2 && c->wXR != 1))
     return reduce_i8t(c, nullptr, count, planes, scale, stream);   // the state errors
   if ((rc = check_x8(c->n, dX8, ldx8))) return rc;
   if ((rc = set_device(c))) return rc;
-  if ((rc = fgls::ensure_buffer(&c->X8t, &c->X8t_cap, fgls_x8t_bytes(c->n, count, planes)))) return rc;
-  fgls_launch_tile_i8(c->n, count, planes, dX8, ldx8, c->X8t, (hipStream_t)stream);
+  const int ncols = count * c->wXR;
+  if ((rc = fgls::ensure_buffer(&c->X8t, &c->X8t_cap, fgls_x8t_bytes(c->n, ncols, planes)))) return rc;
+  fgls_launch_tile_i8(c->n, ncols, planes, dX8, ldx8, c->X8t, (hipStream_t)stream);
   HIP_TRY(hipGetLastError());
   return reduce_i8t(c, c->X8t, count, planes, scale, stream);
 }

@@ -413,14 +413,16 @@ class DeviceContext:
         return B, V
 
     def block_reduce_i8(self, X8, count: int | None = None, stream=None):
-        count = X8.shape[0] if count is None else count
+        """X8: (count*wXR, ld8) int8 genotypes, SNP i's columns at rows i*wXR .."""
+        count = X8.shape[0] // self.wXR if count is None else count
         _check(lib().fgls_hip_block_reduce_i8(self._h, _ptr(X8), X8.shape[1], count, _stream(stream)))
         return count
 
     def solve_block_i8(self, X8, count: int | None = None, B=None, V=None, snp_base: int = 0, stream=None):
+        """X8: (count*wXR, ld8) int8 genotypes.  Returns (B (count, p), V (count, p*p))."""
         import torch
         if count is None:
-            count = X8.shape[0]
+            count = X8.shape[0] // self.wXR
         if B is None:
             B = torch.empty((count, self.p), dtype=torch.float64, device=X8.device)
         if V is None:

@@ -24,7 +24,7 @@
  *   - p = wXL + wXR <= 16 (the per-SNP p x p stage is unrolled per p);
  *     larger p is an error ("[Error]: ..." + exit), which the reference
  *     does not have.  precision=mixed (fgls_chol_gpu_ex options) further
- *     needs wXR = 1 and n <= 133,000.
+ *     needs n <= 133,000, and wXR = 1 for dosages.
  * y_b is ignored with the reference's warning (fgls_chol_gpu.c:185-186);
  * num_threads sets the host BLAS threads for the per-trait Cholesky of M.
  */
@@ -65,8 +65,8 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *   dosage_scale=<s>  with precision=mixed (or with xr_format=i16): X_R values are dosages with a
  *                  fixed number of decimals (s = 1000 for 3 decimals), run as
  *                  two int8 digits per value (twice the int8 work of
- *                  genotypes); a value that is not a multiple of 1/s is an
- *                  FGLS_ERR_ARG
+ *                  genotypes; wXR = 1); a value that is not a multiple of 1/s
+ *                  is an FGLS_ERR_ARG
  *   xr_format=i8   XR_path holds int8 values (m*wXR*n bytes, the same order
  *                  as the FP64 file): 8x fewer bytes from disk and over
  *                  PCIe; used as is by precision=mixed, converted to FP64 on
@@ -92,9 +92,11 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *                  default potrf=gpu uploads Phi once and builds and factors
  *                  M on the device (fgls_hip_set_trait_phi)
  *   precision=mixed  int8 genotypes x six int8 digit slices of L^-1 on the int8
- *                  MFMA (wXR = 1; every X_R value must be an integer in
- *                  [-127, 127], else FGLS_ERR_ARG); precision=fp64 (default)
- *                  is the reference's arithmetic
+ *                  MFMA (every X_R value must be an integer in [-127, 127],
+ *                  else FGLS_ERR_ARG; wXR = 1 reduces in the product's
+ *                  epilogue, wXR > 1 stores L^-1 X_R for the FP64 per-SNP
+ *                  stage); precision=fp64 (default) is the reference's
+ *                  arithmetic
  * No counterpart in the reference (its GPU entry only exits on error).
  */
 int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR,

@@ -118,9 +118,13 @@ int fgls_hip_block_finalize(fgls_hip_ctx *ctx, int count, double *dB,
 /*
  * Mixed mode (no counterpart in the reference, whose arithmetic is FP64 only).
  * set_mode(FGLS_MODE_MIXED_I8) allocates ldn*ldn*6 bytes of slices of L^-1
- * (built at set_trait, or at once if a trait is set) and needs wXR = 1.
+ * (built at set_trait, or at once if a trait is set).
  * The genotypes of a block are int8, column-major, ld ldx8 >=
- * ceil(n/64)*64 with ldx8 % 16 == 0 (rows n..ld-1 are ignored).  b and V
+ * ceil(n/64)*64 with ldx8 % 16 == 0 (rows n..ld-1 are ignored), SNP i's
+ * wXR columns at column i*wXR as for fgls_hip_solve_block.  wXR = 1 reduces
+ * in the int8 product's epilogue; wXR > 1 stores L^-1 X_R of the block
+ * (ldn x count*wXR FP64, allocated at set_mode) for the per-SNP stage, and
+ * dosage digits need wXR = 1.  b and V
  * have the layout of fgls_hip_solve_block; their error against the FP64
  * path is set by the slice count (b within 1e-9 of max(|b|, se), see the tests).
  */
@@ -165,7 +169,9 @@ int fgls_hip_solve_block_dosage(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ld
  * fgls_hip_x8t_bytes(n, count, planes) bytes, 16-byte aligned; planes = 2 for
  * dosage digit planes laid out as for fgls_hip_solve_block_dosage) and pass
  * dX8T to the *_i8t entry points (planes = 1 genotypes, 2 dosages with
- * x = (128 hi + lo) / scale; scale is ignored for planes = 1).  No
+ * x = (128 hi + lo) / scale; scale is ignored for planes = 1).  For
+ * wXR > 1 the tiles hold the block's count*wXR columns (tile them with
+ * count*wXR columns) and `count` of the *_i8t entry points is SNPs.  No
  * counterpart in the reference.
  */
 size_t fgls_hip_x8t_bytes(int n, int count, int planes);
@@ -188,7 +194,7 @@ int fgls_hip_solve_block_i8t(fgls_hip_ctx *ctx, const int8_t *dX8T, int count, i
  * fgls_hip_u2_to_i8 (ceil(n/4) bytes per column at dP + c*ldp); a value
  * outside 0..3 (int8 input) or a code 3 (packed input) in rows < n is
  * counted into *dbad (device counter, caller-cleared) and written as 0.
- * reduce/solve take the tiles in the mixed mode (wXR = 1) as the *_i8t entry
+ * reduce/solve take the tiles (count*wXR columns) in the mixed mode as the *_i8t entry
  * points do; eig_solve_block_u2t is the eigen variant's.  No counterpart in
  * the reference (its X_R is FP64, io.c:35-55).  Asynchronous.
  */

@@ -370,3 +370,37 @@ def test_i16_dosage_file(pkg, gpu, tmp_path, precision):
                               f("XR16bad.in"), f("Y.in"), f("B.out"), f("V.out"),
                               options=dict(base, xr_format="i16", dosage_scale=1000))
         assert ei.value.code == pkg.FGLS_ERR_ARG and "outside" in ei.value.msg
+
+
+def test_fgls_chol_gpu_mixed_several_columns_per_snp(pkg, gpu, tmp_path):
+    """precision=mixed with wXR = 2 through the file pipeline: the FP64, int8
+    and 2-bit X_R files give bitwise the same files, within the mixed bound
+    of the oracle; dosages and the eigen variant with wXR > 1 in the mixed
+    mode are defined errors."""
+    print("fgls_chol_gpu precision=mixed wXR=2", flush=True)
+    n, m, t, wXL, wXR = 611, 700, 2, 3, 2
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=41, t=t, wXR=wXR)
+    d = str(tmp_path)
+    H, Sig = np.array([0.45, 0.6]), np.array([1.0, 1.4])
+    write_problem_dir(prob, d, H=H, Sig=Sig)
+    f = lambda s: os.path.join(d, s)
+    prob.XR.astype(np.int8).ravel(order="F").tofile(f("XR8.in"))
+    pkg.pack_u2(prob.XR).tofile(f("XR2.in"))
+    p = wXL + wXR
+    out = {}
+    for fmt, xr in (("f64", "XR.in"), ("i8", "XR8.in"), ("u2", "XR2.in")):
+        pkg.fgls_chol_gpu(n, p, m, t, wXL, wXR, 128, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f(xr), f("Y.in"), f(f"B_{fmt}.out"), f(f"V_{fmt}.out"),
+                          options={"precision": "mixed", "xr_format": fmt, "block": 256})
+        out[fmt] = (read_out(d, f"B_{fmt}.out", t * m * p), read_out(d, f"V_{fmt}.out", t * m * p * p))
+    for fmt in ("i8", "u2"):
+        assert np.array_equal(out[fmt][0], out["f64"][0]) and np.array_equal(out[fmt][1], out["f64"][1]), fmt
+    Bo, Vo = orc.fgls_chol(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y, wXR=wXR)
+    e = orc.parity_errors(out["f64"][0], out["f64"][1], Bo, Vo, p)
+    assert e["B_scaled"] < 2e-11 and e["V_scaled"] < 2e-11 and e["var_rel"] < 1e-12 and e["V_upper_zero"], e
+    for call, opts in ((pkg.fgls_chol_gpu, {"precision": "mixed", "dosage_scale": 1000}),
+                       (pkg.fgls_eigen, {"precision": "mixed"})):
+        with pytest.raises(pkg.FglsError) as ei:
+            call(n, p, m, t, wXL, wXR, 128, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"), f("XL.in"),
+                 f("XR.in"), f("Y.in"), f("Be.out"), f("Ve.out"), options=opts)
+        assert ei.value.code == pkg.FGLS_ERR_ARG and "wXR = 1" in ei.value.msg

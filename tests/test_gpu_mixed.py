@@ -226,3 +226,54 @@ def test_u2_tiles_reject_values_outside_0_3(pkg, gpu):
     with pytest.raises(pkg.FglsError) as ei:
         pkg.tile_u2_packed(P, n)
     assert "1 2-bit codes" in ei.value.msg
+
+
+@pytest.mark.parametrize("n,m,wXL,wXR", [(650, 120, 3, 2), (1001, 77, 13, 3), (64, 20, 1, 2), (333, 257, 2, 4)])
+def test_mixed_several_columns_per_snp(pkg, gpu, n, m, wXL, wXR):
+    """wXR > 1 in the mixed mode: the int8 product stores L^-1 X_R, the FP64
+    per-SNP stage follows; against the oracle within the mixed bound, the
+    FP64 path of the same context within the same bound, and the column-major,
+    8-bit tiled and 2-bit tiled inputs bitwise equal to one another."""
+    import torch
+    print(f"mixed wXR={wXR} n={n} m={m} wXL={wXL}", flush=True)
+    prob = make_problem(n=n, wXL=wXL, m=m, seed=3 * n + wXR, wXR=wXR)
+    tr, ctx = setup(pkg, prob, wXR=wXR, max_block=max(m, 64))
+    ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)
+    b_ref, V_ref = orc.solve_snps(prob.XR, tr, wXR)
+    p = wXL + wXR
+    X8 = ctx.to_i8(pkg.colmajor(prob.XR))
+    assert X8.shape[0] == m * wXR
+    B, V = ctx.solve_block_i8(X8, snp_base=5)
+    ctx.check()
+    e = orc.parity_errors(B.cpu().numpy(), V.cpu().numpy(), b_ref, v_flat(V_ref), p)
+    assert e["V_upper_zero"]
+    assert e["B_scaled"] < TOL_SCALED and e["V_scaled"] < TOL_SCALED, e
+    assert e["var_rel"] < TOL_VAR, e
+    B2, V2 = ctx.solve_block_i8t(pkg.tile_i8(X8, n), m)
+    assert torch.equal(B, B2) and torch.equal(V, V2)
+    T2 = pkg.tile_u2(X8, n)
+    B3, V3 = ctx.solve_block_u2t(T2, m)
+    assert torch.equal(B, B3) and torch.equal(V, V3)
+    P = torch.from_numpy(pkg.pack_u2(prob.XR)).cuda()
+    assert torch.equal(T2, pkg.tile_u2_packed(P, n))
+    # two ragged blocks through reduce / finalize
+    h = (m + 1) // 2
+    Bs, Vs = [], []
+    for i0, c in ((0, h), (h, m - h)):
+        ctx.block_reduce_i8(X8[i0 * wXR:(i0 + c) * wXR], c)
+        Bb = torch.empty((c, p), dtype=torch.float64, device="cuda")
+        Vb = torch.empty((c, p * p), dtype=torch.float64, device="cuda")
+        ctx.block_finalize(c, Bb, Vb, snp_base=i0)
+        Bs.append(Bb)
+        Vs.append(Vb)
+    assert torch.equal(torch.cat(Bs), B) and torch.equal(torch.cat(Vs), V)
+    # dosage digits need wXR = 1
+    D8 = ctx.to_dosage_i8(pkg.colmajor(prob.XR[:, :wXR * 4]), 1000.0)
+    with pytest.raises(pkg.FglsError) as ei:
+        ctx.solve_block_dosage(D8, 1000.0, count=4)
+    assert ei.value.code == pkg.FGLS_ERR_ARG and "wXR = 1" in ei.value.msg
+    ctx.set_mode(pkg.FGLS_MODE_FP64)
+    B64, V64 = ctx.solve_block(pkg.colmajor(prob.XR, ld=(n + 3) // 4 * 4), count=m)
+    e64 = orc.parity_errors(B.cpu().numpy(), V.cpu().numpy(), B64.cpu().numpy(), V64.cpu().numpy(), p)
+    assert e64["B_scaled"] < TOL_SCALED and e64["V_scaled"] < TOL_SCALED, e64
+    ctx.close()
