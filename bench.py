@@ -64,6 +64,77 @@ TRAFFIC_SOURCE = ("profile constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of 
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
+def smi_sample(gpu: int = 0):
+    """One amd-smi reading: (socket power W, mean gfx clock MHz over the XCDs,
+    max gfx clock MHz), or None where the tool or a field is missing."""
+    import shutil
+    import subprocess
+    exe = shutil.which("amd-smi") or "/opt/rocm/bin/amd-smi"
+    try:
+        r = subprocess.run([exe, "metric", "-g", str(gpu), "-p", "-c", "--json"], capture_output=True,
+                           text=True, timeout=20)
+        d = json.loads(r.stdout)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        return None
+    if isinstance(d, dict):
+        d = d.get("gpu_data", [d])
+    d = d[0] if isinstance(d, list) and d else {}
+
+    def val(x):
+        v = x.get("value") if isinstance(x, dict) else None
+        return float(v) if isinstance(v, (int, float)) else None
+
+    power = val(d.get("power", {}).get("socket_power"))
+    clk = [c for c in d.get("clock", {}).items() if c[0].startswith("gfx_") and isinstance(c[1], dict)]
+    cur = [val(c[1].get("clk")) for c in clk]
+    top = [val(c[1].get("max_clk")) for c in clk]
+    cur = [c for c in cur if c]
+    top = [c for c in top if c]
+    return (power, sum(cur) / len(cur) if cur else None, max(top) if top else None)
+
+
+def sustained_clock(torch, launch, seconds: float):
+    """Runs `launch` back to back for `seconds` while a thread samples
+    amd-smi; returns the medians of socket power and gfx clock and the launch
+    time over the window (None when amd-smi gives nothing)."""
+    import statistics
+    import threading
+    samples, stop = [], threading.Event()
+
+    def sampler():
+        while not stop.is_set():
+            s = smi_sample()
+            if s is not None:
+                samples.append((time.perf_counter() - t0, s))
+            stop.wait(0.25)
+
+    launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    th = threading.Thread(target=sampler, daemon=True)
+    th.start()
+    k = 0
+    while time.perf_counter() - t0 < seconds or (k < 3 and time.perf_counter() - t0 < 4 * seconds):
+        launch()
+        torch.cuda.synchronize()
+        k += 1
+    dt = time.perf_counter() - t0
+    stop.set()
+    th.join()
+    # drop the readings of the first 1.5 s (the clock settles)
+    late = [s for t, s in samples if t > 1.5]
+    pw = [s[0] for s in late if s[0]]
+    ck = [s[1] for s in late if s[1]]
+    top = [s[2] for _, s in samples if s[2]]
+    if not ck:
+        return None
+    return {"gfx_clock_mhz": statistics.median(ck), "gfx_clock_mhz_range": [min(ck), max(ck)],
+            "socket_power_w": statistics.median(pw) if pw else None, "max_clock_mhz": max(top) if top else None,
+            "samples": len(ck), "launches": k, "launch_ms": dt / k * 1e3,
+            "source": f"amd-smi metric (socket power, gfx clock averaged over the XCDs) sampled every "
+                      f"0.25 s while the kernel ran back to back for {dt:.1f} s"}
+
+
 def orc_parity(res, ref, p):
     """Errors of one device path against another over all SNPs."""
     from oracle import fgls_oracle as orc
@@ -556,6 +627,9 @@ def main():
                     help="X_R files the pipeline leg writes and runs: f64 (the reference's XR.in), i8, u2")
     ap.add_argument("--pipeline-traits", type=int, default=1,
                     help="traits in the pipeline leg's files (all in one pass over X_R)")
+    ap.add_argument("--telemetry-s", type=float, default=6.0,
+                    help="seconds of back-to-back reduce launches (fp64 and mixed) with amd-smi power and "
+                         "clock readings; 0 skips")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (a known-size kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -711,9 +785,19 @@ def main():
         ctx.check()
         if mode == "mixed" and int(bad.item()):
             raise SystemExit(f"{int(bad.item())} genotypes outside 0..3")
+        clock = None
+        if args.telemetry_s > 0 and mode in ("fp64", "mixed") and m >= block:
+            # the reduce kernel alone, back to back on one full block, with
+            # the GPU's power and clock read meanwhile
+            if tile is not None:
+                tile(0, block)
+            clock = sustained_clock(torch, lambda: reduce(0, block), args.telemetry_s)
+            if clock:
+                log(f"{mode} reduce back to back: {clock['launch_ms']:.2f} ms, gfx clock "
+                    f"{clock['gfx_clock_mhz']:.0f} MHz, socket power {clock['socket_power_w']} W")
         return {"B": Bo, "V": Vo, "elapsed": elapsed, "snps_per_s": m * args.steps / elapsed,
                 "reduce_ms": red_ms / max(nfull, 1), "finalize_ms": fin_ms / max(nfull, 1),
-                "tile_ms": tile_ms / max(nfull, 1), "setup_s": setup_s}
+                "tile_ms": tile_ms / max(nfull, 1), "setup_s": setup_s, "clock": clock}
 
     paths = {}
     for mode in args.paths.split(","):
@@ -803,6 +887,17 @@ def main():
         f, w = TRAFFIC_KIB[mode]
         return (f * FETCH_SCALE + w * WRITE_SCALE) * 1024.0
 
+    def at_clock(res, ach, peak):
+        """The sustained clock of the kernel and the peak it implies: the
+        published peaks are at the 2.4 GHz maximum gfx clock."""
+        c = res.get("clock")
+        if not c or not c.get("max_clock_mhz"):
+            return None
+        c = dict(c)
+        c["peak_at_this_clock"] = peak * c["gfx_clock_mhz"] / c["max_clock_mhz"]
+        c["frac_of_peak_at_this_clock"] = ach / c["peak_at_this_clock"]
+        return c
+
     def roofline(mode, res):
         flop = float(n) * n * block                 # algorithmic: n^2 per SNP (TRSM count)
         if mode == "fp64":
@@ -811,7 +906,8 @@ def main():
                     "frac": ach / PEAK_FP64_TFLOPS, "traffic": traffic(mode),
                     "traffic_is": TRAFFIC_SOURCE if traffic(mode) is not None else None,
                     "kernel": "k_trmm<EPI_REDUCE> (fgls_hip_block_reduce)",
-                    "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block, "flop_per_snp": float(n) * n}
+                    "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block, "flop_per_snp": float(n) * n,
+                    "sustained_clock": at_clock(res, ach, PEAK_FP64_TFLOPS)}
         planes = 2 if mode == "dosage" else 1
         ops = planes * NSLICE * flop                # int8 ops the 6-slice product executes
         ach = ops / (res["reduce_ms"] * 1e-3) / 1e12
@@ -824,7 +920,8 @@ def main():
                 "achieved_is": "int8 TOP/s",
                 "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block,
                 "int8_ops_per_snp": planes * NSLICE * float(n) * n,
-                "fp64_equivalent_tflops": flop / (res["reduce_ms"] * 1e-3) / 1e12}
+                "fp64_equivalent_tflops": flop / (res["reduce_ms"] * 1e-3) / 1e12,
+                "sustained_clock": at_clock(res, ach, PEAK_INT8_TOPS)}
 
     head = next(h for h in ("mixed", "fp64", "mixed8", "dosage") if h in paths)
     res = paths[head]
