@@ -435,6 +435,53 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
 }
 
 // ---------------------------------------------------------------------------
+// Skinny triangular product Z = Linv X for q <= 16 columns (the trait's
+// [X~_L, y~]): bound by reading the lower triangle of Linv (row-major, ld
+// ldli, strict upper triangle zero) once from HBM.  One wave per 4
+// consecutive rows, lanes striding k; every row r < RT*16 of Z is written
+// with the sum over k < min(last row of the wave + 1, n), reduced across the
+// wave in a fixed order.  Grid: RT blocks of 16 rows, 256 threads.
+// ---------------------------------------------------------------------------
+constexpr int TRMV_ROWS = 4;
+constexpr int TRMV_QMAX = 16;
+
+__global__ __launch_bounds__(256) void k_trmv_lower(int n, int q, const double* __restrict__ Linv,
+                                                    int64_t ldli, const double* __restrict__ X, int64_t ldx,
+                                                    double* __restrict__ Z, int64_t ldz) {
+  const int lane = threadIdx.x & 63;
+  const int r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * TRMV_ROWS;
+  const int kend = min(r0 + TRMV_ROWS, n);
+  double acc[TRMV_ROWS][TRMV_QMAX];
+#pragma unroll
+  for (int i = 0; i < TRMV_ROWS; ++i)
+#pragma unroll
+    for (int j = 0; j < TRMV_QMAX; ++j) acc[i][j] = 0.0;
+  for (int k = lane; k < kend; k += 64) {
+    double a[TRMV_ROWS];
+#pragma unroll
+    for (int i = 0; i < TRMV_ROWS; ++i) a[i] = Linv[(size_t)(r0 + i) * ldli + k];
+#pragma unroll
+    for (int j = 0; j < TRMV_QMAX; ++j) {
+      if (j < q) {
+        const double x = X[(size_t)j * ldx + k];
+#pragma unroll
+        for (int i = 0; i < TRMV_ROWS; ++i) acc[i][j] = fma(a[i], x, acc[i][j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < TRMV_QMAX; ++j) {
+    if (j < q) {
+#pragma unroll
+      for (int i = 0; i < TRMV_ROWS; ++i) {
+        const double s = wave_sum(acc[i][j]);
+        if (lane == i) Z[(size_t)j * ldz + r0 + i] = s;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Mixed precision: L^-1 as S int8 digit slices, genotypes as int8.
 //
 // Row i of L^-1 is scaled by sc_i = 127 / max_k |L^-1_ik| and written in
@@ -1189,10 +1236,12 @@ int finish_trait(fgls_hip_ctx* c, const double* dXL, int64_t ldxl, const double*
   if (!potrf_info && (rc = chol_inv(c, false, st))) return rc;
   // inverses of L's 64 x 64 diagonal blocks, for fgls_hip_trsm
   hipLaunchKernelGGL(k_diag_inv, dim3(c->nblk), dim3(64), 0, st, n, c->L, (int64_t)ldn, c->Dinv);
-  // [X~_L | y~] = L^-1 [X_L | y] through the explicit inverse: wXL + 1 columns
-  // of the triangular product spread over all row tiles (a TRSM would run
-  // the n rows one after another in a single workgroup)
-  launch_product_store(c, c->Linv, false, c->Wtmp, (int64_t)ldn, wXL + 1, c->W, (int64_t)ldn, st);
+  // [X~_L | y~] = L^-1 [X_L | y] through the explicit inverse: wXL + 1 columns,
+  // one pass over the triangle of L^-1 with every row independent (a TRSM
+  // would run the n rows one after another)
+  static_assert(PMAX <= TRMV_QMAX, "the skinny product holds p columns");
+  hipLaunchKernelGGL(k_trmv_lower, dim3(ldn / (4 * TRMV_ROWS)), dim3(256), 0, st, n, wXL + 1, c->Linv,
+                     (int64_t)ldn, c->Wtmp, (int64_t)ldn, c->W, (int64_t)ldn);
   hipLaunchKernelGGL(k_trait_stats, dim3(1), dim3(256), 0, st, n, wXL, c->W, (int64_t)ldn, c->trait,
                      c->status);
   HIP_TRY(hipGetLastError());
