@@ -45,7 +45,15 @@ METRIC = "SNPs/s at n=10k,p=4,m=1M: FP64 vs mixed-prec (rel err ≤1e-8); MFMA %
 PEAK_FP64_TFLOPS = 78.6
 PEAK_INT8_TOPS = 5000.0
 PEAK_HBM_GBS = 8000.0
-NSLICE = 6
+NSLICE = 6       # digits of the six-digit form of L^-1
+TOP_NSLICE = 5   # digits of the five-digit form (plus a top digit on each diagonal 64-block)
+
+
+def int8_ops_per_snp(n, digits, planes=1):
+    """int8 operations per SNP of the triangular int8 product: 2 per
+    multiply-add, n^2 / 2 multiply-adds per digit, and for the five-digit form
+    the top digit on the diagonal 64 x 64 blocks (n 64 / 2 multiply-adds)."""
+    return planes * (digits * float(n) * n + (64.0 * n if digits == TOP_NSLICE else 0.0))
 
 # Bytes per launch of the reduce kernels at n = 10,000 and 65,536 SNPs per
 # launch, from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of
@@ -56,8 +64,9 @@ NSLICE = 6
 # 1,048,576 KiB (x 1.0).  FETCH_SIZE counts L2 misses sent to the fabric
 # (some are served by the MALL), so this is an upper bound on HBM reads.
 TRAFFIC_N, TRAFFIC_BLOCK = 10000, 65536
-TRAFFIC_KIB = {"fp64": (106711329.6, 202240.0), "mixed": (12695667.6, 808960.0),
-               "mixed8": (20186549.5, 808960.0)}
+TRAFFIC_KIB = {"fp64": (106695702.9, 202240.0), "mixed": (11012501.7, 808960.0),
+               "mixed6": (12614833.0, 808960.0), "mixed8": (36806635.6, 808960.0)}
+TRAFFIC_DIGITS = {"fp64": 0, "mixed": 5, "mixed6": 6, "mixed8": 5}   # digit form of L^-1 profiled
 TRAFFIC_SOURCE = ("profile constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel at "
                   "n=10000, 65,536 SNPs per launch, scaled by a 1 GiB copy (x2.0 / x1.0); "
                   "not measured by this run")
@@ -608,10 +617,10 @@ def main():
     ap.add_argument("--check-snps", type=int, default=4096)
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "none"], default="auto")
     ap.add_argument("--cpu-snps", type=int, default=16384)
-    ap.add_argument("--paths", default="fp64,mixed,mixed8,dosage",
-                    help="comma-separated: fp64, mixed (hard-call genotypes as 2-bit tiles), mixed8 "
-                         "(the same genotypes as 8-bit tiles), dosage (3-decimal dosages as two int8 "
-                         "digit planes)")
+    ap.add_argument("--paths", default="fp64,mixed,mixed6,mixed8,dosage",
+                    help="comma-separated: fp64, mixed (hard-call genotypes as 2-bit tiles, the automatic "
+                         "digit form of L^-1), mixed6 (the same with six digits forced), mixed8 (the same "
+                         "genotypes as 8-bit tiles), dosage (3-decimal dosages as two int8 digit planes)")
     ap.add_argument("--eigen-traits", type=int, default=4,
                     help="traits of the eigenbasis variant's run over the same genotypes (0: skip)")
     ap.add_argument("--eigen-paths", default="fp64,mixed")
@@ -665,7 +674,7 @@ def main():
     # a dosage-only run draws no genotypes (a 2,000,000-SNP n = 20,000 FP64
     # genotype matrix would not fit in HBM next to its dosage planes)
     path_list = args.paths.split(",")
-    geno = any(pth in ("fp64", "mixed", "mixed8") for pth in path_list)
+    geno = any(pth in ("fp64", "mixed", "mixed6", "mixed8") for pth in path_list)
     t0 = time.perf_counter()
     Phi, L, XL, y, X, X8 = make_inputs(torch, n, m if geno else 0, wXL, args.h2, args.seed, ldx, dev)
     torch.cuda.synchronize()
@@ -714,7 +723,10 @@ def main():
         timed apart."""
         nonlocal X8T
         tile = None
-        if mode == "mixed":
+        # mixed6 is the mixed path with the six-digit form forced; the others
+        # take the automatic form (five digits where the error bound allows)
+        ctx.set_slice_form(pkg.FGLS_SLICES_SIX if mode == "mixed6" else pkg.FGLS_SLICES_AUTO)
+        if mode in ("mixed", "mixed6"):
             # hard-call genotypes as 2-bit tiles (fgls_hip_tile_u2): the int8
             # product loads a quarter of the genotype bytes
             t0 = time.perf_counter()
@@ -783,7 +795,8 @@ def main():
             log(f"{mode} step {k + 1}/{args.steps} done")
         elapsed = time.perf_counter() - t0
         ctx.check()
-        if mode == "mixed" and int(bad.item()):
+        digits, loss = ctx.slice_info() if mode != "fp64" else (0, -1.0)
+        if mode in ("mixed", "mixed6") and int(bad.item()):
             raise SystemExit(f"{int(bad.item())} genotypes outside 0..3")
         clock = None
         if args.telemetry_s > 0 and mode in ("fp64", "mixed") and m >= block:
@@ -797,7 +810,8 @@ def main():
                     f"{clock['gfx_clock_mhz']:.0f} MHz, socket power {clock['socket_power_w']} W")
         return {"B": Bo, "V": Vo, "elapsed": elapsed, "snps_per_s": m * args.steps / elapsed,
                 "reduce_ms": red_ms / max(nfull, 1), "finalize_ms": fin_ms / max(nfull, 1),
-                "tile_ms": tile_ms / max(nfull, 1), "setup_s": setup_s, "clock": clock}
+                "tile_ms": tile_ms / max(nfull, 1), "setup_s": setup_s, "clock": clock,
+                "digits": digits, "slice_loss": loss}
 
     paths = {}
     for mode in args.paths.split(","):
@@ -881,8 +895,9 @@ def main():
         cpu = cpu_baseline(args.cpu_baseline, Phi, L, XL, y, X, n, wXL, args.h2, args.cpu_snps, threads)
         log(f"CPU baseline: {cpu['value']:.1f} SNPs/s")
 
-    def traffic(mode):
-        if n != TRAFFIC_N or block != TRAFFIC_BLOCK or mode not in TRAFFIC_KIB:
+    def traffic(mode, res):
+        if (n != TRAFFIC_N or block != TRAFFIC_BLOCK or mode not in TRAFFIC_KIB
+                or res["digits"] != TRAFFIC_DIGITS[mode]):
             return None
         f, w = TRAFFIC_KIB[mode]
         return (f * FETCH_SCALE + w * WRITE_SCALE) * 1024.0
@@ -903,23 +918,25 @@ def main():
         if mode == "fp64":
             ach = flop / (res["reduce_ms"] * 1e-3) / 1e12
             return {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": traffic(mode),
-                    "traffic_is": TRAFFIC_SOURCE if traffic(mode) is not None else None,
+                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": traffic(mode, res),
+                    "traffic_is": TRAFFIC_SOURCE if traffic(mode, res) is not None else None,
                     "kernel": "k_trmm<EPI_REDUCE> (fgls_hip_block_reduce)",
                     "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block, "flop_per_snp": float(n) * n,
                     "sustained_clock": at_clock(res, ach, PEAK_FP64_TFLOPS)}
         planes = 2 if mode == "dosage" else 1
-        ops = planes * NSLICE * flop                # int8 ops the 6-slice product executes
+        ops = int8_ops_per_snp(n, res["digits"], planes) * block   # int8 ops of the digits in use
         ach = ops / (res["reduce_ms"] * 1e-3) / 1e12
         return {"bound": "mfma", "achieved": ach, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_INT8_TOPS, "traffic": traffic(mode),
-                "traffic_is": TRAFFIC_SOURCE if traffic(mode) is not None else None,
+                "frac": ach / PEAK_INT8_TOPS, "traffic": traffic(mode, res),
+                "traffic_is": TRAFFIC_SOURCE if traffic(mode, res) is not None else None,
                 "kernel": {"dosage": "k_trmm_i8 dosage planes (fgls_hip_block_reduce_i8t, planes 2)",
                            "mixed": "k_trmm_i8 on 2-bit genotype tiles (fgls_hip_block_reduce_u2t)",
+                           "mixed6": "k_trmm_i8 on 2-bit genotype tiles, six digits (fgls_hip_block_reduce_u2t)",
                            "mixed8": "k_trmm_i8 on 8-bit genotype tiles (fgls_hip_block_reduce_i8t)"}[mode],
                 "achieved_is": "int8 TOP/s",
                 "avg_launch_ms": res["reduce_ms"], "snps_per_launch": block,
-                "int8_ops_per_snp": planes * NSLICE * float(n) * n,
+                "digits": res["digits"], "slice_loss": res["slice_loss"],
+                "int8_ops_per_snp": int8_ops_per_snp(n, res["digits"], planes),
                 "fp64_equivalent_tflops": flop / (res["reduce_ms"] * 1e-3) / 1e12,
                 "sustained_clock": at_clock(res, ach, PEAK_INT8_TOPS)}
 
@@ -933,6 +950,7 @@ def main():
         fin_bytes = block * ((p + 1) * tiles * 8 + (p + p * p) * 8)
         fin_gbs = fin_bytes / (r["finalize_ms"] * 1e-3) / 1e9 if r["finalize_ms"] > 0 else None
         summary[mode] = {"snps_per_s": r["snps_per_s"], "ms_per_step": r["elapsed"] / args.steps * 1e3,
+                         "digits": r["digits"], "slice_loss": r["slice_loss"],
                          "reduce_kernel_ms": r["reduce_ms"], "finalize_kernel_ms": r["finalize_ms"],
                          "tile_kernel_ms": r["tile_ms"],
                          "finalize_hbm": {"bytes_per_launch": fin_bytes, "achieved_GBps": fin_gbs,
@@ -948,6 +966,9 @@ def main():
             paths["fp64"]["B"].cpu().numpy(), paths["fp64"]["V"].cpu().numpy(), p)
         summary["speedup_mixed_over_fp64"] = paths["mixed"]["snps_per_s"] / paths["fp64"]["snps_per_s"]
         log(f"mixed vs fp64 over all {m} SNPs: {e}; speedup {summary['speedup_mixed_over_fp64']:.2f}x")
+    if "mixed" in paths and "mixed6" in paths:
+        summary["mixed6_vs_fp64_all_snps"] = orc_parity(paths["mixed6"], paths["fp64"], p) if "fp64" in paths else None
+        summary["speedup_mixed_over_mixed6"] = paths["mixed"]["snps_per_s"] / paths["mixed6"]["snps_per_s"]
     if "mixed" in paths and "mixed8" in paths:
         summary["mixed_2bit_tiles_bitwise_equal_8bit_tiles"] = bool(
             torch.equal(paths["mixed"]["B"], paths["mixed8"]["B"]) and torch.equal(paths["mixed"]["V"], paths["mixed8"]["V"]))
@@ -996,8 +1017,11 @@ def main():
         "config": {"workload": f"Cholesky FGLS, n={n}, p={p}, m={m}, t=1, h2={args.h2}; value = {head} path",
                    "n": n, "p": p, "m": m, "block": block, "path": head,
                    "paths": {"fp64": "FP64 genotypes, FP64 MFMA triangular product through L^-1",
-                             "mixed": "int8 genotypes x 6 int8 digit slices (radix 254) of the FP64 L^-1 on the int8 "
-                                      "MFMA, exact int32 accumulation, FP64 recombination and p x p solves",
+                             "mixed": "int8 genotypes x int8 digit slices (radix 254) of the FP64 L^-1 on the int8 "
+                                      "MFMA (five digits per entry with a top digit on the diagonal 64-blocks "
+                                      "where the error bound allows, else six), exact int32 accumulation, FP64 "
+                                      "recombination and p x p solves",
+                             "mixed6": "the mixed path with six digits per entry of L^-1",
                              "dosage": "dosages round(U(0,2),3) as two int8 digit planes (1000 x = 128 hi + lo) "
                                        "through the same int8 product, twice its int8 work"}},
         "roofline": roofline(head, res),

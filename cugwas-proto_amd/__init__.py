@@ -36,5 +36,7 @@ from .fgls import (  # noqa: F401
     FGLS_ERR_STATE,
     FGLS_MODE_FP64,
     FGLS_MODE_MIXED_I8,
+    FGLS_SLICES_AUTO,
+    FGLS_SLICES_SIX,
 )
 from .adjoint import SnpLoss, snp_loss  # noqa: F401

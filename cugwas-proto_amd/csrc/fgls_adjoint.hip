@@ -431,7 +431,7 @@ int fgls_hip_adjoint_block_i8(fgls_hip_ctx* c, const int8_t* dX8, int64_t ldx8, 
   const int n = c->n, ldn = c->ldn, p = c->p;
   fgls_launch_tile_i8(n, count, 1, dX8, ldx8, c->X8t, st);
   // Z = L^-1 X_R on the int8 slices, then b and V of every SNP of the block
-  fgls_launch_i8_store(n, count, c->RT8, ldn / 64, c->A8, c->invsc, c->X8t, ld8, c->z, ldn, st);
+  fgls_launch_i8_store(n, count, c->RT8, ldn / 64, c->A8, c->invsc, c->X8t, ld8, c->z, ldn, c->nslice, st);
   fgls::launch_snp_stage_any(c, c->z, ldn, count, a->Bs, a->Vs, snp_base, st);
   // Y = H X_R = G Z on the slices of H: t_i = y_i' z_i, s_i = W' y_i
   fgls_launch_i8_dot(p, n, count, c->RT8, ldn / 64, a->H8, a->hinvsc, c->X8t, ld8, c->W, ldn, c->z,

@@ -30,7 +30,10 @@ struct fgls_hip_ctx {
   int8_t* A8;     // slices of L^-1 (mixed mode)
   int8_t* X8t;    // tiled genotypes of the current block (mixed mode)
   size_t X8t_cap;
-  double* invsc;  // per-row 2^(e_i - 6)
+  double* invsc;  // per-row 1 / sc_i of the slices (ldn + 1 doubles: the last is build_slices' loss cell)
+  int nslice;     // digits of the slices in A8: FGLS_NSLICE or FGLS_TOP_NSLICE (five-digit form)
+  int slice_form; // FGLS_SLICES_AUTO or FGLS_SLICES_SIX
+  double slice_loss;  // error bound of the five-digit form over the six-digit one (-1: not tried)
   unsigned int* bad;  // count of genotypes that are not int8-exact
   double* L;
   double* Linv;

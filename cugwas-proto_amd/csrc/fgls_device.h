@@ -17,6 +17,15 @@ constexpr int FGLS_NSLICE = 6;
 // largest n for which the int32 sums stay exact: 127 * 127 * n < 2^31.
 constexpr int FGLS_SLICE_RADIX = 254;
 constexpr int FGLS_I8_MAX_N = 133000;
+// Five-digit form of a triangular L^-1 (k_slice_linv_top): row i scaled by
+// its largest entry outside its diagonal 64-k step, five digits everywhere
+// and one more (top) digit on the diagonal step.  Used when its error bound,
+// at most FGLS_TOP_MAX_LOSS times the six-digit one, allows, and for
+// n <= FGLS_I8_TOP_MAX_N, where the fold of the top digit into digit 0
+// (127 * 127 * (n + 254 * 64) < 2^31) stays exact.
+constexpr int FGLS_TOP_NSLICE = 5;
+constexpr double FGLS_TOP_MAX_LOSS = 16.0;
+constexpr int FGLS_I8_TOP_MAX_N = 116000;
 // The grouping is a build-time tunable (-DFGLS_I8_GR=.. -DFGLS_I8_GC=..).
 #ifndef FGLS_I8_GR
 #define FGLS_I8_GR 16
@@ -58,6 +67,17 @@ __host__ __device__ constexpr size_t fgls_a8_offset(int row, int s, int kc, int 
   return (((((size_t)(row >> 4) * KS + (kc >> 2)) * S + s) * 64) + 16 * (kc & 3) + (row & 15)) * 16;
 }
 
+// Top-digit blocks of the five-digit form: one 1 KB block per 16-row tile
+// (its diagonal 64-k step), in the same lane order; they follow the five
+// slices (fgls_a8_offset with S = 5) in the slice buffer, at byte
+// fgls_a8top_base(KS).
+__host__ __device__ constexpr size_t fgls_a8top_offset(int row, int kc) {
+  return ((size_t)(row >> 4) * 64 + 16 * (kc & 3) + (row & 15)) * 16;
+}
+__host__ __device__ constexpr size_t fgls_a8top_base(int KS) {
+  return (size_t)KS * 64 * KS * 64 * FGLS_TOP_NSLICE;
+}
+
 // Bytes of the tiled genotypes of `count` columns and `planes` planes.
 __host__ __device__ constexpr size_t fgls_x8t_bytes(int n, int count, int planes) {
   return (size_t)planes * (size_t)((count + 15) / 16) * 16 * (size_t)((n + 63) / 64) * 64;
@@ -89,7 +109,7 @@ void fgls_launch_tile_u2_packed(int n, int count, const uint8_t* P, int64_t ldp,
 // layout; genotypes only).
 void fgls_launch_trmm_u2(int p, int n, int count, int RT32, int KS, const int8_t* A8,
                          const double* inv_scale, const uint8_t* X2T, const double* W, int64_t ldw,
-                         double* part, int64_t ldp, int eig_ntr, hipStream_t st);
+                         double* part, int64_t ldp, int eig_ntr, int nslice, hipStream_t st);
 // Column-major genotypes (ld ldx8 >= ceil(n/64)*64, ldx8 % 16 == 0, 16-byte
 // aligned; `planes` planes of count columns, count ldx8 bytes apart) -> the
 // tiled layout of the int8 product (fgls_i8.hip).
@@ -102,20 +122,23 @@ void fgls_launch_tile_i8(int n, int count, int planes, const int8_t* X8, int64_t
 // x = (128 hi + lo) * inv_xscale).  eig_ntr > 0: A8
 // holds a full matrix and W has eig_ntr groups of p+1 weight columns (the
 // eigenbasis variant; part then holds eig_ntr (p+1) sums per SNP and tile).
+// nslice: the digits of a triangular A8, FGLS_NSLICE (k_slice_linv) or
+// FGLS_TOP_NSLICE (k_slice_linv_top: five digits, the top digits at
+// fgls_a8top_base); a full A8 (eig_ntr > 0) always has FGLS_NSLICE.
 void fgls_launch_trmm_i8(int p, int n, int count, int RT32, int KS, const int8_t* A8,
                          const double* inv_scale, const int8_t* X8, int64_t ldx8, const double* W,
                          int64_t ldw, double* part, int64_t ldp, int64_t plane_off,
-                         double inv_xscale, int eig_ntr, hipStream_t st);
+                         double inv_xscale, int eig_ntr, int nslice, hipStream_t st);
 
 // The int8 product z = A x (triangular A, as for the reductions) written to
 // Z (column-major, ld ldz >= RT32 * 32, 32-byte aligned); and, for a full A,
 // y = A x reduced per SNP against Zin and W: part[(rt (p+1) + 0) ldp + col] =
 // y' Zin[:, col], part[(rt (p+1) + 1 + k) ldp + col] = y' W[:, k] (k < p).
 void fgls_launch_i8_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
-                          const int8_t* X8T, int64_t ld8, double* Z, int64_t ldz, hipStream_t st);
+                          const int8_t* X8T, int64_t ld8, double* Z, int64_t ldz, int nslice, hipStream_t st);
 // The same store from 2-bit genotype tiles (fgls_x2t_bytes layout).
 void fgls_launch_u2_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
-                          const uint8_t* X2T, double* Z, int64_t ldz, hipStream_t st);
+                          const uint8_t* X2T, double* Z, int64_t ldz, int nslice, hipStream_t st);
 void fgls_launch_i8_dot(int p, int n, int count, int RT32, int KS, const int8_t* A8,
                         const double* inv_scale, const int8_t* X8T, int64_t ld8, const double* W,
                         int64_t ldw, const double* Zin, int64_t ldz, double* part, int64_t ldp,

@@ -164,7 +164,7 @@ int reduce_i8t(fgls_hip_eig* e, const int8_t* dX8T, int count, int planes, doubl
   fgls_launch_trmm_i8(e->p, e->n, count, e->RT8, e->ldn / 64, e->A8, e->invsc, dX8T,
                       round_up(e->n, 64), e->Wg, (int64_t)e->ldn, e->part, (int64_t)e->max_block,
                       planes == 2 ? fgls_x8t_plane(e->n, count) : 0, planes == 2 ? 1.0 / scale : 1.0,
-                      e->ntr, st);
+                      e->ntr, FGLS_NSLICE, st);
   HIP_TRY(hipGetLastError());
   e->last_rt = e->RT8;
   return FGLS_OK;
@@ -175,7 +175,7 @@ int reduce_u2t(fgls_hip_eig* e, const uint8_t* dX2T, int count, hipStream_t st) 
   if (e->mode != FGLS_MODE_MIXED_I8) return fail(FGLS_ERR_STATE, "eigen context is not in the mixed mode");
   if (!dX2T || ((uintptr_t)dX2T & 3)) return fail(FGLS_ERR_ARG, "2-bit tiles need 4-byte alignment");
   fgls_launch_trmm_u2(e->p, e->n, count, e->RT8, e->ldn / 64, e->A8, e->invsc, dX2T, e->Wg, (int64_t)e->ldn,
-                      e->part, (int64_t)e->max_block, e->ntr, st);
+                      e->part, (int64_t)e->max_block, e->ntr, FGLS_NSLICE, st);
   HIP_TRY(hipGetLastError());
   e->last_rt = e->RT8;
   return FGLS_OK;

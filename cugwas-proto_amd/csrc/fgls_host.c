@@ -6,7 +6,7 @@
  * single-GPU streaming pipeline:
  *
  *   reader thread:  pread(XR.in block c)  -> pinned host slot c%2
- *                   (32 MB pieces over io_threads OpenMP threads)
+ *                   (pieces of up to 32 MB over io_threads OpenMP threads)
  *   copy stream:    hipMemcpy2DAsync slot -> device X_R buffer c%2
  *   compute stream: conversions / tiles, then fgls_hip_solve_block* (product
  *                   through L^-1 + per-SNP stage) per trait -> device B/V c%2
@@ -76,6 +76,8 @@ typedef struct {
   double dosage_scale;  /* precision=mixed for dosages: x * scale = 128 hi + lo (0: genotypes) */
   int eig_host;      /* eig=host: eigendecomposition of Phi by host LAPACK dsyevr (default: device) */
   int no_roctx;      /* roctx=0: no roctx ranges around the pipeline stages */
+  int slices_six;    /* slices=6: six digits of L^-1 in the mixed mode (default auto: five where the
+                        error bound allows, fgls_hip_ctx_set_slice_form) */
 } options_t;
 
 /* roctx ranges around the pipeline's stages, the counterpart of the
@@ -158,6 +160,10 @@ static void parse_options(const char *s, options_t *o) {
     }
     if (!strcmp(tok, "eig")) {
       o->eig_host = !strcmp(eq + 1, "host") ? 1 : (!strcmp(eq + 1, "gpu") ? 0 : -1);
+      continue;
+    }
+    if (!strcmp(tok, "slices")) {
+      o->slices_six = !strcmp(eq + 1, "6") ? 1 : (!strcmp(eq + 1, "auto") ? 0 : -1);
       continue;
     }
     if (!strcmp(tok, "precision")) {
@@ -361,12 +367,18 @@ static void *reader_main(void *arg) {
     const off_t off = (off_t)i0 * R->wXR * (off_t)col_bytes;
     double t0 = now_s();
     range_push(R->trace, "fgls: read X_R block");
-    /* 32 MB preads handed out to io_threads threads in turn (page-cache
-     * reads are memory-copy bound per thread; small pieces keep every
-     * thread busy to the end of the block) */
+    /* preads of at most 32 MB handed out to io_threads threads in turn
+     * (page-cache reads are memory-copy bound per thread; small pieces keep
+     * every thread busy to the end of the block): about two pieces per
+     * thread, whole MBs, at least 1 MB, so that the blocks of the compact
+     * files (an int8 or 2-bit block is 8 or 32 times smaller than an FP64
+     * one) still spread over all threads */
     size_t got = 0;
     const int T = R->io_threads > 0 ? R->io_threads : 1;
-    const size_t piece = (size_t)32 << 20;
+    const size_t mb = (size_t)1 << 20;
+    size_t piece = (bytes / (2 * (size_t)T) + mb - 1) / mb * mb;
+    if (piece < mb) piece = mb;
+    if (piece > 32 * mb) piece = 32 * mb;
     const long np = (long)((bytes + piece - 1) / piece);
 #pragma omp parallel for num_threads(T) schedule(dynamic, 1) reduction(+ : got)
     for (long i = 0; i < np; ++i) {
@@ -489,6 +501,7 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   parse_options(options ? options : getenv("FGLS_OPTIONS"), &opt);
   if (opt.eig_host < 0) return fgls_set_error(FGLS_ERR_ARG, "eig must be gpu or host");
   if (opt.mixed < 0) return fgls_set_error(FGLS_ERR_ARG, "precision must be fp64 or mixed");
+  if (opt.slices_six < 0) return fgls_set_error(FGLS_ERR_ARG, "slices must be auto or 6");
   if (opt.host_potrf < 0) return fgls_set_error(FGLS_ERR_ARG, "potrf must be gpu or host");
   if (opt.xr_i8 < 0) return fgls_set_error(FGLS_ERR_ARG, "xr_format must be f64, i8, u2 or i16");
   if (opt.dosage_scale < 0.0) return fgls_set_error(FGLS_ERR_ARG, "dosage_scale must be positive");
@@ -640,7 +653,10 @@ static int fgls_run(int eig, int n, int p, int m, int t, int wXL, int wXR, int x
   } else {
     for (int k = 0; k < K; ++k) {
       RCHK(fgls_hip_ctx_create(&ctx[k], n, wXL, wXR, (int)(block > 0 ? block : 1), opt.device));
-      if (opt.mixed) RCHK(fgls_hip_ctx_set_mode(ctx[k], FGLS_MODE_MIXED_I8));
+      if (opt.mixed) {
+        if (opt.slices_six) RCHK(fgls_hip_ctx_set_slice_form(ctx[k], FGLS_SLICES_SIX));
+        RCHK(fgls_hip_ctx_set_mode(ctx[k], FGLS_MODE_MIXED_I8));
+      }
     }
   }
   HCHK(hipStreamCreateWithFlags(&s_copy, hipStreamNonBlocking));

@@ -1,6 +1,7 @@
-// fgls_i8.hip -- the mixed-precision hot kernel: int8 genotypes times six
-// int8 digit slices of L^-1 on v_mfma_i32_16x16x64_i8, exact int32
-// accumulation, FP64 recombination and the per-SNP reductions.
+// fgls_i8.hip -- the mixed-precision hot kernel: int8 genotypes times the
+// int8 digit slices of L^-1 (six, or five with a top digit on the diagonal
+// 64-k steps) on v_mfma_i32_16x16x64_i8, exact int32 accumulation, FP64
+// recombination and the per-SNP reductions.
 //
 // This translation unit is compiled with -mllvm -amdgpu-mfma-vgpr-form=1:
 // the int32 accumulators then stay in place in VGPRs (with AGPR accumulators
@@ -11,7 +12,11 @@
 // L^-1 times sc_i = 127 / (row maximum) is sum_s a_s R^(-s), R = 254,
 // a_s in [-127, 127], stored in 1 KB blocks (fgls_a8_offset).  For integer
 // genotypes x, (A_s x)_i is an exact int32 (|a x| <= 127*127 per term, exact
-// for n <= 133,000), and z_i = sum_s R^(-s) (A_s x)_i / sc_i.
+// for n <= 133,000), and z_i = sum_s R^(-s) (A_s x)_i / sc_i.  The
+// five-digit form (k_slice_linv_top, TOP below) scales row i by its largest
+// entry outside its diagonal 64-k step instead, keeps five digits, and adds
+// a top digit t (weight R) on that step only: z_i = (R (T x)_i +
+// sum_{s<5} R^(-s) (A_s x)_i) / sc_i.
 
 #include "fgls_device.h"
 
@@ -91,17 +96,24 @@ constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); }
 // half cache lines ld apart); KSX = ld8 / 64 steps per column group, rows
 // >= n meet all-zero slice columns.  fgls_launch_tile_i8 builds X8T from the
 // column-major genotypes of the C-ABI.
-template <int S>
+// Slice blocks of one 64-k step in LDS: block ti * S + s holds slice s of
+// the tile's 16-row group ti; with the top digit (TOP), blocks 2 S + ti hold
+// the top digits of the diagonal step.
+template <int S, bool TOP = false>
 struct SliceStage {
-  fgls_v4i blk[2 * S][64];
+  fgls_v4i blk[2 * S + (TOP ? 2 : 0)][64];
 };
 
-template <int S>
-__device__ __forceinline__ void slice_dma(fgls_v4i rs, const int (&vo)[S / 2], int off,
-                                          SliceStage<S>& dst, int w) {
+// Slice blocks a wave moves per step: blocks w, w + 4, w + 8, ... < 2 S.
+constexpr int slice_dma_max(int S) { return (2 * S + 3) / 4; }
+
+template <int S, bool TOP>
+__device__ __forceinline__ void slice_dma(fgls_v4i rs, const int (&vo)[slice_dma_max(S)], int off,
+                                          SliceStage<S, TOP>& dst, int w) {
 #pragma unroll
-  for (int j = 0; j < S / 2; ++j)
-    fgls_buffer_load_lds(rs, FGLS_LDS(&dst.blk[w + 4 * j][0]), 16, vo[j], off, 0, 0);
+  for (int j = 0; j < slice_dma_max(S); ++j)
+    if ((2 * S) % 4 == 0 || w + 4 * j < 2 * S)
+      fgls_buffer_load_lds(rs, FGLS_LDS(&dst.blk[w + 4 * j][0]), 16, vo[j], off, 0, 0);
 }
 
 // Genotype (B) fragments: 16 int8 values per lane (fgls_v4i) from 1 KB
@@ -123,12 +135,23 @@ __device__ __forceinline__ fgls_v4i unpack_b(unsigned int x) {
 
 // One 64-k step: rd = LDS half of step k, wr = half that receives step k + 1,
 // b = genotype fragments of step k, bn receives those of step k + 1.
-template <int S, bool U2>
+//
+// TOP (five digits, S = 5, and a top digit on the diagonal step): the ten
+// slice blocks of a step are moved by waves 0, 1 (three each) and 2, 3 (two
+// each); for the diagonal step (the row tile's last) waves 2 and 3 also move
+// the top-digit block of 16-row group w - 2 (from tsrc at tvo).  DIAG marks
+// that last step: nothing further is issued, and the products of its two
+// extra fragments are added to digit 0's sums times R (exact in int32 for
+// the n the host admits to this form).
+template <int S, bool U2, bool TOP = false, bool DIAG = false>
 __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const typename BFrag<U2>::T (&b)[4],
-                                        typename BFrag<U2>::T (&bn)[4], const SliceStage<S>& rd,
-                                        SliceStage<S>& wr, fgls_v4i asrc, const int (&avo)[S / 2],
+                                        typename BFrag<U2>::T (&bn)[4], const SliceStage<S, TOP>& rd,
+                                        SliceStage<S, TOP>& wr, fgls_v4i asrc,
+                                        const int (&avo)[slice_dma_max(S)], fgls_v4i tsrc, int tvo,
                                         const int8_t* Xw, const int (&boff)[4], int k, int last,
                                         int w, int lane) {
+  static_assert(!TOP || S == 5, "the top-digit form has five digits below the top one");
+  static_assert(TOP || !DIAG, "only the top-digit form has a diagonal step");
   // this wave's slices of step k are in LDS (its genotype loads of step k,
   // issued after them, may still be in flight)
   __builtin_amdgcn_s_waitcnt(vmcnt_imm(4));
@@ -136,33 +159,56 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const typename
   __builtin_amdgcn_sched_barrier(0);
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_sched_barrier(0);
-  slice_dma<S>(asrc, avo, k1 * S * 1024, wr, w);
-  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (!DIAG) {
+    slice_dma<S, TOP>(asrc, avo, k1 * S * 1024, wr, w);
+    if constexpr (TOP) {
+      if (w >= 2 && k1 == last) fgls_buffer_load_lds(tsrc, FGLS_LDS(&wr.blk[2 * S + w - 2][0]), 16, tvo, 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int tj = 0; tj < 4; ++tj) bn[tj] = load_b<U2>(Xw + BFrag<U2>::BLOCK * k1 + boff[tj]);
-  __builtin_amdgcn_sched_barrier(0);
-  // the genotype fragments of step k (older than the S/2 + 4 loads just issued)
-  __builtin_amdgcn_s_waitcnt(vmcnt_imm(S / 2 + 4));
+    for (int tj = 0; tj < 4; ++tj) bn[tj] = load_b<U2>(Xw + BFrag<U2>::BLOCK * k1 + boff[tj]);
+    __builtin_amdgcn_sched_barrier(0);
+    // the genotype fragments of step k (older than the slice and 4 genotype
+    // loads just issued)
+    if constexpr (!TOP) {
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm(slice_dma_max(S) + 4));
+    } else {
+      if (w < 2 || k1 == last) __builtin_amdgcn_s_waitcnt(vmcnt_imm(3 + 4));
+      else __builtin_amdgcn_s_waitcnt(vmcnt_imm(2 + 4));
+    }
+  } else {
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+  }
   __builtin_amdgcn_sched_barrier(0);
   fgls_v4i bu[4];
 #pragma unroll
   for (int tj = 0; tj < 4; ++tj) bu[tj] = unpack_b(b[tj]);
-  // A fragments one ahead: fragment f = 2 s + ti
+  // A fragments one ahead: fragment f = 2 s + ti (block ti S + s); on the
+  // diagonal step fragments 2 S + ti are the top digits (block 2 S + ti)
+  constexpr int NF = 2 * S + (DIAG ? 2 : 0);
+  auto blk_of = [](int f) { return f < 2 * S ? (f & 1) * S + (f >> 1) : f; };
   fgls_v4i af[2];
   af[0] = rd.blk[0][lane];
   __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
 #pragma unroll
-  for (int f = 0; f < 2 * S; ++f) {
-    const int s2 = f >> 1, ti = f & 1;
-    if (f + 1 < 2 * S) af[(f + 1) & 1] = rd.blk[((f + 1) & 1) * S + ((f + 1) >> 1)][lane];
+  for (int f = 0; f < NF; ++f) {
+    if (f + 1 < NF) af[(f + 1) & 1] = rd.blk[blk_of(f + 1)][lane];
+    if (f < 2 * S) {
+      const int s2 = f >> 1, ti = f & 1;
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(af[f & 1], bu[tj], acc[s2][ti][tj]);
+      for (int tj = 0; tj < 4; ++tj) acc[s2][ti][tj] = mfma_i8(af[f & 1], bu[tj], acc[s2][ti][tj]);
+    } else {   // the top digits (one step): R times their products join digit 0's sums
+      const int ti = f - 2 * S;
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj)
+        acc[0][ti][tj] += FGLS_SLICE_RADIX * mfma_i8(af[f & 1], bu[tj], fgls_v4i{0, 0, 0, 0});
+    }
   }
   // issue order: one LDS read (the next fragment), then the four MFMAs of
   // the current one
 #pragma unroll
-  for (int f = 0; f < 2 * S; ++f) {
-    if (f + 1 < 2 * S) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+  for (int f = 0; f < NF; ++f) {
+    if (f + 1 < NF) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
     __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
   }
   __builtin_amdgcn_sched_barrier(0);
@@ -187,19 +233,24 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const typename
 // into part as I8_RED does z'z and z'W[:, k].
 enum { I8_RED = 0, I8_STORE = 1, I8_DOT = 2 };
 
-template <int NQ, int S, bool DOS = false, bool EIG = false, int MODE = I8_RED, bool U2 = false>
+// TOP: A8 holds five digits per entry (fgls_a8_offset with S = 5) and A8top
+// the top digits of every 16-row group's diagonal 64-k step (the group's
+// last step: fgls_a8top_offset); triangular A only.
+template <int NQ, int S, bool DOS = false, bool EIG = false, int MODE = I8_RED, bool U2 = false,
+          bool TOP = false>
 __global__ __launch_bounds__(256, 2) void k_trmm_i8(
     int n, int count, int RT32, int CT, int KS, const int8_t* __restrict__ A8,
     const double* __restrict__ inv_scale, const int8_t* __restrict__ X8T, int64_t ld8,
     const double* __restrict__ W, int64_t ldw, double* __restrict__ part, int64_t ldp,
     int64_t plane_off = 0, double inv_xscale = 1.0, int ntr = 1, double* __restrict__ Zio = nullptr,
-    int64_t ldz = 0) {
-  static_assert(S % 2 == 0, "the 2 x S slice blocks are spread over 4 waves");
+    int64_t ldz = 0, const int8_t* __restrict__ A8top = nullptr) {
+  static_assert(TOP ? S == 5 : S % 2 == 0, "the 2 x S slice blocks are spread over 4 waves");
+  static_assert(!(TOP && (EIG || MODE == I8_DOT)), "the top digit is for the triangular L^-1");
   static_assert(!(U2 && DOS), "2-bit tiles hold genotypes, not dosage digit planes");
   using BT = typename BFrag<U2>::T;
   constexpr int BB = BFrag<U2>::BLOCK, BL = BFrag<U2>::LANE;
   constexpr int WN = DOS ? 32 : 64;   // SNP columns per wave
-  __shared__ SliceStage<S> st0, st1;
+  __shared__ SliceStage<S, TOP> st0, st1;
   int rt, ct;
   if (!fgls_tile_of_block<I8_GR, I8_GC>(blockIdx.x, RT32, CT, rt, ct)) return;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -218,11 +269,20 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   const uint64_t abase = (uint64_t)(A8 + a0);
   const fgls_v4i asrc = {(int)(uint32_t)abase, (int)(uint32_t)(abase >> 32) & 0xFFFF,
                          2 * KS * S * 1024, 0x00020000};
-  int avo[S / 2];
+  int avo[slice_dma_max(S)];
 #pragma unroll
-  for (int j = 0; j < S / 2; ++j) {
-    const int blk = w + 4 * j, ti = blk / S, s2 = blk % S;
+  for (int j = 0; j < slice_dma_max(S); ++j) {
+    const int blk = min(w + 4 * j, 2 * S - 1), ti = blk / S, s2 = blk % S;
     avo[j] = (int)(fgls_a8_offset(r0 + 16 * ti, s2, 0, S, KS) - a0) + 16 * lane;
+  }
+  // TOP: the two top-digit blocks of the tile (2 KB from its first 16-row
+  // group's); waves 2 and 3 move group w - 2's
+  fgls_v4i tsrc = {0, 0, 0, 0};
+  int tvo = 0;
+  if constexpr (TOP) {
+    const uint64_t tbase = (uint64_t)(A8top + fgls_a8top_offset(r0, 0));
+    tsrc = fgls_v4i{(int)(uint32_t)tbase, (int)(uint32_t)(tbase >> 32) & 0xFFFF, 2 * 1024, 0x00020000};
+    tvo = max(w - 2, 0) * 1024 + 16 * lane;
   }
   // genotype fragments of this wave: Xw + boff[tj] (+ 1024 k) for fragment tj;
   // column groups past the last one re-read it (plane_off < 2^31 is checked
@@ -250,19 +310,49 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   if (nsteps > 0) {
     const int last = nsteps - 1;
     BT b0[4], b1[4];
-    slice_dma<S>(asrc, avo, 0, st0, w);
-    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (TOP) {
+      // the diagonal step always reads (b0, st0): with an odd number of steps
+      // before it, step 0 goes to (b1, st1) and runs alone first
+      auto prologue = [&](BT (&b)[4], SliceStage<S, TOP>& st) {
+        slice_dma<S, TOP>(asrc, avo, 0, st, w);
+        if (w >= 2 && last == 0) fgls_buffer_load_lds(tsrc, FGLS_LDS(&st.blk[2 * S + w - 2][0]), 16, tvo, 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj) b0[tj] = load_b<U2>(Xw + boff[tj]);
-    __builtin_amdgcn_sched_barrier(0);
-    int k = 0;
-    for (; k + 2 <= nsteps; k += 2) {
-      step_i8<S, U2>(acc, b0, b1, st0, st1, asrc, avo, Xw, boff, k, last, w, lane);
-      step_i8<S, U2>(acc, b1, b0, st1, st0, asrc, avo, Xw, boff, k + 1, last, w, lane);
+        for (int tj = 0; tj < 4; ++tj) b[tj] = load_b<U2>(Xw + boff[tj]);
+        __builtin_amdgcn_sched_barrier(0);
+      };
+      int k = 0;
+      if (last & 1) {
+        prologue(b1, st1);
+        step_i8<S, U2, true>(acc, b1, b0, st1, st0, asrc, avo, tsrc, tvo, Xw, boff, 0, last, w, lane);
+        k = 1;
+      } else {
+        prologue(b0, st0);
+      }
+      for (; k < last; k += 2) {
+        step_i8<S, U2, true>(acc, b0, b1, st0, st1, asrc, avo, tsrc, tvo, Xw, boff, k, last, w, lane);
+        step_i8<S, U2, true>(acc, b1, b0, st1, st0, asrc, avo, tsrc, tvo, Xw, boff, k + 1, last, w,
+                             lane);
+      }
+      // the diagonal step waits for all of its loads: nothing is in flight after it
+      step_i8<S, U2, true, true>(acc, b0, b1, st0, st1, asrc, avo, tsrc, tvo, Xw, boff, last, last, w,
+                                 lane);
+    } else {
+      slice_dma<S, TOP>(asrc, avo, 0, st0, w);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) b0[tj] = load_b<U2>(Xw + boff[tj]);
+      __builtin_amdgcn_sched_barrier(0);
+      int k = 0;
+      for (; k + 2 <= nsteps; k += 2) {
+        step_i8<S, U2>(acc, b0, b1, st0, st1, asrc, avo, tsrc, tvo, Xw, boff, k, last, w, lane);
+        step_i8<S, U2>(acc, b1, b0, st1, st0, asrc, avo, tsrc, tvo, Xw, boff, k + 1, last, w, lane);
+      }
+      if (k < nsteps)
+        step_i8<S, U2>(acc, b0, b1, st0, st1, asrc, avo, tsrc, tvo, Xw, boff, k, last, w, lane);
+      // no LDS load may still be in flight when the workgroup ends
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
     }
-    if (k < nsteps) step_i8<S, U2>(acc, b0, b1, st0, st1, asrc, avo, Xw, boff, k, last, w, lane);
-    // no LDS load may still be in flight when the workgroup ends
-    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
   }
 
   if constexpr (MODE != I8_RED) {
@@ -631,32 +721,44 @@ __global__ __launch_bounds__(256) void k_tile_u2_packed(int n, int count, int KS
 template <int P, bool EIG>
 void launch(int n, int count, int RT32, int KS, const int8_t* A8, const double* isc,
             const int8_t* X8, int64_t ldx8, const double* W, int64_t ldw, double* part, int64_t ldp,
-            int64_t plane_off, double inv_xscale, int ntr, hipStream_t st) {
+            int64_t plane_off, double inv_xscale, int ntr, int nslice, hipStream_t st) {
+  constexpr int T = FGLS_TOP_NSLICE;
+  const int8_t* top = A8 + fgls_a8top_base(KS);
   if (plane_off > 0) {
     const int CT = (count + 127) / 128;
     const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
-    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, true, EIG>), dim3(G), dim3(256), 0, st, n,
-                       count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off,
-                       inv_xscale, ntr);
+    if (!EIG && nslice == T)
+      hipLaunchKernelGGL((k_trmm_i8<P + 1, T, true, false, I8_RED, false, true>), dim3(G), dim3(256), 0, st, n,
+                         count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off, inv_xscale, 1,
+                         (double*)nullptr, (int64_t)0, top);
+    else
+      hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, true, EIG>), dim3(G), dim3(256), 0, st, n,
+                         count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off,
+                         inv_xscale, ntr);
   } else {
     const int CT = (count + I8_TN - 1) / I8_TN;
     const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
-    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, EIG>), dim3(G), dim3(256), 0, st, n,
-                       count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, (int64_t)0, 1.0,
-                       ntr);
+    if (!EIG && nslice == T)
+      hipLaunchKernelGGL((k_trmm_i8<P + 1, T, false, false, I8_RED, false, true>), dim3(G), dim3(256), 0, st,
+                         n, count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, (int64_t)0, 1.0, 1,
+                         (double*)nullptr, (int64_t)0, top);
+    else
+      hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, EIG>), dim3(G), dim3(256), 0, st, n,
+                         count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, (int64_t)0, 1.0,
+                         ntr);
   }
 }
 
 template <int P>
 void launch_any(int n, int count, int RT32, int KS, const int8_t* A8, const double* isc,
                 const int8_t* X8, int64_t ldx8, const double* W, int64_t ldw, double* part,
-                int64_t ldp, int64_t plane_off, double inv_xscale, int eig_ntr, hipStream_t st) {
+                int64_t ldp, int64_t plane_off, double inv_xscale, int eig_ntr, int nslice, hipStream_t st) {
   if (eig_ntr > 0)
     launch<P, true>(n, count, RT32, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off,
-                    inv_xscale, eig_ntr, st);
+                    inv_xscale, eig_ntr, FGLS_NSLICE, st);
   else
     launch<P, false>(n, count, RT32, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off,
-                     inv_xscale, 1, st);
+                     inv_xscale, 1, nslice, st);
 }
 
 }  // namespace
@@ -665,7 +767,8 @@ namespace {
 
 template <int P>
 void launch_u2(int n, int count, int RT32, int KS, const int8_t* A8, const double* isc, const uint8_t* X2T,
-               const double* W, int64_t ldw, double* part, int64_t ldp, int eig_ntr, hipStream_t st) {
+               const double* W, int64_t ldw, double* part, int64_t ldp, int eig_ntr, int nslice,
+               hipStream_t st) {
   const int CT = (count + I8_TN - 1) / I8_TN;
   const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
   const int64_t ld8 = ((int64_t)n + 63) / 64 * 64;
@@ -673,6 +776,10 @@ void launch_u2(int n, int count, int RT32, int KS, const int8_t* A8, const doubl
     hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, true, I8_RED, true>), dim3(G), dim3(256), 0, st,
                        n, count, RT32, CT, KS, A8, isc, (const int8_t*)X2T, ld8, W, ldw, part, ldp, (int64_t)0,
                        1.0, eig_ntr);
+  else if (nslice == FGLS_TOP_NSLICE)
+    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_TOP_NSLICE, false, false, I8_RED, true, true>), dim3(G),
+                       dim3(256), 0, st, n, count, RT32, CT, KS, A8, isc, (const int8_t*)X2T, ld8, W, ldw, part,
+                       ldp, (int64_t)0, 1.0, 1, (double*)nullptr, (int64_t)0, A8 + fgls_a8top_base(KS));
   else
     hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, false, I8_RED, true>), dim3(G), dim3(256), 0, st,
                        n, count, RT32, CT, KS, A8, isc, (const int8_t*)X2T, ld8, W, ldw, part, ldp, (int64_t)0,
@@ -690,25 +797,32 @@ void launch_dot(int n, int count, int RT32, int KS, const int8_t* A8, const doub
                      const_cast<double*>(Zin), ldz);
 }
 
-}  // namespace
-
-void fgls_launch_i8_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
-                          const int8_t* X8T, int64_t ld8, double* Z, int64_t ldz, hipStream_t st) {
-  const int CT = (count + I8_TN - 1) / I8_TN;
-  const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
-  hipLaunchKernelGGL((k_trmm_i8<2, FGLS_NSLICE, false, false, I8_STORE>), dim3(G), dim3(256), 0, st, n,
-                     count, RT32, CT, KS, A8, inv_scale, X8T, ld8, (const double*)nullptr, (int64_t)0,
-                     (double*)nullptr, (int64_t)0, (int64_t)0, 1.0, 1, Z, ldz);
-}
-
-void fgls_launch_u2_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
-                          const uint8_t* X2T, double* Z, int64_t ldz, hipStream_t st) {
+template <bool U2>
+void launch_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
+                  const int8_t* X, double* Z, int64_t ldz, int nslice, hipStream_t st) {
   const int CT = (count + I8_TN - 1) / I8_TN;
   const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
   const int64_t ld8 = ((int64_t)n + 63) / 64 * 64;
-  hipLaunchKernelGGL((k_trmm_i8<2, FGLS_NSLICE, false, false, I8_STORE, true>), dim3(G), dim3(256), 0, st, n,
-                     count, RT32, CT, KS, A8, inv_scale, (const int8_t*)X2T, ld8, (const double*)nullptr,
-                     (int64_t)0, (double*)nullptr, (int64_t)0, (int64_t)0, 1.0, 1, Z, ldz);
+  if (nslice == FGLS_TOP_NSLICE)
+    hipLaunchKernelGGL((k_trmm_i8<2, FGLS_TOP_NSLICE, false, false, I8_STORE, U2, true>), dim3(G), dim3(256), 0,
+                       st, n, count, RT32, CT, KS, A8, inv_scale, X, ld8, (const double*)nullptr, (int64_t)0,
+                       (double*)nullptr, (int64_t)0, (int64_t)0, 1.0, 1, Z, ldz, A8 + fgls_a8top_base(KS));
+  else
+    hipLaunchKernelGGL((k_trmm_i8<2, FGLS_NSLICE, false, false, I8_STORE, U2>), dim3(G), dim3(256), 0, st, n,
+                       count, RT32, CT, KS, A8, inv_scale, X, ld8, (const double*)nullptr, (int64_t)0,
+                       (double*)nullptr, (int64_t)0, (int64_t)0, 1.0, 1, Z, ldz);
+}
+
+}  // namespace
+
+void fgls_launch_i8_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
+                          const int8_t* X8T, int64_t ld8, double* Z, int64_t ldz, int nslice, hipStream_t st) {
+  launch_store<false>(n, count, RT32, KS, A8, inv_scale, X8T, Z, ldz, nslice, st);
+}
+
+void fgls_launch_u2_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
+                          const uint8_t* X2T, double* Z, int64_t ldz, int nslice, hipStream_t st) {
+  launch_store<true>(n, count, RT32, KS, A8, inv_scale, (const int8_t*)X2T, Z, ldz, nslice, st);
 }
 
 void fgls_launch_i8_dot(int p, int n, int count, int RT32, int KS, const int8_t* A8,
@@ -721,8 +835,9 @@ void fgls_launch_i8_dot(int p, int n, int count, int RT32, int KS, const int8_t*
 
 void fgls_launch_trmm_u2(int p, int n, int count, int RT32, int KS, const int8_t* A8,
                          const double* inv_scale, const uint8_t* X2T, const double* W, int64_t ldw,
-                         double* part, int64_t ldp, int eig_ntr, hipStream_t st) {
-  FGLS_I8_P_SWITCH(p, launch_u2, n, count, RT32, KS, A8, inv_scale, X2T, W, ldw, part, ldp, eig_ntr, st);
+                         double* part, int64_t ldp, int eig_ntr, int nslice, hipStream_t st) {
+  FGLS_I8_P_SWITCH(p, launch_u2, n, count, RT32, KS, A8, inv_scale, X2T, W, ldw, part, ldp, eig_ntr, nslice,
+                   st);
 }
 
 void fgls_launch_tile_u2(int n, int count, const int8_t* X8, int64_t ldx8, uint8_t* X2T, unsigned int* bad,
@@ -749,11 +864,11 @@ void fgls_launch_tile_i8(int n, int count, int planes, const int8_t* X8, int64_t
 void fgls_launch_trmm_i8(int p, int n, int count, int RT32, int KS, const int8_t* A8,
                          const double* inv_scale, const int8_t* X8, int64_t ldx8, const double* W,
                          int64_t ldw, double* part, int64_t ldp, int64_t plane_off,
-                         double inv_xscale, int eig_ntr, hipStream_t st) {
+                         double inv_xscale, int eig_ntr, int nslice, hipStream_t st) {
 #define FGLS_I8_CASE(P)                                                                      \
   case P:                                                                                    \
     launch_any<P>(n, count, RT32, KS, A8, inv_scale, X8, ldx8, W, ldw, part, ldp, plane_off, \
-                  inv_xscale, eig_ntr, st);                                                  \
+                  inv_xscale, eig_ntr, nslice, st);                                          \
     break;
   switch (p) {
     FGLS_I8_CASE(1) FGLS_I8_CASE(2) FGLS_I8_CASE(3) FGLS_I8_CASE(4) FGLS_I8_CASE(5)

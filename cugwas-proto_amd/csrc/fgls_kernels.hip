@@ -541,6 +541,82 @@ __global__ __launch_bounds__(256) void k_slice_linv(int ldn, const double* __res
   }
 }
 
+// Five-digit form of the triangular L^-1 (S = 5).  Row i is scaled by
+//   sc_i = min(127 / off_i, 127 R / max_i),
+// off_i the largest |L^-1_ik| outside the row's diagonal 64-k step and max_i
+// the largest of the row: outside that step |L^-1_ik sc_i| <= 127 and five
+// digits a_0..a_4 follow as in k_slice_linv; inside it |L^-1_ik sc_i| <=
+// 127 R, and a top digit t = rint(r / R) (|t| <= 127) is split off first
+// (r - R t is exact in FP64 and within [-127, 127]), then a_0..a_4.  The
+// residual is 127 R^-5 / sc_i per entry, R off_i / max_i (<= the loss) times
+// the six-digit form's 127 R^-6 / (127 / max_i); the largest such factor over
+// the rows goes to *loss_bits (non-negative doubles order as their bits), and
+// the host keeps this form only while it is at most FGLS_TOP_MAX_LOSS.  The
+// top digits of a 16-row group's diagonal step go to A8top
+// (fgls_a8top_offset), its five digits to A8 like every other step.
+template <int S>
+__global__ __launch_bounds__(256) void k_slice_linv_top(int ldn, const double* __restrict__ Linv,
+                                                        int8_t* __restrict__ A8, int8_t* __restrict__ A8top,
+                                                        double* __restrict__ inv_scale,
+                                                        unsigned long long* __restrict__ loss_bits) {
+  __shared__ double red[2][4];
+  const int row = blockIdx.x, tid = threadIdx.x;
+  const int d0 = row & ~63;   // the diagonal step's first column
+  const double* Lr = Linv + (size_t)row * ldn;
+  double mx = 0.0, off = 0.0;
+  for (int k = tid; k < ldn; k += 256) {
+    const double a = fabs(Lr[k]);
+    mx = fmax(mx, a);
+    if (k < d0 || k >= d0 + 64) off = fmax(off, a);
+  }
+  mx = wave_max(mx);
+  off = wave_max(off);
+  if ((tid & 63) == 0) {
+    red[0][tid >> 6] = mx;
+    red[1][tid >> 6] = off;
+  }
+  __syncthreads();
+  mx = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+  off = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
+  constexpr double R = (double)FGLS_SLICE_RADIX;
+  double sc = 1.0;
+  if (mx > 0.0) {
+    sc = 127.0 * R / mx;
+    if (off > 0.0) sc = fmin(sc, 127.0 / off);
+  }
+  if (tid == 0) {
+    inv_scale[row] = (mx > 0.0) ? 1.0 / sc : 0.0;
+    const double loss = (mx > 0.0) ? R * off / mx : 0.0;
+    atomicMax(loss_bits, (unsigned long long)__double_as_longlong(loss));
+  }
+  const int KC = ldn / 16;
+  for (int kc = tid; kc < KC; kc += 256) {
+    const bool diag = (kc >> 2) == (row >> 6);
+    int8_t out[S][16], top[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      double r = Lr[kc * 16 + b] * sc;
+      double t = 0.0;
+      if (diag) {
+        t = fmin(fmax(rint(r / R), -127.0), 127.0);
+        r -= R * t;
+      }
+      top[b] = (int8_t)(int)t;
+#pragma unroll
+      for (int s2 = 0; s2 < S; ++s2) {
+        const double a = fmin(fmax(rint(r), -127.0), 127.0);
+        out[s2][b] = (int8_t)(int)a;
+        r = (r - a) * R;
+      }
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < S; ++s2)
+      *reinterpret_cast<v4i*>(A8 + fgls_a8_offset(row, s2, kc, S, ldn / 64)) =
+          *reinterpret_cast<const v4i*>(out[s2]);
+    if (diag) *reinterpret_cast<v4i*>(A8top + fgls_a8top_offset(row, kc)) = *reinterpret_cast<const v4i*>(top);
+  }
+}
+
 // FP64 genotypes -> int8 (column-major, ld ldx8), exact only: a value that is
 // not an integer in [-127, 127] is counted in *bad.
 __global__ __launch_bounds__(256) void k_to_i8(int n, const double* __restrict__ X,
@@ -1060,7 +1136,29 @@ void launch_trmm_reduce(const fgls_hip_ctx* c, const double* X, int64_t ldx, int
 
 constexpr int NSLICE = FGLS_NSLICE;   // int8 digit slices of L^-1 in the mixed mode
 
+// The slices of L^-1 in the form the context uses: the five-digit form
+// (k_slice_linv_top) when the slice form is automatic, n admits it and its
+// error bound over the six-digit one (the loss) is at most
+// FGLS_TOP_MAX_LOSS; otherwise six digits (k_slice_linv).  Synchronizes `st`
+// (the loss is read back) when it tries the five-digit form.
 int build_slices(fgls_hip_ctx* c, hipStream_t st) {
+  c->nslice = NSLICE;
+  c->slice_loss = -1.0;
+  if (c->slice_form == FGLS_SLICES_AUTO && c->n <= FGLS_I8_TOP_MAX_N) {
+    unsigned long long* lb = reinterpret_cast<unsigned long long*>(c->invsc + c->ldn);
+    HIP_TRY(hipMemsetAsync(lb, 0, sizeof(*lb), st));
+    hipLaunchKernelGGL((k_slice_linv_top<FGLS_TOP_NSLICE>), dim3(c->ldn), dim3(256), 0, st, c->ldn, c->Linv,
+                       c->A8, c->A8 + fgls_a8top_base(c->ldn / 64), c->invsc, lb);
+    HIP_TRY(hipGetLastError());
+    double loss = 0.0;
+    HIP_TRY(hipMemcpyAsync(&loss, lb, sizeof(loss), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    c->slice_loss = loss;
+    if (loss <= FGLS_TOP_MAX_LOSS) {
+      c->nslice = FGLS_TOP_NSLICE;
+      return FGLS_OK;
+    }
+  }
   hipLaunchKernelGGL((k_slice_linv<NSLICE>), dim3(c->ldn), dim3(256), 0, st, c->ldn, c->Linv, c->A8,
                      c->invsc);
   HIP_TRY(hipGetLastError());
@@ -1301,6 +1399,9 @@ int fgls_hip_ctx_create(fgls_hip_ctx** out, int n, int wXL, int wXR, int max_blo
   c->RT8 = c->ldn / I8_TM;
   c->part_rt = c->RT;
   c->mode = FGLS_MODE_FP64;
+  c->nslice = FGLS_NSLICE;
+  c->slice_form = FGLS_SLICES_AUTO;
+  c->slice_loss = -1.0;
   const size_t ldn = (size_t)c->ldn;
   const size_t nq = (size_t)c->p + 1;
   struct { void** ptr; size_t bytes; } allocs[] = {
@@ -1454,6 +1555,29 @@ int fgls_hip_block_reduce(fgls_hip_ctx* c, const double* dXR, int64_t ldxr, int 
   return FGLS_OK;
 }
 
+int fgls_hip_ctx_set_slice_form(fgls_hip_ctx* c, int form) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (form != FGLS_SLICES_AUTO && form != FGLS_SLICES_SIX) return fail(FGLS_ERR_ARG, "unknown slice form %d", form);
+  if (form == c->slice_form) return FGLS_OK;
+  c->slice_form = form;
+  if (c->mode == FGLS_MODE_MIXED_I8 && c->have_trait) {
+    if ((rc = set_device(c))) return rc;
+    if ((rc = build_slices(c, nullptr))) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+  }
+  return FGLS_OK;
+}
+
+int fgls_hip_ctx_slice_info(const fgls_hip_ctx* c, int* nslice, double* loss) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  const bool built = c->A8 && c->have_trait && c->mode == FGLS_MODE_MIXED_I8;
+  if (nslice) *nslice = built ? c->nslice : 0;
+  if (loss) *loss = built ? c->slice_loss : -1.0;
+  return FGLS_OK;
+}
+
 int fgls_hip_ctx_set_mode(fgls_hip_ctx* c, int mode) {
   int rc = check_ctx(c);
   if (rc) return rc;
@@ -1466,7 +1590,8 @@ int fgls_hip_ctx_set_mode(fgls_hip_ctx* c, int mode) {
       const size_t bytes = (size_t)c->ldn * c->ldn * NSLICE;
       if (hipMalloc((void**)&c->A8, bytes) != hipSuccess)
         return fail(FGLS_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
-      if (hipMalloc((void**)&c->invsc, (size_t)c->ldn * sizeof(double)) != hipSuccess)
+      // the row scales and, after them, the loss cell of build_slices
+      if (hipMalloc((void**)&c->invsc, ((size_t)c->ldn + 1) * sizeof(double)) != hipSuccess)
         return fail(FGLS_ERR_NOMEM, "hipMalloc of the row scales failed");
     }
     if ((rc = fgls::ensure_buffer(&c->X8t, &c->X8t_cap, fgls_x8t_bytes(c->n, c->max_block * c->wXR, 1))))
@@ -1586,12 +1711,12 @@ int reduce_i8t(fgls_hip_ctx* c, const int8_t* dX8T, int count, int planes, doubl
   if (c->wXR > 1) {
     if ((rc = ensure_z(c))) return rc;
     fgls_launch_i8_store(c->n, count * c->wXR, c->RT8, c->ldn / 64, c->A8, c->invsc, dX8T, round_up(c->n, 64),
-                         c->z, (int64_t)c->ldn, st);
+                         c->z, (int64_t)c->ldn, c->nslice, st);
   } else {
     fgls_launch_trmm_i8(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, dX8T,
                         round_up(c->n, 64), c->W, (int64_t)c->ldn, c->part, (int64_t)c->max_block,
                         planes == 2 ? fgls_x8t_plane(c->n, count) : 0, planes == 2 ? 1.0 / scale : 1.0,
-                        0, st);
+                        0, c->nslice, st);
   }
   HIP_TRY(hipGetLastError());
   c->reduced = count;
@@ -1622,10 +1747,10 @@ int reduce_u2t(fgls_hip_ctx* c, const uint8_t* dX2T, int count, void* stream) {
   if (c->wXR > 1) {
     if ((rc = ensure_z(c))) return rc;
     fgls_launch_u2_store(c->n, count * c->wXR, c->RT8, c->ldn / 64, c->A8, c->invsc, dX2T, c->z,
-                         (int64_t)c->ldn, st);
+                         (int64_t)c->ldn, c->nslice, st);
   } else {
     fgls_launch_trmm_u2(c->p, c->n, count, c->RT8, c->ldn / 64, c->A8, c->invsc, dX2T, c->W, (int64_t)c->ldn,
-                        c->part, (int64_t)c->max_block, 0, st);
+                        c->part, (int64_t)c->max_block, 0, c->nslice, st);
   }
   HIP_TRY(hipGetLastError());
   c->reduced = count;

@@ -37,6 +37,9 @@ FGLS_ERR_STATE = -6
 
 FGLS_MODE_FP64 = 0
 FGLS_MODE_MIXED_I8 = 1
+# digit form of the slices of L^-1 in the mixed mode (fgls_hip_ctx_set_slice_form)
+FGLS_SLICES_AUTO = 0
+FGLS_SLICES_SIX = 6
 
 _lib = None
 
@@ -78,6 +81,8 @@ def lib() -> ctypes.CDLL:
             "fgls_hip_block_reduce": (i32, [vp, dp, i64, i32, vp]),
             "fgls_hip_block_finalize": (i32, [vp, i32, dp, dp, i64, vp]),
             "fgls_hip_ctx_set_mode": (i32, [vp, i32]),
+            "fgls_hip_ctx_set_slice_form": (i32, [vp, i32]),
+            "fgls_hip_ctx_slice_info": (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double)]),
             "fgls_hip_block_reduce_i8": (i32, [vp, dp, i64, i32, vp]),
             "fgls_hip_solve_block_i8": (i32, [vp, dp, i64, i32, dp, dp, i64, vp]),
             "fgls_hip_to_i8": (i32, [dp, i64, i32, i32, dp, i64, dp, vp]),
@@ -151,6 +156,7 @@ def exported_symbols() -> list[str]:
                         "fgls_hip_get_trait", "fgls_hip_solve_block", "fgls_hip_solve_block_noalloc",
                         "fgls_hip_block_reduce",
                         "fgls_hip_block_finalize", "fgls_hip_ctx_set_mode", "fgls_hip_block_reduce_i8",
+                        "fgls_hip_ctx_set_slice_form", "fgls_hip_ctx_slice_info",
                         "fgls_hip_solve_block_i8", "fgls_hip_to_i8", "fgls_hip_trsm",
                         "fgls_hip_trmm", "fgls_hip_snp_stage", "fgls_hip_check",
                         "fgls_hip_adjoint_begin", "fgls_hip_adjoint_block", "fgls_hip_adjoint_end",
@@ -332,6 +338,16 @@ class DeviceContext:
     def set_mode(self, mode: int):
         _check(lib().fgls_hip_ctx_set_mode(self._h, mode))
         self.mode = mode
+
+    def set_slice_form(self, form: int):
+        """FGLS_SLICES_AUTO (five digits of L^-1 where the error bound allows) or FGLS_SLICES_SIX."""
+        _check(lib().fgls_hip_ctx_set_slice_form(self._h, form))
+
+    def slice_info(self) -> tuple[int, float]:
+        """(digits of the slices in use: 5, 6, or 0 before any; loss of the five-digit form, -1 if not tried)."""
+        ns, loss = ctypes.c_int(0), ctypes.c_double(0.0)
+        _check(lib().fgls_hip_ctx_slice_info(self._h, ctypes.byref(ns), ctypes.byref(loss)))
+        return ns.value, loss.value
 
     @property
     def ld8(self) -> int:

@@ -91,12 +91,16 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  *                  as the reference) and upload the factor per trait; the
  *                  default potrf=gpu uploads Phi once and builds and factors
  *                  M on the device (fgls_hip_set_trait_phi)
- *   precision=mixed  int8 genotypes x six int8 digit slices of L^-1 on the int8
+ *   precision=mixed  int8 genotypes x int8 digit slices of L^-1 on the int8
  *                  MFMA (every X_R value must be an integer in [-127, 127],
  *                  else FGLS_ERR_ARG; wXR = 1 reduces in the product's
  *                  epilogue, wXR > 1 stores L^-1 X_R for the FP64 per-SNP
  *                  stage); precision=fp64 (default) is the reference's
  *                  arithmetic
+ *   slices=6       with precision=mixed: six digits per entry of L^-1; the
+ *                  default slices=auto keeps five (with a top digit on the
+ *                  diagonal 64-blocks) wherever the error bound stays within
+ *                  16 times the six-digit one (fgls_hip_ctx_set_slice_form)
  * No counterpart in the reference (its GPU entry only exits on error).
  */
 int fgls_chol_gpu_ex(int n, int p, int m, int t, int wXL, int wXR,

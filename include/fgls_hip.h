@@ -34,7 +34,7 @@ extern "C" {
 
 /* Arithmetic of the per-SNP hot path (fgls_hip_ctx_set_mode). */
 #define FGLS_MODE_FP64     0   /* FP64 genotypes, FP64 MFMA                    */
-#define FGLS_MODE_MIXED_I8 1   /* int8 genotypes; L^-1 as 6 int8 digit slices
+#define FGLS_MODE_MIXED_I8 1   /* int8 genotypes; L^-1 as 5 or 6 int8 digit slices
                                   (balanced radix 254); exact int32 accumulation
                                   on the int8 MFMA; FP64 per-row recombination
                                   and per-SNP solve; n <= 133,000 */
@@ -129,6 +129,25 @@ int fgls_hip_block_finalize(fgls_hip_ctx *ctx, int count, double *dB,
  * path is set by the slice count (b within 1e-9 of max(|b|, se), see the tests).
  */
 int fgls_hip_ctx_set_mode(fgls_hip_ctx *ctx, int mode);
+/*
+ * Digit form of the slices of L^-1 in the mixed mode (no counterpart in the
+ * reference).  FGLS_SLICES_AUTO (the default): row i of L^-1 is scaled by
+ * its largest entry outside its diagonal 64-column block, and five digits
+ * per entry plus one top digit on that diagonal block are kept whenever the
+ * resulting error bound is at most 16 times the six-digit form's (the
+ * "loss": 254 max_i (off-block maximum / row maximum) of row i) and
+ * n <= 116,000; five-sixths of the int8 work of six digits.  Otherwise, and
+ * always with FGLS_SLICES_SIX, six digits of the row scaled by its maximum.
+ * set_slice_form takes effect at the next set_trait (and rebuilds the slices
+ * at once if a trait is set and the context is in the mixed mode).
+ * slice_info gives the digits in use (5 or 6; 0 before any slices exist)
+ * and the loss measured at the last build (-1 if the five-digit form was not
+ * tried).
+ */
+#define FGLS_SLICES_AUTO 0
+#define FGLS_SLICES_SIX  6
+int fgls_hip_ctx_set_slice_form(fgls_hip_ctx *ctx, int form);
+int fgls_hip_ctx_slice_info(const fgls_hip_ctx *ctx, int *nslice, double *loss);
 int fgls_hip_block_reduce_i8(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ldx8,
                              int count, void *stream);
 int fgls_hip_solve_block_i8(fgls_hip_ctx *ctx, const int8_t *dX8, int64_t ldx8,

@@ -136,6 +136,17 @@ def test_fgls_chol_gpu_mixed_precision(pkg, gpu, tmp_path, n, m, t, x_b):
     V = read_out(d, "V.out", t * m * p * p)
     e = orc.parity_errors(B, V, Bo, Vo, p)
     assert e["B_scaled"] < 2e-11 and e["var_rel"] < 1e-12 and e["V_upper_zero"], e
+    # slices=6: the six-digit form of L^-1, to the same bar
+    pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, x_b, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                      f("XL.in"), f("XR.in"), f("Y.in"), f("B6.out"), f("V6.out"),
+                      options={"precision": "mixed", "block": 2 * x_b, "slices": 6})
+    e6 = orc.parity_errors(read_out(d, "B6.out", t * m * p), read_out(d, "V6.out", t * m * p * p), Bo, Vo, p)
+    assert e6["B_scaled"] < 2e-11 and e6["var_rel"] < 1e-12 and e6["V_upper_zero"], e6
+    with pytest.raises(pkg.FglsError) as ei:
+        pkg.fgls_chol_gpu(n, p, m, t, wXL, 1, x_b, 1, 4, f("Phi.in"), f("H.in"), f("Sig.in"),
+                          f("XL.in"), f("XR.in"), f("Y.in"), f("B6.out"), f("V6.out"),
+                          options={"precision": "mixed", "slices": 7})
+    assert ei.value.code == pkg.FGLS_ERR_ARG and "slices" in ei.value.msg
 
 
 def test_fgls_chol_gpu_mixed_rejects_dosages(pkg, gpu, tmp_path):
