@@ -42,7 +42,7 @@ constexpr int I8_TM = 32, I8_TN = 256, I8_GR = FGLS_I8_GR, I8_GC = FGLS_I8_GC;
 // in turn, heaviest row tiles (longest K) first.  Returns false for padding
 // workgroups.
 template <int GR, int GC>
-__device__ __forceinline__ bool fgls_tile_of_block(int b, int RT, int CT, int& rt, int& ct) {
+__host__ __device__ __forceinline__ bool fgls_tile_of_block(int b, int RT, int CT, int& rt, int& ct) {
   constexpr int GS = GR * GC;
   const int xcd = b & 7, k = b >> 3;
   const int group = (k / GS) * 8 + xcd, j = k % GS;

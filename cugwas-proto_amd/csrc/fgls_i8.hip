@@ -233,6 +233,22 @@ __device__ __forceinline__ void step_i8(fgls_v4i (&acc)[S][2][4], const typename
 // into part as I8_RED does z'z and z'W[:, k].
 enum { I8_RED = 0, I8_STORE = 1, I8_DOT = 2 };
 
+// XCD group shape of each variant (row tiles x column tiles that one XCD
+// takes together, fgls_tile_of_block): 16 x 8 on 2-bit tiles (1-1.5 %
+// faster than 16 x 4 with five or six digits), 32 x 2 for the five-digit
+// product on 8-bit tiles and dosage planes (4 % faster than 16 x 4: 32 row
+// tiles share each genotype panel in the XCD's L2; with six digits 16 x 4
+// is faster), I8_GR x I8_GC otherwise and for full matrices (FULL).
+template <bool TOP, bool U2, bool FULL>
+struct I8Group {
+  static constexpr int R = FULL ? I8_GR : U2 ? 16 : TOP ? 32 : I8_GR;
+  static constexpr int C = FULL ? I8_GC : U2 ? 8 : TOP ? 2 : I8_GC;
+};
+template <bool TOP, bool U2, bool FULL>
+int i8_grid(int RT32, int CT) {
+  return fgls_tile_grid(RT32, CT, I8Group<TOP, U2, FULL>::R, I8Group<TOP, U2, FULL>::C);
+}
+
 // TOP: A8 holds five digits per entry (fgls_a8_offset with S = 5) and A8top
 // the top digits of every 16-row group's diagonal 64-k step (the group's
 // last step: fgls_a8top_offset); triangular A only.
@@ -252,7 +268,8 @@ __global__ __launch_bounds__(256, 2) void k_trmm_i8(
   constexpr int WN = DOS ? 32 : 64;   // SNP columns per wave
   __shared__ SliceStage<S, TOP> st0, st1;
   int rt, ct;
-  if (!fgls_tile_of_block<I8_GR, I8_GC>(blockIdx.x, RT32, CT, rt, ct)) return;
+  using Grp = I8Group<TOP, U2, EIG || MODE == I8_DOT>;
+  if (!fgls_tile_of_block<Grp::R, Grp::C>(blockIdx.x, RT32, CT, rt, ct)) return;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lg = lane >> 4;
@@ -726,26 +743,24 @@ void launch(int n, int count, int RT32, int KS, const int8_t* A8, const double* 
   const int8_t* top = A8 + fgls_a8top_base(KS);
   if (plane_off > 0) {
     const int CT = (count + 127) / 128;
-    const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
     if (!EIG && nslice == T)
-      hipLaunchKernelGGL((k_trmm_i8<P + 1, T, true, false, I8_RED, false, true>), dim3(G), dim3(256), 0, st, n,
+      hipLaunchKernelGGL((k_trmm_i8<P + 1, T, true, false, I8_RED, false, true>), dim3(i8_grid<true, false, false>(RT32, CT)), dim3(256), 0, st, n,
                          count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off, inv_xscale, 1,
                          (double*)nullptr, (int64_t)0, top);
     else
-      hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, true, EIG>), dim3(G), dim3(256), 0, st, n,
-                         count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off,
+      hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, true, EIG>), dim3(i8_grid<false, false, EIG>(RT32, CT)),
+                         dim3(256), 0, st, n, count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, plane_off,
                          inv_xscale, ntr);
   } else {
     const int CT = (count + I8_TN - 1) / I8_TN;
-    const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
     if (!EIG && nslice == T)
-      hipLaunchKernelGGL((k_trmm_i8<P + 1, T, false, false, I8_RED, false, true>), dim3(G), dim3(256), 0, st,
+      hipLaunchKernelGGL((k_trmm_i8<P + 1, T, false, false, I8_RED, false, true>), dim3(i8_grid<true, false, false>(RT32, CT)), dim3(256), 0, st,
                          n, count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, (int64_t)0, 1.0, 1,
                          (double*)nullptr, (int64_t)0, top);
     else
-      hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, EIG>), dim3(G), dim3(256), 0, st, n,
-                         count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, (int64_t)0, 1.0,
-                         ntr);
+      hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, EIG>), dim3(i8_grid<false, false, EIG>(RT32, CT)),
+                         dim3(256), 0, st, n, count, RT32, CT, KS, A8, isc, X8, ldx8, W, ldw, part, ldp, (int64_t)0,
+                         1.0, ntr);
   }
 }
 
@@ -770,20 +785,20 @@ void launch_u2(int n, int count, int RT32, int KS, const int8_t* A8, const doubl
                const double* W, int64_t ldw, double* part, int64_t ldp, int eig_ntr, int nslice,
                hipStream_t st) {
   const int CT = (count + I8_TN - 1) / I8_TN;
-  const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
   const int64_t ld8 = ((int64_t)n + 63) / 64 * 64;
   if (eig_ntr > 0)
-    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, true, I8_RED, true>), dim3(G), dim3(256), 0, st,
+    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, true, I8_RED, true>), dim3(i8_grid<false, true, true>(RT32, CT)), dim3(256), 0, st,
                        n, count, RT32, CT, KS, A8, isc, (const int8_t*)X2T, ld8, W, ldw, part, ldp, (int64_t)0,
                        1.0, eig_ntr);
   else if (nslice == FGLS_TOP_NSLICE)
-    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_TOP_NSLICE, false, false, I8_RED, true, true>), dim3(G),
-                       dim3(256), 0, st, n, count, RT32, CT, KS, A8, isc, (const int8_t*)X2T, ld8, W, ldw, part,
+    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_TOP_NSLICE, false, false, I8_RED, true, true>),
+                       dim3(i8_grid<true, true, false>(RT32, CT)), dim3(256), 0, st, n, count, RT32, CT, KS, A8,
+                       isc, (const int8_t*)X2T, ld8, W, ldw, part,
                        ldp, (int64_t)0, 1.0, 1, (double*)nullptr, (int64_t)0, A8 + fgls_a8top_base(KS));
   else
-    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, false, I8_RED, true>), dim3(G), dim3(256), 0, st,
-                       n, count, RT32, CT, KS, A8, isc, (const int8_t*)X2T, ld8, W, ldw, part, ldp, (int64_t)0,
-                       1.0, 1);
+    hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, false, I8_RED, true>),
+                       dim3(i8_grid<false, true, false>(RT32, CT)), dim3(256), 0, st, n, count, RT32, CT, KS, A8,
+                       isc, (const int8_t*)X2T, ld8, W, ldw, part, ldp, (int64_t)0, 1.0, 1);
 }
 
 template <int P>
@@ -791,8 +806,7 @@ void launch_dot(int n, int count, int RT32, int KS, const int8_t* A8, const doub
                 const int8_t* X8T, int64_t ld8, const double* W, int64_t ldw, const double* Zin,
                 int64_t ldz, double* part, int64_t ldp, hipStream_t st) {
   const int CT = (count + I8_TN - 1) / I8_TN;
-  const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
-  hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, false, I8_DOT>), dim3(G), dim3(256), 0, st,
+  hipLaunchKernelGGL((k_trmm_i8<P + 1, FGLS_NSLICE, false, false, I8_DOT>), dim3(i8_grid<false, false, true>(RT32, CT)), dim3(256), 0, st,
                      n, count, RT32, CT, KS, A8, isc, X8T, ld8, W, ldw, part, ldp, (int64_t)0, 1.0, 1,
                      const_cast<double*>(Zin), ldz);
 }
@@ -801,14 +815,15 @@ template <bool U2>
 void launch_store(int n, int count, int RT32, int KS, const int8_t* A8, const double* inv_scale,
                   const int8_t* X, double* Z, int64_t ldz, int nslice, hipStream_t st) {
   const int CT = (count + I8_TN - 1) / I8_TN;
-  const int G = fgls_tile_grid(RT32, CT, I8_GR, I8_GC);
   const int64_t ld8 = ((int64_t)n + 63) / 64 * 64;
   if (nslice == FGLS_TOP_NSLICE)
-    hipLaunchKernelGGL((k_trmm_i8<2, FGLS_TOP_NSLICE, false, false, I8_STORE, U2, true>), dim3(G), dim3(256), 0,
+    hipLaunchKernelGGL((k_trmm_i8<2, FGLS_TOP_NSLICE, false, false, I8_STORE, U2, true>),
+                       dim3(i8_grid<true, U2, false>(RT32, CT)), dim3(256), 0,
                        st, n, count, RT32, CT, KS, A8, inv_scale, X, ld8, (const double*)nullptr, (int64_t)0,
                        (double*)nullptr, (int64_t)0, (int64_t)0, 1.0, 1, Z, ldz, A8 + fgls_a8top_base(KS));
   else
-    hipLaunchKernelGGL((k_trmm_i8<2, FGLS_NSLICE, false, false, I8_STORE, U2>), dim3(G), dim3(256), 0, st, n,
+    hipLaunchKernelGGL((k_trmm_i8<2, FGLS_NSLICE, false, false, I8_STORE, U2>), dim3(i8_grid<false, U2, false>(RT32, CT)),
+                       dim3(256), 0, st, n,
                        count, RT32, CT, KS, A8, inv_scale, X, ld8, (const double*)nullptr, (int64_t)0,
                        (double*)nullptr, (int64_t)0, (int64_t)0, 1.0, 1, Z, ldz);
 }
