@@ -26,7 +26,9 @@ constexpr int FGLS_I8_MAX_N = 133000;
 constexpr int FGLS_TOP_NSLICE = 5;
 constexpr double FGLS_TOP_MAX_LOSS = 16.0;
 constexpr int FGLS_I8_TOP_MAX_N = 116000;
-// The grouping is a build-time tunable (-DFGLS_I8_GR=.. -DFGLS_I8_GC=..).
+// The default grouping is a build-time tunable (-DFGLS_I8_GR=.. -DFGLS_I8_GC=..);
+// the variants of the int8 product that measured faster with another shape
+// set their own (I8Group in fgls_i8.hip).
 #ifndef FGLS_I8_GR
 #define FGLS_I8_GR 16
 #endif
