@@ -362,7 +362,8 @@ def run_pipeline(torch, pkg, args, Phi, XL, y, X, X8, dev_paths, dev):
         # 16,384, io_threads = the host cores up to 16) on the reference's FP64 XR.in
         runs = [("f64", "fp64", 0, 16384), ("f64", "mixed", 0, 16384), ("f64", "fp64", 4, 16384),
                 ("f64", "fp64", 8, 16384), ("f64", "fp64", 0, 65536), ("i8", "fp64", 0, 16384),
-                ("i8", "mixed", 0, 16384), ("u2", "mixed", 0, 16384), ("u2", "fp64", 0, 16384)]
+                ("i8", "mixed", 0, 16384), ("i8", "mixed", 0, 65536), ("u2", "mixed", 0, 16384),
+                ("u2", "mixed", 0, 65536), ("u2", "fp64", 0, 16384)]
         runs = [r for r in runs if r[0] in fmts]
         # trait 0 of B.out against the device-resident path of the same precision
         Bref = {prec: dev_paths[key]["B"][:m].cpu().numpy().ravel()

@@ -52,7 +52,8 @@ int fgls_chol_gpu(int n, int p, int m, int t, int wXL, int wXR,
  * of key=value settings; when NULL the FGLS_OPTIONS environment variable is
  * read instead:
  *   block=<SNPs>   SNPs per device launch (default: a multiple of x_b near
- *                  16384, at most m)
+ *                  16384, near 65536 for xr_format=u2 with precision=mixed;
+ *                  at most m)
  *   device=<id>    HIP device (default 0)
  *   serial=1       synchronise after every stage (debug aid, like the
  *                  reference's FGLS_GPU_SERIAL, fgls_chol_gpu.c:44-46)
