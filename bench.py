@@ -682,6 +682,18 @@ def main():
     log(f"inputs ready: n={n} m={m} X_R {X.numel() * 8 / 1e9:.1f} GB FP64 + {X8.numel() / 1e9:.1f} GB int8 "
         f"in HBM ({time.perf_counter() - t0:.1f}s)")
 
+    # the mixed path's input: the same genotypes as 2-bit columns, ceil(n/4)
+    # bytes per SNP, individual r in bits 2 (r % 4) of byte r / 4
+    P2 = None
+    if any(pth in ("mixed", "mixed6") for pth in path_list):
+        nb = (n + 3) // 4
+        P2 = torch.empty((m, nb), dtype=torch.uint8, device=dev)
+        for i0 in range(0, m, 65536):
+            q = X8[i0:i0 + 65536, :4 * nb].to(torch.uint8).view(-1, nb, 4)
+            P2[i0:i0 + 65536] = q[:, :, 0] | (q[:, :, 1] << 2) | (q[:, :, 2] << 4) | (q[:, :, 3] << 6)
+            del q
+        torch.cuda.synchronize()
+
     block = min(args.block, m)
     ctx = pkg.DeviceContext(n, wXL, 1, max_block=block)
     # per-trait one-off on the device: M = h2 Phi + (1 - h2) I, its Cholesky
@@ -728,15 +740,17 @@ def main():
         # take the automatic form (five digits where the error bound allows)
         ctx.set_slice_form(pkg.FGLS_SLICES_SIX if mode == "mixed6" else pkg.FGLS_SLICES_AUTO)
         if mode in ("mixed", "mixed6"):
-            # hard-call genotypes as 2-bit tiles (fgls_hip_tile_u2): the int8
-            # product loads a quarter of the genotype bytes
+            # hard-call genotypes resident as 2-bit columns (the layout of a
+            # 2-bit X_R file, xr_format=u2), tiled per block for the int8
+            # product (fgls_hip_tile_u2_packed), which loads a quarter of the
+            # int8 genotype bytes
             t0 = time.perf_counter()
             ctx.set_mode(pkg.FGLS_MODE_MIXED_I8)
             torch.cuda.synchronize()
             setup_s = time.perf_counter() - t0
             X2T = torch.empty(pkg.x2t_bytes(n, block), dtype=torch.uint8, device=dev)
             bad = torch.zeros(1, dtype=torch.int32, device=dev)
-            tile = lambda i0, cnt: pkg.tile_u2(X8[i0:i0 + cnt], n, cnt, X2T=X2T, bad=bad)
+            tile = lambda i0, cnt: pkg.tile_u2_packed(P2[i0:i0 + cnt], n, cnt, X2T=X2T, bad=bad)
             reduce = lambda i0, cnt: ctx.block_reduce_u2t(X2T, cnt)
         elif mode in ("dosage", "mixed8"):
             t0 = time.perf_counter()
@@ -991,7 +1005,7 @@ def main():
             log(f"eigen leg failed: {err}")
     adjoint = None
     if args.adjoint_m > 0:
-        del X, X8, D8, paths
+        del X, X8, D8, P2, paths
         torch.cuda.empty_cache()
         try:
             adjoint = run_adjoint(torch, pkg, dev, args.adjoint_m, args.steps, args.adjoint_n, args.adjoint_wxl)

@@ -701,36 +701,56 @@ __global__ __launch_bounds__(256) void k_tile_u2(int n, int count, int KSX, cons
 }
 
 // 2-bit genotype file columns (ceil(n/4) bytes per column at P + c ldp; the
-// file's genotype r in bits 2 (r % 4) of byte r / 4) -> 2-bit tiles: the 16
-// genotypes of a lane are 4 file bytes (genotype 4 m + q' at file bit
-// 8 m + 2 q'), moved to bit 8 q' + 2 m.  Code 3 in rows < n is counted in
-// *bad and written as 0; rows >= n and bytes past the column are 0.
+// file's genotype r in bits 2 (r % 4) of byte r / 4, so the 16 genotypes
+// j = 0..15 of a dword sit at bits 2 j) -> 2-bit tiles, whose lane dword
+// holds genotype j at bit 8 (j % 4) + 2 (j / 4): a transpose of the dword's
+// 4 x 4 two-bit fields.  One workgroup per 16-column group and 16 steps;
+// thread t takes column t % 16 and step 16 blockIdx.y + t / 16: it reads the
+// column's 16 bytes of the step as 4 dwords (bytes when the columns are not
+// 4-byte aligned or at the end of a column) and writes lanes 16 g + t % 16
+// of the step's block, so the 16 threads of a column group write 64
+// contiguous bytes.  Code 3 in rows < n is counted in *bad and written as 0;
+// rows >= n and bytes past the column are 0; columns past count repeat the
+// last column.
 __global__ __launch_bounds__(256) void k_tile_u2_packed(int n, int count, int KSX, const uint8_t* __restrict__ P,
                                                         int64_t ldp, unsigned int* __restrict__ X2T,
                                                         unsigned int* __restrict__ bad) {
   const int cg = blockIdx.x;
   const int nbytes = (n + 3) / 4;
+  const bool al4 = ((ldp & 3) == 0) && ((reinterpret_cast<uintptr_t>(P) & 3) == 0);
+  const int c = threadIdx.x & 15;
+  const int col = min(cg * 16 + c, count - 1);
+  const uint8_t* src = P + (size_t)col * ldp;
   unsigned int* dst = X2T + (size_t)cg * KSX * 64;
   unsigned int nb = 0;
-  for (int t = threadIdx.x; t < KSX * 64; t += 256) {
-    const int ks = t >> 6, g = (t >> 4) & 3, i = t & 15;
-    const int col = min(cg * 16 + i, count - 1);
-    const int r0 = 64 * ks + 16 * g;
-    const uint8_t* src = P + (size_t)col * ldp;
-    unsigned int x = 0;
+  const int ks = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (ks < KSX) {
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int bi = r0 / 4 + m;
-      const unsigned int byte = bi < nbytes ? src[bi] : 0u;
+    for (int g = 0; g < 4; ++g) {
+      const int bi = 16 * ks + 4 * g;   // bytes of genotypes r = 64 ks + 16 g + 0..15
+      unsigned int x = 0;
+      if (al4 && bi + 3 < nbytes) {
+        x = *reinterpret_cast<const unsigned int*>(src + bi);
+      } else {
 #pragma unroll
-      for (int q2 = 0; q2 < 4; ++q2) {
-        unsigned int c = (byte >> (2 * q2)) & 3u;
-        if (r0 + 4 * m + q2 >= n) c = 0;
-        else if (c == 3u) { ++nb; c = 0; }
-        x |= c << (8 * q2 + 2 * m);
+        for (int m = 0; m < 4; ++m)
+          if (bi + m < nbytes) x |= (unsigned int)src[bi + m] << (8 * m);
       }
+      // fields of rows >= n cleared, code 3 counted and cleared
+      const int valid = n - 4 * bi;
+      if (valid < 16) x &= valid <= 0 ? 0u : (1u << (2 * valid)) - 1u;
+      const unsigned int three = x & (x >> 1) & 0x55555555u;
+      nb += __builtin_popcount(three);
+      x &= ~(three | (three << 1));
+      // transpose the 4 x 4 two-bit fields (field (m, q) at bit 8 m + 2 q
+      // goes to bit 8 q + 2 m): swap the off-diagonal 2 x 2 blocks, then
+      // the off-diagonal fields inside each block
+      unsigned int t = ((x >> 12) ^ x) & 0x0000F0F0u;
+      x ^= t ^ (t << 12);
+      t = ((x >> 6) ^ x) & 0x00CC00CCu;
+      x ^= t ^ (t << 6);
+      dst[(size_t)ks * 64 + 16 * g + c] = x;
     }
-    dst[t] = x;
   }
   if (nb) atomicAdd(bad, nb);
 }
@@ -865,7 +885,7 @@ void fgls_launch_tile_u2(int n, int count, const int8_t* X8, int64_t ldx8, uint8
 void fgls_launch_tile_u2_packed(int n, int count, const uint8_t* P, int64_t ldp, uint8_t* X2T,
                                 unsigned int* bad, hipStream_t st) {
   const int KSX = (n + 63) / 64;
-  hipLaunchKernelGGL(k_tile_u2_packed, dim3((count + 15) / 16), dim3(256), 0, st, n, count, KSX, P, ldp,
+  hipLaunchKernelGGL(k_tile_u2_packed, dim3((count + 15) / 16, (KSX + 15) / 16), dim3(256), 0, st, n, count, KSX, P, ldp,
                      reinterpret_cast<unsigned int*>(X2T), bad);
 }
 

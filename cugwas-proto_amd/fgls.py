@@ -757,17 +757,21 @@ def tile_u2(X8, n: int, count: int | None = None, X2T=None, bad=None, stream=Non
     return X2T
 
 
-def tile_u2_packed(P, n: int, count: int | None = None, X2T=None, stream=None):
+def tile_u2_packed(P, n: int, count: int | None = None, X2T=None, bad=None, stream=None):
     """2-bit genotype file columns P ((count, ldp) uint8 tensor, pack_u2's
-    layout) -> the 2-bit tiles of the int8 product; raises if a code is 3."""
+    layout) -> the 2-bit tiles of the int8 product.  Raises if a code is 3,
+    unless a device counter `bad` (int32, caller-cleared) is given, which then
+    receives the count without a synchronisation."""
     import torch
     count = P.shape[0] if count is None else count
     nbytes = x2t_bytes(n, count)
     if X2T is None:
         X2T = torch.empty(nbytes, dtype=torch.uint8, device=P.device)
-    bad = torch.zeros(1, dtype=torch.int32, device=P.device)
+    own = bad is None
+    if own:
+        bad = torch.zeros(1, dtype=torch.int32, device=P.device)
     _check(lib().fgls_hip_tile_u2_packed(_ptr(P), P.shape[1], n, count, _ptr(X2T), _ptr(bad), _stream(stream)))
-    if int(bad.item()):
+    if own and int(bad.item()):
         raise FglsError(FGLS_ERR_ARG, f"{int(bad.item())} 2-bit codes are 3 (not a genotype)")
     return X2T
 
