@@ -237,12 +237,20 @@ enum { I8_RED = 0, I8_STORE = 1, I8_DOT = 2 };
 // takes together, fgls_tile_of_block): 16 x 8 on 2-bit tiles (1-1.5 %
 // faster than 16 x 4 with five or six digits), 32 x 2 for the five-digit
 // product on 8-bit tiles and dosage planes (4 % faster than 16 x 4: 32 row
-// tiles share each genotype panel in the XCD's L2; with six digits 16 x 4
-// is faster), I8_GR x I8_GC otherwise and for full matrices (FULL).
+// tiles share each genotype panel in the XCD's L2; 64 x 1, 48 x 1, 32 x 1
+// and 16 x 2 measured 1-4 % slower; with six digits 16 x 4 is faster;
+// -DFGLS_I8_GR8= -DFGLS_I8_GC8= set it at build time), I8_GR x I8_GC
+// otherwise and for full matrices (FULL).
+#ifndef FGLS_I8_GR8
+#define FGLS_I8_GR8 32
+#endif
+#ifndef FGLS_I8_GC8
+#define FGLS_I8_GC8 2
+#endif
 template <bool TOP, bool U2, bool FULL>
 struct I8Group {
-  static constexpr int R = FULL ? I8_GR : U2 ? 16 : TOP ? 32 : I8_GR;
-  static constexpr int C = FULL ? I8_GC : U2 ? 8 : TOP ? 2 : I8_GC;
+  static constexpr int R = FULL ? I8_GR : U2 ? 16 : TOP ? FGLS_I8_GR8 : I8_GR;
+  static constexpr int C = FULL ? I8_GC : U2 ? 8 : TOP ? FGLS_I8_GC8 : I8_GC;
 };
 template <bool TOP, bool U2, bool FULL>
 int i8_grid(int RT32, int CT) {
