@@ -858,31 +858,50 @@ __device__ __forceinline__ void record_failure(unsigned long long* err, int64_t 
 //   [wXL + wXL^2]          res_sigma
 //   [wXL + wXL^2 + 1, +wXL) b_L
 
-// Finalize from the TRMM partials (wXR = 1, p = P): thread per SNP, one
+// Finalize from the TRMM partials (wXR = 1, p = P): lane per SNP, one
 // grid row per trait k of ntr (partials of trait k at
 // part[((rt*ntr + k)*NQ + q)*ldp + i]; its scalars at trait + k*trait_ld,
 // its outputs at Bout + k*ldb, Vout + k*ldv, its failure record at err[k]).
+//
+// Workgroup: FIN_SPLIT waves over the same 64 SNPs (lane = SNP), wave w
+// summing the row tiles rt = w, w + FIN_SPLIT, ... (FIN_SPLIT times the
+// loads in flight of one wave per SNP); wave 0 then adds the other waves'
+// sums in wave order (a fixed order: the result does not depend on timing)
+// and runs the p x p stage.
+constexpr int FIN_SPLIT = 4;
 template <int P>
-__global__ __launch_bounds__(128) void k_finalize(int count, int RT, const double* __restrict__ part,
+__global__ __launch_bounds__(64 * FIN_SPLIT) void k_finalize(int count, int RT, const double* __restrict__ part,
                                                   int64_t ldp, int ntr, const double* __restrict__ trait,
                                                   int64_t trait_ld, double* __restrict__ Bout, int64_t ldb,
                                                   double* __restrict__ Vout, int64_t ldv,
                                                   unsigned long long* __restrict__ err, int64_t snp_base) {
   constexpr int NQ = P + 1;
   constexpr int WXL = P - 1;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double red[FIN_SPLIT - 1][NQ][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
   const int k = blockIdx.y;
-  if (i >= count) return;
-  trait += (size_t)k * trait_ld;
-  Bout += (size_t)k * ldb;
-  Vout += (size_t)k * ldv;
+  const int ic = min(i, count - 1);   // lanes past count read a valid column and store nothing
   double sums[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) sums[q] = 0.0;
-  for (int rt = 0; rt < RT; ++rt) {
+  for (int rt = w; rt < RT; rt += FIN_SPLIT) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) sums[q] += part[(((size_t)rt * ntr + k) * NQ + q) * ldp + i];
+    for (int q = 0; q < NQ; ++q) sums[q] += part[(((size_t)rt * ntr + k) * NQ + q) * ldp + ic];
   }
+  if (w > 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) red[w - 1][q][lane] = sums[q];
+  }
+  __syncthreads();
+  if (w > 0 || i >= count) return;
+#pragma unroll
+  for (int j = 0; j < FIN_SPLIT - 1; ++j)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) sums[q] += red[j][q][lane];
+  trait += (size_t)k * trait_ld;
+  Bout += (size_t)k * ldb;
+  Vout += (size_t)k * ldv;
   double S[P][P], b[P];
 #pragma unroll
   for (int r = 0; r < P; ++r)
@@ -1106,8 +1125,9 @@ int ensure_z(fgls_hip_ctx* c) {
 template <int P>
 void launch_finalize_from(const fgls_hip_ctx* c, int RT, const double* part, int64_t ldp, int count,
                           double* dB, double* dV, int64_t base, hipStream_t st) {
-  dim3 grid((count + 127) / 128);
-  hipLaunchKernelGGL(k_finalize<P>, grid, dim3(128), 0, st, count, RT, part, ldp, 1, c->trait,
+  if (count <= 0) return;
+  dim3 grid((count + 63) / 64);
+  hipLaunchKernelGGL(k_finalize<P>, grid, dim3(64 * FIN_SPLIT), 0, st, count, RT, part, ldp, 1, c->trait,
                      (int64_t)0, dB, (int64_t)0, dV, (int64_t)0, c->err, base);
 }
 
@@ -1269,8 +1289,9 @@ void launch_finalize_multi_p(int count, int RT, const double* part, int64_t ldp,
                              const double* trait, int64_t trait_ld, double* dB, int64_t ldb,
                              double* dV, int64_t ldv, unsigned long long* err, int64_t base,
                              hipStream_t st) {
-  dim3 grid((count + 127) / 128, ntr);
-  hipLaunchKernelGGL(k_finalize<P>, grid, dim3(128), 0, st, count, RT, part, ldp, ntr, trait,
+  if (count <= 0) return;
+  dim3 grid((count + 63) / 64, ntr);
+  hipLaunchKernelGGL(k_finalize<P>, grid, dim3(64 * FIN_SPLIT), 0, st, count, RT, part, ldp, ntr, trait,
                      trait_ld, dB, ldb, dV, ldv, err, base);
 }
 
