@@ -75,7 +75,8 @@ constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 0xF) | ((n >> 4) << 14); }
 // finished reading the other half in step k - 1), issue the slices and then
 // the genotype loads of step k + 1, wait for this wave's genotype fragments
 // of step k (issued after its slices in step k - 1, so the barrier never
-// waits for genotype loads), then 12 LDS fragment reads and 48 MFMAs, each
+// waits for genotype loads), then 12 LDS fragment reads and 48 MFMAs (six
+// digits; five: 10 and 40, and 2 and 8 more on the diagonal step), each
 // read one fragment ahead of the four MFMAs that use it (vmcnt retires in
 // issue order: the 4 genotype loads follow the slices, and the S/2 + 4 loads
 // of step k + 1 follow them).  The loop runs two steps per trip so that the
