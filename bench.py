@@ -64,8 +64,8 @@ def int8_ops_per_snp(n, digits, planes=1):
 # 1,048,576 KiB (x 1.0).  FETCH_SIZE counts L2 misses sent to the fabric
 # (some are served by the MALL), so this is an upper bound on HBM reads.
 TRAFFIC_N, TRAFFIC_BLOCK = 10000, 65536
-TRAFFIC_KIB = {"fp64": (106695702.9, 202240.0), "mixed": (11012501.7, 808960.0),
-               "mixed6": (12614833.0, 808960.0), "mixed8": (36806635.6, 808960.0)}
+TRAFFIC_KIB = {"fp64": (106709599.6, 202240.0), "mixed": (8380560.1, 808960.0),
+               "mixed6": (9237405.8, 808960.0), "mixed8": (30692676.5, 808960.0)}
 TRAFFIC_DIGITS = {"fp64": 0, "mixed": 5, "mixed6": 6, "mixed8": 5}   # digit form of L^-1 profiled
 TRAFFIC_SOURCE = ("profile constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel at "
                   "n=10000, 65,536 SNPs per launch, scaled by a 1 GiB copy (x2.0 / x1.0); "
