@@ -20,20 +20,20 @@
 //             X    = L21 T11                      (product, B lower)
 //             T21  = -T22 X     (as T21' = -X' T22', product, B upper)
 //
-// 64 x 64 leaves factor and invert in one workgroup (k_leaf).  Every product runs
-// on k_tgemm, an FP64 MFMA kernel whose operands are both read along k
-// (A(r, k) at A[r ld + k], B(k, c) at B[c ld + k]) with a column-major
-// result, or, when it has at most 4,096 64 x 64 blocks of output, on
-// k_tgemm_sk, which splits K over the four waves of a block.  To give every product such operands the two n x n buffers are
-// kept symmetric-stored while the recursion runs ("R-view" = the buffer read
-// row-major, element (i, j) at i ld + j):
-//   L buffer: the R-view lower triangle holds the Schur complements S, then
-//             L row-major; the R-view upper holds L column-major (the final
-//             factor; its strict upper triangle in column-major terms is L').
-//   T buffer: R-view lower = T row-major (the final L^-1), R-view upper =
-//             T column-major while the recursion runs (zeroed at the end).
+// 64 x 64 leaves factor and invert in one workgroup (k_leaf), 128-row nodes
+// in one workgroup (k_node128).  Every product runs on k_tgemm, an FP64 MFMA
+// kernel that reads each operand along k or, transposed, across k, with a
+// column-major result, or, when it has at most 4,096 64 x 64 blocks of
+// output, on k_tgemm_sk, which splits K over the four waves of a block.  So
+// each factor needs one layout ("R-view" = the buffer read row-major, element
+// (i, j) at i ld + j):
+//   L buffer: L column-major in the R-view upper triangle (the final factor);
+//             the R-view lower triangle holds the Schur complements S (and
+//             L row-major on the diagonal 128 x 128 blocks).
+//   T buffer: R-view lower = T row-major (the final L^-1); the R-view upper
+//             holds temporaries and is zeroed at the end.
 // The temporary X of a node lives in the T buffer's R-view upper-right block
-// of that node, which holds T21 column-major only once the node is done.
+// of that node.
 // Rows and columns n..N-1 are identity in the leaves' arithmetic and zero in
 // the stored factor and inverse.  The first non-positive pivot is recorded as
 // LAPACK's info (1-based order of the failing leading minor); later leaves
@@ -108,6 +108,23 @@ __device__ __forceinline__ double lane_bcast(double x, int l) {
   return __hiloint2double(hi, lo);
 }
 
+// acc + C for one 16 x 16 block by one wave on FP64 MFMA, C(r, c) =
+// sum_{q < 16} U(r, q) V(c, q) with U(r, q) = U[ur + r][uc + q] and V(c, q) =
+// V[vr + c][vc + q] (VT: V[vr + q][vc + c]), both in LDS.  Element v of the
+// result is (row li, column lg + 4 v), li = lane % 16, lg = lane / 16.
+template <bool VT, typename TU, typename TV>
+__device__ __forceinline__ v4d blk16(const TU& U, int ur, int uc, const TV& V, int vr, int vc, int li, int lg,
+                                     v4d acc) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int q = 4 * lg + s;
+    const double a = U[ur + li][uc + q];
+    const double b = VT ? V[vr + q][vc + li] : V[vr + li][vc + q];
+    acc = mfma_f64(b, a, acc);
+  }
+  return acc;
+}
+
 // ---------------------------------------------------------------------------
 // 64 x 64 diagonal block at o = o0 + 64 blockIdx.x, one workgroup of 4 waves,
 // blocked by 16 with the block in LDS (A: S, then L; Tm: T = L^-1, lower
@@ -115,13 +132,15 @@ __device__ __forceinline__ double lane_bcast(double x, int l) {
 // FACTOR: S from the L buffer's R-view lower triangle; for each 16-column
 //         panel jb: wave 0 factors the 16 x 16 diagonal block in registers
 //         (lane r holds row r; the column and the pivot are scalar
-//         broadcasts, v_readlane) and inverts it (lane c: column c of T_jj);
-//         then all waves form the panel below, L_ij = S_ij T_jj', and
-//         update the trailing lower triangle, S_ik -= L_ij L_kj'.
+//         broadcasts, v_readlane; 1 / sqrt of the pivot from v_rsq_f64 and
+//         two Newton steps) and inverts it (lane c: column c of T_jj);
+//         then the waves form the panel below, L_ij = S_ij T_jj', and
+//         update the trailing lower triangle, S_ik -= L_ij L_kj', a 16 x 16
+//         block per wave on FP64 MFMA.
 // else:   L from the caller's factor (column-major lower triangle); each
 //         wave inverts one 16 x 16 diagonal block.
 // Then the off-diagonal blocks of T by block rows, T_IJ = -T_II sum_{J<=K<I}
-// L_IK T_KJ.  L (FACTOR) and T are written to both triangles of their
+// L_IK T_KJ (wave J, FP64 MFMA).  L (FACTOR) and T are written to both triangles of their
 // buffers, every store instruction covering contiguous row segments.  Rows
 // >= n are identity rows; their stored values are zero.  The first
 // non-positive pivot is recorded in info (1-based), and the pivot is replaced
@@ -146,6 +165,7 @@ __device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, do
   auto& X = sh.X;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 15, lg = lane >> 4;
   const int nl = min(64, n - o);   // live rows of the block (may be <= 0)
   for (int idx = tid; idx < 4096; idx += 256) {
     // FACTOR: row-major source, c fastest; else column-major, r fastest
@@ -162,11 +182,9 @@ __device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, do
   __syncthreads();
 
   // T_jj = L_jj^-1 of the diagonal block at c0 by one wave: lane c (< 16)
-  // forms column c by forward substitution; a[] is row `lane` of L_jj
-  auto diag_inverse = [&](int c0, const double (&a)[16]) {
-    double irr[16];   // 1 / L(r, r): independent of the substitution chain
-#pragma unroll
-    for (int r = 0; r < 16; ++r) irr[r] = 1.0 / lane_bcast(a[r], r);
+  // forms column c by forward substitution; a[] is row `lane` of L_jj and
+  // irr[r] = 1 / L(r, r) (the same in every lane)
+  auto diag_inverse = [&](int c0, const double (&a)[16], const double (&irr)[16]) {
     double t[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -188,7 +206,7 @@ __device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, do
     for (int jb = 0; jb < 4; ++jb) {
       const int c0 = 16 * jb;
       if (w == 0) {
-        double a[16];
+        double a[16], ip[16];
         const int rr = min(lane, 15);
 #pragma unroll
         for (int j = 0; j < 16; ++j) a[j] = A[c0 + rr][c0 + j];
@@ -199,8 +217,15 @@ __device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, do
             if (bad == 0) bad = o + c0 + k + 1;
             d = 1.0;
           }
-          const double p = sqrt(d), ip = 1.0 / p;
-          const double lk = (lane == k) ? p : a[k] * ip;   // L(lane, k)
+          // 1 / sqrt(d) from the hardware estimate (v_rsq_f64) and two
+          // Newton steps y += y (1 - d y^2) / 2, each of which squares the
+          // relative error: 7 dependent operations on the pivot chain in
+          // place of a square root and a division
+          double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+          for (int it = 0; it < 2; ++it) y = fma(0.5 * y, fma(-d * y, y, 1.0), y);
+          ip[k] = y;
+          const double lk = (lane == k) ? d * y : a[k] * y;   // L(lane, k)
           a[k] = lk;
 #pragma unroll
           for (int j = k + 1; j < 16; ++j) a[j] = fma(-lk, lane_bcast(lk, j), a[j]);
@@ -210,71 +235,65 @@ __device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, do
           for (int j = 0; j < 16; ++j)
             if (j <= lane) A[c0 + lane][c0 + j] = a[j];
         }
-        diag_inverse(c0, a);
+        diag_inverse(c0, a, ip);
       }
       __syncthreads();
-      // panel rows below: L(r, c0 + c) = sum_{q <= c} S(r, c0 + q) T_jj(c, q)
-      const int pr = 48 - c0;
-      double pv[3];
+      // panel: L_Ij = S_Ij T_jj' for the blocks I > jb below the diagonal
+      // block, one per wave on FP64 MFMA (T_jj(c, q) = 0 for q > c); each
+      // wave overwrites only the block it read
+      if (jb + 1 + w < 4) {
+        const int I = jb + 1 + w;
+        const v4d pl = blk16<false>(A, 16 * I, c0, Tm, c0, c0, li, lg, v4d{0.0, 0.0, 0.0, 0.0});
 #pragma unroll
-      for (int e = 0; e < 3; ++e) {
-        const int idx = tid + 256 * e;
-        pv[e] = 0.0;
-        if (idx < pr * 16) {
-          const int r = c0 + 16 + (idx >> 4), c = idx & 15;
-          double s4[4] = {0.0, 0.0, 0.0, 0.0};   // T_jj(c, q) = 0 for q > c
-#pragma unroll
-          for (int q = 0; q < 16; ++q) s4[q & 3] = fma(A[r][c0 + q], Tm[c0 + c][c0 + q], s4[q & 3]);
-          pv[e] = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-        }
+        for (int v = 0; v < 4; ++v) A[16 * I + li][c0 + lg + 4 * v] = pl[v];
       }
       __syncthreads();
+      // trailing lower triangle: S_IJ -= L_Ij L_Jj' for jb < J <= I < 4, the
+      // blocks of the lower triangle dealt round the waves (only the lower
+      // half of a diagonal block is written)
+      {
+        const int m = 3 - jb;
+        for (int t = w; t < m * (m + 1) / 2; t += 4) {
+          int I = 0;
+          while ((I + 1) * (I + 2) / 2 <= t) ++I;
+          const int J = t - I * (I + 1) / 2 + jb + 1;
+          I += jb + 1;
+          const v4d u = blk16<false>(A, 16 * I, c0, A, 16 * J, c0, li, lg, v4d{0.0, 0.0, 0.0, 0.0});
 #pragma unroll
-      for (int e = 0; e < 3; ++e) {
-        const int idx = tid + 256 * e;
-        if (idx < pr * 16) A[c0 + 16 + (idx >> 4)][c0 + (idx & 15)] = pv[e];
-      }
-      __syncthreads();
-      // trailing lower triangle: S(r, c) -= sum_q L(r, c0 + q) L(c, c0 + q)
-      for (int idx = tid; idx < pr * pr; idx += 256) {
-        const int r = c0 + 16 + idx / pr, c = c0 + 16 + idx % pr;
-        if (c <= r) {
-          double s4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-          for (int q = 0; q < 16; ++q) s4[q & 3] = fma(A[r][c0 + q], A[c][c0 + q], s4[q & 3]);
-          A[r][c] -= (s4[0] + s4[1]) + (s4[2] + s4[3]);
+          for (int v = 0; v < 4; ++v) {
+            const int r = 16 * I + li, c = 16 * J + lg + 4 * v;
+            if (I != J || c <= r) A[r][c] -= u[v];
+          }
         }
       }
       __syncthreads();
     }
     if (tid == 0 && bad != 0 && *info == 0) *info = bad;
   } else {
-    double a[16];
+    double a[16], irr[16];
     const int c0 = 16 * w, rr = min(lane, 15);
 #pragma unroll
     for (int j = 0; j < 16; ++j) a[j] = A[c0 + rr][c0 + j];
-    diag_inverse(c0, a);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) irr[r] = 1.0 / lane_bcast(a[r], r);
+    diag_inverse(c0, a, irr);
     __syncthreads();
   }
 
-  // off-diagonal blocks of T, block row I at a time
+  // off-diagonal blocks of T, block row I at a time, wave J < I forming
+  // T_IJ on FP64 MFMA: X_J = sum_{J <= K < I} L_IK T_KJ, then T_IJ = -T_II X_J
   for (int I = 1; I < 4; ++I) {
-    for (int idx = tid; idx < I * 256; idx += 256) {
-      const int J = idx >> 8, r = (idx >> 4) & 15, c = idx & 15;
-      double s4[4] = {0.0, 0.0, 0.0, 0.0};   // T(kk, 16 J + c) = 0 for kk < 16 J + c
-      for (int k0 = 16 * J; k0 < 16 * I; k0 += 16) {
+    if (w < I) {
+      v4d x = {0.0, 0.0, 0.0, 0.0};
+      for (int K = w; K < I; ++K) x = blk16<true>(A, 16 * I, 16 * K, Tm, 16 * K, 16 * w, li, lg, x);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) s4[q & 3] = fma(A[16 * I + r][k0 + q], Tm[k0 + q][16 * J + c], s4[q & 3]);
-      }
-      X[J][r][c] = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+      for (int v = 0; v < 4; ++v) X[w][li][lg + 4 * v] = x[v];
     }
     __syncthreads();
-    for (int idx = tid; idx < I * 256; idx += 256) {
-      const int J = idx >> 8, r = (idx >> 4) & 15, c = idx & 15;
-      double s4[4] = {0.0, 0.0, 0.0, 0.0};   // T_II(r, q) = 0 for q > r
+    if (w < I) {
+      const v4d t = blk16<true>(Tm, 16 * I, 16 * I, X[w], 0, 0, li, lg, v4d{0.0, 0.0, 0.0, 0.0});
 #pragma unroll
-      for (int q = 0; q < 16; ++q) s4[q & 3] = fma(Tm[16 * I + r][16 * I + q], X[J][q][c], s4[q & 3]);
-      Tm[16 * I + r][16 * J + c] = -((s4[0] + s4[1]) + (s4[2] + s4[3]));
+      for (int v = 0; v < 4; ++v) Tm[16 * I + li][16 * w + lg + 4 * v] = -t[v];
     }
     __syncthreads();
   }
@@ -373,13 +392,37 @@ __global__ __launch_bounds__(256) void k_node128(int n, int o, double* __restric
   const int o2 = o + 64;
   const int nl1 = min(64, n - o), nl2 = min(64, n - o2);
 
+  // S21 (R-view lower-left, row-major) and this wave's quadrant of S22 are
+  // loaded before the first leaf, so that their latency overlaps the leaf's
+  // own load (nothing in this node writes them before they are used)
+  double s21[16], s22[2][2][4];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int idx = tid + 256 * e, r = idx >> 6, c = idx & 63;
+    s21[e] = Lb[(size_t)(o2 + r) * ld + o + c];
+  }
+  {
+    const int li = lane & 15, lg = lane >> 4;
+    const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1);
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = r0 + 16 * ti + li, c = c0 + 16 * tj + lg + 4 * v;
+          s22[ti][tj][v] = (r < nl2 && c <= r) ? Lb[(size_t)(o2 + r) * ld + o2 + c] : 0.0;
+        }
+  }
+
   leaf64<true, false>(n, o, Lb, Tb, ld, info, sh.leaf);
   __syncthreads();   // the leaf's last reads of Tm are done
-  // T11 as stored: rows >= n zero; S21 (R-view lower-left, row-major)
-  for (int idx = tid; idx < 4096; idx += 256) {
-    const int r = idx >> 6, c = idx & 63;
+  // T11 as stored: rows >= n zero; S21 into P
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int idx = tid + 256 * e, r = idx >> 6, c = idx & 63;
     if (r >= nl1) Tm[r][c] = 0.0;
-    sh.P[r][c] = Lb[(size_t)(o2 + r) * ld + o + c];
+    sh.P[r][c] = s21[e];
   }
   __syncthreads();
   // L21(r, c) = sum_{k <= c} S21(r, k) T11(c, k)
@@ -398,11 +441,21 @@ __global__ __launch_bounds__(256) void k_node128(int n, int o, double* __restric
   // S22(r, c) - sum_k L21(r, k) L21(c, k): the second leaf's input in A (its
   // lower triangle; rows >= n identity rows, as the leaf's own load)
   lds_product<false>(sh.P, sh.P, w, lane, acc);
-  for_quadrant(acc, w, lane, [&](int r, int c, double v) {
-    double a = (r == c) ? 1.0 : 0.0;
-    if (r < nl2 && c <= r) a = Lb[(size_t)(o2 + r) * ld + o2 + c] - v;
-    A[r][c] = a;
-  });
+  {
+    const int li = lane & 15, lg = lane >> 4;
+    const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1);
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = r0 + 16 * ti + li, c = c0 + 16 * tj + lg + 4 * v;
+          double a = (r == c) ? 1.0 : 0.0;
+          if (r < nl2 && c <= r) a = s22[ti][tj][v] - acc[ti][tj][v];
+          A[r][c] = a;
+        }
+  }
   __syncthreads();   // A complete; the first leaf's T11 in Tm is no longer needed
   leaf64<true, true>(n, o2, Lb, Tb, ld, info, sh.leaf);
   __syncthreads();
@@ -421,22 +474,82 @@ __global__ __launch_bounds__(256) void k_node128(int n, int o, double* __restric
 
 // ---------------------------------------------------------------------------
 // C = alpha A B (+ C with ACC) on an M x N block, K terms, FP64 MFMA, over
-// 128 x 128 tiles (2 x 2 waves of 64 x 64).  A(r, k) = A[r ld + k],
-// B(k, c) = B[c ld + k], C(r, c) = C[c ld + r]; M, N, K multiples of 64.
+// 128 x 128 tiles (2 x 2 waves of 64 x 64).  C(r, c) = C[c ld + r]; M, N, K
+// multiples of 64.  The operands are read along k, A(r, k) = A[r ld + k] and
+// B(k, c) = B[c ld + k], or across k (transposed storage: ATR, BTR):
+// A(r, k) = A[k ld + r], B(k, c) = B[k ld + c] (16 lanes read 16 consecutive
+// rows or columns of one k).
 // TRI selects the terms and tiles that can be non-zero:
 //   TG_BLOWER  B lower triangular (k >= c): k from the wave's first column;
 //   TG_BUPPER  B upper triangular (k <= c): k up to the wave's last column;
 //   TG_CUPPER  only the blocks with r <= c (one triangle of a symmetric C).
-// The triangular B is a diagonal block of a symmetric-stored buffer, so its
-// other triangle is not zero: inside the wave's 64 x 64 diagonal block the
-// terms of that triangle are masked.
+// The triangular B is a diagonal block of a buffer whose other triangle is
+// not zero: inside the wave's 64 x 64 diagonal block the terms of that
+// triangle are masked.
 // Grid: (row tiles, column tiles), heaviest column tiles first.  Operand
-// fetch and MFMA order as the triangular product of the block loop: one
-// 32-byte load per lane and operand per K = 16 step, one step ahead.
+// fetch and MFMA order as the triangular product of the block loop: per K = 16
+// step one 32-byte load per lane and operand (transposed: four 8-byte
+// loads), one step ahead.
 // ---------------------------------------------------------------------------
 enum { TG_BLOWER = 1, TG_BUPPER = 2, TG_CUPPER = 3 };
+#ifndef FGLS_TG_SK_BLOCKS
+#define FGLS_TG_SK_BLOCKS 4096
+#endif
 
-template <int TRI, bool ACC>
+// The operands of one wave: rows r0 + 16 t + li of A and columns c0 + 16 t +
+// li of B, lane (li, lg) holding k = k0 + 4 lg + s in element s.
+template <bool ATR, bool BTR>
+struct TgOperands {
+  const double* Ap[4];
+  const double* Bp[4];
+  int64_t ld;
+  __device__ __forceinline__ TgOperands(const double* A, const double* B, int64_t ld_, int r0, int c0, int li,
+                                        int lg)
+      : ld(ld_) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      Ap[t] = ATR ? A + (size_t)(4 * lg) * ld + r0 + 16 * t + li : A + (size_t)(r0 + 16 * t + li) * ld + 4 * lg;
+      Bp[t] = BTR ? B + (size_t)(4 * lg) * ld + c0 + 16 * t + li : B + (size_t)(c0 + 16 * t + li) * ld + 4 * lg;
+    }
+  }
+  template <bool TR>
+  __device__ __forceinline__ v4d get(const double* p, int k0) const {
+    if constexpr (TR) {
+      const double* q = p + (size_t)k0 * ld;
+      return v4d{q[0], q[ld], q[2 * ld], q[3 * ld]};
+    } else {
+      return ld4(p + k0);
+    }
+  }
+  __device__ __forceinline__ void load(int k0, v4d (&a)[4], v4d (&b)[4]) const {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) a[t] = get<ATR>(Ap[t], k0);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) b[t] = get<BTR>(Bp[t], k0);
+  }
+};
+
+// A wave's 64 x 64 block of alpha acc (+ C): element v of acc[ti][tj] is (row
+// r0 + 16 ti + li, column c0 + 16 tj + lg + 4 v), so 16 lanes write 16
+// consecutive rows of one column.
+template <bool ACC>
+__device__ __forceinline__ void tg_store(const v4d (&acc)[4][4], double alpha, double* __restrict__ C, int64_t ld,
+                                         int r0, int c0, int li, int lg) {
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double* Cc = C + (size_t)(c0 + 16 * tj + lg + 4 * v) * ld + r0 + li;
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) {
+        const double z = alpha * acc[ti][tj][v];
+        Cc[16 * ti] = ACC ? Cc[16 * ti] + z : z;
+      }
+    }
+  }
+}
+
+template <int TRI, bool ACC, bool ATR, bool BTR>
 __global__ __launch_bounds__(256) void k_tgemm(int M, int N, int K, double alpha,
                                                const double* __restrict__ A,
                                                const double* __restrict__ B,
@@ -454,13 +567,7 @@ __global__ __launch_bounds__(256) void k_tgemm(int M, int N, int K, double alpha
   // unmasked range [kb, ke); a triangular B adds the diagonal block [c0, c0 + 64)
   const int kb = (TRI == TG_BLOWER) ? c0 + 64 : 0;
   const int ke = (TRI == TG_BUPPER) ? c0 : K;
-
-  const double* Ap[4];
-  const double* Bp[4];
-#pragma unroll
-  for (int ti = 0; ti < 4; ++ti) Ap[ti] = A + (size_t)(r0 + 16 * ti + li) * ld + 4 * lg;
-#pragma unroll
-  for (int tj = 0; tj < 4; ++tj) Bp[tj] = B + (size_t)(c0 + 16 * tj + li) * ld + 4 * lg;
+  const TgOperands<ATR, BTR> op(A, B, ld, r0, c0, li, lg);
 
   v4d acc[4][4];
 #pragma unroll
@@ -481,11 +588,9 @@ __global__ __launch_bounds__(256) void k_tgemm(int M, int N, int K, double alpha
   auto diag_steps = [&]() {
 #pragma unroll
     for (int k0 = c0; k0 < c0 + 64; k0 += 16) {
-#pragma unroll
-      for (int ti = 0; ti < 4; ++ti) a[ti] = ld4(Ap[ti] + k0);
+      op.load(k0, a, b);
 #pragma unroll
       for (int tj = 0; tj < 4; ++tj) {
-        b[tj] = ld4(Bp[tj] + k0);
         const int c = c0 + 16 * tj + li;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -507,16 +612,10 @@ __global__ __launch_bounds__(256) void k_tgemm(int M, int N, int K, double alpha
     constexpr int DK = (TRI == TG_BUPPER) ? -16 : 16;
     const int kfirst = (TRI == TG_BUPPER) ? ke - 16 : kb;
     const int nst = (ke - kb) / 16;
-#pragma unroll
-    for (int ti = 0; ti < 4; ++ti) a[ti] = ld4(Ap[ti] + kfirst);
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj) b[tj] = ld4(Bp[tj] + kfirst);
+    op.load(kfirst, a, b);
     for (int st = 1, k0 = kfirst + DK; st < nst; ++st, k0 += DK) {
       v4d an[4], bn[4];
-#pragma unroll
-      for (int ti = 0; ti < 4; ++ti) an[ti] = ld4(Ap[ti] + k0);
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj) bn[tj] = ld4(Bp[tj] + k0);
+      op.load(k0, an, bn);
       mfma_step();
 #pragma unroll
       for (int ti = 0; ti < 4; ++ti) a[ti] = an[ti];
@@ -525,21 +624,7 @@ __global__ __launch_bounds__(256) void k_tgemm(int M, int N, int K, double alpha
     }
     mfma_step();
   }
-
-  // element v of acc[ti][tj] is (row r0+16ti+li, column c0+16tj+lg+4v): 16
-  // lanes write 16 consecutive rows of one column
-#pragma unroll
-  for (int tj = 0; tj < 4; ++tj) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      double* Cc = C + (size_t)(c0 + 16 * tj + lg + 4 * v) * ld + r0 + li;
-#pragma unroll
-      for (int ti = 0; ti < 4; ++ti) {
-        const double z = alpha * acc[ti][tj][v];
-        Cc[16 * ti] = ACC ? Cc[16 * ti] + z : z;
-      }
-    }
-  }
+  tg_store<ACC>(acc, alpha, C, ld, r0, c0, li, lg);
 }
 
 // The same product for small blocks, split over K: one workgroup per 64 x 64
@@ -547,7 +632,7 @@ __global__ __launch_bounds__(256) void k_tgemm(int M, int N, int K, double alpha
 // (in 16-steps) and their partial sums are added through LDS in a fixed
 // order.  Four times the waves of k_tgemm on the levels of the recursion
 // whose products have few 64 x 64 blocks.  Grid: (row blocks, column blocks).
-template <int TRI, bool ACC>
+template <int TRI, bool ACC, bool ATR, bool BTR>
 __global__ __launch_bounds__(256) void k_tgemm_sk(int M, int N, int K, double alpha,
                                                   const double* __restrict__ A,
                                                   const double* __restrict__ B,
@@ -562,13 +647,8 @@ __global__ __launch_bounds__(256) void k_tgemm_sk(int M, int N, int K, double al
   const int ke = (TRI == TG_BUPPER) ? min(c0 + 64, K) : K;
   const int q = (((ke - kb) / 16 + 3) / 4) * 16;      // this wave's share, whole 16-steps
   const int kbw = min(kb + w * q, ke), kew = min(kbw + q, ke);
+  const TgOperands<ATR, BTR> op(A, B, ld, r0, c0, li, lg);
 
-  const double* Ap[4];
-  const double* Bp[4];
-#pragma unroll
-  for (int ti = 0; ti < 4; ++ti) Ap[ti] = A + (size_t)(r0 + 16 * ti + li) * ld + 4 * lg;
-#pragma unroll
-  for (int tj = 0; tj < 4; ++tj) Bp[tj] = B + (size_t)(c0 + 16 * tj + li) * ld + 4 * lg;
   v4d acc[4][4];
 #pragma unroll
   for (int ti = 0; ti < 4; ++ti)
@@ -577,10 +657,7 @@ __global__ __launch_bounds__(256) void k_tgemm_sk(int M, int N, int K, double al
   v4d a[4], b[4];
   // operands of step k0; inside the diagonal block B's other triangle is masked
   auto load = [&](int k0, v4d (&av)[4], v4d (&bv)[4]) {
-#pragma unroll
-    for (int ti = 0; ti < 4; ++ti) av[ti] = ld4(Ap[ti] + k0);
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj) bv[tj] = ld4(Bp[tj] + k0);
+    op.load(k0, av, bv);
     if ((TRI == TG_BLOWER || TRI == TG_BUPPER) && k0 >= c0 && k0 < c0 + 64) {
 #pragma unroll
       for (int tj = 0; tj < 4; ++tj) {
@@ -637,19 +714,13 @@ __global__ __launch_bounds__(256) void k_tgemm_sk(int M, int N, int K, double al
     __syncthreads();
   }
   if (w != 0) return;
-  // element v of acc[ti][tj] is (row r0+16ti+li, column c0+16tj+lg+4v)
-#pragma unroll
-  for (int tj = 0; tj < 4; ++tj) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      double* Cc = C + (size_t)(c0 + 16 * tj + lg + 4 * v) * ld + r0 + li;
-#pragma unroll
-      for (int ti = 0; ti < 4; ++ti) {
-        const double z = alpha * acc[ti][tj][v];
-        Cc[16 * ti] = ACC ? Cc[16 * ti] + z : z;
-      }
-    }
-  }
+  tg_store<ACC>(acc, alpha, C, ld, r0, c0, li, lg);
+}
+
+// Zero the strict upper triangle of the row-major N x N matrix T.
+__global__ __launch_bounds__(256) void k_zero_upper(int N, double* __restrict__ T, int64_t ld) {
+  const int r = blockIdx.x;
+  for (int c = r + 1 + threadIdx.x; c < N; c += 256) T[(size_t)r * ld + c] = 0.0;
 }
 
 // dst(c, r) = src(r, c) for an R x C block (64-multiples): src(r, c) at
@@ -669,39 +740,27 @@ __global__ __launch_bounds__(256) void k_tcopy(const double* __restrict__ src,
   }
 }
 
-// Zero the strict upper triangle of the row-major N x N matrix T.
-__global__ __launch_bounds__(256) void k_zero_upper(int N, double* __restrict__ T, int64_t ld) {
-  const int r = blockIdx.x;
-  for (int c = r + 1 + threadIdx.x; c < N; c += 256) T[(size_t)r * ld + c] = 0.0;
-}
-
-void tgemm(int tri, int M, int N, int K, double alpha, const double* A, const double* B, double* C,
-           int64_t ld, hipStream_t st) {
-  const dim3 blk(256);
-  if ((M / 64) * (N / 64) <= 4096) {   // few 64 x 64 blocks: split K over the waves
-    const dim3 grid(M / 64, N / 64);
-    if (tri == TG_BLOWER)
-      hipLaunchKernelGGL((k_tgemm_sk<TG_BLOWER, false>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
-    else if (tri == TG_BUPPER)
-      hipLaunchKernelGGL((k_tgemm_sk<TG_BUPPER, false>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
-    else
-      hipLaunchKernelGGL((k_tgemm_sk<TG_CUPPER, true>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
-    return;
-  }
-  const dim3 grid((M + 127) / 128, (N + 127) / 128);
-  if (tri == TG_BLOWER)
-    hipLaunchKernelGGL((k_tgemm<TG_BLOWER, false>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
-  else if (tri == TG_BUPPER)
-    hipLaunchKernelGGL((k_tgemm<TG_BUPPER, false>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
-  else   // the symmetric update, accumulated into C
-    hipLaunchKernelGGL((k_tgemm<TG_CUPPER, true>), grid, blk, 0, st, M, N, K, alpha, A, B, C, ld);
-}
-
-void tcopy(const double* src, double* dst, int R, int C, int64_t ld, hipStream_t st) {
-  hipLaunchKernelGGL(k_tcopy, dim3(R / 64, C / 64), dim3(256), 0, st, src, dst, ld);
+template <int TRI, bool ACC, bool ATR, bool BTR>
+void tgemm_launch(int M, int N, int K, double alpha, const double* A, const double* B, double* C, int64_t ld,
+                  hipStream_t st) {
+  // 64 x 64 blocks that hold work (the symmetric update forms one triangle)
+  const long blocks = (TRI == TG_CUPPER) ? (long)(M / 64) * (M / 64 + 1) / 2 : (long)(M / 64) * (N / 64);
+  if (blocks <= FGLS_TG_SK_BLOCKS)   // few 64 x 64 blocks: split K over the waves
+    hipLaunchKernelGGL((k_tgemm_sk<TRI, ACC, ATR, BTR>), dim3(M / 64, N / 64), dim3(256), 0, st, M, N, K, alpha,
+                       A, B, C, ld);
+  else
+    hipLaunchKernelGGL((k_tgemm<TRI, ACC, ATR, BTR>), dim3((M + 127) / 128, (N + 127) / 128), dim3(256), 0, st,
+                       M, N, K, alpha, A, B, C, ld);
 }
 
 // One node of the recursion on rows/columns [o, o + size) (see the top).
+// The products: L21 = S21 T11' and T21' = -X' T22' (B upper, operands along
+// k), X = L21 T11 (B lower, T11 read across k from its row-major lower
+// triangle) and S22 -= L21 L21' (the symmetric update, accumulated).  L21 is
+// read from its column-major copy across k on the small nodes; on the nodes
+// whose products fill the device (more than FGLS_TG_SK_BLOCKS 64 x 64 blocks)
+// it is first copied row-major over S21 and read along k, which is faster
+// there than the transposed reads.
 void chol_inv_node(int n, int o, int size, bool factor, double* L, double* T, int64_t ld,
                    int* info, hipStream_t st) {
   if (size == 64) {
@@ -715,24 +774,25 @@ void chol_inv_node(int n, int o, int size, bool factor, double* L, double* T, in
   const int s = (size <= 128) ? 64 : ((size / 2 + 127) / 128) * 128;
   const int s2 = size - s;
   chol_inv_node(n, o, s, factor, L, T, ld, info, st);
-  double* L21r = L + (size_t)(o + s) * ld + o;   // R-view lower-left: S21, then L21 row-major
+  double* L21r = L + (size_t)(o + s) * ld + o;   // R-view lower-left: S21 (then L21 row-major)
   double* L21c = L + (size_t)o * ld + o + s;     // R-view upper-right: L21 column-major
   const double* T11 = T + (size_t)o * ld + o;
   const double* T22 = T + (size_t)(o + s) * ld + o + s;
-  double* Xc = T + (size_t)o * ld + o + s;       // R-view upper-right: X, then T21 column-major
+  double* Xc = T + (size_t)o * ld + o + s;       // R-view upper-right: X column-major
   double* T21r = T + (size_t)(o + s) * ld + o;   // R-view lower-left: T21 row-major
+  const bool big = (long)(s2 / 64) * (s / 64) > FGLS_TG_SK_BLOCKS;
+  if (factor)
+    tgemm_launch<TG_BUPPER, false, false, false>(s2, s, s, 1.0, L21r, T11, L21c, ld, st);   // L21 = S21 T11'
+  if (big) hipLaunchKernelGGL(k_tcopy, dim3(s / 64, s2 / 64), dim3(256), 0, st, L21c, L21r, ld);
   if (factor) {
-    tgemm(TG_BUPPER, s2, s, s, 1.0, L21r, T11, L21c, ld, st);        // L21 = S21 T11'
-    tcopy(L21c, L21r, s, s2, ld, st);
-    tgemm(TG_CUPPER, s2, s2, s, -1.0, L21r, L21r, L + (size_t)(o + s) * ld + o + s, ld,
-          st);                                                               // S22 -= L21 L21'
-  } else {
-    tcopy(L21c, L21r, s, s2, ld, st);
+    double* S22 = L + (size_t)(o + s) * ld + o + s;   // S22 -= L21 L21'
+    if (big) tgemm_launch<TG_CUPPER, true, false, false>(s2, s2, s, -1.0, L21r, L21r, S22, ld, st);
+    else tgemm_launch<TG_CUPPER, true, true, true>(s2, s2, s, -1.0, L21c, L21c, S22, ld, st);
   }
   chol_inv_node(n, o + s, s2, factor, L, T, ld, info, st);
-  tgemm(TG_BLOWER, s2, s, s, 1.0, L21r, T11, Xc, ld, st);            // X = L21 T11
-  tgemm(TG_BUPPER, s, s2, s2, -1.0, Xc, T22, T21r, ld, st);          // T21' = -X' T22'
-  tcopy(T21r, Xc, s2, s, ld, st);
+  if (big) tgemm_launch<TG_BLOWER, false, false, true>(s2, s, s, 1.0, L21r, T11, Xc, ld, st);   // X = L21 T11
+  else tgemm_launch<TG_BLOWER, false, true, true>(s2, s, s, 1.0, L21c, T11, Xc, ld, st);
+  tgemm_launch<TG_BUPPER, false, false, false>(s, s2, s2, -1.0, Xc, T22, T21r, ld, st);   // T21' = -X' T22'
 }
 
 int factor_on_device(fgls_hip_ctx* c, const double* dPhi, int64_t ldphi, double h2, double sigma2,
