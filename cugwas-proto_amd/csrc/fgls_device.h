@@ -17,7 +17,7 @@ constexpr int FGLS_NSLICE = 6;
 // largest n for which the int32 sums stay exact: 127 * 127 * n < 2^31.
 constexpr int FGLS_SLICE_RADIX = 254;
 constexpr int FGLS_I8_MAX_N = 133000;
-// Five-digit form of a triangular L^-1 (k_slice_linv_top): row i scaled by
+// Five-digit form of a triangular L^-1 (k_slice_tri, five digits): row i scaled by
 // its largest entry outside its diagonal 64-k step, five digits everywhere
 // and one more (top) digit on the diagonal step.  Used when its error bound,
 // at most FGLS_TOP_MAX_LOSS times the six-digit one, allows, and for
@@ -124,8 +124,8 @@ void fgls_launch_tile_i8(int n, int count, int planes, const int8_t* X8, int64_t
 // x = (128 hi + lo) * inv_xscale).  eig_ntr > 0: A8
 // holds a full matrix and W has eig_ntr groups of p+1 weight columns (the
 // eigenbasis variant; part then holds eig_ntr (p+1) sums per SNP and tile).
-// nslice: the digits of a triangular A8, FGLS_NSLICE (k_slice_linv) or
-// FGLS_TOP_NSLICE (k_slice_linv_top: five digits, the top digits at
+// nslice: the digits of a triangular A8, FGLS_NSLICE (k_slice_tri, six digits) or
+// FGLS_TOP_NSLICE (five digits, the top digits at
 // fgls_a8top_base); a full A8 (eig_ntr > 0) always has FGLS_NSLICE.
 void fgls_launch_trmm_i8(int p, int n, int count, int RT32, int KS, const int8_t* A8,
                          const double* inv_scale, const int8_t* X8, int64_t ldx8, const double* W,

@@ -8,12 +8,12 @@
 // the allocator copies accumulators through scratch AGPRs around the MFMAs
 // once more than a few dozen of them are live).
 //
-// Slice representation (built by k_slice_linv in fgls_kernels.hip): row i of
+// Slice representation (built by k_slice_tri in fgls_kernels.hip): row i of
 // L^-1 times sc_i = 127 / (row maximum) is sum_s a_s R^(-s), R = 254,
 // a_s in [-127, 127], stored in 1 KB blocks (fgls_a8_offset).  For integer
 // genotypes x, (A_s x)_i is an exact int32 (|a x| <= 127*127 per term, exact
 // for n <= 133,000), and z_i = sum_s R^(-s) (A_s x)_i / sc_i.  The
-// five-digit form (k_slice_linv_top, TOP below) scales row i by its largest
+// five-digit form (k_slice_tri with TOP; TOP below) scales row i by its largest
 // entry outside its diagonal 64-k step instead, keeps five digits, and adds
 // a top digit t (weight R) on that step only: z_i = (R (T x)_i +
 // sum_{s<5} R^(-s) (A_s x)_i) / sc_i.

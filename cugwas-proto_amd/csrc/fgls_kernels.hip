@@ -553,66 +553,95 @@ __global__ __launch_bounds__(256) void k_slice_linv(int ldn, const double* __res
 // the rows goes to *loss_bits (non-negative doubles order as their bits), and
 // the host keeps this form only while it is at most FGLS_TOP_MAX_LOSS.  The
 // top digits of a 16-row group's diagonal step go to A8top
-// (fgls_a8top_offset), its five digits to A8 like every other step.
-template <int S>
-__global__ __launch_bounds__(256) void k_slice_linv_top(int ldn, const double* __restrict__ Linv,
-                                                        int8_t* __restrict__ A8, int8_t* __restrict__ A8top,
-                                                        double* __restrict__ inv_scale,
-                                                        unsigned long long* __restrict__ loss_bits) {
-  __shared__ double red[2][4];
-  const int row = blockIdx.x, tid = threadIdx.x;
-  const int d0 = row & ~63;   // the diagonal step's first column
-  const double* Lr = Linv + (size_t)row * ldn;
-  double mx = 0.0, off = 0.0;
-  for (int k = tid; k < ldn; k += 256) {
-    const double a = fabs(Lr[k]);
-    mx = fmax(mx, a);
-    if (k < d0 || k >= d0 + 64) off = fmax(off, a);
-  }
-  mx = wave_max(mx);
-  off = wave_max(off);
-  if ((tid & 63) == 0) {
-    red[0][tid >> 6] = mx;
-    red[1][tid >> 6] = off;
+// (fgls_a8top_offset), its five digits to A8 like every other step
+// (k_slice_tri with TOP).
+// The slices of the triangular L^-1 (row-major, ld ldn, zero above the
+// diagonal) in either form: the digits of k_slice_linv (S = 6, TOP false)
+// or the five-digit form above (S = 5, TOP), for the steps the triangular int8
+// product reads: a 16-row group's 64-k steps up to its diagonal step (the
+// later ones are neither read nor written).  One workgroup per 16-row group.
+// Phase 1: wave w takes rows 4 w .. 4 w + 3, its lanes along k < the end of
+// the diagonal step, for the row maximum (TOP: and the maximum before the
+// diagonal step).  Phase 2: wave w forms the steps w, w + 4, ...; lane
+// 16 q + r takes row r and the 16 k's of chunk 4 step + q, so that the wave
+// writes each digit's 1 KB block of the step contiguously.
+template <int S, bool TOP>
+__global__ __launch_bounds__(256) void k_slice_tri(int ldn, const double* __restrict__ Linv,
+                                                   int8_t* __restrict__ A8, int8_t* __restrict__ A8top,
+                                                   double* __restrict__ inv_scale,
+                                                   unsigned long long* __restrict__ loss_bits) {
+  __shared__ double shsc[16];
+  constexpr double R = (double)FGLS_SLICE_RADIX;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int row0 = 16 * (int)blockIdx.x;
+  const int d0 = row0 & ~63;     // the group's diagonal step
+  const int nst = d0 / 64 + 1;   // steps 0 .. diagonal
+  const int KS = ldn / 64;
+  for (int i = 0; i < 4; ++i) {
+    const int r = 4 * w + i;
+    const double* Lr = Linv + (size_t)(row0 + r) * ldn;
+    double mx = 0.0, off = 0.0;
+    for (int k = lane; k < d0 + 64; k += 64) {
+      const double a = fabs(Lr[k]);
+      mx = fmax(mx, a);
+      if (TOP && k < d0) off = fmax(off, a);
+    }
+    mx = wave_max(mx);
+    if (TOP) off = wave_max(off);
+    if (lane == 0) {
+      double sc, isc;
+      if constexpr (TOP) {
+        sc = 1.0;
+        if (mx > 0.0) {
+          sc = 127.0 * R / mx;
+          if (off > 0.0) sc = fmin(sc, 127.0 / off);
+        }
+        isc = (mx > 0.0) ? 1.0 / sc : 0.0;
+        const double loss = (mx > 0.0) ? R * off / mx : 0.0;
+        atomicMax(loss_bits, (unsigned long long)__double_as_longlong(loss));
+      } else {
+        sc = (mx > 0.0) ? 127.0 / mx : 1.0;   // |L^-1_ik| sc <= 127
+        isc = (mx > 0.0) ? mx / 127.0 : 0.0;
+      }
+      shsc[r] = sc;
+      inv_scale[row0 + r] = isc;
+    }
   }
   __syncthreads();
-  mx = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
-  off = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
-  constexpr double R = (double)FGLS_SLICE_RADIX;
-  double sc = 1.0;
-  if (mx > 0.0) {
-    sc = 127.0 * R / mx;
-    if (off > 0.0) sc = fmin(sc, 127.0 / off);
-  }
-  if (tid == 0) {
-    inv_scale[row] = (mx > 0.0) ? 1.0 / sc : 0.0;
-    const double loss = (mx > 0.0) ? R * off / mx : 0.0;
-    atomicMax(loss_bits, (unsigned long long)__double_as_longlong(loss));
-  }
-  const int KC = ldn / 16;
-  for (int kc = tid; kc < KC; kc += 256) {
-    const bool diag = (kc >> 2) == (row >> 6);
+  const int r = lane & 15, q = lane >> 4;
+  const int row = row0 + r;
+  const double sc = shsc[r];
+  const double* Lr = Linv + (size_t)row * ldn;
+  for (int st = w; st < nst; st += 4) {
+    const int kc = 4 * st + q;
+    const bool diag = TOP && st == nst - 1;
+    double v[16];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const v4d x = *reinterpret_cast<const v4d*>(Lr + 16 * kc + 4 * j);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[4 * j + e] = x[e];
+    }
     int8_t out[S][16], top[16];
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
-      double r = Lr[kc * 16 + b] * sc;
+      double x = v[b] * sc;
       double t = 0.0;
       if (diag) {
-        t = fmin(fmax(rint(r / R), -127.0), 127.0);
-        r -= R * t;
+        t = fmin(fmax(rint(x / R), -127.0), 127.0);
+        x -= R * t;
       }
       top[b] = (int8_t)(int)t;
 #pragma unroll
       for (int s2 = 0; s2 < S; ++s2) {
-        const double a = fmin(fmax(rint(r), -127.0), 127.0);
+        const double a = fmin(fmax(rint(x), -127.0), 127.0);
         out[s2][b] = (int8_t)(int)a;
-        r = (r - a) * R;
+        x = (x - a) * R;
       }
     }
 #pragma unroll
     for (int s2 = 0; s2 < S; ++s2)
-      *reinterpret_cast<v4i*>(A8 + fgls_a8_offset(row, s2, kc, S, ldn / 64)) =
-          *reinterpret_cast<const v4i*>(out[s2]);
+      *reinterpret_cast<v4i*>(A8 + fgls_a8_offset(row, s2, kc, S, KS)) = *reinterpret_cast<const v4i*>(out[s2]);
     if (diag) *reinterpret_cast<v4i*>(A8top + fgls_a8top_offset(row, kc)) = *reinterpret_cast<const v4i*>(top);
   }
 }
@@ -1157,9 +1186,9 @@ void launch_trmm_reduce(const fgls_hip_ctx* c, const double* X, int64_t ldx, int
 constexpr int NSLICE = FGLS_NSLICE;   // int8 digit slices of L^-1 in the mixed mode
 
 // The slices of L^-1 in the form the context uses: the five-digit form
-// (k_slice_linv_top) when the slice form is automatic, n admits it and its
+// (k_slice_tri with TOP) when the slice form is automatic, n admits it and its
 // error bound over the six-digit one (the loss) is at most
-// FGLS_TOP_MAX_LOSS; otherwise six digits (k_slice_linv).  Synchronizes `st`
+// FGLS_TOP_MAX_LOSS; otherwise six digits.  Synchronizes `st`
 // (the loss is read back) when it tries the five-digit form.
 int build_slices(fgls_hip_ctx* c, hipStream_t st) {
   c->nslice = NSLICE;
@@ -1167,8 +1196,8 @@ int build_slices(fgls_hip_ctx* c, hipStream_t st) {
   if (c->slice_form == FGLS_SLICES_AUTO && c->n <= FGLS_I8_TOP_MAX_N) {
     unsigned long long* lb = reinterpret_cast<unsigned long long*>(c->invsc + c->ldn);
     HIP_TRY(hipMemsetAsync(lb, 0, sizeof(*lb), st));
-    hipLaunchKernelGGL((k_slice_linv_top<FGLS_TOP_NSLICE>), dim3(c->ldn), dim3(256), 0, st, c->ldn, c->Linv,
-                       c->A8, c->A8 + fgls_a8top_base(c->ldn / 64), c->invsc, lb);
+    hipLaunchKernelGGL((k_slice_tri<FGLS_TOP_NSLICE, true>), dim3(c->ldn / 16), dim3(256), 0, st, c->ldn,
+                       c->Linv, c->A8, c->A8 + fgls_a8top_base(c->ldn / 64), c->invsc, lb);
     HIP_TRY(hipGetLastError());
     double loss = 0.0;
     HIP_TRY(hipMemcpyAsync(&loss, lb, sizeof(loss), hipMemcpyDeviceToHost, st));
@@ -1179,8 +1208,8 @@ int build_slices(fgls_hip_ctx* c, hipStream_t st) {
       return FGLS_OK;
     }
   }
-  hipLaunchKernelGGL((k_slice_linv<NSLICE>), dim3(c->ldn), dim3(256), 0, st, c->ldn, c->Linv, c->A8,
-                     c->invsc);
+  hipLaunchKernelGGL((k_slice_tri<NSLICE, false>), dim3(c->ldn / 16), dim3(256), 0, st, c->ldn, c->Linv, c->A8,
+                     (int8_t*)nullptr, c->invsc, (unsigned long long*)nullptr);
   HIP_TRY(hipGetLastError());
   return FGLS_OK;
 }

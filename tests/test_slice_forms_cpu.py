@@ -1,7 +1,7 @@
 """The arithmetic of the two digit forms of L^-1 used by the mixed path,
-restated in numpy on the CPU (the device kernels k_slice_linv and
-k_slice_linv_top follow the same steps; their outputs are checked on the
-GPU against the oracle in test_gpu_slice_forms.py).
+restated in numpy on the CPU (the device kernel k_slice_tri follows the same
+steps for both forms; its outputs are checked on the GPU against the oracle
+in test_gpu_slice_forms.py).
 
 Six digits: row i scaled by sc_i = 127 / max_i, balanced radix-254 digits
 a_s = rint(r_s), r_{s+1} = 254 (r_s - a_s).  Five digits with a top digit:
