@@ -64,39 +64,25 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 
 // ---------------------------------------------------------------------------
-// Inverses of the 64x64 diagonal blocks of L, in LDS, column-parallel
-// forward substitution.  Block I covers rows/cols [64I, 64I+64); rows beyond
-// n are completed with the identity so that the padded solve is harmless.
-// One wave per block; Dinv block I is stored column-major at Dinv + I*4096.
+// Inverses of the 64x64 diagonal blocks of L: the diagonal blocks of
+// T = L^-1 (row-major lower triangle of Linv, zero above), copied into Dinv
+// (block I column-major at Dinv + I*4096); rows beyond n get the identity so
+// that the padded solve is harmless.  One workgroup per block.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_diag_inv(int n, const double* __restrict__ L,
-                                                 int64_t ldl, double* __restrict__ Dinv) {
-  __shared__ double Lb[NB][NB + 1];
-  __shared__ double Db[NB][NB + 1];
-  const int I = blockIdx.x, t = threadIdx.x;
-  const int base = I * NB;
-  const int r = base + t;
-  for (int c = 0; c < NB; ++c) {
-    double v;
-    if (c > t) v = 0.0;
-    else if (r < n) v = L[(size_t)(base + c) * ldl + r];
-    else v = (c == t) ? 1.0 : 0.0;
-    Lb[t][c] = v;
+__global__ __launch_bounds__(256) void k_diag_copy(int n, const double* __restrict__ Linv, int64_t ldli,
+                                                   double* __restrict__ Dinv) {
+  __shared__ double t[NB][NB + 1];
+  const int base = blockIdx.x * NB;
+  for (int idx = threadIdx.x; idx < NB * NB; idx += 256) {
+    const int r = idx / NB, c = idx % NB;
+    t[r][c] = (base + r < n) ? Linv[(size_t)(base + r) * ldli + base + c] : (r == c ? 1.0 : 0.0);
   }
   __syncthreads();
-  // thread t owns column t of the inverse: Lb * d = e_t
-  for (int row = 0; row < NB; ++row) {
-    double d = 0.0;
-    if (row >= t) {
-      double s = (row == t) ? 1.0 : 0.0;
-      for (int q = t; q < row; ++q) s -= Lb[row][q] * Db[q][t];
-      d = s / Lb[row][row];
-    }
-    Db[row][t] = d;
+  double* out = Dinv + (size_t)blockIdx.x * NB * NB;
+  for (int idx = threadIdx.x; idx < NB * NB; idx += 256) {
+    const int c = idx / NB, r = idx % NB;
+    out[(size_t)c * NB + r] = t[r][c];
   }
-  __syncthreads();
-  double* out = Dinv + (size_t)I * NB * NB;
-  for (int c = 0; c < NB; ++c) out[(size_t)c * NB + t] = Db[t][c];
 }
 
 // ---------------------------------------------------------------------------
@@ -437,10 +423,12 @@ __global__ __launch_bounds__(256) void k_trmm(int n, int count, int RT,
 // ---------------------------------------------------------------------------
 // Skinny triangular product Z = Linv X for q <= 16 columns (the trait's
 // [X~_L, y~]): bound by reading the lower triangle of Linv (row-major, ld
-// ldli, strict upper triangle zero) once from HBM.  One wave per 4
-// consecutive rows, lanes striding k; every row r < RT*16 of Z is written
-// with the sum over k < min(last row of the wave + 1, n), reduced across the
-// wave in a fixed order.  Grid: RT blocks of 16 rows, 256 threads.
+// ldli, strict upper triangle zero) once from HBM.  One workgroup per 4
+// consecutive rows, its 4 waves taking interleaved 64-column slices of the
+// k range (so that the longest rows are split four ways); every row
+// r < gridDim.x * 4 of Z is written with the sum over k < min(last row of the
+// group + 1, n), reduced across each wave and then over the waves in a fixed
+// order.  Grid: ldn / 4 blocks, 256 threads.
 // ---------------------------------------------------------------------------
 constexpr int TRMV_ROWS = 4;
 constexpr int TRMV_QMAX = 16;
@@ -448,15 +436,16 @@ constexpr int TRMV_QMAX = 16;
 __global__ __launch_bounds__(256) void k_trmv_lower(int n, int q, const double* __restrict__ Linv,
                                                     int64_t ldli, const double* __restrict__ X, int64_t ldx,
                                                     double* __restrict__ Z, int64_t ldz) {
-  const int lane = threadIdx.x & 63;
-  const int r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * TRMV_ROWS;
+  __shared__ double red[4][TRMV_ROWS][TRMV_QMAX];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r0 = blockIdx.x * TRMV_ROWS;
   const int kend = min(r0 + TRMV_ROWS, n);
   double acc[TRMV_ROWS][TRMV_QMAX];
 #pragma unroll
   for (int i = 0; i < TRMV_ROWS; ++i)
 #pragma unroll
     for (int j = 0; j < TRMV_QMAX; ++j) acc[i][j] = 0.0;
-  for (int k = lane; k < kend; k += 64) {
+  for (int k = threadIdx.x; k < kend; k += 256) {
     double a[TRMV_ROWS];
 #pragma unroll
     for (int i = 0; i < TRMV_ROWS; ++i) a[i] = Linv[(size_t)(r0 + i) * ldli + k];
@@ -475,10 +464,14 @@ __global__ __launch_bounds__(256) void k_trmv_lower(int n, int q, const double* 
 #pragma unroll
       for (int i = 0; i < TRMV_ROWS; ++i) {
         const double s = wave_sum(acc[i][j]);
-        if (lane == i) Z[(size_t)j * ldz + r0 + i] = s;
+        if (lane == 0) red[w][i][j] = s;
       }
     }
   }
+  __syncthreads();
+  const int t = threadIdx.x, i = t / TRMV_QMAX, j = t % TRMV_QMAX;
+  if (t < TRMV_ROWS * TRMV_QMAX && j < q)
+    Z[(size_t)j * ldz + r0 + i] = ((red[0][i][j] + red[1][i][j]) + red[2][i][j]) + red[3][i][j];
 }
 
 // ---------------------------------------------------------------------------
@@ -1382,13 +1375,14 @@ int finish_trait(fgls_hip_ctx* c, const double* dXL, int64_t ldxl, const double*
                          hipMemcpyDeviceToDevice, st));
   // L^-1 (already formed with the device factor)
   if (!potrf_info && (rc = chol_inv(c, false, st))) return rc;
-  // inverses of L's 64 x 64 diagonal blocks, for fgls_hip_trsm
-  hipLaunchKernelGGL(k_diag_inv, dim3(c->nblk), dim3(64), 0, st, n, c->L, (int64_t)ldn, c->Dinv);
+  // inverses of L's 64 x 64 diagonal blocks, for fgls_hip_trsm: the diagonal
+  // blocks of L^-1
+  hipLaunchKernelGGL(k_diag_copy, dim3(c->nblk), dim3(256), 0, st, n, c->Linv, (int64_t)ldn, c->Dinv);
   // [X~_L | y~] = L^-1 [X_L | y] through the explicit inverse: wXL + 1 columns,
   // one pass over the triangle of L^-1 with every row independent (a TRSM
   // would run the n rows one after another)
   static_assert(PMAX <= TRMV_QMAX, "the skinny product holds p columns");
-  hipLaunchKernelGGL(k_trmv_lower, dim3(ldn / (4 * TRMV_ROWS)), dim3(256), 0, st, n, wXL + 1, c->Linv,
+  hipLaunchKernelGGL(k_trmv_lower, dim3(ldn / TRMV_ROWS), dim3(256), 0, st, n, wXL + 1, c->Linv,
                      (int64_t)ldn, c->Wtmp, (int64_t)ldn, c->W, (int64_t)ldn);
   hipLaunchKernelGGL(k_trait_stats, dim3(1), dim3(256), 0, st, n, wXL, c->W, (int64_t)ldn, c->trait,
                      c->status);

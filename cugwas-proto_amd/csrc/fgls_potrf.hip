@@ -70,12 +70,13 @@ __device__ __forceinline__ v4d ld4(const double* p) {
   return *reinterpret_cast<const v4d*>(p);
 }
 
-// M (ldn x ldn, column-major) = alpha Phi + beta I on the n x n block, zero
-// elsewhere (fgls_chol_gpu.c:406-414: alpha = sigma2 h2, beta = sigma2 (1-h2)),
-// from the lower triangle of Phi only, written to both triangles of M (the
-// recursion reads its row-major lower triangle).  One workgroup per 64 x 64
-// block (I, J) with I >= J: the block is read once (coalesced down the
-// columns), written in place and, transposed through LDS, at (J, I).
+// M = alpha Phi + beta I on the n x n block, zero elsewhere
+// (fgls_chol_gpu.c:406-414: alpha = sigma2 h2, beta = sigma2 (1-h2)), from the
+// lower triangle of Phi only, written to the lower triangle of the ldn x ldn
+// buffer read row-major, which is what the recursion reads (it overwrites
+// the other triangle with the column-major factor).  One workgroup per
+// 64 x 64 block (I, J) with I >= J: the block is read once (coalesced down
+// the columns of Phi) and written transposed through LDS.
 __global__ __launch_bounds__(256) void k_build_M(int n, int ldn, const double* __restrict__ Phi,
                                                  int64_t ldphi, double alpha, double beta,
                                                  double* __restrict__ M) {
@@ -91,13 +92,12 @@ __global__ __launch_bounds__(256) void k_build_M(int n, int ldn, const double* _
     if (r < n && c < n && (!diag || rr >= cc))
       v = alpha * Phi[(size_t)c * ldphi + r] + (r == c ? beta : 0.0);
     t[rr][cc] = v;
-    if (!diag || rr >= cc) M[(size_t)c * ldn + r] = v;
   }
   __syncthreads();
-  // M(c, r) = M(r, c): block (J, I), the strict upper half of a diagonal block
+  // M(r, c) at M[r ldn + c], r >= c
   for (int idx = threadIdx.x; idx < 4096; idx += 256) {
     const int rr = idx >> 6, cc = idx & 63;
-    if (!diag || rr > cc) M[(size_t)(r0 + rr) * ldn + c0 + cc] = t[rr][cc];
+    if (!diag || rr >= cc) M[(size_t)(r0 + rr) * ldn + c0 + cc] = t[rr][cc];
   }
 }
 
@@ -495,6 +495,9 @@ enum { TG_BLOWER = 1, TG_BUPPER = 2, TG_CUPPER = 3 };
 #ifndef FGLS_TG_SK_BLOCKS
 #define FGLS_TG_SK_BLOCKS 4096
 #endif
+#ifndef FGLS_TG_COPY_BLOCKS
+#define FGLS_TG_COPY_BLOCKS 1024
+#endif
 
 // The operands of one wave: rows r0 + 16 t + li of A and columns c0 + 16 t +
 // li of B, lane (li, lg) holding k = k0 + 4 lg + s in element s.
@@ -758,9 +761,9 @@ void tgemm_launch(int M, int N, int K, double alpha, const double* A, const doub
 // k), X = L21 T11 (B lower, T11 read across k from its row-major lower
 // triangle) and S22 -= L21 L21' (the symmetric update, accumulated).  L21 is
 // read from its column-major copy across k on the small nodes; on the nodes
-// whose products fill the device (more than FGLS_TG_SK_BLOCKS 64 x 64 blocks)
-// it is first copied row-major over S21 and read along k, which is faster
-// there than the transposed reads.
+// with more than FGLS_TG_COPY_BLOCKS 64 x 64 blocks of L21 it is first copied
+// row-major over S21 and read along k, which is faster there than the
+// transposed reads (n = 10,000: the top two levels).
 void chol_inv_node(int n, int o, int size, bool factor, double* L, double* T, int64_t ld,
                    int* info, hipStream_t st) {
   if (size == 64) {
@@ -780,7 +783,7 @@ void chol_inv_node(int n, int o, int size, bool factor, double* L, double* T, in
   const double* T22 = T + (size_t)(o + s) * ld + o + s;
   double* Xc = T + (size_t)o * ld + o + s;       // R-view upper-right: X column-major
   double* T21r = T + (size_t)(o + s) * ld + o;   // R-view lower-left: T21 row-major
-  const bool big = (long)(s2 / 64) * (s / 64) > FGLS_TG_SK_BLOCKS;
+  const bool big = (long)(s2 / 64) * (s / 64) > FGLS_TG_COPY_BLOCKS;
   if (factor)
     tgemm_launch<TG_BUPPER, false, false, false>(s2, s, s, 1.0, L21r, T11, L21c, ld, st);   // L21 = S21 T11'
   if (big) hipLaunchKernelGGL(k_tcopy, dim3(s / 64, s2 / 64), dim3(256), 0, st, L21c, L21r, ld);
