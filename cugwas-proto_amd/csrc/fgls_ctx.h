@@ -46,6 +46,11 @@ struct fgls_hip_ctx {
   fgls_adjoint_state* adj;  // lazily allocated (adjoint)
   unsigned long long* err;
   int* status;
+  // the factor recursion's second stream (the small nodes' X = L21 T11 runs
+  // there beside the node's right child) and its fork / join events;
+  // created on first use
+  hipStream_t side;
+  hipEvent_t ev_fork, ev_join;
 };
 
 // Calls FN<p>(...) for p = 1..16.

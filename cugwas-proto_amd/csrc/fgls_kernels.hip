@@ -1485,6 +1485,9 @@ int fgls_hip_ctx_destroy(fgls_hip_ctx* c) {
   for (void* p : ptrs)
     if (p) hipFree(p);
   fgls_adjoint_free(c->adj);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->side) hipStreamDestroy(c->side);
   delete c;
   return FGLS_OK;
 }

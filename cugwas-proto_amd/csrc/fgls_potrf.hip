@@ -756,6 +756,21 @@ void tgemm_launch(int M, int N, int K, double alpha, const double* A, const doub
                        M, N, K, alpha, A, B, C, ld);
 }
 
+// The second stream of the recursion: on the nodes of at most
+// FGLS_SIDE_MAX_SIZE rows, X = L21 T11 needs only L21 and T11, so it runs on
+// `s` (forked after the L21 product) while the main stream recurses into
+// S22, and the main stream joins before T21' = -X' T22'.  One fork and one
+// join event serve every node: each wait is enqueued right after the record
+// it needs, and a later record of `join` follows every earlier X on the
+// in-order side stream.
+struct Side {
+  hipStream_t s;
+  hipEvent_t fork, join;
+};
+#ifndef FGLS_SIDE_MAX_SIZE
+#define FGLS_SIDE_MAX_SIZE 5120
+#endif
+
 // One node of the recursion on rows/columns [o, o + size) (see the top).
 // The products: L21 = S21 T11' and T21' = -X' T22' (B upper, operands along
 // k), X = L21 T11 (B lower, T11 read across k from its row-major lower
@@ -764,19 +779,20 @@ void tgemm_launch(int M, int N, int K, double alpha, const double* A, const doub
 // with more than FGLS_TG_COPY_BLOCKS 64 x 64 blocks of L21 it is first copied
 // row-major over S21 and read along k, which is faster there than the
 // transposed reads (n = 10,000: the top two levels).
-void chol_inv_node(int n, int o, int size, bool factor, double* L, double* T, int64_t ld,
-                   int* info, hipStream_t st) {
+int chol_inv_node(int n, int o, int size, bool factor, double* L, double* T, int64_t ld, int* info,
+                  hipStream_t st, const Side* sd) {
   if (size == 64) {
     if (factor) hipLaunchKernelGGL(k_leaf<true>, dim3(1), dim3(256), 0, st, n, o, L, T, ld, info);
-    return;   // without FACTOR every leaf ran up front
+    return FGLS_OK;   // without FACTOR every leaf ran up front
   }
   if (size == 128 && factor) {
     hipLaunchKernelGGL(k_node128, dim3(1), dim3(256), 0, st, n, o, L, T, ld, info);
-    return;
+    return FGLS_OK;
   }
   const int s = (size <= 128) ? 64 : ((size / 2 + 127) / 128) * 128;
   const int s2 = size - s;
-  chol_inv_node(n, o, s, factor, L, T, ld, info, st);
+  int rc = chol_inv_node(n, o, s, factor, L, T, ld, info, st, sd);
+  if (rc) return rc;
   double* L21r = L + (size_t)(o + s) * ld + o;   // R-view lower-left: S21 (then L21 row-major)
   double* L21c = L + (size_t)o * ld + o + s;     // R-view upper-right: L21 column-major
   const double* T11 = T + (size_t)o * ld + o;
@@ -784,18 +800,28 @@ void chol_inv_node(int n, int o, int size, bool factor, double* L, double* T, in
   double* Xc = T + (size_t)o * ld + o + s;       // R-view upper-right: X column-major
   double* T21r = T + (size_t)(o + s) * ld + o;   // R-view lower-left: T21 row-major
   const bool big = (long)(s2 / 64) * (s / 64) > FGLS_TG_COPY_BLOCKS;
+  const bool aside = sd && size <= FGLS_SIDE_MAX_SIZE;
   if (factor)
     tgemm_launch<TG_BUPPER, false, false, false>(s2, s, s, 1.0, L21r, T11, L21c, ld, st);   // L21 = S21 T11'
   if (big) hipLaunchKernelGGL(k_tcopy, dim3(s / 64, s2 / 64), dim3(256), 0, st, L21c, L21r, ld);
+  // X = L21 T11
+  const hipStream_t xs = aside ? sd->s : st;
+  if (aside) {
+    HIP_TRY(hipEventRecord(sd->fork, st));
+    HIP_TRY(hipStreamWaitEvent(sd->s, sd->fork, 0));
+  }
+  if (big) tgemm_launch<TG_BLOWER, false, false, true>(s2, s, s, 1.0, L21r, T11, Xc, ld, xs);
+  else tgemm_launch<TG_BLOWER, false, true, true>(s2, s, s, 1.0, L21c, T11, Xc, ld, xs);
+  if (aside) HIP_TRY(hipEventRecord(sd->join, sd->s));
   if (factor) {
     double* S22 = L + (size_t)(o + s) * ld + o + s;   // S22 -= L21 L21'
     if (big) tgemm_launch<TG_CUPPER, true, false, false>(s2, s2, s, -1.0, L21r, L21r, S22, ld, st);
     else tgemm_launch<TG_CUPPER, true, true, true>(s2, s2, s, -1.0, L21c, L21c, S22, ld, st);
   }
-  chol_inv_node(n, o + s, s2, factor, L, T, ld, info, st);
-  if (big) tgemm_launch<TG_BLOWER, false, false, true>(s2, s, s, 1.0, L21r, T11, Xc, ld, st);   // X = L21 T11
-  else tgemm_launch<TG_BLOWER, false, true, true>(s2, s, s, 1.0, L21c, T11, Xc, ld, st);
+  if ((rc = chol_inv_node(n, o + s, s2, factor, L, T, ld, info, st, sd))) return rc;
+  if (aside) HIP_TRY(hipStreamWaitEvent(st, sd->join, 0));
   tgemm_launch<TG_BUPPER, false, false, false>(s, s2, s2, -1.0, Xc, T22, T21r, ld, st);   // T21' = -X' T22'
+  return FGLS_OK;
 }
 
 int factor_on_device(fgls_hip_ctx* c, const double* dPhi, int64_t ldphi, double h2, double sigma2,
@@ -815,10 +841,17 @@ namespace fgls {
 int chol_inv(fgls_hip_ctx* c, bool factor, hipStream_t st) {
   const int n = c->n, N = c->ldn;
   const int64_t ld = N;
+  if (!c->side) {
+    HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  }
+  const Side sd{c->side, c->ev_fork, c->ev_join};
   if (!factor)
     hipLaunchKernelGGL(k_leaf<false>, dim3(N / 64), dim3(256), 0, st, n, 0, c->L, c->Linv, ld,
                        (int*)nullptr);
-  chol_inv_node(n, 0, N, factor, c->L, c->Linv, ld, c->status + 1, st);
+  int rc = chol_inv_node(n, 0, N, factor, c->L, c->Linv, ld, c->status + 1, st, &sd);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_zero_upper, dim3(N), dim3(256), 0, st, N, c->Linv, ld);
   HIP_TRY(hipGetLastError());
   return FGLS_OK;
