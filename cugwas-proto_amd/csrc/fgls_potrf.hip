@@ -167,15 +167,23 @@ __device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, do
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lg = lane >> 4;
   const int nl = min(64, n - o);   // live rows of the block (may be <= 0)
-  for (int idx = tid; idx < 4096; idx += 256) {
-    // FACTOR: row-major source, c fastest; else column-major, r fastest
-    const int r = FACTOR ? idx >> 6 : idx & 63, c = FACTOR ? idx & 63 : idx >> 6;
-    if (!FROM_LDS) {
-      double v = (r == c) ? 1.0 : 0.0;
-      if (r < nl && c <= r)
-        v = FACTOR ? Lb[(size_t)(o + r) * ld + o + c] : Lb[(size_t)(o + c) * ld + o + r];
-      A[r][c] = v;
+  // all 16 loads of a thread in flight at once (every address is inside the
+  // buffer; the values outside the live lower triangle are replaced)
+  double sv[16];
+  if (!FROM_LDS) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      // FACTOR: row-major source, c fastest; else column-major, r fastest
+      const int idx = tid + 256 * e;
+      const int r = FACTOR ? idx >> 6 : idx & 63, c = FACTOR ? idx & 63 : idx >> 6;
+      sv[e] = FACTOR ? Lb[(size_t)(o + r) * ld + o + c] : Lb[(size_t)(o + c) * ld + o + r];
     }
+  }
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int idx = tid + 256 * e;
+    const int r = FACTOR ? idx >> 6 : idx & 63, c = FACTOR ? idx & 63 : idx >> 6;
+    if (!FROM_LDS) A[r][c] = (r < nl && c <= r) ? sv[e] : ((r == c) ? 1.0 : 0.0);
     Tm[r][c] = 0.0;   // the strict upper triangle of T stays zero: the block
                       // products below run over whole 16-blocks
   }
@@ -194,10 +202,9 @@ __device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, do
       const double sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
       t[r] = (r == lane) ? irr[r] : ((r > lane) ? -sum * irr[r] : 0.0);
     }
-    if (lane < 16) {
+    if (lane < 16) {   // whole columns: t[r] = 0 for r < lane
 #pragma unroll
-      for (int r = 0; r < 16; ++r)
-        if (r >= lane) Tm[c0 + r][c0 + lane] = t[r];
+      for (int r = 0; r < 16; ++r) Tm[c0 + r][c0 + lane] = t[r];
     }
   };
 
@@ -230,10 +237,11 @@ __device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, do
 #pragma unroll
           for (int j = k + 1; j < 16; ++j) a[j] = fma(-lk, lane_bcast(lk, j), a[j]);
         }
+        // whole rows (unmasked stores): the strict upper triangle of a
+        // diagonal block of A is never read
         if (lane < 16) {
 #pragma unroll
-          for (int j = 0; j < 16; ++j)
-            if (j <= lane) A[c0 + lane][c0 + j] = a[j];
+          for (int j = 0; j < 16; ++j) A[c0 + lane][c0 + j] = a[j];
         }
         diag_inverse(c0, a, ip);
       }
@@ -411,7 +419,7 @@ __global__ __launch_bounds__(256) void k_node128(int n, int o, double* __restric
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int r = r0 + 16 * ti + li, c = c0 + 16 * tj + lg + 4 * v;
-          s22[ti][tj][v] = (r < nl2 && c <= r) ? Lb[(size_t)(o2 + r) * ld + o2 + c] : 0.0;
+          s22[ti][tj][v] = Lb[(size_t)(o2 + r) * ld + o2 + c];   // in the buffer; used where c <= r < nl2
         }
   }
 
