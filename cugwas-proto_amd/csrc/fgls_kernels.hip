@@ -574,10 +574,24 @@ __global__ __launch_bounds__(256) void k_slice_tri(int ldn, const double* __rest
     const int r = 4 * w + i;
     const double* Lr = Linv + (size_t)(row0 + r) * ldn;
     double mx = 0.0, off = 0.0;
-    for (int k = lane; k < d0 + 64; k += 64) {
-      const double a = fabs(Lr[k]);
+    // 64-column chunks j = 0 .. nst - 1 (the last is the diagonal step),
+    // eight loads in flight per lane
+    int j = 0;
+    for (; j + 8 <= nst; j += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = Lr[64 * (j + u) + lane];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const double a = fabs(v[u]);
+        mx = fmax(mx, a);
+        if (TOP && j + u < nst - 1) off = fmax(off, a);
+      }
+    }
+    for (; j < nst; ++j) {
+      const double a = fabs(Lr[64 * j + lane]);
       mx = fmax(mx, a);
-      if (TOP && k < d0) off = fmax(off, a);
+      if (TOP && j < nst - 1) off = fmax(off, a);
     }
     mx = wave_max(mx);
     if (TOP) off = wave_max(off);
