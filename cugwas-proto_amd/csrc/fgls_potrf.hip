@@ -766,7 +766,7 @@ void tgemm_launch(int M, int N, int K, double alpha, const double* A, const doub
 
 // The second stream of the recursion: on the nodes of at most
 // FGLS_SIDE_MAX_SIZE rows, X = L21 T11 needs only L21 and T11, so it runs on
-// `s` (forked after the L21 product) while the main stream recurses into
+// `s` (forked after S22 -= L21 L21') while the main stream recurses into
 // S22, and the main stream joins before T21' = -X' T22'.  One fork and one
 // join event serve every node: each wait is enqueued right after the record
 // it needs, and a later record of `join` follows every earlier X on the
@@ -812,7 +812,13 @@ int chol_inv_node(int n, int o, int size, bool factor, double* L, double* T, int
   if (factor)
     tgemm_launch<TG_BUPPER, false, false, false>(s2, s, s, 1.0, L21r, T11, L21c, ld, st);   // L21 = S21 T11'
   if (big) hipLaunchKernelGGL(k_tcopy, dim3(s / 64, s2 / 64), dim3(256), 0, st, L21c, L21r, ld);
-  // X = L21 T11
+  if (factor) {
+    double* S22 = L + (size_t)(o + s) * ld + o + s;   // S22 -= L21 L21'
+    if (big) tgemm_launch<TG_CUPPER, true, false, false>(s2, s2, s, -1.0, L21r, L21r, S22, ld, st);
+    else tgemm_launch<TG_CUPPER, true, true, true>(s2, s2, s, -1.0, L21c, L21c, S22, ld, st);
+  }
+  // X = L21 T11, forked after the symmetric update (beside which it would
+  // only share the device) so that it runs beside the right child
   const hipStream_t xs = aside ? sd->s : st;
   if (aside) {
     HIP_TRY(hipEventRecord(sd->fork, st));
@@ -821,11 +827,6 @@ int chol_inv_node(int n, int o, int size, bool factor, double* L, double* T, int
   if (big) tgemm_launch<TG_BLOWER, false, false, true>(s2, s, s, 1.0, L21r, T11, Xc, ld, xs);
   else tgemm_launch<TG_BLOWER, false, true, true>(s2, s, s, 1.0, L21c, T11, Xc, ld, xs);
   if (aside) HIP_TRY(hipEventRecord(sd->join, sd->s));
-  if (factor) {
-    double* S22 = L + (size_t)(o + s) * ld + o + s;   // S22 -= L21 L21'
-    if (big) tgemm_launch<TG_CUPPER, true, false, false>(s2, s2, s, -1.0, L21r, L21r, S22, ld, st);
-    else tgemm_launch<TG_CUPPER, true, true, true>(s2, s2, s, -1.0, L21c, L21c, S22, ld, st);
-  }
   if ((rc = chol_inv_node(n, o + s, s2, factor, L, T, ld, info, st, sd))) return rc;
   if (aside) HIP_TRY(hipStreamWaitEvent(st, sd->join, 0));
   tgemm_launch<TG_BUPPER, false, false, false>(s, s2, s2, -1.0, Xc, T22, T21r, ld, st);   // T21' = -X' T22'
