@@ -28,8 +28,8 @@
 // each factor needs one layout ("R-view" = the buffer read row-major, element
 // (i, j) at i ld + j):
 //   L buffer: L column-major in the R-view upper triangle (the final factor);
-//             the R-view lower triangle holds the Schur complements S (and
-//             L row-major on the diagonal 128 x 128 blocks).
+//             the R-view lower triangle holds M and then the Schur
+//             complements S.
 //   T buffer: R-view lower = T row-major (the final L^-1); the R-view upper
 //             holds temporaries and is zeroed at the end.
 // The temporary X of a node lives in the T buffer's R-view upper-right block
@@ -140,9 +140,9 @@ __device__ __forceinline__ v4d blk16(const TU& U, int ur, int uc, const TV& V, i
 // else:   L from the caller's factor (column-major lower triangle); each
 //         wave inverts one 16 x 16 diagonal block.
 // Then the off-diagonal blocks of T by block rows, T_IJ = -T_II sum_{J<=K<I}
-// L_IK T_KJ (wave J, FP64 MFMA).  L (FACTOR) and T are written to both triangles of their
-// buffers, every store instruction covering contiguous row segments.  Rows
-// >= n are identity rows; their stored values are zero.  The first
+// L_IK T_KJ (wave J, FP64 MFMA).  L (FACTOR) is written column-major and T
+// row-major, every store instruction covering a contiguous row segment.
+// Rows >= n are identity rows; their stored values are zero.  The first
 // non-positive pivot is recorded in info (1-based), and the pivot is replaced
 // by 1 so that the later arithmetic stays finite.
 // ---------------------------------------------------------------------------
@@ -306,14 +306,12 @@ __device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, do
     __syncthreads();
   }
 
-  // buffer row o + a, column o + b: b <= a holds the row-major entry (a, b),
-  // b > a the column-major entry (b, a); L only for live rows, T everywhere
-  // (zero for rows >= n)
+  // buffer row o + a, column o + b: L column-major (b >= a holds L(b, a)),
+  // live rows only; T row-major (b <= a holds T(a, b)), zero for rows >= n
   for (int idx = tid; idx < 4096; idx += 256) {
     const int ra = idx >> 6, cb = idx & 63;
-    const int r = max(ra, cb), c = min(ra, cb);   // the lower-triangle entry
-    if (FACTOR && r < nl) Lb[(size_t)(o + ra) * ld + o + cb] = A[r][c];
-    Tb[(size_t)(o + ra) * ld + o + cb] = (r < nl) ? Tm[r][c] : 0.0;
+    if (FACTOR && cb >= ra && cb < nl) Lb[(size_t)(o + ra) * ld + o + cb] = A[cb][ra];
+    if (cb <= ra) Tb[(size_t)(o + ra) * ld + o + cb] = (ra < nl) ? Tm[ra][cb] : 0.0;
   }
 }
 
@@ -332,9 +330,8 @@ __global__ __launch_bounds__(256) void k_leaf(int n, int o0, double* __restrict_
 //   (into LDS);  leaf(S22) -> L22, T22;  T21 = -T22 X.
 // Everything the node produces for itself stays in LDS (L21, X, the updated
 // S22, T11 and T22); only S11 and S21, written by earlier launches, are read
-// from the buffers.  The buffers end as after the separate launches: L and
-// T in both triangles of the two diagonal blocks, L21 row-major (R-view
-// lower-left) and column-major (R-view upper-right), T21 likewise.  The
+// from the buffers.  The node writes L column-major (its two diagonal blocks
+// and L21) and T row-major (its two diagonal blocks and T21).  The
 // 64 x 64 x 64 products run on FP64 MFMA with both operands in LDS (a
 // quadrant per wave), T's rows >= n zeroed first so that they match the
 // stored factor.
@@ -439,7 +436,6 @@ __global__ __launch_bounds__(256) void k_node128(int n, int o, double* __restric
   __syncthreads();   // every wave is done reading S21
   for_quadrant(acc, w, lane, [&](int r, int c, double v) {
     sh.P[r][c] = v;
-    Lb[(size_t)(o2 + r) * ld + o + c] = v;   // row-major (R-view lower-left)
     Lb[(size_t)(o + c) * ld + o2 + r] = v;   // column-major (R-view upper-right)
   });
   __syncthreads();
@@ -476,7 +472,6 @@ __global__ __launch_bounds__(256) void k_node128(int n, int o, double* __restric
   lds_product<true>(Tm, sh.Q, w, lane, acc);
   for_quadrant(acc, w, lane, [&](int c, int r, double v) {
     Tb[(size_t)(o2 + c) * ld + o + r] = -v;   // row-major (R-view lower-left)
-    Tb[(size_t)(o + r) * ld + o2 + c] = -v;   // column-major (R-view upper-right)
   });
 }
 
