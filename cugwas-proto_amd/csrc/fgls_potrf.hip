@@ -133,12 +133,13 @@ __device__ __forceinline__ v4d blk16(const TU& U, int ur, int uc, const TV& V, i
 //         panel jb: wave 0 factors the 16 x 16 diagonal block in registers
 //         (lane r holds row r; the column and the pivot are scalar
 //         broadcasts, v_readlane; 1 / sqrt of the pivot from v_rsq_f64 and
-//         two Newton steps) and inverts it (lane c: column c of T_jj);
-//         then the waves form the panel below, L_ij = S_ij T_jj', and
-//         update the trailing lower triangle, S_ik -= L_ij L_kj', a 16 x 16
-//         block per wave on FP64 MFMA.
-// else:   L from the caller's factor (column-major lower triangle); each
-//         wave inverts one 16 x 16 diagonal block.
+//         two Newton steps), the rows below it in the other lanes of the
+//         wave, so that the same 16 pivot steps form the whole column panel;
+//         then the waves update the trailing lower triangle,
+//         S_ik -= L_ij L_kj', a 16 x 16 block per wave on FP64 MFMA.
+// else:   L from the caller's factor (column-major lower triangle).
+// Each wave then inverts one 16 x 16 diagonal block (lane c: column c of
+// T_jj), off the chain of the factorisation.
 // Then the off-diagonal blocks of T by block rows, T_IJ = -T_II sum_{J<=K<I}
 // L_IK T_KJ (wave J, FP64 MFMA).  L (FACTOR) is written column-major and T
 // row-major, every store instruction covering a contiguous row segment.
@@ -152,6 +153,7 @@ struct LeafLds {
   double A[64][65];
   double Tm[64][65];
   double X[3][16][17];
+  double dg[64];   // 1 / L(k, k)
 };
 
 // Leaf body for the block at o, run by all 256 threads of the workgroup.
@@ -213,8 +215,12 @@ __device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, do
     for (int jb = 0; jb < 4; ++jb) {
       const int c0 = 16 * jb;
       if (w == 0) {
-        double a[16], ip[16];
-        const int rr = min(lane, 15);
+        // the 16-column panel from row c0 down, lane i holding row c0 + i:
+        // the diagonal block's rows and the 48 - c0 rows below it, all
+        // factored by the same pivot steps (the lanes past the panel repeat
+        // its last row)
+        double a[16];
+        const int rr = min(lane, 63 - c0);
 #pragma unroll
         for (int j = 0; j < 16; ++j) a[j] = A[c0 + rr][c0 + j];
 #pragma unroll
@@ -231,29 +237,18 @@ __device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, do
           double y = __builtin_amdgcn_rsq(d);
 #pragma unroll
           for (int it = 0; it < 2; ++it) y = fma(0.5 * y, fma(-d * y, y, 1.0), y);
-          ip[k] = y;
-          const double lk = (lane == k) ? d * y : a[k] * y;   // L(lane, k)
+          if (lane == 0) sh.dg[c0 + k] = y;
+          const double lk = (lane == k) ? d * y : a[k] * y;   // L(c0 + lane, c0 + k)
           a[k] = lk;
 #pragma unroll
           for (int j = k + 1; j < 16; ++j) a[j] = fma(-lk, lane_bcast(lk, j), a[j]);
         }
         // whole rows (unmasked stores): the strict upper triangle of a
         // diagonal block of A is never read
-        if (lane < 16) {
+        if (lane < 64 - c0) {
 #pragma unroll
           for (int j = 0; j < 16; ++j) A[c0 + lane][c0 + j] = a[j];
         }
-        diag_inverse(c0, a, ip);
-      }
-      __syncthreads();
-      // panel: L_Ij = S_Ij T_jj' for the blocks I > jb below the diagonal
-      // block, one per wave on FP64 MFMA (T_jj(c, q) = 0 for q > c); each
-      // wave overwrites only the block it read
-      if (jb + 1 + w < 4) {
-        const int I = jb + 1 + w;
-        const v4d pl = blk16<false>(A, 16 * I, c0, Tm, c0, c0, li, lg, v4d{0.0, 0.0, 0.0, 0.0});
-#pragma unroll
-        for (int v = 0; v < 4; ++v) A[16 * I + li][c0 + lg + 4 * v] = pl[v];
       }
       __syncthreads();
       // trailing lower triangle: S_IJ -= L_Ij L_Jj' for jb < J <= I < 4, the
@@ -277,13 +272,16 @@ __device__ __forceinline__ void leaf64(int n, int o, double* __restrict__ Lb, do
       __syncthreads();
     }
     if (tid == 0 && bad != 0 && *info == 0) *info = bad;
-  } else {
+  }
+  // the diagonal blocks of T, off the factorisation's chain: wave w inverts
+  // block w (FACTOR: 1 / L(r, r) kept from the pivots)
+  {
     double a[16], irr[16];
     const int c0 = 16 * w, rr = min(lane, 15);
 #pragma unroll
     for (int j = 0; j < 16; ++j) a[j] = A[c0 + rr][c0 + j];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) irr[r] = 1.0 / lane_bcast(a[r], r);
+    for (int r = 0; r < 16; ++r) irr[r] = FACTOR ? sh.dg[c0 + r] : 1.0 / lane_bcast(a[r], r);
     diag_inverse(c0, a, irr);
     __syncthreads();
   }
