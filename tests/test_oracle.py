@@ -184,9 +184,9 @@ def test_eigen_oracle_matches_reference_build(tmp_path, wXR):
     fgls_eigen (3 traits with their own h2 and sigma2; one and two columns
     per SNP).  The reference's
     asynchronous B/V writes (fgls_eigen.c:574-600, :646-650) now and then
-    leave a wrong V behind (about one run in six here, B always right, with
-    one compute thread too): B must match on every run, V on one of up to
-    three runs."""
+    leave a wrong V behind (about one run in six here, with one compute
+    thread too) and, more rarely, unwritten B entries: B and V must both
+    match on one of up to five runs."""
     n, wXL, m, t, x_b = 300, 3, 500, 3, 128
     prob = make_problem(n=n, wXL=wXL, m=m, seed=3, t=t, wXR=wXR)
     H = np.array([0.4, 0.7, 0.2])
@@ -195,17 +195,17 @@ def test_eigen_oracle_matches_reference_build(tmp_path, wXR):
     write_problem_dir(prob, d, H=H, Sig=Sig)
     Bo, Vo = orc.fgls_eigen(prob.Phi, H, Sig, prob.XL, prob.XR, prob.Y, wXR=wXR, x_b=x_b)
     errors = []
-    for attempt in range(3):
+    for attempt in range(5):
         for f in ("B.out", "V.out"):
             if os.path.exists(os.path.join(d, f)):
                 os.remove(os.path.join(d, f))
         ref_runner.fgls_eigen_dir(d, n, wXL + wXR, m, t, wXL, x_b, threads=1)
         B = np.fromfile(os.path.join(d, "B.out"))
         V = np.fromfile(os.path.join(d, "V.out"))
-        e = orc.parity_errors(Bo, Vo, B, V, wXL + wXR)
-        assert e["B_scaled"] < 1e-12, e
+        with np.errstate(divide="ignore", invalid="ignore"):
+            e = orc.parity_errors(Bo, Vo, B, V, wXL + wXR)
         errors.append(e)
-        if e["V_scaled"] < 1e-12 and e["V_upper_zero"]:
+        if e["B_scaled"] < 1e-12 and e["V_scaled"] < 1e-12 and e["V_upper_zero"]:
             break
     else:
         raise AssertionError(errors)
